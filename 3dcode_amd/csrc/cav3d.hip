@@ -1,0 +1,1544 @@
+// cav3d: the pseudo-time LU-SGS step on MI355X (gfx950).
+//
+// Layout: every block's ghosted cell arrays (-1:imax+3 etc.) are concatenated
+// in one allocation per field and stored structure-of-arrays, i fastest, so a
+// wavefront reads 64 consecutive cells of one variable.  Persistent state is
+// pv (npv fields) and the eddy viscosity; the 13 cell properties the path
+// reads (rho, h, c^2, the eight EOS derivatives, mu, kappa) are kept per cell
+// as the reference keeps dv/tv, and are re-derived from pv wherever the
+// reference calls the EOS.
+//
+// One step (t_lusgs%timeinteg, src/update_module.f90:455-733):
+//   k_timestep            local dt + global min          (timestep_module)
+//   k_bc / k_halo         faces, interfaces, edges, corners (bc_module setbc)
+//   k_face<D>             reconstruction + EOS + inviscid/viscous face flux
+//   k_combine             flux differences in the reference's i,j,k order
+//                         + Merkle / SST / BDF sources   (rhs_module calrhs)
+//   k_sweep<fwd>          one hyperplane i+j+k=s of the L / U sweep
+//   k_update, k_eddy      pv += dq, clip, EOS, norms; SST eddy viscosity
+#include "phys.hpp"
+#include "../../include/cav3d.h"
+
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <thread>
+#include <vector>
+
+using namespace cav;
+
+#define MAXB 16
+
+namespace {
+
+thread_local std::string g_err;
+int fail(const std::string &m, int code = -1) { g_err = m; return code; }
+
+#define HIPCHK(x)                                                                    \
+    do {                                                                             \
+        hipError_t e_ = (x);                                                         \
+        if (e_ != hipSuccess) return fail(std::string(#x) + ": " + hipGetErrorString(e_), -3); \
+    } while (0)
+
+struct BlkD {
+    int I, J, K, nx, ny;
+    long g0, c0, w0, f0[3], cx0, ex0, tx0;
+    long ng, nc, nw, nf[3], ncx, nex, ntx;
+};
+
+struct Seg {          // one boundary range (face BC, edge or corner)
+    int b, fn, face, pad;
+    int is[3], ie[3], org[3], dir[3], nb[6][3];
+    long start;
+};
+
+struct Con {          // one interface connection
+    int src, dst;
+    int is[3], ie[3], isd[3], tm[9];
+    long start;
+};
+
+struct DS {           // device state, passed by value
+    int nb, npv, nbcseg[3], ncon;
+    long NG, NC, NW, NF[3], NCX, NEX, NTX, NBC[3], NHALO;
+    double *pv, *dc, *mut, *vol, *ydn, *cx, *ex, *tx;
+    double *res, *dt, *icav, *itt, *ocut, *rcav, *qq;
+    double *dqf, *dqb, *dcv;
+    double *ff[3];    // face fluxes [2*npv][NF]
+    double *red;      // norm partials
+    double *scal;     // [0] dt min bits, [1] accumulated time, [2..] norms
+    Seg *seg[3];
+    Con *con;
+    BlkD b[MAXB];
+    Par P;
+};
+
+enum { FN_NONE = 0, FN_WALLINV, FN_WALLVIS, FN_WALLVISKW, FN_INSUB, FN_OUTSUB, FN_FARFIELD, FN_SYM,
+       FN_INSUPER, FN_COPY, FN_EMPTY, FN_EDGEWW, FN_EDGEWWKW, FN_EDGENOWALL, FN_CORNERWW,
+       FN_CORNERWWKW, FN_CORNERNOWALL };
+enum { F_NONE = -1, F_IMIN = 0, F_IMAX, F_JMIN, F_JMAX, F_KMIN, F_KMAX };
+
+// ------------------------------------------------------------ index helpers
+__device__ __forceinline__ long gidx(const BlkD &B, int i, int j, int k)
+{
+    return B.g0 + (long)(i + 1) + (long)B.nx * ((long)(j + 1) + (long)B.ny * (long)(k + 1));
+}
+__device__ __forceinline__ long cidx(const BlkD &B, int i, int j, int k)
+{
+    return B.c0 + (long)(i - 2) + (long)(B.I - 1) * ((long)(j - 2) + (long)(B.J - 1) * (long)(k - 2));
+}
+__device__ __forceinline__ long widx(const BlkD &B, int i, int j, int k)
+{
+    return B.w0 + (long)(i - 1) + (long)(B.I + 1) * ((long)(j - 1) + (long)(B.J + 1) * (long)(k - 1));
+}
+// face-vector arrays by axis: 0 -> cx(1:I,1:J+1,1:K+1), 1 -> ex(1:I+1,1:J,1:K+1), 2 -> tx(1:I+1,1:J+1,1:K)
+__device__ __forceinline__ void face_vec(const DS &S, const BlkD &B, int ax, int i, int j, int k, double *v)
+{
+    const double *a;
+    long n, o;
+    if (ax == 0) { a = S.cx; n = S.NCX; o = B.cx0 + (i - 1) + (long)B.I * ((j - 1) + (long)(B.J + 1) * (k - 1)); }
+    else if (ax == 1) { a = S.ex; n = S.NEX; o = B.ex0 + (i - 1) + (long)(B.I + 1) * ((j - 1) + (long)B.J * (k - 1)); }
+    else { a = S.tx; n = S.NTX; o = B.tx0 + (i - 1) + (long)(B.I + 1) * ((j - 1) + (long)(B.J + 1) * (k - 1)); }
+    v[0] = a[o]; v[1] = a[n + o]; v[2] = a[2 * n + o];
+}
+
+__device__ __forceinline__ int find_blk(const DS &S, long t, int which)
+{
+    int b = 0;
+    for (int q = 1; q < S.nb; q++) {
+        long o = which == 0 ? S.b[q].c0 : (which == 1 ? S.b[q].g0 : S.b[q].f0[which - 2]);
+        if (t >= o) b = q;
+    }
+    return b;
+}
+
+__device__ __forceinline__ void load_cell(const DS &S, const BlkD &B, int i, int j, int k, CellJ &c)
+{
+    long g = gidx(B, i, j, k);
+    for (int n = 0; n < S.npv; n++) c.pv[n] = S.pv[n * S.NG + g];
+    for (int m = 0; m < ND; m++) c.d[m] = S.dc[m * S.NG + g];
+    c.mut = S.mut ? S.mut[g] : 0.0;
+    c.vol = S.vol[widx(B, i, j, k)];
+}
+
+__device__ __forceinline__ void cell_faces(const DS &S, const BlkD &B, int i, int j, int k, double f6[6][3])
+{
+    face_vec(S, B, 0, i - 1, j, k, f6[0]);
+    face_vec(S, B, 0, i, j, k, f6[1]);
+    face_vec(S, B, 1, i, j - 1, k, f6[2]);
+    face_vec(S, B, 1, i, j, k, f6[3]);
+    face_vec(S, B, 2, i, j, k - 1, f6[4]);
+    face_vec(S, B, 2, i, j, k, f6[5]);
+}
+
+__device__ __forceinline__ void cell_of(const BlkD &B, long r, int &i, int &j, int &k)
+{
+    int ni = B.I - 1, nj = B.J - 1;
+    i = 2 + (int)(r % ni);
+    j = 2 + (int)((r / ni) % nj);
+    k = 2 + (int)(r / ((long)ni * nj));
+}
+
+// ------------------------------------------------------------ EOS kernels
+// dc (and mut = emutref) of interior cells from pv
+__global__ void k_eos_cells(DS S, int interior_only, int set_mut)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long n = interior_only ? S.NC : S.NG;
+    if (t >= n) return;
+    int b = find_blk(S, t, interior_only ? 0 : 1);
+    const BlkD &B = S.b[b];
+    long g;
+    if (interior_only) {
+        int i, j, k;
+        cell_of(B, t - B.c0, i, j, k);
+        g = gidx(B, i, j, k);
+    } else g = t;
+    double d[ND];
+    mixture_eos<double>(S.pv[g] + S.P.pref, S.pv[4 * S.NG + g], S.pv[5 * S.NG + g], S.pv[6 * S.NG + g], d, S.P.iturb != -3);
+    for (int m = 0; m < ND; m++) S.dc[m * S.NG + g] = d[m];
+    if (set_mut && S.mut) S.mut[g] = S.P.emutref;
+}
+
+// full dv(18), tv(1:2) of every ghosted cell, AoS for the host
+__global__ void k_eos_full(DS S, long g0, long n, double *dv, double *tv2)
+{
+    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    long g = g0 + q;
+    double d[ND], full[18];
+    mixture_eos<double>(S.pv[g] + S.P.pref, S.pv[4 * S.NG + g], S.pv[5 * S.NG + g], S.pv[6 * S.NG + g], d,
+                        S.P.iturb != -3, full);
+    for (int m = 0; m < 18; m++) dv[18 * q + m] = full[m];
+    tv2[2 * q] = d[D_MU];
+    tv2[2 * q + 1] = d[D_KAP];
+}
+
+__global__ void k_eos_points(Par P, int n, const double *p, const double *t, const double *y1, const double *y2,
+                             double *dv, double *tv)
+{
+    int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    double d[ND], full[18];
+    mixture_eos<double>(p[q], t[q], y1[q], y2[q], d, true, full);
+    for (int m = 0; m < 18; m++) dv[18 * q + m] = full[m];
+    tv[2 * q] = d[D_MU];
+    tv[2 * q + 1] = d[D_KAP];
+}
+
+// ------------------------------------------------------------ time step
+__global__ void k_reset_scal(DS S)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        unsigned long long big = 0x7ff0000000000000ull;   // +inf
+        reinterpret_cast<unsigned long long *>(S.scal)[0] = big;
+    }
+}
+
+__global__ void k_timestep(DS S)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double dt = 1.0e300;
+    if (t < S.NC) {
+        const BlkD &B = S.b[find_blk(S, t, 0)];
+        int i, j, k;
+        cell_of(B, t - B.c0, i, j, k);
+        CellJ c;
+        load_cell(S, B, i, j, k, c);
+        double f6[6][3];
+        cell_faces(S, B, i, j, k, f6);
+        dt = (S.P.local == -1) ? S.P.cfl : local_dt(S.P, c, f6);
+        S.dt[t] = dt;
+    }
+    // wavefront min, then one atomic per wavefront (positive doubles order as integers)
+    for (int o = 32; o > 0; o >>= 1) dt = fmin(dt, __shfl_xor(dt, o));
+    if ((threadIdx.x & 63) == 0 && dt < 1.0e299)
+        atomicMin(reinterpret_cast<unsigned long long *>(S.scal), (unsigned long long)__double_as_longlong(dt));
+}
+
+__global__ void k_time_acc(DS S)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        double m = __longlong_as_double((long long)reinterpret_cast<unsigned long long *>(S.scal)[0]);
+        if (S.P.local == -1) m = S.P.cfl;
+        S.scal[1] += m;
+        S.scal[2] = m;
+    }
+}
+
+__global__ void k_dt_fill(DS S)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < S.NC) S.dt[t] = __longlong_as_double((long long)reinterpret_cast<unsigned long long *>(S.scal)[0]);
+}
+
+// ------------------------------------------------------------ boundary conditions
+__device__ __forceinline__ void put_cell(const DS &S, long g, const double *pv, const double *d, double mut)
+{
+    for (int n = 0; n < S.npv; n++) S.pv[n * S.NG + g] = pv[n];
+    for (int m = 0; m < ND; m++) S.dc[m * S.NG + g] = d[m];
+    if (S.mut) S.mut[g] = mut;
+}
+
+__device__ void surface_of(const Seg &s, int i, int j, int k, int *o)
+{
+    o[0] = s.org[0] + s.dir[0] * i;
+    o[1] = s.org[1] + s.dir[1] * j;
+    o[2] = s.org[2] + s.dir[2] * k;
+    switch (s.face) {
+    case F_IMIN: o[0] = s.org[0] + s.dir[0] * s.ie[0]; break;
+    case F_IMAX: o[0] = s.org[0] + s.dir[0] * s.is[0]; break;
+    case F_JMIN: o[1] = s.org[1] + s.dir[1] * s.ie[1]; break;
+    case F_JMAX: o[1] = s.org[1] + s.dir[1] * s.is[1]; break;
+    case F_KMIN: o[2] = s.org[2] + s.dir[2] * s.ie[2]; break;
+    case F_KMAX: o[2] = s.org[2] + s.dir[2] * s.is[2]; break;
+    default: break;
+    }
+}
+
+// outward area vector of a boundary face at its surface cell
+__device__ void outward_of(const DS &S, const BlkD &B, int face, const int *s, double *n)
+{
+    double sg = 1.0;
+    switch (face) {
+    case F_IMIN: face_vec(S, B, 0, s[0] - 1, s[1], s[2], n); sg = -1.0; break;
+    case F_IMAX: face_vec(S, B, 0, s[0], s[1], s[2], n); break;
+    case F_JMIN: face_vec(S, B, 1, s[0], s[1] - 1, s[2], n); sg = -1.0; break;
+    case F_JMAX: face_vec(S, B, 1, s[0], s[1], s[2], n); break;
+    case F_KMIN: face_vec(S, B, 2, s[0], s[1], s[2] - 1, n); sg = -1.0; break;
+    default: face_vec(S, B, 2, s[0], s[1], s[2], n); break;
+    }
+    n[0] *= sg; n[1] *= sg; n[2] *= sg;
+}
+
+__device__ void apply_bc_cell(const DS &S, const Seg &s, int i, int j, int k)
+{
+    const BlkD &B = S.b[s.b];
+    const Par &P = S.P;
+    const int npv = S.npv;
+    const double *rp = P.bc_pv;
+    int m[3] = {s.org[0] + s.dir[0] * i, s.org[1] + s.dir[1] * j, s.org[2] + s.dir[2] * k};
+    long gm = gidx(B, m[0], m[1], m[2]);
+    long gg = gidx(B, i, j, k);
+    double pv[NPV], d[ND], mut = 0.0;
+    for (int n = 0; n < npv; n++) pv[n] = S.pv[n * S.NG + gm];
+    for (int q = 0; q < ND; q++) d[q] = S.dc[q * S.NG + gm];
+    if (S.mut) mut = S.mut[gm];
+    switch (s.fn) {
+    case FN_WALLVIS:
+        pv[1] = -pv[1]; pv[2] = -pv[2]; pv[3] = -pv[3];
+        put_cell(S, gg, pv, d, mut);
+        break;
+    case FN_WALLVISKW: {
+        int o[3];
+        surface_of(s, i, j, k, o);
+        long gs = gidx(B, o[0], o[1], o[2]);
+        double yd = S.ydn[widx(B, o[0], o[1], o[2])];
+        double var = 1600.0 * S.dc[D_MU * S.NG + gs] / S.dc[D_RHO * S.NG + gs] / (yd * yd) - pv[8];
+        pv[1] = -pv[1]; pv[2] = -pv[2]; pv[3] = -pv[3]; pv[7] = -pv[7]; pv[8] = var;
+        put_cell(S, gg, pv, d, -mut);
+        break;
+    }
+    case FN_WALLINV: case FN_SYM: {
+        int o[3];
+        double n[3];
+        surface_of(s, i, j, k, o);
+        outward_of(S, B, s.face, o, n);
+        n[0] = -n[0]; n[1] = -n[1]; n[2] = -n[2];
+        long gs = gidx(B, o[0], o[1], o[2]);
+        double ps1 = S.pv[1 * S.NG + gs], ps2 = S.pv[2 * S.NG + gs], ps3 = S.pv[3 * S.NG + gs];
+        double dn = ps1 * n[0] + ps2 * n[1] + ps3 * n[2];
+        double nn = n[0] * n[0] + n[1] * n[1] + n[2] * n[2];
+        double v1 = 2.0 * (ps1 - n[0] * dn / nn) - pv[1];
+        double v2 = 2.0 * (ps2 - n[1] * dn / nn) - pv[2];
+        double v3 = 2.0 * (ps3 - n[2] * dn / nn) - pv[3];
+        pv[1] = v1; pv[2] = v2; pv[3] = v3;
+        put_cell(S, gg, pv, d, mut);
+        break;
+    }
+    case FN_INSUB: case FN_OUTSUB: case FN_FARFIELD: case FN_INSUPER: {
+        double press = P.bc_pressure;
+        if (s.fn == FN_INSUPER) {
+            pv[0] = 2.0 * (press - rp[0]) - pv[0];
+            for (int q = 1; q < npv; q++) pv[q] = 2.0 * rp[q] - pv[q];
+            pv[5] = mn(mx(pv[5], 0.0), 1.0);
+            pv[6] = mn(mx(pv[6], 0.0), 1.0);
+        } else {
+            int o[3];
+            double n[3], ps[NPV];
+            surface_of(s, i, j, k, o);
+            outward_of(S, B, s.face, o, n);
+            long gs = gidx(B, o[0], o[1], o[2]);
+            for (int q = 0; q < npv; q++) ps[q] = S.pv[q * S.NG + gs];
+            double rs = S.dc[D_RHO * S.NG + gs], cs = S.dc[D_C2 * S.NG + gs];
+            double dl = 1.0 / sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+            double vel = (ps[1] * n[0] + ps[2] * n[1] + ps[3] * n[2]) * dl;
+            double a = sqrt(sndp2(P, cs, vel * vel));
+            int mode = (s.fn == FN_INSUB) ? 0 : 1;
+            if (s.fn == FN_FARFIELD) {
+                double mach = vel / sqrt(cs);
+                if (mach >= 0.0) mode = (mach >= 1.0) ? 2 : 1;
+                else mode = (mach <= -1.0) ? 3 : 0;
+            }
+            if (mode == 0) {
+                double var = press - rp[0] + ps[0]
+                           - rs * a * (n[0] * (rp[1] - ps[1]) + n[1] * (rp[2] - ps[2]) + n[2] * (rp[3] - ps[3])) * dl;
+                pv[0] = var - pv[0];
+                pv[1] = 2.0 * (rp[1] + n[0] * 0.5 * var / rs / a * dl) - pv[1];
+                pv[2] = 2.0 * (rp[2] + n[1] * 0.5 * var / rs / a * dl) - pv[2];
+                pv[3] = 2.0 * (rp[3] + n[2] * 0.5 * var / rs / a * dl) - pv[3];
+                for (int q = 4; q < npv; q++) pv[q] = 2.0 * rp[q] - pv[q];
+                pv[5] = mn(mx(pv[5], 0.0), 1.0);
+                pv[6] = mn(mx(pv[6], 0.0), 1.0);
+            } else if (mode == 1) {
+                double dp = ps[0] + rp[0] - press;
+                pv[0] = 2.0 * (press - rp[0]) - pv[0];
+                pv[1] = 2.0 * (ps[1] + n[0] * dp / rs / a * dl) - pv[1];
+                pv[2] = 2.0 * (ps[2] + n[1] * dp / rs / a * dl) - pv[2];
+                pv[3] = 2.0 * (ps[3] + n[2] * dp / rs / a * dl) - pv[3];
+            } else if (mode == 3) {
+                pv[0] = 2.0 * (press - rp[0]) - pv[0];
+                for (int q = 1; q < npv; q++) pv[q] = 2.0 * rp[q] - pv[q];
+                pv[5] = mn(mx(pv[5], 0.0), 1.0);
+                pv[6] = mn(mx(pv[6], 0.0), 1.0);
+            }
+        }
+        double mu_keep[2] = {d[D_MU], d[D_KAP]};
+        mixture_eos<double>(pv[0] + rp[0], pv[4], pv[5], pv[6], d, P.iturb != -3);
+        if (P.iturb == -3) { d[D_MU] = mu_keep[0]; d[D_KAP] = mu_keep[1]; }
+        put_cell(S, gg, pv, d, mut);
+        break;
+    }
+    case FN_COPY:
+        put_cell(S, gg, pv, d, mut);
+        break;
+    case FN_EDGEWW: case FN_EDGEWWKW: case FN_CORNERWW: case FN_CORNERWWKW: {
+        const bool corner = (s.fn == FN_CORNERWW || s.fn == FN_CORNERWWKW);
+        const int nn = corner ? 6 : 3;
+        double pp[NPV];
+        for (int q = 0; q < npv; q++) pp[q] = -pv[q];
+        for (int t = 0; t < nn; t++) {
+            const int *nb = s.nb[t];
+            if (!corner && nb[0] == 0 && nb[1] == 0 && nb[2] == 0) continue;
+            long gn = gidx(B, nb[0] + s.dir[0] * i, nb[1] + s.dir[1] * j, nb[2] + s.dir[2] * k);
+            for (int q = 0; q < npv; q++) pp[q] = pp[q] - S.pv[q * S.NG + gn];
+        }
+        pv[1] = pp[1]; pv[2] = pp[2]; pv[3] = pp[3];
+        if (s.fn == FN_EDGEWW || s.fn == FN_CORNERWW) {
+            if (npv > 7) { pv[7] = pp[7]; pv[8] = pp[8]; }
+        } else {
+            double yd = S.ydn[widx(B, m[0], m[1], m[2])];
+            double var = 800.0 * d[D_MU] / d[D_RHO] / (yd * yd);
+            pv[7] = pp[7];
+            pv[8] = (corner ? 8.0 : 4.0) * var + pp[8];
+        }
+        put_cell(S, gg, pv, d, mut);
+        break;
+    }
+    case FN_EDGENOWALL: case FN_CORNERNOWALL: {
+        const bool corner = (s.fn == FN_CORNERNOWALL);
+        const int nn = corner ? 6 : 3;
+        double mt = mut;
+        for (int t = 0; t < nn; t++) {
+            const int *nb = s.nb[t];
+            if (!corner && nb[0] == 0 && nb[1] == 0 && nb[2] == 0) continue;
+            long gn = gidx(B, nb[0] + s.dir[0] * i, nb[1] + s.dir[1] * j, nb[2] + s.dir[2] * k);
+            for (int q = 0; q < npv; q++) pv[q] = pv[q] + S.pv[q * S.NG + gn];
+            if (S.mut) mt = mt + S.mut[gn];
+        }
+        double f = corner ? 1.0 / 7.0 : 1.0 / 3.0;
+        for (int q = 0; q < npv; q++) pv[q] = f * pv[q];
+        mt = f * mt;
+        mixture_eos<double>(pv[0] + rp[0], pv[4], pv[5], pv[6], d, P.iturb != -3);
+        put_cell(S, gg, pv, d, mt);
+        break;
+    }
+    default: break;
+    }
+}
+
+__global__ void k_bc(DS S, int which)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= S.NBC[which]) return;
+    const Seg *sg = S.seg[which];
+    int lo = 0, hi = S.nbcseg[which] - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (sg[mid].start <= t) lo = mid; else hi = mid - 1;
+    }
+    const Seg &s = sg[lo];
+    long r = t - s.start;
+    int ni = s.ie[0] - s.is[0] + 1, nj = s.ie[1] - s.is[1] + 1;
+    int i = s.is[0] + (int)(r % ni);
+    int j = s.is[1] + (int)((r / ni) % nj);
+    int k = s.is[2] + (int)(r / ((long)ni * nj));
+    apply_bc_cell(S, s, i, j, k);
+}
+
+// interface copy: the sending block's cells to the donor block's ghost cells
+__global__ void k_halo(DS S)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= S.NHALO) return;
+    int lo = 0, hi = S.ncon - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (S.con[mid].start <= t) lo = mid; else hi = mid - 1;
+    }
+    const Con &c = S.con[lo];
+    long r = t - c.start;
+    int ni = c.ie[0] - c.is[0] + 1, nj = c.ie[1] - c.is[1] + 1;
+    int di = (int)(r % ni), dj = (int)((r / ni) % nj), dk = (int)(r / ((long)ni * nj));
+    int ii = c.tm[0] * di + c.tm[1] * dj + c.tm[2] * dk + c.isd[0];
+    int jj = c.tm[3] * di + c.tm[4] * dj + c.tm[5] * dk + c.isd[1];
+    int kk = c.tm[6] * di + c.tm[7] * dj + c.tm[8] * dk + c.isd[2];
+    long gs = gidx(S.b[c.src], c.is[0] + di, c.is[1] + dj, c.is[2] + dk);
+    long gd = gidx(S.b[c.dst], ii, jj, kk);
+    for (int n = 0; n < S.npv; n++) S.pv[n * S.NG + gd] = S.pv[n * S.NG + gs];
+    for (int m = 0; m < ND; m++) S.dc[m * S.NG + gd] = S.dc[m * S.NG + gs];
+    if (S.mut) S.mut[gd] = S.mut[gs];
+}
+
+// ------------------------------------------------------------ face fluxes
+// Direction D: normal axis D, first transverse axis E=(D+1)%3, second T=(D+2)%3,
+// matching the reference's stencil orderings (src/rhs_module.f90:288-312,
+// 374-398, 460-484): ll = 1 + dn + 2(de+1) + 6(dt+1) for dn in {0,1}.
+template <int D>
+__global__ __launch_bounds__(128) void k_face(DS S)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= S.NF[D]) return;
+    int b = 0;
+    for (int q = 1; q < S.nb; q++) if (t >= S.b[q].f0[D]) b = q;
+    const BlkD &B = S.b[b];
+    const Par &P = S.P;
+    const int npv = S.npv;
+    long r = t - B.f0[D];
+    int i, j, k;
+    if (D == 0) { i = 1 + (int)(r % B.I); j = 2 + (int)((r / B.I) % (B.J - 1)); k = 2 + (int)(r / ((long)B.I * (B.J - 1))); }
+    else if (D == 1) { i = 2 + (int)(r % (B.I - 1)); j = 1 + (int)((r / (B.I - 1)) % B.J); k = 2 + (int)(r / ((long)(B.I - 1) * B.J)); }
+    else { i = 2 + (int)(r % (B.I - 1)); j = 2 + (int)((r / (B.I - 1)) % (B.J - 1)); k = 1 + (int)(r / ((long)(B.I - 1) * (B.J - 1))); }
+    const long st[3] = {1, (long)B.nx, (long)B.nx * B.ny};
+    const long sN = st[D], sE = st[(D + 1) % 3], sT = st[(D + 2) % 3];
+    const long g9 = gidx(B, i, j, k);
+    auto off = [&](int ll) -> long {
+        int q = ll - 1;
+        if (q < 18) return (long)(q & 1) * sN + (long)((q >> 1) % 3 - 1) * sE + (long)(q / 6 - 1) * sT;
+        if (q < 27) { q -= 18; return -sN + (long)(q % 3 - 1) * sE + (long)(q / 3 - 1) * sT; }
+        if (q < 36) { q -= 27; return 2 * sN + (long)(q % 3 - 1) * sE + (long)(q / 3 - 1) * sT; }
+        return q == 36 ? -2 * sN : 3 * sN;
+    };
+    const bool visc = (P.iturb >= -2);
+    double c9[NPV], c10[NPV], xl[NPV], xr[NPV];
+    double g3[6][3] = {};
+    int vq = 0;
+    for (int n = 0; n < npv; n++) {
+        const double *f = S.pv + (long)n * S.NG + g9;
+        auto xs = [&](int ll) -> double { return f[off(ll)]; };
+        c9[n] = f[0];
+        c10[n] = f[sN];
+        if (P.nmuscl > 0) muscl_var<double>(P.nlim, P.nmuscl == 2, xs, xl[n], xr[n]);
+        else { xl[n] = c9[n]; xr[n] = c10[n]; }
+        if (visc && (n >= 1 && n <= 4 || n >= 7)) {
+            g3[vq][0] = xs(10) - xs(9);
+            g3[vq][1] = 0.0625 * (-xs(1) - xs(2) + xs(5) + xs(6) + 2.0 * (-xs(7) - xs(8) + xs(11) + xs(12))
+                                  - xs(13) - xs(14) + xs(17) + xs(18));
+            g3[vq][2] = 0.0625 * (-xs(1) - xs(2) + xs(13) + xs(14) + 2.0 * (-xs(3) - xs(4) + xs(15) + xs(16))
+                                  - xs(5) - xs(6) + xs(17) + xs(18));
+            vq++;
+        }
+    }
+    int ip = i, jp = j, kp = k;
+    if (D == 0) ip++; else if (D == 1) jp++; else kp++;
+    const long gq = g9 + sN;
+    FaceState<double> L, R;
+    double n3[3];
+    face_vec(S, B, D, i, j, k, n3);
+    if (P.nmuscl > 0) {
+        muscl_clamps<double>(npv, P.pref, c9, c10, xl, xr);
+        face_rho_h<double>(xl[0] + P.pref, xl[4], xl[5], xl[6], L.rho, L.h);
+        face_rho_h<double>(xr[0] + P.pref, xr[4], xr[5], xr[6], R.rho, R.h);
+    } else {
+        L.rho = S.dc[D_RHO * S.NG + g9]; L.h = S.dc[D_H * S.NG + g9];
+        R.rho = S.dc[D_RHO * S.NG + gq]; R.h = S.dc[D_H * S.NG + gq];
+    }
+    for (int n = 0; n < npv; n++) { L.p[n] = xl[n]; R.p[n] = xr[n]; }
+    double sl = sqrt(L.rho), sr = sqrt(R.rho), rd = 1.0 / (sl + sr);
+    double ya1 = (sl * xl[5] + sr * xr[5]) * rd, ya2 = (sl * xl[6] + sr * xr[6]) * rd;
+    double c2a = face_c2<double>(0.5 * (xl[0] + xr[0]) + P.pref, 0.5 * (xl[4] + xr[4]), ya1, ya2);
+    double fx[NPV];
+    if (P.nscheme == 1) roe_flux<double>(P, npv, n3, L, R, c2a, fx);
+    else ausm_flux<double>(P, npv, n3, L, R, c2a, fx);
+    double *out = S.ff[D];
+    const long NF = S.NF[D];
+    for (int n = 0; n < npv; n++) out[n * NF + t] = fx[n];
+    if (visc) {
+        double e4[4][3], t4[4][3];
+        const int E = (D + 1) % 3, T = (D + 2) % 3;
+        int p0[3] = {i, j, k}, p1[3] = {ip, jp, kp};
+        int a0[3], a1[3];
+        // e-family: (P,e-1), (P+1,e-1), (P,e), (P+1,e)
+        for (int q = 0; q < 3; q++) { a0[q] = p0[q]; a1[q] = p1[q]; }
+        a0[E]--; a1[E]--;
+        face_vec(S, B, E, a0[0], a0[1], a0[2], e4[0]);
+        face_vec(S, B, E, a1[0], a1[1], a1[2], e4[1]);
+        face_vec(S, B, E, p0[0], p0[1], p0[2], e4[2]);
+        face_vec(S, B, E, p1[0], p1[1], p1[2], e4[3]);
+        // t-family: i faces (P,t-1),(P+1,t-1),(P,t),(P+1,t); j and k faces (P,t-1),(P,t),(P+1,t-1),(P+1,t)
+        for (int q = 0; q < 3; q++) { a0[q] = p0[q]; a1[q] = p1[q]; }
+        a0[T]--; a1[T]--;
+        face_vec(S, B, T, a0[0], a0[1], a0[2], t4[0]);
+        if (D == 0) {
+            face_vec(S, B, T, a1[0], a1[1], a1[2], t4[1]);
+            face_vec(S, B, T, p0[0], p0[1], p0[2], t4[2]);
+        } else {
+            face_vec(S, B, T, p0[0], p0[1], p0[2], t4[1]);
+            face_vec(S, B, T, a1[0], a1[1], a1[2], t4[2]);
+        }
+        face_vec(S, B, T, p1[0], p1[1], p1[2], t4[3]);
+        double gl[2], gr[2], dl[ND], dr[ND];
+        long wl = widx(B, i, j, k), wr = widx(B, ip, jp, kp);
+        gl[0] = S.vol[wl]; gr[0] = S.vol[wr];
+        gl[1] = S.ydn ? S.ydn[wl] : 1.0; gr[1] = S.ydn ? S.ydn[wr] : 1.0;
+        for (int m = 0; m < ND; m++) { dl[m] = S.dc[m * S.NG + g9]; dr[m] = S.dc[m * S.NG + gq]; }
+        double ml = S.mut ? S.mut[g9] : 0.0, mr = S.mut ? S.mut[gq] : 0.0;
+        double cav[5] = {0.5 * (c9[1] + c10[1]), 0.5 * (c9[2] + c10[2]), 0.5 * (c9[3] + c10[3]), 0.0, 0.0};
+        if (npv > 7) { cav[3] = 0.5 * (c9[7] + c10[7]); cav[4] = 0.5 * (c9[8] + c10[8]); }
+        double fv[NPV];
+        visc_flux<double>(P, n3, e4, t4, gl, gr, g3, cav, dl, dr, ml, mr, fv);
+        for (int n = 0; n < npv; n++) out[(npv + n) * NF + t] = fv[n];
+    }
+}
+
+__device__ __forceinline__ long fidx(const BlkD &B, int D, int i, int j, int k)
+{
+    if (D == 0) return B.f0[0] + (i - 1) + (long)B.I * ((j - 2) + (long)(B.J - 1) * (k - 2));
+    if (D == 1) return B.f0[1] + (i - 2) + (long)(B.I - 1) * ((j - 1) + (long)B.J * (k - 2));
+    return B.f0[2] + (i - 2) + (long)(B.I - 1) * ((j - 2) + (long)(B.J - 1) * (k - 1));
+}
+
+// flux differences in the reference's order, then the cell sources
+__global__ void k_combine(DS S)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= S.NC) return;
+    const BlkD &B = S.b[find_blk(S, t, 0)];
+    const Par &P = S.P;
+    const int npv = S.npv;
+    const bool visc = (P.iturb >= -2);
+    int i, j, k;
+    cell_of(B, t - B.c0, i, j, k);
+    double r[NPV];
+    for (int D = 0; D < 3; D++) {
+        long f1 = fidx(B, D, i, j, k);
+        long f0 = fidx(B, D, i - (D == 0), j - (D == 1), k - (D == 2));
+        const double *F = S.ff[D];
+        long NF = S.NF[D];
+        for (int n = 0; n < npv; n++) {
+            double a = -(F[n * NF + f1] - F[n * NF + f0]);
+            r[n] = (D == 0) ? a : r[n] + a;
+            if (visc) r[n] = r[n] + (F[(npv + n) * NF + f1] - F[(npv + n) * NF + f0]);
+        }
+    }
+    long g = gidx(B, i, j, k);
+    double d[ND], pc[NPV];
+    for (int m = 0; m < ND; m++) d[m] = S.dc[m * S.NG + g];
+    for (int n = 0; n < npv; n++) pc[n] = S.pv[n * S.NG + g];
+    double vol = S.vol[widx(B, i, j, k)];
+    if (P.ncav) {
+        double rc, rv, ic[4];
+        merkle(P, pc, d, vol, rc, rv, ic);
+        r[5] = r[5] + rv - rc;
+        for (int q = 0; q < 4; q++) S.icav[q * S.NC + t] = ic[q];
+        S.rcav[t] = rc;
+        S.rcav[S.NC + t] = rv;
+    }
+    if (P.iturb == 0) {
+        double x7[7][NPV], f6[6][3], src[2], it[4], oc;
+        const int di[7] = {-1, 0, 1, 0, 0, 0, 0}, dj[7] = {0, 0, 0, -1, 1, 0, 0}, dk[7] = {0, 0, 0, 0, 0, -1, 1};
+        for (int q = 0; q < 7; q++) {
+            long gn = gidx(B, i + di[q], j + dj[q], k + dk[q]);
+            for (int n = 0; n < npv; n++) x7[q][n] = S.pv[n * S.NG + gn];
+        }
+        cell_faces(S, B, i, j, k, f6);
+        sst_source(x7, f6, vol, S.ydn[widx(B, i, j, k)], d, S.mut[g], src, it, oc);
+        S.ocut[t] = oc;
+        r[7] = r[7] + src[0];
+        r[8] = r[8] + src[1];
+        for (int q = 0; q < 4; q++) S.itt[q * S.NC + t] = it[q];
+    }
+    if (P.nsteady > 0) {
+        double fx[NPV];
+        fx[0] = d[D_RHO];
+        fx[1] = d[D_RHO] * pc[1];
+        fx[2] = d[D_RHO] * pc[2];
+        fx[3] = d[D_RHO] * pc[3];
+        fx[4] = d[D_RHO] * (d[D_H] + 0.5 * (pc[1] * pc[1] + pc[2] * pc[2] + pc[3] * pc[3])) - (pc[0] + P.pref);
+        for (int n = 5; n < npv; n++) fx[n] = d[D_RHO] * pc[n];
+        for (int n = 0; n < npv; n++) {
+            double q1 = S.qq[(long)n * S.NC + t], q2 = S.qq[(long)(npv + n) * S.NC + t];
+            double v = (double)P.nsteady * (P.bdf_b * fx[n] - q1) + 0.5 * ((double)P.nsteady - 1.0) * q2;
+            r[n] = r[n] - v * vol / P.dt_phy;
+        }
+    }
+    for (int n = 0; n < npv; n++) S.res[n * S.NC + t] = r[n];
+}
+
+// ------------------------------------------------------------ LU-SGS sweeps
+// One hyperplane i+j+k = s of block b.  Thread -> (j,k); i = s - j - k.
+template <bool FWD>
+__global__ __launch_bounds__(64) void k_sweep(DS S, int b, int s)
+{
+    const BlkD &B = S.b[b];
+    const Par &P = S.P;
+    const int npv = S.npv;
+    int nj = B.J - 1;
+    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    int j = 2 + (int)(q % nj), k = 2 + (int)(q / nj);
+    if (k > B.K) return;
+    int i = s - j - k;
+    if (i < 2 || i > B.I) return;
+    long c = cidx(B, i, j, k);
+    long w = widx(B, i, j, k);
+    const double *dq = FWD ? S.dqf : S.dqb;
+    double y[NPV];
+    if (FWD) for (int n = 0; n < npv; n++) y[n] = S.res[n * S.NC + c];
+    else for (int n = 0; n < npv; n++) y[n] = S.dcv[n * S.NW + w];
+    const int sg = FWD ? 1 : -1;
+    for (int ax = 0; ax < 3; ax++) {
+        int a[3] = {i, j, k};
+        a[ax] -= sg;
+        const int lim = (ax == 0) ? B.I : (ax == 1 ? B.J : B.K);
+        if (a[ax] < 2 || a[ax] > lim) continue;       // ghost neighbour: its dq is zero
+        CellJ nb;
+        load_cell(S, B, a[0], a[1], a[2], nb);
+        double n3[3];
+        if (FWD) face_vec(S, B, ax, a[0], a[1], a[2], n3);
+        else face_vec(S, B, ax, i, j, k, n3);
+        long wn = widx(B, a[0], a[1], a[2]);
+        double x[NPV];
+        for (int n = 0; n < npv; n++) x[n] = dq[n * S.NW + wn];
+        jac_apply(P, npv, nb, n3, sg, x, y, FWD ? 1.0 : -1.0);
+    }
+    if (FWD) for (int n = 0; n < npv; n++) S.dcv[n * S.NW + w] = y[n];
+    CellJ cc;
+    load_cell(S, B, i, j, k, cc);
+    double f6[6][3];
+    cell_faces(S, B, i, j, k, f6);
+    double ic[4] = {0, 0, 0, 0}, it[4] = {0, 0, 0, 0};
+    if (P.ncav) for (int m = 0; m < 4; m++) ic[m] = S.icav[m * S.NC + c];
+    if (P.iturb == 0) for (int m = 0; m < 4; m++) it[m] = S.itt[m * S.NC + c];
+    double x[NPV];
+    getx(P, npv, cc, f6, S.dt[c], ic, it, y, x);
+    double *out = FWD ? S.dqf : S.dqb;
+    for (int n = 0; n < npv; n++) out[n * S.NW + w] = x[n];
+}
+
+// ------------------------------------------------------------ update + norms
+__global__ __launch_bounds__(256) void k_update(DS S)
+{
+    __shared__ double sred[4][NPV + 1];
+    __shared__ long sidx[4];
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const Par &P = S.P;
+    const int npv = S.npv;
+    double dsq[NPV + 1];
+    for (int n = 0; n <= NPV; n++) dsq[n] = 0.0;
+    double amax = -1.0;
+    long aidx = 0x7fffffffffffffffL;
+    if (t < S.NC) {
+        int bb = find_blk(S, t, 0);
+        const BlkD &B = S.b[bb];
+        int i, j, k;
+        cell_of(B, t - B.c0, i, j, k);
+        long g = gidx(B, i, j, k), w = widx(B, i, j, k);
+        double p[NPV], old[NPV];
+        for (int n = 0; n < npv; n++) { old[n] = S.pv[n * S.NG + g]; p[n] = old[n] + S.dqb[n * S.NW + w]; }
+        p[0] = mx(-P.pref + 1.0e1, p[0]);
+        p[5] = mn(1.0, mx(0.0, p[5]));
+        p[6] = mn(1.0, mx(0.0, p[6]));
+        if (P.iturb == 0) {
+            p[7] = mx(p[7], 1.0e-16);
+            p[8] = mx(mx(p[8], 1.0e-16), S.ocut[t]);
+        }
+        for (int n = 0; n < npv; n++) S.pv[n * S.NG + g] = p[n];
+        double d[ND];
+        mixture_eos<double>(p[0] + P.pref, p[4], p[5], p[6], d, P.iturb != -3);
+        for (int m = 0; m < ND; m++) S.dc[m * S.NG + g] = d[m];
+        for (int n = 0; n < npv; n++) { double q = fabs(old[n] - p[n]); dsq[n] = q * q; }
+        double q0 = fabs(old[0] - p[0]);
+        if (q0 > 1.0e-12) {
+            amax = q0;
+            // linear order of the reference's scan: block, then k, j, i
+            aidx = ((long)bb << 48) | ((long)k << 32) | ((long)j << 16) | (long)i;
+        }
+    }
+    // wavefront reduction
+    for (int o = 32; o > 0; o >>= 1) {
+        for (int n = 0; n < npv; n++) dsq[n] += __shfl_xor(dsq[n], o);
+        double om = __shfl_xor(amax, o);
+        long oi = __shfl_xor(aidx, o);
+        if (om > amax || (om == amax && oi < aidx)) { amax = om; aidx = oi; }
+    }
+    int wv = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        for (int n = 0; n < npv; n++) sred[wv][n] = dsq[n];
+        sred[wv][NPV] = amax;
+        sidx[wv] = aidx;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double acc[NPV + 1];
+        double bm = sred[0][NPV];
+        long bi = sidx[0];
+        for (int n = 0; n < npv; n++) acc[n] = sred[0][n];
+        for (int q = 1; q < (int)(blockDim.x >> 6); q++) {
+            for (int n = 0; n < npv; n++) acc[n] += sred[q][n];
+            if (sred[q][NPV] > bm || (sred[q][NPV] == bm && sidx[q] < bi)) { bm = sred[q][NPV]; bi = sidx[q]; }
+        }
+        double *o = S.red + (long)blockIdx.x * (NPV + 2);
+        for (int n = 0; n < npv; n++) o[n] = acc[n];
+        o[NPV] = bm;
+        reinterpret_cast<long *>(o)[NPV + 1] = bi;
+    }
+}
+
+// fixed-order final reduction of the per-workgroup partials (deterministic)
+__global__ void k_norm_final(DS S, int nwg)
+{
+    __shared__ double part[256][NPV];
+    __shared__ double pm[256];
+    __shared__ long pi[256];
+    int tid = threadIdx.x;
+    double acc[NPV];
+    for (int n = 0; n < NPV; n++) acc[n] = 0.0;
+    double bm = -1.0;
+    long bi = 0x7fffffffffffffffL;
+    for (int q = tid; q < nwg; q += 256) {
+        const double *o = S.red + (long)q * (NPV + 2);
+        for (int n = 0; n < S.npv; n++) acc[n] += o[n];
+        long oi = reinterpret_cast<const long *>(o)[NPV + 1];
+        if (o[NPV] > bm || (o[NPV] == bm && oi < bi)) { bm = o[NPV]; bi = oi; }
+    }
+    for (int n = 0; n < NPV; n++) part[tid][n] = acc[n];
+    pm[tid] = bm;
+    pi[tid] = bi;
+    __syncthreads();
+    if (tid == 0) {
+        for (int q = 1; q < 256; q++) {
+            for (int n = 0; n < S.npv; n++) acc[n] += part[q][n];
+            if (pm[q] > bm || (pm[q] == bm && pi[q] < bi)) { bm = pm[q]; bi = pi[q]; }
+        }
+        for (int n = 0; n < S.npv; n++) S.scal[4 + n] = sqrt(acc[n]);
+        S.scal[4 + NPV] = bm;
+        reinterpret_cast<long *>(S.scal)[5 + NPV] = bi;
+    }
+}
+
+__global__ void k_eddy(DS S)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= S.NC) return;
+    const BlkD &B = S.b[find_blk(S, t, 0)];
+    int i, j, k;
+    cell_of(B, t - B.c0, i, j, k);
+    double x7[7][NPV], f6[6][3], d[ND];
+    const int di[7] = {-1, 0, 1, 0, 0, 0, 0}, dj[7] = {0, 0, 0, -1, 1, 0, 0}, dk[7] = {0, 0, 0, 0, 0, -1, 1};
+    for (int q = 0; q < 7; q++) {
+        long gn = gidx(B, i + di[q], j + dj[q], k + dk[q]);
+        for (int n = 0; n < S.npv; n++) x7[q][n] = S.pv[n * S.NG + gn];
+    }
+    long g = gidx(B, i, j, k);
+    for (int m = 0; m < ND; m++) d[m] = S.dc[m * S.NG + g];
+    cell_faces(S, B, i, j, k, f6);
+    long w = widx(B, i, j, k);
+    S.mut[g] = sst_eddy(x7, f6, S.vol[w], S.ydn[w], d);
+}
+
+// dual time: qq(2) = qq(1), qq(1) = Q(pv, dv)
+__global__ void k_unsteady_shift(DS S)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= S.NC) return;
+    const BlkD &B = S.b[find_blk(S, t, 0)];
+    int i, j, k;
+    cell_of(B, t - B.c0, i, j, k);
+    long g = gidx(B, i, j, k);
+    const int npv = S.npv;
+    double pc[NPV];
+    for (int n = 0; n < npv; n++) pc[n] = S.pv[n * S.NG + g];
+    double rho = S.dc[D_RHO * S.NG + g], h = S.dc[D_H * S.NG + g];
+    for (int n = 0; n < npv; n++) S.qq[(long)(npv + n) * S.NC + t] = S.qq[(long)n * S.NC + t];
+    double q[NPV];
+    q[0] = rho;
+    q[1] = rho * pc[1];
+    q[2] = rho * pc[2];
+    q[3] = rho * pc[3];
+    q[4] = rho * (h + 0.5 * (pc[1] * pc[1] + pc[2] * pc[2] + pc[3] * pc[3])) - pc[0] - S.P.pref;
+    for (int n = 5; n < npv; n++) q[n] = rho * pc[n];
+    for (int n = 0; n < npv; n++) S.qq[(long)n * S.NC + t] = q[n];
+}
+
+// ------------------------------------------------------------ layout transposes
+// host AoS (m components fastest) <-> device SoA for one block's ghosted cells
+__global__ void k_aos_to_soa(const double *src, int m, int msrc, int moff, double *dst, long nstride, long g0, long n)
+{
+    long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    for (int q = 0; q < m; q++) dst[q * nstride + g0 + g] = src[g * msrc + moff + q];
+}
+__global__ void k_soa_to_aos(const double *src, int m, long nstride, long g0, double *dst, long n)
+{
+    long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    for (int q = 0; q < m; q++) dst[g * m + q] = src[q * nstride + g0 + g];
+}
+// dv(18) AoS -> compact dc, tv(ntv) AoS -> mu, kappa, mut
+__global__ void k_dv_to_dc(const double *dv, const double *tv, int ntv, double *dc, double *mut, long nstride, long g0, long n)
+{
+    long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    const int map[11] = {0, 1, 5, 6, 7, 8, 9, 10, 11, 12, 13};
+    for (int q = 0; q < 11; q++) dc[q * nstride + g0 + g] = dv[18 * g + map[q]];
+    if (ntv >= 2) {
+        dc[D_MU * nstride + g0 + g] = tv[ntv * g];
+        dc[D_KAP * nstride + g0 + g] = tv[ntv * g + 1];
+    }
+    if (ntv == 3 && mut) mut[g0 + g] = tv[ntv * g + 2];
+}
+// interior-cell arrays (c-indexed SoA, m components) -> reference AoS (m,2:imax,...)
+__global__ void k_cell_to_aos(const double *src, int m, long nstride, long c0, double *dst, long n)
+{
+    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    for (int a = 0; a < m; a++) dst[q * m + a] = src[a * nstride + c0 + q];
+}
+__global__ void k_cell_from_aos(const double *src, int m, long nstride, long c0, double *dst, long n)
+{
+    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    for (int a = 0; a < m; a++) dst[a * nstride + c0 + q] = src[q * m + a];
+}
+__global__ void k_fill(double *p, double v, long n)
+{
+    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) p[q] = v;
+}
+
+inline unsigned nblk(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+}  // namespace
+
+// ============================================================ host context
+struct cav3d_ctx {
+    cav3d_params prm;
+    Par P;
+    DS S;
+    int nb = 0, npv = 0, ntv = 0, ngrd = 0;
+    std::vector<BlkD> blk;
+    std::vector<std::vector<Seg>> segs = std::vector<std::vector<Seg>>(3);
+    std::vector<Con> cons;
+    std::vector<std::vector<int>> bcdesc;   // per block: nbc*7 raw
+    hipStream_t st = nullptr;
+    double *stage = nullptr;
+    size_t stage_n = 0;
+    int nwg_update = 0;
+    bool profile = false;
+    hipEvent_t ev[8] = {};
+    double prof_ms[6] = {};
+    std::vector<void *> allocs;
+};
+
+namespace {
+
+template <class T> int dalloc(cav3d_ctx *c, T **p, size_t n)
+{
+    *p = nullptr;
+    if (n == 0) n = 1;
+    HIPCHK(hipMalloc((void **)p, n * sizeof(T)));
+    HIPCHK(hipMemsetAsync(*p, 0, n * sizeof(T), c->st));
+    c->allocs.push_back(*p);
+    return 0;
+}
+
+int ensure_stage(cav3d_ctx *c, size_t n)
+{
+    if (c->stage_n >= n) return 0;
+    if (c->stage) { HIPCHK(hipStreamSynchronize(c->st)); HIPCHK(hipFree(c->stage)); c->stage = nullptr; }
+    HIPCHK(hipMalloc((void **)&c->stage, n * sizeof(double)));
+    c->stage_n = n;
+    return 0;
+}
+
+int wall_fn(int iturb) { return iturb == 0 ? FN_WALLVISKW : (iturb == -2 ? FN_WALLVIS : FN_WALLINV); }
+void set3(int *a, int x, int y, int z) { a[0] = x; a[1] = y; a[2] = z; }
+
+// BC construction (src/bc_module.f90:157-336 faces, :502-866 edges, :870-1185 corners)
+int build_bcs(cav3d_ctx *c)
+{
+    const int iturb = c->prm.iturb;
+    for (int w = 0; w < 3; w++) c->segs[w].clear();
+    for (int b = 0; b < c->nb; b++) {
+        const BlkD &B = c->blk[b];
+        const std::vector<int> &dsc = c->bcdesc[b];
+        int nbc = (int)dsc.size() / 7;
+        std::vector<Seg> faces;
+        for (int n = 0; n < nbc; n++) {
+            Seg s;
+            memset(&s, 0, sizeof(s));
+            s.b = b;
+            int kind = dsc[7 * n];
+            for (int d = 0; d < 3; d++) { s.is[d] = dsc[7 * n + 1 + d]; s.ie[d] = dsc[7 * n + 4 + d]; }
+            if (s.is[0] == s.ie[0]) {
+                set3(s.dir, -1, 1, 1);
+                if (s.is[0] == 1) { s.face = F_IMIN; s.is[0] -= 2; s.org[0] = 2 * s.ie[0] + 1; }
+                else { s.face = F_IMAX; s.ie[0] += 2; s.org[0] = 2 * s.is[0] - 1; }
+            } else if (s.is[1] == s.ie[1]) {
+                set3(s.dir, 1, -1, 1);
+                if (s.is[1] == 1) { s.face = F_JMIN; s.is[1] -= 2; s.org[1] = 2 * s.ie[1] + 1; }
+                else { s.face = F_JMAX; s.ie[1] += 2; s.org[1] = 2 * s.is[1] - 1; }
+            } else if (s.is[2] == s.ie[2]) {
+                set3(s.dir, 1, 1, -1);
+                if (s.is[2] == 1) { s.face = F_KMIN; s.is[2] -= 2; s.org[2] = 2 * s.ie[2] + 1; }
+                else { s.face = F_KMAX; s.ie[2] += 2; s.org[2] = 2 * s.is[2] - 1; }
+            } else return fail("boundary range of block " + std::to_string(b) + " is not a face", -2);
+            switch (kind) {
+            case CAV3D_BC_WALL: s.fn = wall_fn(iturb); break;
+            case CAV3D_BC_INFLOW_SUBSONIC: s.fn = FN_INSUB; break;
+            case CAV3D_BC_OUTFLOW_SUBSONIC: s.fn = FN_OUTSUB; break;
+            case CAV3D_BC_FARFIELD: s.fn = FN_FARFIELD; break;
+            case CAV3D_BC_SYMMETRY: s.fn = FN_SYM; break;
+            case CAV3D_BC_INFLOW_SUPERSONIC: s.fn = FN_INSUPER; break;
+            case CAV3D_BC_OUTFLOW_SUPERSONIC: case CAV3D_BC_DEGENERATE: s.fn = FN_COPY; break;
+            case CAV3D_BC_INFLOW: case CAV3D_BC_OUTFLOW: s.fn = FN_EMPTY; break;
+            default: return fail("unknown boundary kind " + std::to_string(kind), -2);
+            }
+            s.pad = kind;
+            for (int d = 0; d < 3; d++) {
+                int lo = (d == 0 ? -1 : -1), hi = (d == 0 ? B.I + 3 : (d == 1 ? B.J + 3 : B.K + 3));
+                if (s.is[d] < lo || s.ie[d] > hi) return fail("boundary range out of the block", -2);
+            }
+            faces.push_back(s);
+        }
+        auto covers = [&](const int *p, int d0, int d1, int d2) {
+            for (const Seg &s : faces) {
+                if (s.pad != CAV3D_BC_WALL && s.pad != CAV3D_BC_SYMMETRY) continue;
+                bool ok = s.is[d0] <= p[d0] && s.ie[d0] >= p[d0] && s.is[d1] <= p[d1] && s.ie[d1] >= p[d1];
+                if (d2 >= 0) ok = ok && s.is[d2] <= p[d2] && s.ie[d2] >= p[d2];
+                if (ok) return true;
+            }
+            return false;
+        };
+        int I = B.I, J = B.J, K = B.K;
+        Seg e[12];
+        memset(e, 0, sizeof(e));
+        set3(e[0].is, -1, -1, 2); set3(e[0].ie, 1, 1, K); set3(e[0].nb[0], 1, 2, 0); set3(e[0].nb[1], 2, 1, 0); set3(e[0].org, 2, 2, 0);
+        set3(e[1].is, I + 1, -1, 2); set3(e[1].ie, I + 3, 1, K); set3(e[1].nb[0], I + 1, 2, 0); set3(e[1].nb[1], I, 1, 0); set3(e[1].org, I, 2, 0);
+        set3(e[2].is, -1, J + 1, 2); set3(e[2].ie, 1, J + 3, K); set3(e[2].nb[0], 1, J, 0); set3(e[2].nb[1], 2, J + 1, 0); set3(e[2].org, 2, J, 0);
+        set3(e[3].is, I + 1, J + 1, 2); set3(e[3].ie, I + 3, J + 3, K); set3(e[3].nb[0], I + 1, J, 0); set3(e[3].nb[1], I, J + 1, 0); set3(e[3].org, I, J, 0);
+        set3(e[4].is, -1, 2, -1); set3(e[4].ie, 1, J, 1); set3(e[4].nb[0], 1, 0, 2); set3(e[4].nb[2], 2, 0, 1); set3(e[4].org, 2, 0, 2);
+        set3(e[5].is, I + 1, 2, -1); set3(e[5].ie, I + 3, J, 1); set3(e[5].nb[0], I + 1, 0, 2); set3(e[5].nb[2], I, 0, 1); set3(e[5].org, I, 0, 2);
+        set3(e[6].is, -1, 2, K + 1); set3(e[6].ie, 1, J, K + 3); set3(e[6].nb[0], 1, 0, K); set3(e[6].nb[2], 2, 0, K + 1); set3(e[6].org, 2, 0, K);
+        set3(e[7].is, I + 1, 2, K + 1); set3(e[7].ie, I + 3, J, K + 3); set3(e[7].nb[0], I + 1, 0, K); set3(e[7].nb[2], I, 0, K + 1); set3(e[7].org, I, 0, K);
+        set3(e[8].is, 2, -1, -1); set3(e[8].ie, I, 1, 1); set3(e[8].nb[1], 0, 1, 2); set3(e[8].nb[2], 0, 2, 1); set3(e[8].org, 0, 2, 2);
+        set3(e[9].is, 2, J + 1, -1); set3(e[9].ie, I, J + 3, 1); set3(e[9].nb[1], 0, J + 1, 2); set3(e[9].nb[2], 0, J, 1); set3(e[9].org, 0, J, 2);
+        set3(e[10].is, 2, -1, K + 1); set3(e[10].ie, I, 1, K + 3); set3(e[10].nb[1], 0, 1, K); set3(e[10].nb[2], 0, 2, K + 1); set3(e[10].org, 0, 2, K);
+        set3(e[11].is, 2, J + 1, K + 1); set3(e[11].ie, I, J + 3, K + 3); set3(e[11].nb[1], 0, J + 1, K); set3(e[11].nb[2], 0, J, K + 1); set3(e[11].org, 0, J, K);
+        const int ww = iturb == 0 ? FN_EDGEWWKW : FN_EDGEWW;
+        for (int m = 0; m < 12; m++) { e[m].b = b; e[m].face = F_NONE; }
+        for (int m = 0; m < 4; m++) {
+            bool is = covers(e[m].nb[0], 0, 1, -1), js = covers(e[m].nb[1], 0, 1, -1);
+            if (is && js) e[m].fn = ww;
+            else if (is) { e[m].face = (m == 0 || m == 2) ? F_IMIN : F_IMAX; e[m].org[0] = e[m].nb[1][0]; e[m].org[1] = e[m].nb[1][1]; e[m].fn = wall_fn(iturb); }
+            else if (js) { e[m].face = (m == 0 || m == 1) ? F_JMIN : F_JMAX; e[m].org[0] = e[m].nb[0][0]; e[m].org[1] = e[m].nb[0][1]; e[m].fn = wall_fn(iturb); }
+            else e[m].fn = FN_EDGENOWALL;
+            set3(e[m].dir, 0, 0, 1);
+        }
+        for (int m = 4; m < 8; m++) {
+            bool is = covers(e[m].nb[0], 0, 2, -1), ks = covers(e[m].nb[2], 0, 2, -1);
+            if (is && ks) e[m].fn = ww;
+            else if (is) { e[m].face = (m == 4 || m == 6) ? F_IMIN : F_IMAX; e[m].org[0] = e[m].nb[2][0]; e[m].org[2] = e[m].nb[2][2]; e[m].fn = wall_fn(iturb); }
+            else if (ks) { e[m].face = (m == 4 || m == 5) ? F_KMIN : F_KMAX; e[m].org[0] = e[m].nb[0][0]; e[m].org[2] = e[m].nb[0][2]; e[m].fn = wall_fn(iturb); }
+            else e[m].fn = FN_EDGENOWALL;
+            set3(e[m].dir, 0, 1, 0);
+        }
+        for (int m = 8; m < 12; m++) {
+            bool js = covers(e[m].nb[1], 1, 2, -1), ks = covers(e[m].nb[2], 1, 2, -1);
+            if (js && ks) e[m].fn = ww;
+            else if (js) { e[m].face = (m == 8 || m == 10) ? F_JMIN : F_JMAX; e[m].org[1] = e[m].nb[2][1]; e[m].org[2] = e[m].nb[2][2]; e[m].fn = wall_fn(iturb); }
+            else if (ks) { e[m].face = (m == 8 || m == 9) ? F_KMIN : F_KMAX; e[m].org[1] = e[m].nb[1][1]; e[m].org[2] = e[m].nb[1][2]; e[m].fn = wall_fn(iturb); }
+            else e[m].fn = FN_EDGENOWALL;
+            set3(e[m].dir, 1, 0, 0);
+        }
+        Seg q[8];
+        memset(q, 0, sizeof(q));
+        for (int m = 0; m < 8; m++) {
+            int ib = m & 1, jb = (m >> 1) & 1, kb = (m >> 2) & 1;
+            int ig = ib ? I + 1 : 1, jg = jb ? J + 1 : 1, kg = kb ? K + 1 : 1;
+            int ic = ib ? I : 2, jc = jb ? J : 2, kc = kb ? K : 2;
+            int hi[3] = {I + 1, J + 1, K + 1}, bit[3] = {ib, jb, kb};
+            for (int d = 0; d < 3; d++) { q[m].is[d] = bit[d] ? hi[d] : -1; q[m].ie[d] = bit[d] ? hi[d] + 2 : 1; }
+            set3(q[m].nb[0], ig, jc, kc); set3(q[m].nb[1], ic, jg, kc); set3(q[m].nb[2], ic, jc, kg);
+            set3(q[m].org, ic, jc, kc);
+            set3(q[m].nb[3], ic, jg, kg); set3(q[m].nb[4], ig, jc, kg); set3(q[m].nb[5], ig, jg, kc);
+            q[m].face = F_NONE;
+            q[m].b = b;
+        }
+        set3(q[5].nb[2], I, 1, K + 1);   // neighbor3 of corner 6 as tabulated at src/bc_module.f90:958-960
+        for (int m = 0; m < 8; m++) {
+            Seg &cm = q[m];
+            bool is = covers(cm.nb[0], 0, 1, 2), js = covers(cm.nb[1], 0, 1, 2), ks = covers(cm.nb[2], 0, 1, 2);
+            if (is && js && ks) cm.fn = iturb == 0 ? FN_CORNERWWKW : FN_CORNERWW;
+            else if (!is && js && ks) { memcpy(cm.org, cm.nb[0], 12); set3(cm.nb[0], 0, 0, 0); memcpy(cm.nb[1], cm.nb[4], 12); memcpy(cm.nb[2], cm.nb[5], 12); cm.fn = ww; }
+            else if (is && !js && ks) { memcpy(cm.org, cm.nb[1], 12); memcpy(cm.nb[0], cm.nb[3], 12); set3(cm.nb[1], 0, 0, 0); memcpy(cm.nb[2], cm.nb[5], 12); cm.fn = ww; }
+            else if (is && js && !ks) { memcpy(cm.org, cm.nb[2], 12); memcpy(cm.nb[0], cm.nb[3], 12); memcpy(cm.nb[1], cm.nb[4], 12); set3(cm.nb[2], 0, 0, 0); cm.fn = ww; }
+            else if (!is && !js && ks) { cm.face = m < 4 ? F_KMIN : F_KMAX; memcpy(cm.org, cm.nb[5], 12); cm.fn = wall_fn(iturb); }
+            else if (is && !js && !ks) { cm.face = (m % 2 == 0) ? F_IMIN : F_IMAX; memcpy(cm.org, cm.nb[3], 12); cm.fn = wall_fn(iturb); }
+            else if (!is && js && !ks) { cm.face = (m == 0 || m == 1 || m == 4 || m == 5) ? F_JMIN : F_JMAX; memcpy(cm.org, cm.nb[4], 12); cm.fn = wall_fn(iturb); }
+            else cm.fn = FN_CORNERNOWALL;
+        }
+        for (Seg &s : faces) if (s.fn != FN_EMPTY) c->segs[0].push_back(s);
+        for (int m = 0; m < 12; m++) c->segs[1].push_back(e[m]);
+        for (int m = 0; m < 8; m++) c->segs[2].push_back(q[m]);
+    }
+    for (int w = 0; w < 3; w++) {
+        long start = 0;
+        for (Seg &s : c->segs[w]) {
+            s.start = start;
+            start += (long)(s.ie[0] - s.is[0] + 1) * (s.ie[1] - s.is[1] + 1) * (s.ie[2] - s.is[2] + 1);
+        }
+        c->S.NBC[w] = start;
+        c->S.nbcseg[w] = (int)c->segs[w].size();
+    }
+    return 0;
+}
+
+int launch_setbc(cav3d_ctx *c)
+{
+    DS &S = c->S;
+    if (S.NBC[0]) hipLaunchKernelGGL(k_bc, dim3(nblk(S.NBC[0], 128)), dim3(128), 0, c->st, S, 0);
+    if (S.NHALO) hipLaunchKernelGGL(k_halo, dim3(nblk(S.NHALO, 128)), dim3(128), 0, c->st, S);
+    if (S.NBC[1]) hipLaunchKernelGGL(k_bc, dim3(nblk(S.NBC[1], 128)), dim3(128), 0, c->st, S, 1);
+    if (S.NBC[2]) hipLaunchKernelGGL(k_bc, dim3(nblk(S.NBC[2], 128)), dim3(128), 0, c->st, S, 2);
+    return 0;
+}
+
+int launch_rhs(cav3d_ctx *c)
+{
+    DS &S = c->S;
+    hipLaunchKernelGGL(k_face<0>, dim3(nblk(S.NF[0], 128)), dim3(128), 0, c->st, S);
+    hipLaunchKernelGGL(k_face<1>, dim3(nblk(S.NF[1], 128)), dim3(128), 0, c->st, S);
+    hipLaunchKernelGGL(k_face<2>, dim3(nblk(S.NF[2], 128)), dim3(128), 0, c->st, S);
+    hipLaunchKernelGGL(k_combine, dim3(nblk(S.NC, 128)), dim3(128), 0, c->st, S);
+    return 0;
+}
+
+int launch_timestep(cav3d_ctx *c)
+{
+    DS &S = c->S;
+    hipLaunchKernelGGL(k_reset_scal, dim3(1), dim3(64), 0, c->st, S);
+    hipLaunchKernelGGL(k_timestep, dim3(nblk(S.NC, 256)), dim3(256), 0, c->st, S);
+    if (c->prm.local == 0) hipLaunchKernelGGL(k_dt_fill, dim3(nblk(S.NC, 256)), dim3(256), 0, c->st, S);
+    hipLaunchKernelGGL(k_time_acc, dim3(1), dim3(64), 0, c->st, S);
+    return 0;
+}
+
+int launch_sweeps(cav3d_ctx *c)
+{
+    DS &S = c->S;
+    for (int b = 0; b < c->nb; b++) {
+        const BlkD &B = c->blk[b];
+        long nt = (long)(B.J - 1) * (B.K - 1);
+        unsigned g = nblk(nt, 64);
+        for (int s = 6; s <= B.I + B.J + B.K; s++)
+            hipLaunchKernelGGL(k_sweep<true>, dim3(g), dim3(64), 0, c->st, S, b, s);
+        for (int s = B.I + B.J + B.K; s >= 6; s--)
+            hipLaunchKernelGGL(k_sweep<false>, dim3(g), dim3(64), 0, c->st, S, b, s);
+    }
+    return 0;
+}
+
+int launch_update(cav3d_ctx *c)
+{
+    DS &S = c->S;
+    hipLaunchKernelGGL(k_update, dim3(c->nwg_update), dim3(256), 0, c->st, S);
+    hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(256), 0, c->st, S, c->nwg_update);
+    if (c->prm.iturb == 0) hipLaunchKernelGGL(k_eddy, dim3(nblk(S.NC, 128)), dim3(128), 0, c->st, S);
+    return 0;
+}
+
+int check_launch(const char *what)
+{
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(std::string(what) + ": " + hipGetErrorString(e), -3);
+    return 0;
+}
+
+}  // namespace
+
+// ============================================================ C ABI
+extern "C" {
+
+const char *cav3d_last_error(void) { return g_err.c_str(); }
+
+int cav3d_create(const cav3d_params *prm, int nblocks, const cav3d_block *blocks, cav3d_ctx **out)
+{
+    if (!prm || !blocks || !out) return fail("null argument", -1);
+    *out = nullptr;
+    if (nblocks < 1 || nblocks > MAXB) return fail("nblocks must be 1.." + std::to_string(MAXB), -1);
+    if (prm->iturb != 0 && prm->iturb != -2 && prm->iturb != -3)
+        return fail("iturb must be 0 (k-omega SST), -2 (laminar) or -3 (Euler)", -1);
+    if (prm->nscheme != 1 && prm->nscheme != 4) return fail("nscheme must be 1 (Roe) or 4 (AUSM+-up)", -1);
+    if (prm->nmuscl < 0 || prm->nmuscl > 2 || prm->nlim < 0 || prm->nlim > 10) return fail("bad nmuscl/nlim", -1);
+    if (prm->ncav != 0 && prm->ncav != 1) return fail("ncav must be 0 or 1 (Merkle)", -1);
+    if (prm->cav_clip != 0 && prm->cav_clip != 1) return fail("cav_clip must be 0 (as written) or 1 (species)", -1);
+    if (prm->nsteady < 0 || prm->nsteady > 2) return fail("nsteady must be 0, 1 or 2", -1);
+    if (!(prm->dt_phy > 0.0)) return fail("dt_phy must be > 0 (it divides the BDF term of getx)", -1);
+    if (prm->precision != CAV3D_FP64) return fail("only the FP64 path is built in this library", -1);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail("no HIP device", -4);
+    cav3d_ctx *c = new cav3d_ctx();
+    c->prm = *prm;
+    c->nb = nblocks;
+    c->npv = prm->iturb == 0 ? 9 : 7;
+    c->ntv = prm->iturb == 0 ? 3 : (prm->iturb == -2 ? 2 : 0);
+    c->ngrd = prm->iturb == 0 ? 5 : 4;
+    Par &P = c->P;
+    memset(&P, 0, sizeof(P));
+    P.npv = c->npv; P.ntv = c->ntv; P.iturb = prm->iturb; P.nscheme = prm->nscheme; P.nmuscl = prm->nmuscl;
+    P.nlim = prm->nlim; P.local = prm->local; P.prec = prm->prec; P.ncav = prm->ncav; P.nsteady = prm->nsteady;
+    P.cav_clip = prm->cav_clip;
+    P.cfl = prm->cfl; P.pref = prm->pref; P.uref = prm->uref; P.str = prm->str; P.dt_phy = prm->dt_phy;
+    P.c_v = prm->c_v; P.c_c = prm->c_c; P.dp_ref = prm->dp_ref; P.t_ref = prm->t_ref; P.t_crit = prm->t_crit;
+    for (int q = 0; q < 9; q++) P.bc_pv[q] = prm->bc_pv[q];
+    P.bc_pressure = prm->bc_pressure;
+    P.emutref = prm->emutref;
+    P.bdf_b = prm->nsteady == 1 ? 1.0 : (prm->nsteady == 2 ? 1.5 * 0.5 : 0.0);
+    if (hipStreamCreate(&c->st) != hipSuccess) { delete c; return fail("hipStreamCreate failed", -3); }
+    DS &S = c->S;
+    memset(&S, 0, sizeof(S));
+    S.nb = nblocks;
+    S.npv = c->npv;
+    S.P = P;
+    long g0 = 0, c0 = 0, w0 = 0, f0[3] = {0, 0, 0}, x0 = 0, e0 = 0, t0 = 0;
+    c->blk.resize(nblocks);
+    c->bcdesc.resize(nblocks);
+    for (int b = 0; b < nblocks; b++) {
+        const cav3d_block &cb = blocks[b];
+        if (cb.imax < 4 || cb.jmax < 4 || cb.kmax < 4) { delete c; return fail("each block needs at least 3 cells per direction", -1); }
+        BlkD &B = c->blk[b];
+        B.I = cb.imax; B.J = cb.jmax; B.K = cb.kmax;
+        B.nx = B.I + 5; B.ny = B.J + 5;
+        B.ng = (long)B.nx * B.ny * (B.K + 5);
+        B.nc = (long)(B.I - 1) * (B.J - 1) * (B.K - 1);
+        B.nw = (long)(B.I + 1) * (B.J + 1) * (B.K + 1);
+        B.nf[0] = (long)B.I * (B.J - 1) * (B.K - 1);
+        B.nf[1] = (long)(B.I - 1) * B.J * (B.K - 1);
+        B.nf[2] = (long)(B.I - 1) * (B.J - 1) * B.K;
+        B.ncx = (long)B.I * (B.J + 1) * (B.K + 1);
+        B.nex = (long)(B.I + 1) * B.J * (B.K + 1);
+        B.ntx = (long)(B.I + 1) * (B.J + 1) * B.K;
+        B.g0 = g0; B.c0 = c0; B.w0 = w0; B.cx0 = x0; B.ex0 = e0; B.tx0 = t0;
+        for (int d = 0; d < 3; d++) { B.f0[d] = f0[d]; f0[d] += B.nf[d]; }
+        g0 += B.ng; c0 += B.nc; w0 += B.nw; x0 += B.ncx; e0 += B.nex; t0 += B.ntx;
+        S.b[b] = B;
+        if (cb.nbc < 0 || (cb.nbc > 0 && !cb.bc)) { delete c; return fail("bad bc list", -1); }
+        c->bcdesc[b].assign(cb.bc, cb.bc + 7 * cb.nbc);
+    }
+    S.NG = g0; S.NC = c0; S.NW = w0; S.NCX = x0; S.NEX = e0; S.NTX = t0;
+    for (int d = 0; d < 3; d++) S.NF[d] = f0[d];
+    // interfaces
+    long hs = 0;
+    for (int b = 0; b < nblocks; b++) {
+        const cav3d_block &cb = blocks[b];
+        for (int n = 0; n < cb.ncon; n++) {
+            const int *d = cb.con + 22 * n;
+            Con cn;
+            memset(&cn, 0, sizeof(cn));
+            cn.src = b; cn.dst = d[0];
+            if (cn.dst < 0 || cn.dst >= nblocks) { delete c; return fail("interface donor out of range", -1); }
+            for (int q = 0; q < 3; q++) { cn.is[q] = d[1 + q]; cn.ie[q] = d[4 + q]; cn.isd[q] = d[7 + q]; }
+            for (int q = 0; q < 9; q++) cn.tm[q] = d[13 + q];
+            long cnt = 1;
+            for (int q = 0; q < 3; q++) cnt *= (cn.ie[q] >= cn.is[q]) ? (cn.ie[q] - cn.is[q] + 1) : 0;
+            if (cnt == 0) continue;
+            cn.start = hs;
+            hs += cnt;
+            c->cons.push_back(cn);
+        }
+    }
+    S.NHALO = hs;
+    S.ncon = (int)c->cons.size();
+    int rc = build_bcs(c);
+    if (rc) { cav3d_destroy(c); return rc; }
+    const int npv = c->npv;
+    if ((rc = dalloc(c, &S.pv, npv * S.NG)) || (rc = dalloc(c, &S.dc, (long)ND * S.NG)) ||
+        (rc = dalloc(c, &S.vol, S.NW)) || (rc = dalloc(c, &S.cx, 3 * S.NCX)) || (rc = dalloc(c, &S.ex, 3 * S.NEX)) ||
+        (rc = dalloc(c, &S.tx, 3 * S.NTX)) || (rc = dalloc(c, &S.res, npv * S.NC)) || (rc = dalloc(c, &S.dt, S.NC)) ||
+        (rc = dalloc(c, &S.dqf, npv * S.NW)) || (rc = dalloc(c, &S.dqb, npv * S.NW)) || (rc = dalloc(c, &S.dcv, npv * S.NW)) ||
+        (rc = dalloc(c, &S.scal, 32))) { cav3d_destroy(c); return rc; }
+    for (int d = 0; d < 3; d++)
+        if ((rc = dalloc(c, &S.ff[d], 2 * npv * S.NF[d]))) { cav3d_destroy(c); return rc; }
+    if (prm->iturb == 0) {
+        if ((rc = dalloc(c, &S.mut, S.NG)) || (rc = dalloc(c, &S.ydn, S.NW)) || (rc = dalloc(c, &S.itt, 4 * S.NC)) ||
+            (rc = dalloc(c, &S.ocut, S.NC))) { cav3d_destroy(c); return rc; }
+    }
+    if (prm->ncav) {
+        if ((rc = dalloc(c, &S.icav, 4 * S.NC)) || (rc = dalloc(c, &S.rcav, 2 * S.NC))) { cav3d_destroy(c); return rc; }
+    }
+    if (prm->nsteady > 0) {
+        if ((rc = dalloc(c, &S.qq, 2 * npv * S.NC))) { cav3d_destroy(c); return rc; }
+    }
+    c->nwg_update = (int)nblk(S.NC, 256);
+    if ((rc = dalloc(c, &S.red, (long)c->nwg_update * (NPV + 2)))) { cav3d_destroy(c); return rc; }
+    for (int w = 0; w < 3; w++) {
+        if ((rc = dalloc(c, &S.seg[w], c->segs[w].size()))) { cav3d_destroy(c); return rc; }
+        if (!c->segs[w].empty() &&
+            hipMemcpyAsync(S.seg[w], c->segs[w].data(), c->segs[w].size() * sizeof(Seg), hipMemcpyHostToDevice, c->st) != hipSuccess) {
+            cav3d_destroy(c); return fail("segment upload failed", -3);
+        }
+    }
+    if ((rc = dalloc(c, &S.con, c->cons.size()))) { cav3d_destroy(c); return rc; }
+    if (!c->cons.empty() &&
+        hipMemcpyAsync(S.con, c->cons.data(), c->cons.size() * sizeof(Con), hipMemcpyHostToDevice, c->st) != hipSuccess) {
+        cav3d_destroy(c); return fail("interface upload failed", -3);
+    }
+    for (int q = 0; q < 8; q++) hipEventCreate(&c->ev[q]);
+    if (hipStreamSynchronize(c->st) != hipSuccess) { cav3d_destroy(c); return fail("device set-up failed", -3); }
+    *out = c;
+    return 0;
+}
+
+void cav3d_destroy(cav3d_ctx *c)
+{
+    if (!c) return;
+    if (c->st) hipStreamSynchronize(c->st);
+    for (void *p : c->allocs) hipFree(p);
+    if (c->stage) hipFree(c->stage);
+    for (int q = 0; q < 8; q++) if (c->ev[q]) hipEventDestroy(c->ev[q]);
+    if (c->st) hipStreamDestroy(c->st);
+    delete c;
+}
+
+int cav3d_set_metrics(cav3d_ctx *c, int b, const double *cx, const double *ex, const double *tx, const double *grd)
+{
+    if (!c || b < 0 || b >= c->nb || !cx || !ex || !tx || !grd) return fail("bad argument to cav3d_set_metrics", -1);
+    const BlkD &B = c->blk[b];
+    DS &S = c->S;
+    int rc;
+    long nmax = std::max(std::max(3 * B.ncx, 3 * B.nex), std::max(3 * B.ntx, (long)c->ngrd * B.nw));
+    if ((rc = ensure_stage(c, nmax))) return rc;
+    struct { const double *h; long n; double *d; long nst; long o; } parts[3] = {
+        {cx, B.ncx, S.cx, S.NCX, B.cx0}, {ex, B.nex, S.ex, S.NEX, B.ex0}, {tx, B.ntx, S.tx, S.NTX, B.tx0}};
+    for (auto &p : parts) {
+        HIPCHK(hipMemcpyAsync(c->stage, p.h, 3 * p.n * sizeof(double), hipMemcpyHostToDevice, c->st));
+        hipLaunchKernelGGL(k_aos_to_soa, dim3(nblk(p.n, 256)), dim3(256), 0, c->st, (const double *)c->stage, 3, 3, 0, p.d, p.nst, p.o, p.n);
+        HIPCHK(hipStreamSynchronize(c->st));
+    }
+    HIPCHK(hipMemcpyAsync(c->stage, grd, (long)c->ngrd * B.nw * sizeof(double), hipMemcpyHostToDevice, c->st));
+    hipLaunchKernelGGL(k_aos_to_soa, dim3(nblk(B.nw, 256)), dim3(256), 0, c->st, (const double *)c->stage, 1, c->ngrd, 0, S.vol, S.NW, B.w0, B.nw);
+    if (S.ydn)
+        hipLaunchKernelGGL(k_aos_to_soa, dim3(nblk(B.nw, 256)), dim3(256), 0, c->st, (const double *)c->stage, 1, c->ngrd, 4, S.ydn, S.NW, B.w0, B.nw);
+    HIPCHK(hipStreamSynchronize(c->st));
+    return check_launch("cav3d_set_metrics");
+}
+
+int cav3d_put_state(cav3d_ctx *c, int b, const double *pv, const double *dv, const double *tv, const double *qq)
+{
+    if (!c || b < 0 || b >= c->nb || !pv) return fail("bad argument to cav3d_put_state", -1);
+    const BlkD &B = c->blk[b];
+    DS &S = c->S;
+    int rc;
+    if ((rc = ensure_stage(c, (long)(18 + 3) * B.ng + 2L * c->npv * B.nc))) return rc;
+    HIPCHK(hipMemcpyAsync(c->stage, pv, (long)c->npv * B.ng * sizeof(double), hipMemcpyHostToDevice, c->st));
+    hipLaunchKernelGGL(k_aos_to_soa, dim3(nblk(B.ng, 256)), dim3(256), 0, c->st, (const double *)c->stage, c->npv, c->npv, 0, S.pv, S.NG, B.g0, B.ng);
+    HIPCHK(hipStreamSynchronize(c->st));
+    if (dv) {
+        HIPCHK(hipMemcpyAsync(c->stage, dv, 18L * B.ng * sizeof(double), hipMemcpyHostToDevice, c->st));
+        double *tvd = c->stage + 18L * B.ng;
+        if (tv && c->ntv) HIPCHK(hipMemcpyAsync(tvd, tv, (long)c->ntv * B.ng * sizeof(double), hipMemcpyHostToDevice, c->st));
+        hipLaunchKernelGGL(k_dv_to_dc, dim3(nblk(B.ng, 256)), dim3(256), 0, c->st, (const double *)c->stage,
+                           (const double *)tvd, (tv ? c->ntv : 0), S.dc, S.mut, S.NG, B.g0, B.ng);
+        HIPCHK(hipStreamSynchronize(c->st));
+    } else {
+        DS S1 = S;   // evaluate this block's interior only
+        S1.nb = 1; S1.b[0] = B; S1.b[0].c0 = 0; S1.NC = B.nc;
+        S1.b[0].g0 = B.g0;
+        hipLaunchKernelGGL(k_eos_cells, dim3(nblk(B.nc, 128)), dim3(128), 0, c->st, S1, 1, 1);
+        HIPCHK(hipStreamSynchronize(c->st));
+    }
+    if (c->prm.nsteady > 0) {
+        if (qq) {
+            HIPCHK(hipMemcpyAsync(c->stage, qq, 2L * c->npv * B.nc * sizeof(double), hipMemcpyHostToDevice, c->st));
+            hipLaunchKernelGGL(k_cell_from_aos, dim3(nblk(B.nc, 256)), dim3(256), 0, c->st, (const double *)c->stage,
+                               2 * c->npv, S.NC, B.c0, S.qq, B.nc);
+        } else {
+            DS S1 = S;
+            S1.nb = 1; S1.b[0] = B; S1.NC = S.NC;
+            // both levels from the current state (initial_module set_others)
+            hipLaunchKernelGGL(k_unsteady_shift, dim3(nblk(S.NC, 256)), dim3(256), 0, c->st, S);
+            hipLaunchKernelGGL(k_unsteady_shift, dim3(nblk(S.NC, 256)), dim3(256), 0, c->st, S);
+        }
+        HIPCHK(hipStreamSynchronize(c->st));
+    }
+    return check_launch("cav3d_put_state");
+}
+
+int cav3d_get_state(cav3d_ctx *c, int b, double *pv, double *dv, double *tv, double *qq)
+{
+    if (!c || b < 0 || b >= c->nb) return fail("bad argument to cav3d_get_state", -1);
+    const BlkD &B = c->blk[b];
+    DS &S = c->S;
+    int rc;
+    HIPCHK(hipStreamSynchronize(c->st));
+    if ((rc = ensure_stage(c, (long)(18 + 3) * B.ng + 2L * c->npv * B.nc))) return rc;
+    if (pv) {
+        hipLaunchKernelGGL(k_soa_to_aos, dim3(nblk(B.ng, 256)), dim3(256), 0, c->st, (const double *)S.pv, c->npv, S.NG, B.g0, c->stage, B.ng);
+        HIPCHK(hipMemcpyAsync(pv, c->stage, (long)c->npv * B.ng * sizeof(double), hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+    }
+    if (dv || tv) {
+        // evaluate the EOS over this block's ghosted cells into AoS staging
+        double *dvd = c->stage, *tvd = c->stage + 18L * B.ng;
+        hipLaunchKernelGGL(k_eos_full, dim3(nblk(B.ng, 128)), dim3(128), 0, c->st, S, B.g0, B.ng, dvd, tvd);
+        HIPCHK(hipStreamSynchronize(c->st));
+        if (dv) HIPCHK(hipMemcpy(dv, dvd, 18L * B.ng * sizeof(double), hipMemcpyDeviceToHost));
+        if (tv && c->ntv) {
+            std::vector<double> t2(2 * B.ng), mt;
+            HIPCHK(hipMemcpy(t2.data(), tvd, 2L * B.ng * sizeof(double), hipMemcpyDeviceToHost));
+            if (c->ntv == 3) {
+                mt.resize(B.ng);
+                HIPCHK(hipMemcpy(mt.data(), S.mut + B.g0, B.ng * sizeof(double), hipMemcpyDeviceToHost));
+            }
+            for (long g = 0; g < B.ng; g++) {
+                tv[c->ntv * g] = t2[2 * g];
+                tv[c->ntv * g + 1] = t2[2 * g + 1];
+                if (c->ntv == 3) tv[3 * g + 2] = mt[g];
+            }
+        }
+    }
+    if (qq && c->prm.nsteady > 0) {
+        hipLaunchKernelGGL(k_cell_to_aos, dim3(nblk(B.nc, 256)), dim3(256), 0, c->st, (const double *)S.qq, 2 * c->npv, S.NC, B.c0, c->stage, B.nc);
+        HIPCHK(hipMemcpyAsync(qq, c->stage, 2L * c->npv * B.nc * sizeof(double), hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+    }
+    return check_launch("cav3d_get_state");
+}
+
+int cav3d_setbc(cav3d_ctx *c)
+{
+    if (!c) return fail("null context", -1);
+    launch_setbc(c);
+    HIPCHK(hipStreamSynchronize(c->st));
+    return check_launch("cav3d_setbc");
+}
+
+int cav3d_timestep(cav3d_ctx *c, int b, double *dt)
+{
+    if (!c || b < 0 || b >= c->nb) return fail("bad argument to cav3d_timestep", -1);
+    launch_timestep(c);
+    const BlkD &B = c->blk[b];
+    if (dt) HIPCHK(hipMemcpyAsync(dt, c->S.dt + B.c0, B.nc * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    return check_launch("cav3d_timestep");
+}
+
+int cav3d_calrhs(cav3d_ctx *c, double time, int b, double *res, double *icav, double *itt, double *omega_cut)
+{
+    (void)time;
+    if (!c || b < 0 || b >= c->nb) return fail("bad argument to cav3d_calrhs", -1);
+    DS &S = c->S;
+    const BlkD &B = c->blk[b];
+    int rc;
+    if ((rc = ensure_stage(c, (long)c->npv * B.nc))) return rc;
+    launch_rhs(c);
+    struct { double *h; const double *d; int m; } outs[4] = {{res, S.res, c->npv}, {icav, S.icav, 4}, {itt, S.itt, 4}, {omega_cut, S.ocut, 1}};
+    for (auto &o : outs) {
+        if (!o.h) continue;
+        if (!o.d) { memset(o.h, 0, sizeof(double) * o.m * B.nc); continue; }
+        hipLaunchKernelGGL(k_cell_to_aos, dim3(nblk(B.nc, 256)), dim3(256), 0, c->st, o.d, o.m, S.NC, B.c0, c->stage, B.nc);
+        HIPCHK(hipMemcpyAsync(o.h, c->stage, (long)o.m * B.nc * sizeof(double), hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+    }
+    HIPCHK(hipStreamSynchronize(c->st));
+    return check_launch("cav3d_calrhs");
+}
+
+int cav3d_timeinteg(cav3d_ctx *c, int nt_phy, int nt, double timeprev, double *time)
+{
+    (void)nt;
+    if (!c) return fail("null context", -1);
+    if (c->profile) hipEventRecord(c->ev[0], c->st);
+    launch_timestep(c);
+    if (c->profile) hipEventRecord(c->ev[1], c->st);
+    launch_setbc(c);
+    if (c->profile) hipEventRecord(c->ev[2], c->st);
+    launch_rhs(c);
+    if (c->profile) hipEventRecord(c->ev[3], c->st);
+    launch_sweeps(c);
+    if (c->profile) hipEventRecord(c->ev[4], c->st);
+    launch_update(c);
+    if (c->profile) {
+        hipEventRecord(c->ev[5], c->st);
+        hipEventSynchronize(c->ev[5]);
+        float ms;
+        for (int q = 0; q < 5; q++) { hipEventElapsedTime(&ms, c->ev[q], c->ev[q + 1]); c->prof_ms[q] = ms; }
+        hipEventElapsedTime(&ms, c->ev[0], c->ev[5]);
+        c->prof_ms[5] = ms;
+    }
+    int rc = check_launch("cav3d_timeinteg");
+    if (rc) return rc;
+    if (time) {
+        double tacc;
+        HIPCHK(hipMemcpyAsync(&tacc, c->S.scal + 1, sizeof(double), hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+        *time = (c->prm.nsteady == 0) ? tacc + timeprev : c->prm.dt_phy * (double)nt_phy;
+    }
+    return 0;
+}
+
+int cav3d_residual(cav3d_ctx *c, double *l2, int *ijk, double *resmax)
+{
+    if (!c) return fail("null context", -1);
+    double h[32];
+    HIPCHK(hipMemcpyAsync(h, c->S.scal, sizeof(h), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    if (l2) for (int n = 0; n < c->npv; n++) l2[n] = h[4 + n];
+    long bi;
+    memcpy(&bi, &h[5 + NPV], sizeof(long));
+    double bm = h[4 + NPV];
+    if (ijk) {
+        if (bm > 1.0e-12) {
+            ijk[0] = (int)(bi >> 48); ijk[1] = (int)(bi & 0xffff); ijk[2] = (int)((bi >> 16) & 0xffff); ijk[3] = (int)((bi >> 32) & 0xffff);
+        } else { ijk[0] = -1; ijk[1] = ijk[2] = ijk[3] = 0; }
+    }
+    if (resmax) *resmax = bm > 1.0e-12 ? bm : 1.0e-12;
+    return 0;
+}
+
+int cav3d_unsteady_shift(cav3d_ctx *c)
+{
+    if (!c) return fail("null context", -1);
+    if (c->prm.nsteady == 0) return 0;
+    hipLaunchKernelGGL(k_unsteady_shift, dim3(nblk(c->S.NC, 256)), dim3(256), 0, c->st, c->S);
+    HIPCHK(hipStreamSynchronize(c->st));
+    return check_launch("cav3d_unsteady_shift");
+}
+
+int cav3d_eos(cav3d_ctx *c, int n, const double *p, const double *t, const double *y1, const double *y2, double *dv, double *tv)
+{
+    if (!c || n < 0 || !p || !t || !y1 || !y2 || !dv || !tv) return fail("bad argument to cav3d_eos", -1);
+    if (n == 0) return 0;
+    int rc;
+    if ((rc = ensure_stage(c, 24L * n))) return rc;
+    double *d = c->stage;
+    HIPCHK(hipMemcpyAsync(d, p, n * 8, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(d + n, t, n * 8, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(d + 2L * n, y1, n * 8, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(d + 3L * n, y2, n * 8, hipMemcpyHostToDevice, c->st));
+    hipLaunchKernelGGL(k_eos_points, dim3(nblk(n, 128)), dim3(128), 0, c->st, c->P, n, (const double *)d, (const double *)(d + n),
+                       (const double *)(d + 2L * n), (const double *)(d + 3L * n), d + 4L * n, d + 22L * n);
+    HIPCHK(hipMemcpyAsync(dv, d + 4L * n, 18L * n * 8, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipMemcpyAsync(tv, d + 22L * n, 2L * n * 8, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    return check_launch("cav3d_eos");
+}
+
+int cav3d_eos_host(int n, const double *p, const double *t, const double *y1, const double *y2, double *dv, double *tv)
+{
+    if (n < 0 || !p || !t || !y1 || !y2 || !dv || !tv) return fail("bad argument to cav3d_eos_host", -1);
+    auto work = [&](int lo, int hi) {
+        for (int q = lo; q < hi; q++) {
+            double d[ND];
+            mixture_eos<double>(p[q], t[q], y1[q], y2[q], d, true, dv + 18L * q);
+            tv[2L * q] = d[D_MU];
+            tv[2L * q + 1] = d[D_KAP];
+        }
+    };
+    int nt = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    if (n < 4096) nt = 1;
+    std::vector<std::thread> th;
+    for (int w = 0; w < nt; w++) th.emplace_back(work, (int)((long)n * w / nt), (int)((long)n * (w + 1) / nt));
+    for (auto &x : th) x.join();
+    return 0;
+}
+
+int cav3d_profile(cav3d_ctx *c, int enable, double *out)
+{
+    if (!c) return fail("null context", -1);
+    c->profile = enable != 0;
+    if (out) for (int q = 0; q < 6; q++) out[q] = c->prof_ms[q];
+    return 0;
+}
+
+int cav3d_synchronize(cav3d_ctx *c)
+{
+    if (!c) return fail("null context", -1);
+    HIPCHK(hipStreamSynchronize(c->st));
+    return check_launch("cav3d_synchronize");
+}
+
+}  // extern "C"

@@ -1,0 +1,869 @@
+// Pointwise physics of the pseudo-time step, written once for the host and
+// the gfx950 device (HD = __host__ __device__).  Templates take the
+// arithmetic type R of the face/flux work (double on the FP64 path, float on
+// the mixed path); the IF97 saturation line, the LU-SGS diagonal inverse and
+// every norm stay in double.
+//
+// Reference formulas (kwonhwi1/3dcode):
+//   IF97 region 1/2/metastable-2    src/eos_module.f90:1331-1461
+//   IAPS-85 transport, ideal gas    src/eos_module.f90:1236-1282, 1463-1659
+//   saturation line                 src/eos_module.f90:2266-2279
+//   mixture rules                   src/eos_module.f90:1064-1118, 1176-1234
+//   MUSCL-TVD / MLP / limiters      src/muscl_module.f90:104-413
+//   preconditioned Roe, AUSM+-up    src/flux_module.f90:198-276, 548-695
+//   viscous flux + SST F1           src/vsflux_module.f90:182-425
+//   Merkle source                   src/cav_module.f90:136-171
+//   SST source / eddy viscosity     src/turbsource_module.f90:269-414,
+//                                   src/eddy_module.f90:135-187
+//   split Jacobian (applied as a    src/jacobian_module.f90:182-465
+//     structured mat-vec)
+//   diagonal inverse getx           src/lhs_module.f90:346-534
+//   local time step                 src/timestep_module.f90:156-198
+#pragma once
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#define HD __host__ __device__ __forceinline__
+
+namespace cav {
+
+constexpr int NPV = 9;
+
+// compact cell properties kept per cell (the dv/tv entries the path reads)
+enum { D_RHO = 0, D_H, D_C2, D_RP, D_RT, D_RY1, D_RY2, D_HP, D_HT, D_HY1, D_HY2, D_MU, D_KAP, ND };
+
+struct Par {
+    int npv, ntv, iturb, nscheme, nmuscl, nlim, local, prec, ncav, nsteady, cav_clip;
+    double cfl, pref, uref, str, dt_phy, c_v, c_c, dp_ref, t_ref, t_crit;
+    double bc_pv[9], bc_pressure, emutref, bdf_b;
+};
+
+template <class R> HD R mx(R a, R b) { return (b > a) ? b : a; }
+template <class R> HD R mn(R a, R b) { return (b < a) ? b : a; }
+
+// x^n for a non-negative exponent that is a compile-time constant after
+// unrolling: folds into a multiplication chain shared by common sub-expressions.
+template <class R> HD R ipow(R x, int n)
+{
+    R y = R(1);
+    while (n) {
+        if (n & 1) y *= x;
+        x *= x;
+        n >>= 1;
+    }
+    return y;
+}
+
+// ------------------------------------------------------------ IF97 tables
+// IAPWS-IF97 (2007) region 1 (table 2), region 2 ideal/residual (tables 10,
+// 11) and metastable-vapour residual (table 16) coefficients.
+struct IJN { int i, j; double n; };
+__device__ __host__ constexpr IJN R1C[34] = {
+    {0, -2, 0.14632971213167e0}, {0, -1, -0.84548187169114e0}, {0, 0, -0.37563603672040e1},
+    {0, 1, 0.33855169168385e1}, {0, 2, -0.95791963387872e0}, {0, 3, 0.15772038513228e0},
+    {0, 4, -0.16616417199501e-1}, {0, 5, 0.81214629983568e-3}, {1, -9, 0.28319080123804e-3},
+    {1, -7, -0.60706301565874e-3}, {1, -1, -0.18990068218419e-1}, {1, 0, -0.32529748770505e-1},
+    {1, 1, -0.21841717175414e-1}, {1, 3, -0.52838357969930e-4}, {2, -3, -0.47184321073267e-3},
+    {2, 0, -0.30001780793026e-3}, {2, 1, 0.47661393906987e-4}, {2, 3, -0.44141845330846e-5},
+    {2, 17, -0.72694996297594e-15}, {3, -4, -0.31679644845054e-4}, {3, 0, -0.28270797985312e-5},
+    {3, 6, -0.85205128120103e-9}, {4, -5, -0.22425281908000e-5}, {4, -2, -0.65171222895601e-6},
+    {4, 10, -0.14341729937924e-12}, {5, -8, -0.40516996860117e-6}, {8, -11, -0.12734301741641e-8},
+    {8, -6, -0.17424871230634e-9}, {21, -29, -0.68762131295531e-18}, {23, -31, 0.14478307828521e-19},
+    {29, -38, 0.26335781662795e-22}, {30, -39, -0.11947622640071e-22}, {31, -40, 0.18228094581404e-23},
+    {32, -41, -0.93537087292458e-25}};
+__device__ __host__ constexpr int R2J0[9] = {0, 1, -5, -4, -3, -2, -1, 2, 3};
+__device__ __host__ constexpr double R2N0[9] = {-0.96927686500217e1, 0.10086655968018e2, -0.56087911283020e-2,
+    0.71452738081455e-1, -0.40710498223928e0, 0.14240819171444e1, -0.43839511319450e1, -0.28408632460772e0,
+    0.21268463753307e-1};
+__device__ __host__ constexpr double R2MN0[2] = {-0.96937268393049e1, 0.10087275970006e2};
+__device__ __host__ constexpr IJN R2C[43] = {
+    {1, 0, -0.17731742473213e-2}, {1, 1, -0.17834862292358e-1}, {1, 2, -0.45996013696365e-1},
+    {1, 3, -0.57581259083432e-1}, {1, 6, -0.50325278727930e-1}, {2, 1, -0.33032641670203e-4},
+    {2, 2, -0.18948987516315e-3}, {2, 4, -0.39392777243355e-2}, {2, 7, -0.43797295650573e-1},
+    {2, 36, -0.26674547914087e-4}, {3, 0, 0.20481737692309e-7}, {3, 1, 0.43870667284435e-6},
+    {3, 3, -0.32277677238570e-4}, {3, 6, -0.15033924542148e-2}, {3, 35, -0.40668253562649e-1},
+    {4, 1, -0.78847309559367e-9}, {4, 2, 0.12790717852285e-7}, {4, 3, 0.48225372718507e-6},
+    {5, 7, 0.22922076337661e-5}, {6, 3, -0.16714766451061e-10}, {6, 16, -0.21171472321355e-2},
+    {6, 35, -0.23895741934104e2}, {7, 0, -0.59059564324270e-17}, {7, 11, -0.12621808899101e-5},
+    {7, 25, -0.38946842435739e-1}, {8, 8, 0.11256211360459e-10}, {8, 36, -0.82311340897998e1},
+    {9, 13, 0.19809712802088e-7}, {10, 4, 0.10406965210174e-18}, {10, 10, -0.10234747095929e-12},
+    {10, 14, -0.10018179379511e-8}, {16, 29, -0.80882908646985e-10}, {16, 50, 0.10693031879409e0},
+    {18, 57, -0.33662250574171e0}, {20, 20, 0.89185845355421e-24}, {20, 35, 0.30629316876232e-12},
+    {20, 48, -0.42002467698208e-5}, {21, 21, -0.59056029685639e-25}, {22, 53, 0.37826947613457e-5},
+    {23, 39, -0.12768608934681e-14}, {24, 26, 0.73087610595061e-28}, {24, 40, 0.55414715350778e-16},
+    {24, 58, -0.94369707241210e-6}};
+__device__ __host__ constexpr IJN R2MC[13] = {
+    {1, 0, -0.73362260186506e-2}, {1, 2, -0.88223831943146e-1}, {1, 5, -0.72334555213245e-1},
+    {1, 11, -0.40813178534455e-2}, {2, 1, 0.20097803380207e-2}, {2, 7, -0.53045921898642e-1},
+    {2, 16, -0.76190409086970e-2}, {3, 4, -0.63498037657313e-2}, {3, 16, -0.86043093028588e-1},
+    {4, 7, 0.75321581522770e-2}, {4, 10, -0.79238375446139e-2}, {5, 9, -0.22888160778447e-3},
+    {5, 10, -0.26456501482810e-2}};
+// IAPS-85 viscosity / conductivity coefficients
+__device__ __host__ constexpr double TPH[4] = {1.0, 0.978197, 0.579829, -0.202354};
+__device__ __host__ constexpr double TPL[4] = {1.0, 6.978267, 2.599096, -0.998254};
+__device__ __host__ constexpr double TPHH[6][7] = {
+    {0.5132047, 0.2151778, -0.2818107, 0.1778064, -0.0417661, 0.0, 0.0},
+    {0.3205656, 0.7317883, -1.070786, 0.460504, 0.0, -0.01578386, 0.0},
+    {0.0, 1.241044, -1.263184, 0.2340379, 0.0, 0.0, 0.0},
+    {0.0, 1.476783, 0.0, -0.4924179, 0.1600435, 0.0, -0.003629481},
+    {-0.7782567, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0},
+    {0.1885447, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0}};
+__device__ __host__ constexpr double TPLL[5][6] = {
+    {1.3293046, -0.40452437, 0.24409490, 0.018660751, -0.12961068, 0.044809953},
+    {1.7018363, -2.2156845, 1.6511057, -0.76736002, 0.37283344, -0.11203160},
+    {5.2246158, -10.124111, 4.9874687, -0.27297694, -0.43083393, 0.13333849},
+    {8.7127675, -9.5000611, 4.3786606, -0.91783782, 0.0, 0.0},
+    {-1.8525999, 0.93404690, 0.0, 0.0, 0.0, 0.0}};
+
+// Folded constants.  Five of them are default-REAL literals in the reference
+// (src/eos_module.f90:1367,1370,1371,1457) and are used at float precision.
+constexpr double C_RHO1 = (double)25.841246054191803727f;   // p*/(R T*) region 1
+constexpr double C_RT1 = (double)0.01864447767257706f;
+constexpr double C_HP1 = (double)0.03869782431941923775f;
+constexpr double C_RT2A = (double)0.00185185185185185185f;  // 1/540
+constexpr double C_RT2B = (double)0.0074304703986495518f;
+
+template <class R> struct E2 { R rho, h, drdp, drdt, dhdp, dhdt; };
+
+// IF97 region-4 saturation pressure, always double (src/eos_module.f90:2266-2279)
+HD double sat_p(double t)
+{
+    double tt = mn(mx(t, 273.15), 647.096);
+    double th = tt - 0.23855557567849e0 / (tt - 0.65017534844798e3);
+    double a = th * th + 0.11670521452767e4 * th - 0.72421316703206e6;
+    double b = -0.17073846940092e2 * (th * th) + 0.12020824702470e5 * th - 0.32325550322333e7;
+    double c = 0.14915108613530e2 * (th * th) - 0.48232657361591e4 * th + 0.40511340542057e6;
+    double x = 2.0 * c / (-b + sqrt(b * b - 4.0 * a * c));
+    double x2 = x * x;
+    return 1.0e6 * (x2 * x2);
+}
+
+// Region 1: the five derivative sums of the Gibbs series are accumulated as
+// weighted sums of the terms and scaled once (sum_k J g / b etc.).
+template <class R> HD E2<R> liquid(R p, R t)
+{
+    R pi = p * R(0.00000006049607);
+    R tau = R(1386.0) / t;
+    R a = R(7.1) - pi, ia = R(1) / a;
+    R b = tau - R(1.222), ib = R(1) / b;
+    R sj = 0, si = 0, sii = 0, sjj = 0, sij = 0;
+#pragma unroll
+    for (int k = 0; k < 34; k++) {
+        const int I = R1C[k].i, J = R1C[k].j;
+        R g = R(R1C[k].n) * ipow(a, I) * (J >= 0 ? ipow(b, J) : ipow(ib, -J));
+        sj += R(J) * g;
+        si += R(I) * g;
+        sii += R(I * (I - 1)) * g;
+        sjj += R(J * (J - 1)) * g;
+        sij += R(I * J) * g;
+    }
+    R gt = sj * ib, gp = -si * ia, gpp = sii * (ia * ia), gtt = sjj * (ib * ib), gpt = -sij * (ia * ib);
+    R gp1 = R(1) / gp;
+    E2<R> e;
+    e.rho = R(C_RHO1) * tau * gp1;
+    e.h = R(639675.036) * gt;
+    e.drdp = -(gpp * R(1.5632937721834122e-6) * tau * (gp1 * gp1));
+    e.drdt = -(e.rho * R(7.215007215007215e-4) * tau) + R(C_RT1) * gpt * (tau * tau * tau) * (gp1 * gp1);
+    e.dhdp = R(C_HP1) * gpt;
+    e.dhdt = -(R(461.526) * (tau * tau) * gtt);
+    return e;
+}
+
+// Region 2 with the metastable-vapour branch for p > p_sat(T).
+template <class R> HD E2<R> vapour(R p, R t, double psat)
+{
+    R pi = p * R(1.0e-6);
+    R tau = R(540.0) / t;
+    R ipi = R(1.0e6) / p, itau = R(1) / tau;
+    R c = tau - R(0.5), ic = R(1) / c;
+    R s0j = 0, s0jj = 0;                // ideal part: sums of J g0, J(J-1) g0
+    R sj = 0, si = 0, sii = 0, sjj = 0, sij = 0;
+    const bool meta = (double)p > psat;
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        const int J = R2J0[k];
+        double n = R2N0[k];
+        if (meta && k < 2) n = R2MN0[k];
+        R g = R(n) * (J >= 0 ? ipow(tau, J) : ipow(itau, -J));
+        s0j += R(J) * g;
+        s0jj += R(J * (J - 1)) * g;
+    }
+    if (meta) {
+#pragma unroll
+        for (int k = 0; k < 13; k++) {
+            const int I = R2MC[k].i, J = R2MC[k].j;
+            R g = R(R2MC[k].n) * ipow(pi, I) * ipow(c, J);
+            sj += R(J) * g; si += R(I) * g; sii += R(I * (I - 1)) * g; sjj += R(J * (J - 1)) * g; sij += R(I * J) * g;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < 43; k++) {
+            const int I = R2C[k].i, J = R2C[k].j;
+            R g = R(R2C[k].n) * ipow(pi, I) * ipow(c, J);
+            sj += R(J) * g; si += R(I) * g; sii += R(I * (I - 1)) * g; sjj += R(J * (J - 1)) * g; sij += R(I * J) * g;
+        }
+    }
+    R g0t = s0j * itau, g0tt = s0jj * (itau * itau);
+    R grt = sj * ic, grp = si * ipi, grpp = sii * (ipi * ipi), grtt = sjj * (ic * ic), grpt = sij * (ipi * ic);
+    R gp1 = R(1) / (ipi + grp);
+    E2<R> e;
+    e.rho = R(4.01245401527075799) * tau * gp1;
+    e.h = R(249224.04) * (g0t + grt);
+    e.drdp = -((-(ipi * ipi) + grpp) * R(4.01245401527075799e-6) * tau * (gp1 * gp1));
+    e.drdt = -(e.rho * R(C_RT2A) * tau) + R(C_RT2B) * grpt * (tau * tau * tau) * (gp1 * gp1);
+    e.dhdp = R(0.24922404) * grpt;
+    e.dhdt = -(R(461.526) * (tau * tau) * (g0tt + grtt));
+    return e;
+}
+
+template <class R> HD E2<R> gas(R p, R t)
+{
+    const R r1 = R(0.00348383500557);
+    R tau = R(1) / t;
+    E2<R> e;
+    e.rho = p * r1 * tau;
+    e.h = R(1004.64) * t;
+    e.drdp = r1 * tau;
+    e.drdt = -(p * r1 * (tau * tau));
+    e.dhdp = R(0);
+    e.dhdt = R(1004.64);
+    return e;
+}
+
+// IAPS-85 mu, kappa of a water phase (no critical enhancement)
+template <class R> HD void iaps85(R t, R rho, R &mu, R &kap)
+{
+    R tt = t * R(0.00154495032985);
+    R rr = mx(R(1.0e-3), mn(R(1332.4), rho)) * R(0.00314699949333);
+    R t1 = R(647.27) / t;
+    R st = sqrt(tt);
+    R x = t1 - R(1), y = rr - R(1);
+    R s = 0;
+#pragma unroll
+    for (int i = 5; i >= 0; i--) {
+        R q = 0;
+#pragma unroll
+        for (int j = 6; j >= 0; j--) q = q * y + R(TPHH[i][j]);
+        s = s * x + q;
+    }
+    mu = R(55.071e-6) * (st / (R(TPH[0]) + t1 * (R(TPH[1]) + t1 * (R(TPH[2]) + t1 * R(TPH[3]))))) * exp(rr * s);
+    s = 0;
+#pragma unroll
+    for (int i = 4; i >= 0; i--) {
+        R q = 0;
+#pragma unroll
+        for (int j = 5; j >= 0; j--) q = q * y + R(TPLL[i][j]);
+        s = s * x + q;
+    }
+    kap = R(0.4945) * (st / (R(TPL[0]) + t1 * (R(TPL[1]) + t1 * (R(TPL[2]) + t1 * R(TPL[3]))))) * exp(rr * s);
+}
+
+// Mixture of liquid, vapour and ideal gas (fluid = IF97, ngas = ideal).
+// d[ND] receives the compact properties; full (if non-null) the 18 dv entries.
+// prop = compute mu, kappa (deteos_prop) as d[D_MU], d[D_KAP].
+template <class R> HD void mixture_eos(R p, R t, R y1, R y2, R *d, bool prop, R *full = nullptr)
+{
+    double psat = sat_p((double)t);
+    E2<R> l = liquid(p, t), v = vapour(p, t, psat), g = gas(p, t);
+    R il = R(1) / l.rho, iv = R(1) / v.rho, ig = R(1) / g.rho;
+    R yl = R(1) - y1 - y2;
+    R rho = R(1) / (yl * il + y1 * iv + y2 * ig);
+    R r2 = rho * rho;
+    d[D_RHO] = rho;
+    d[D_H] = yl * l.h + y1 * v.h + y2 * g.h;
+    d[D_RP] = r2 * (yl * (il * il) * l.drdp + y1 * (iv * iv) * v.drdp + y2 * (ig * ig) * g.drdp);
+    d[D_RT] = r2 * (yl * (il * il) * l.drdt + y1 * (iv * iv) * v.drdt + y2 * (ig * ig) * g.drdt);
+    d[D_RY1] = r2 * (il - iv);
+    d[D_RY2] = r2 * (il - ig);
+    d[D_HP] = yl * l.dhdp + y1 * v.dhdp + y2 * g.dhdp;
+    d[D_HT] = yl * l.dhdt + y1 * v.dhdt + y2 * g.dhdt;
+    d[D_HY1] = v.h - l.h;
+    d[D_HY2] = g.h - l.h;
+    R c2 = rho * d[D_HT] / (rho * d[D_RP] * d[D_HT] + d[D_RT] * (R(1) - rho * d[D_HP]));
+    if (c2 <= R(0)) {   // src/eos_module.f90:1104-1111
+        R cl = (l.rho * l.drdp * l.dhdt + l.drdt * (R(1) - l.rho * l.dhdp)) / (l.rho * l.dhdt);
+        R cv = (v.rho * v.drdp * v.dhdt + v.drdt * (R(1) - v.rho * v.dhdp)) / (v.rho * v.dhdt);
+        R cg = (g.rho * g.drdp * g.dhdt + g.drdt * (R(1) - g.rho * g.dhdp)) / (g.rho * g.dhdt);
+        c2 = R(1) / (r2 * (yl * cl * (il * il) + y1 * cv * (iv * iv) + y2 * cg * (ig * ig)));
+    }
+    d[D_C2] = c2;
+    if (prop) {
+        R ml, kl, mv, kv;
+        iaps85(t, l.rho, ml, kl);
+        iaps85(t, v.rho, mv, kv);
+        R mg = R(1.716e-5) * pow(t / R(273.11), R(1.5)) * R(383.67) / (t + R(110.56));
+        R kg = R(1395.333333) * mg;
+        d[D_MU] = rho * (ml * yl * il + mv * y1 * iv + mg * y2 * ig);
+        d[D_KAP] = rho * (kl * yl * il + kv * y1 * iv + kg * y2 * ig);
+    }
+    if (full) {
+        full[0] = rho; full[1] = d[D_H]; full[2] = l.rho; full[3] = v.rho; full[4] = g.rho; full[5] = c2;
+        full[6] = d[D_RP]; full[7] = d[D_RT]; full[8] = d[D_RY1]; full[9] = d[D_RY2];
+        full[10] = d[D_HP]; full[11] = d[D_HT]; full[12] = d[D_HY1]; full[13] = d[D_HY2];
+        full[14] = v.drdp; full[15] = v.drdt; full[16] = l.drdp; full[17] = l.drdt;
+    }
+}
+
+// Density and static enthalpy of a face state (all the flux reads from the
+// reconstructed states).  Phases with zero mass fraction contribute exact
+// zeros in the reference's mixture sums, so they are skipped.
+template <class R> HD void face_rho_h(R p, R t, R y1, R y2, R &rho, R &h)
+{
+    E2<R> l = liquid(p, t);
+    R yl = R(1) - y1 - y2;
+    R iv = 0, ig = 0, hv = 0, hg = 0;
+    if (y1 != R(0)) { E2<R> v = vapour(p, t, sat_p((double)t)); iv = R(1) / v.rho; hv = v.h; }
+    if (y2 != R(0)) { E2<R> g = gas(p, t); ig = R(1) / g.rho; hg = g.h; }
+    rho = R(1) / (yl * (R(1) / l.rho) + y1 * iv + y2 * ig);
+    h = yl * l.h + y1 * hv + y2 * hg;
+}
+
+// Mixture c^2 of the Roe-averaged state (src/flux_module.f90:237,588).
+template <class R> HD R face_c2(R p, R t, R y1, R y2)
+{
+    E2<R> l = liquid(p, t), v{}, g{};
+    const bool hv = (y1 != R(0)), hg = (y2 != R(0));
+    if (hv) v = vapour(p, t, sat_p((double)t));
+    if (hg) g = gas(p, t);
+    R il = R(1) / l.rho, iv = hv ? R(1) / v.rho : R(0), ig = hg ? R(1) / g.rho : R(0);
+    R yl = R(1) - y1 - y2;
+    R rho = R(1) / (yl * il + y1 * iv + y2 * ig);
+    R r2 = rho * rho;
+    R rp = r2 * (yl * (il * il) * l.drdp + (hv ? y1 * (iv * iv) * v.drdp : R(0)) + (hg ? y2 * (ig * ig) * g.drdp : R(0)));
+    R rt = r2 * (yl * (il * il) * l.drdt + (hv ? y1 * (iv * iv) * v.drdt : R(0)) + (hg ? y2 * (ig * ig) * g.drdt : R(0)));
+    R hp = yl * l.dhdp + (hv ? y1 * v.dhdp : R(0)) + (hg ? y2 * g.dhdp : R(0));
+    R ht = yl * l.dhdt + (hv ? y1 * v.dhdt : R(0)) + (hg ? y2 * g.dhdt : R(0));
+    R c2 = rho * ht / (rho * rp * ht + rt * (R(1) - rho * hp));
+    if (c2 <= R(0)) {
+        R cl = (l.rho * l.drdp * l.dhdt + l.drdt * (R(1) - l.rho * l.dhdp)) / (l.rho * l.dhdt);
+        R cv = hv ? (v.rho * v.drdp * v.dhdt + v.drdt * (R(1) - v.rho * v.dhdp)) / (v.rho * v.dhdt) : R(0);
+        R cg = hg ? (g.rho * g.drdp * g.dhdt + g.drdt * (R(1) - g.rho * g.dhdp)) / (g.rho * g.dhdt) : R(0);
+        c2 = R(1) / (r2 * (yl * cl * (il * il) + y1 * cv * (iv * iv) + y2 * cg * (ig * ig)));
+    }
+    return c2;
+}
+
+// ------------------------------------------------------------ preconditioning
+template <class R> HD R sndp2(const Par &P, R snd2, R u2)
+{
+    R ur2 = R(P.uref) * R(P.uref);
+    if (P.prec == 0) return snd2;
+    if (P.prec == 1) return mn(snd2, mx(u2, ur2));
+    return mn(snd2, mx(mx(u2, ur2), R(P.str) * R(P.str)));
+}
+
+// ------------------------------------------------------------ reconstruction
+// x[ll][n]: 38-point stencil (ll = Fortran index - 1)
+template <class R> struct LimState { R r[2], r1[2], r2[2], alp[2]; };
+
+template <class R> HD R limiter(int nlim, const LimState<R> &m, int s)
+{
+    R r = m.r[s], a = m.alp[s], c;
+    switch (nlim) {
+    case 0: return R(1);
+    case 1: return mx(R(0), mn(R(1), r));
+    case 2: return mx(mx(R(0), mn(R(1), a * r)), mn(a, r));
+    case 3: c = (r + fabs(r)) / (R(1) + fabs(r)); return mx(R(0), mn(mn(a, a * r), c));
+    case 4: c = (R(1) + r) / R(2); return mx(R(0), mn(mn(a, a * r), c));
+    case 5: c = (r * r + r) / (R(1) + r * r); return mx(mx(R(0), mn(c * r, R(1))), mn(r, c));
+    case 6: c = mn(R(1.5), a); return mx(mx(R(0), mn(c * r, R(1))), mn(r, c));
+    case 7: c = (R(1) + R(2) * r) / R(3); return mx(R(0), mn(mn(a, a * r), c));
+    case 8: c = (-(R(2) * m.r1[s]) + R(11) + R(24) * r - R(3) * m.r2[s]) / R(30); return mx(R(0), mn(mn(a, a * r), c));
+    case 9: return (R(1) + R(2) * r) / R(3);
+    case 10: return (-(R(2) * m.r1[s]) + R(11) + R(24) * r - R(3) * m.r2[s]) / R(30);
+    }
+    return R(1);
+}
+
+// One variable of the MUSCL-TVD (mlp=false) or MLP reconstruction.
+// xs(ll) gives the stencil value of this variable at Fortran index ll.
+template <class R, class F> HD void muscl_var(int nlim, bool mlp, F xs, R &xl, R &xr)
+{
+    const R eps = R(1.0e-30);
+    R x9 = xs(9), x10 = xs(10), x23 = xs(23), x32 = xs(32);
+    R dq = x10 - x9, dqm = x9 - x23, dqm1 = x23 - xs(37), dqp = x32 - x10, dqp1 = xs(38) - x32;
+    LimState<R> m;
+    R dqmm = R(1) / (dqm + eps);
+    m.r[0] = dq * dqmm; m.r1[0] = dqm1 * dqmm; m.r2[0] = dqp * dqmm;
+    R dqpp = R(1) / (dqp + eps);
+    m.r[1] = dq * dqpp; m.r1[1] = dqp1 * dqpp; m.r2[1] = dqm * dqpp;
+    if (!mlp) {
+        m.alp[0] = R(2); m.alp[1] = R(2);
+    } else {
+        const R eps2 = R(1.0e-3);
+        R rxy_l = fabs((xs(11) - xs(7)) / (x10 - x23 + eps));
+        R rxz_l = fabs((xs(15) - xs(3)) / (x10 - x23 + eps));
+        R rxy_r = fabs((xs(12) - xs(8)) / (x32 - x9 + eps));
+        R rxz_r = fabs((xs(16) - xs(4)) / (x32 - x9 + eps));
+        // 18-point min/max over the two cells of the face and their rings
+        R a18 = xs(1), b18 = xs(1);
+#pragma unroll
+        for (int q = 2; q <= 18; q++) { a18 = mn(a18, xs(q)); b18 = mx(b18, xs(q)); }
+        // the nine upwind-side planes ll = 19..27 and 28..36
+        R aL = xs(19), bL = xs(19), aR = xs(28), bR = xs(28);
+#pragma unroll
+        for (int q = 20; q <= 27; q++) { aL = mn(aL, xs(q)); bL = mx(bL, xs(q)); }
+#pragma unroll
+        for (int q = 29; q <= 36; q++) { aR = mn(aR, xs(q)); bR = mx(bR, xs(q)); }
+        // odd ring of cell i (ll = 1,3,..,17) and even ring of cell i+1 (2,..,18)
+        R ao = xs(1), bo = xs(1), ae = xs(2), be = xs(2);
+#pragma unroll
+        for (int q = 3; q <= 17; q += 2) { ao = mn(ao, xs(q)); bo = mx(bo, xs(q)); }
+#pragma unroll
+        for (int q = 4; q <= 18; q += 2) { ae = mn(ae, xs(q)); be = mx(be, xs(q)); }
+        R qmin, qmax;
+        if ((rxy_l < eps2) && (rxz_l < eps2)) { qmin = mn(a18, aL); qmax = mx(b18, bL); }
+        else if ((x10 - x23) > R(0)) { qmax = b18; qmin = mn(ao, aL); }
+        else { qmin = a18; qmax = mx(bo, bL); }
+        R qml = mn(fabs(qmax - x9), fabs(qmin - x9));
+        if ((rxy_r < eps2) && (rxz_r < eps2)) { qmin = mn(a18, aR); qmax = mx(b18, bR); }
+        else if ((x32 - x9) > R(0)) { qmax = mx(be, bR); qmin = a18; }
+        else { qmin = mn(ae, aR); qmax = b18; }
+        R qmr = mn(fabs(qmax - x10), fabs(qmin - x10));
+        R d1 = R(1) + rxy_l + rxz_l, d2 = R(1) + rxy_r + rxz_r;
+        m.alp[0] = R(2) * mx(R(1), m.r[0]) / (fabs(dq) + eps) / d1 * qml;
+        if (fabs(dq) <= eps) m.alp[1] = R(2) * mx(R(1), dqp / eps) * fabs(dqpp) / d2 * qmr;
+        else m.alp[1] = R(2) * mx(R(1), R(1) / m.r[1]) * fabs(dqpp) / d2 * qmr;
+        m.alp[0] = mx(R(1), mn(R(2), m.alp[0]));
+        m.alp[1] = mx(R(1), mn(R(2), m.alp[1]));
+    }
+    xl = x9 + R(0.5) * limiter(nlim, m, 0) * dqm;
+    xr = x10 - R(0.5) * limiter(nlim, m, 1) * dqp;
+}
+
+// positivity / fraction / turbulence clamps on the face states
+template <class R> HD void muscl_clamps(int npv, R pref, const R *c9, const R *c10, R *xl, R *xr)
+{
+    xl[0] = mx(-pref + R(10), xl[0]);
+    xr[0] = mx(-pref + R(10), xr[0]);
+    if ((xl[5] < R(0)) || (xl[5] > R(1))) xl[5] = c9[5];
+    if ((xr[5] < R(0)) || (xr[5] > R(1))) xr[5] = c10[5];
+    if ((xl[6] < R(0)) || (xl[6] > R(1))) xl[6] = c9[6];
+    if ((xr[6] < R(0)) || (xr[6] > R(1))) xr[6] = c10[6];
+    if ((xl[5] + xl[6]) > R(1)) { xl[5] = c9[5]; xl[6] = c9[6]; }
+    if ((xr[5] + xr[6]) > R(1)) { xr[5] = c10[5]; xr[6] = c10[6]; }
+    if (npv > 7) {
+        if ((xl[7] < R(0)) && (c9[7] > R(0))) xl[7] = c9[7];
+        if ((xr[7] < R(0)) && (c10[7] > R(0))) xr[7] = c10[7];
+        if ((xl[8] < R(0)) && (c9[8] > R(0))) xl[8] = c9[8];
+        if ((xr[8] < R(0)) && (c10[8] > R(0))) xr[8] = c10[8];
+    }
+}
+
+// ------------------------------------------------------------ inviscid flux
+// pl/pr primitive face states (pl[0] gauge p), rl/rr densities, hl/hr static
+// enthalpies, c2m the mixture c^2 at the Roe-averaged state.
+template <class R> struct FaceState { R p[NPV]; R rho, h; };
+
+template <class R> HD void roe_flux(const Par &P, int npv, const R *n3, const FaceState<R> &L,
+                                    const FaceState<R> &Rt, R c2avg, R *fx)
+{
+    R dl = sqrt(n3[0] * n3[0] + n3[1] * n3[1] + n3[2] * n3[2]);
+    if (dl == R(0)) { for (int k = 0; k < npv; k++) fx[k] = 0; return; }
+    R nx = n3[0] / dl, ny = n3[1] / dl, nz = n3[2] / dl;
+    const R *pl = L.p, *pr = Rt.p;
+    R uull = nx * pl[1] + ny * pl[2] + nz * pl[3];
+    R uurr = nx * pr[1] + ny * pr[2] + nz * pr[3];
+    R sl = sqrt(L.rho), sr = sqrt(Rt.rho);
+    R ravd = R(1) / (sl + sr);
+    R ra[NPV];
+#pragma unroll
+    for (int k = 1; k < NPV; k++) if (k != 4 && k < npv) ra[k] = (sl * pl[k] + sr * pr[k]) * ravd;
+    R hl = L.h + R(0.5) * (pl[1] * pl[1] + pl[2] * pl[2] + pl[3] * pl[3]);
+    R hr = Rt.h + R(0.5) * (pr[1] * pr[1] + pr[2] * pr[2] + pr[3] * pr[3]);
+    R raht = (sl * hl + sr * hr) * ravd;
+    R u2 = ra[1] * ra[1] + ra[2] * ra[2] + ra[3] * ra[3];
+    R rr = sqrt(L.rho * Rt.rho);
+    R uuu = nx * ra[1] + ny * ra[2] + nz * ra[3];
+    R sp = sndp2(P, c2avg, u2);
+    R q = sp / c2avg, omq = R(1) - q;
+    R uup = R(0.5) * (R(1) + q) * uuu;
+    R ddd = R(0.5) * sqrt(omq * omq * (uuu * uuu) + R(4) * sp);
+    R cs = R(0.5) * (fabs(uup + ddd) + fabs(uup - ddd));
+    R ms = R(0.5) * (fabs(uup + ddd) - fabs(uup - ddd)) / ddd;
+    R dpp = pr[0] - pl[0], dun = uurr - uull;
+    R du = ms * dun + (cs - q * fabs(uuu) - R(0.5) * omq * uuu * ms) * dpp / rr / sp;
+    R dp = ms * dpp + (cs - fabs(uuu) + R(0.5) * omq * uuu * ms) * rr * dun;
+    R au = fabs(uuu), dur = du * rr;
+    R ml = L.rho * uull, mr = Rt.rho * uurr;
+    R pla = pl[0] + R(P.pref), pra = pr[0] + R(P.pref);
+    fx[0] = R(0.5) * (ml + mr - (au * (Rt.rho - L.rho) + dur)) * dl;
+    fx[1] = R(0.5) * (ml * pl[1] + nx * pla + mr * pr[1] + nx * pra - (au * (Rt.rho * pr[1] - L.rho * pl[1]) + dur * ra[1] + dp * nx)) * dl;
+    fx[2] = R(0.5) * (ml * pl[2] + ny * pla + mr * pr[2] + ny * pra - (au * (Rt.rho * pr[2] - L.rho * pl[2]) + dur * ra[2] + dp * ny)) * dl;
+    fx[3] = R(0.5) * (ml * pl[3] + nz * pla + mr * pr[3] + nz * pra - (au * (Rt.rho * pr[3] - L.rho * pl[3]) + dur * ra[3] + dp * nz)) * dl;
+    fx[4] = R(0.5) * (ml * hl + mr * hr - (au * (Rt.rho * hr - pr[0] - L.rho * hl + pl[0]) + dur * raht + dp * uuu)) * dl;
+#pragma unroll
+    for (int k = 5; k < NPV; k++)
+        if (k < npv) fx[k] = R(0.5) * (ml * pl[k] + mr * pr[k] - (au * (Rt.rho * pr[k] - L.rho * pl[k]) + dur * ra[k])) * dl;
+}
+
+template <class R> HD void ausm_flux(const Par &P, int npv, const R *n3, const FaceState<R> &L,
+                                     const FaceState<R> &Rt, R c2avg, R *fx)
+{
+    const R beta = R(0.125), kp = R(0.25), ku = R(1);
+    R dl = sqrt(n3[0] * n3[0] + n3[1] * n3[1] + n3[2] * n3[2]);
+    if (dl == R(0)) { for (int k = 0; k < npv; k++) fx[k] = 0; return; }
+    R nx = n3[0] / dl, ny = n3[1] / dl, nz = n3[2] / dl;
+    const R *pl = L.p, *pr = Rt.p;
+    R uull = nx * pl[1] + ny * pl[2] + nz * pl[3];
+    R uurr = nx * pr[1] + ny * pr[2] + nz * pr[3];
+    R sl = sqrt(L.rho), sr = sqrt(Rt.rho);
+    R ravd = R(1) / (sl + sr);
+    R ua = (sl * pl[1] + sr * pr[1]) * ravd, va = (sl * pl[2] + sr * pr[2]) * ravd, wa = (sl * pl[3] + sr * pr[3]) * ravd;
+    R rr = sqrt(L.rho * Rt.rho);
+    R amid = sqrt(c2avg);
+    R zmr = uurr / amid, zml = uull / amid;
+    R un = nx * ua + ny * va + nz * wa;
+    R am2 = un * un;
+    R am2r = sndp2(P, c2avg, am2) / c2avg;
+    R fmid = sqrt(am2r) * (R(2) - sqrt(am2r));
+    R alpha = R(0.1875) * (R(-4) + R(5) * (fmid * fmid));
+    alpha = mx(mn(alpha, R(0.1875)), R(-0.75));
+    R zmmr, pmr, zmpl, ppl;
+    if (fabs(zmr) >= R(1)) {
+        zmmr = R(0.5) * (zmr - fabs(zmr));
+        pmr = R(0.5) * (R(1) - zmr / fabs(zmr));
+    } else {
+        R t1 = (zmr - R(1)) * (zmr - R(1)), t2 = (zmr * zmr - R(1)) * (zmr * zmr - R(1));
+        zmmr = -(R(0.25) * t1) - beta * t2;
+        pmr = R(0.25) * t1 * (R(2) + zmr) - alpha * zmr * t2;
+    }
+    if (fabs(zml) >= R(1)) {
+        zmpl = R(0.5) * (zml + fabs(zml));
+        ppl = R(0.5) * (R(1) + zml / fabs(zml));
+    } else {
+        R t1 = (zml + R(1)) * (zml + R(1)), t2 = (zml * zml - R(1)) * (zml * zml - R(1));
+        zmpl = R(0.25) * t1 + beta * t2;
+        ppl = R(0.25) * t1 * (R(2) - zml) + alpha * zml * t2;
+    }
+    R zmid = zmpl + zmmr - kp * mx(R(1) - am2 / c2avg, R(0)) * (pr[0] - pl[0]) / rr / c2avg / fmid;
+    R pmid = ppl * (pl[0] + R(P.pref)) + pmr * (pr[0] + R(P.pref)) - R(2) * ku * ppl * pmr * rr * fmid * amid * (uurr - uull);
+    R ml = (zmid > R(0)) ? zmid * amid * L.rho : R(0);
+    R mr = (zmid > R(0)) ? R(0) : zmid * amid * Rt.rho;
+    R hl = L.h + R(0.5) * (pl[1] * pl[1] + pl[2] * pl[2] + pl[3] * pl[3]);
+    R hr = Rt.h + R(0.5) * (pr[1] * pr[1] + pr[2] * pr[2] + pr[3] * pr[3]);
+    fx[0] = (ml + mr) * dl;
+    fx[1] = (ml * pl[1] + mr * pr[1] + pmid * nx) * dl;
+    fx[2] = (ml * pl[2] + mr * pr[2] + pmid * ny) * dl;
+    fx[3] = (ml * pl[3] + mr * pr[3] + pmid * nz) * dl;
+    fx[4] = (ml * hl + mr * hr) * dl;
+#pragma unroll
+    for (int k = 5; k < NPV; k++) if (k < npv) fx[k] = (ml * pl[k] + mr * pr[k]) * dl;
+}
+
+// ------------------------------------------------------------ Jacobian / getx
+// Cell data the split Jacobian and the diagonal inverse read.
+struct CellJ {
+    double pv[NPV];
+    double d[ND];
+    double mut, vol;
+};
+
+HD double eigenvis(const Par &P, const double *d, double mut)
+{
+    if (P.iturb == -3) return 0.0;
+    double den = d[D_RT] + d[D_HT] * d[D_RP] * d[D_RHO] - d[D_HP] * d[D_RT] * d[D_RHO];
+    if (P.iturb == -2) return mx(d[D_RP] * d[D_KAP] / den, 4.0 / 3.0 * d[D_MU] / d[D_RHO]);
+    return mx(d[D_RP] * (d[D_KAP] + d[D_HT] * mut / 0.9) / den, 4.0 / 3.0 * (d[D_MU] + mut) / d[D_RHO]);
+}
+
+// y += sgn * A(cell, n3, sign) x, with A the split flux Jacobian of the
+// reference assembled implicitly (row 1 vector plus rank-one rows).
+HD void jac_apply(const Par &P, int npv, const CellJ &c, const double *n3, int sign, const double *x, double *y, double sgn)
+{
+    const double *pv = c.pv, *d = c.d;
+    double ds = n3[0] * n3[0] + n3[1] * n3[1] + n3[2] * n3[2];
+    double u = n3[0] * pv[1] + n3[1] * pv[2] + n3[2] * pv[3];
+    double v2 = pv[1] * pv[1] + pv[2] * pv[2] + pv[3] * pv[3];
+    double sp = sndp2(P, d[D_C2], v2);
+    double q = sp / d[D_C2], omq = 1.0 - q;
+    double up = 0.5 * (1.0 + q) * u;
+    double dd = 0.5 * sqrt((u * u) * (omq * omq) + 4.0 * sp * ds);
+    double lam = (double)sign * 1.05 * (fabs(up) + dd);
+    double vis = (double)sign * 2.0 * eigenvis(P, d, c.mut) * ds / c.vol;
+    double beta = 1.0 / sp - 1.0 / d[D_C2] + d[D_RP];
+    double lv = lam + vis;
+    double r0 = u * d[D_RP] + lam * beta + vis * d[D_RP];
+    double r4 = (u + lv) * d[D_RT], r5 = (u + lv) * d[D_RY1], r6 = (u + lv) * d[D_RY2];
+    double s = r0 * x[0] + d[D_RHO] * (n3[0] * x[1] + n3[1] * x[2] + n3[2] * x[3]) + r4 * x[4] + r5 * x[5] + r6 * x[6];
+    double ru = d[D_RHO] * (u + lv);
+    double hh = d[D_H] + 0.5 * v2;
+    double w = 0.5 * sgn;
+    y[0] += w * s;
+    y[1] += w * (pv[1] * s + n3[0] * x[0] + ru * x[1]);
+    y[2] += w * (pv[2] * s + n3[1] * x[0] + ru * x[2]);
+    y[3] += w * (pv[3] * s + n3[2] * x[0] + ru * x[3]);
+    y[4] += w * (hh * s + ru * (d[D_HP] * x[0] + pv[1] * x[1] + pv[2] * x[2] + pv[3] * x[3] + d[D_HT] * x[4]
+                                + d[D_HY1] * x[5] + d[D_HY2] * x[6]) - lv * x[0]);
+    y[5] += w * (pv[5] * s + ru * x[5]);
+    y[6] += w * (pv[6] * s + ru * x[6]);
+    if (npv > 7) {
+        y[7] += w * (pv[7] * s + ru * x[7]);
+        y[8] += w * (pv[8] * s + ru * x[8]);
+    }
+}
+
+// Closed-form inverse of the preconditioned diagonal block with the Merkle
+// (icav) and SST (itt) couplings.  f6 = the cell's six face vectors.
+HD void getx(const Par &P, int npv, const CellJ &c, const double f6[6][3], double dt,
+             const double *cc, const double *tt4, const double *res, double *x)
+{
+    const double *pv = c.pv, *d = c.d;
+    double v2 = pv[1] * pv[1] + pv[2] * pv[2] + pv[3] * pv[3];
+    double sp = sndp2(P, d[D_C2], v2);
+    double q = sp / d[D_C2], omq = 1.0 - q;
+    double eig = 0.0, msum = 0.0;
+    for (int dd = 0; dd < 3; dd++) {
+        double o0 = 0.5 * (f6[2 * dd][0] + f6[2 * dd + 1][0]);
+        double o1 = 0.5 * (f6[2 * dd][1] + f6[2 * dd + 1][1]);
+        double o2 = 0.5 * (f6[2 * dd][2] + f6[2 * dd + 1][2]);
+        double ds = o0 * o0 + o1 * o1 + o2 * o2;
+        double u = o0 * pv[1] + o1 * pv[2] + o2 * pv[3];
+        double up = 0.5 * (1.0 + q) * u;
+        double e = 0.5 * sqrt((u * u) * (omq * omq) + 4.0 * sp * ds);
+        eig += 1.05 * (fabs(up) + e);
+        msum += ds;
+    }
+    double ev = eigenvis(P, d, c.mut) * msum;
+    double a = 1.0 / (c.vol / dt + eig);
+    double b = ((double)P.nsteady * P.bdf_b * c.vol / P.dt_phy + 2.0 * ev / c.vol) * a;
+    double va = c.vol * a;
+    double c1 = cc[0] * va, c2 = cc[1] * va, c3 = cc[2] * va, c4 = cc[3] * va;
+    double b1 = 1.0 + b;
+    double rb = b1 * d[D_RHO];
+    double i_b1 = 1.0 / rb;
+    double ss = d[D_RHO] * d[D_HT] * (1.0 / sp + b / d[D_C2]);
+    double rr = 1.0 / sp - 1.0 / d[D_C2] + b1 * d[D_RP];
+    double hst = d[D_H] + 0.5 * v2 - v2;
+    double cross = d[D_HY2] * d[D_RY1] - d[D_HY1] * d[D_RY2];
+    double cross1 = d[D_HY1] * d[D_RT] - d[D_HT] * d[D_RY1];
+    double cross2 = d[D_HY2] * d[D_RT] - d[D_HT] * d[D_RY2];
+    double dhdp1 = 1.0 - d[D_HP] * d[D_RHO];
+    double ssy1 = d[D_HY1] * d[D_RHO] * rr + b1 * d[D_RY1] * dhdp1;
+    double ssy2 = d[D_HY2] * d[D_RHO] * rr + b1 * d[D_RY2] * dhdp1;
+    double ssh = d[D_RHO] * rr * hst - b1 * dhdp1 * d[D_RHO];
+    double aaa1 = d[D_RY1] * hst + d[D_HY1] * d[D_RHO] - cross * pv[6];
+    double aaa2 = d[D_RT] * hst + d[D_HT] * d[D_RHO] - cross2 * pv[6];
+    double uuu = res[1] * pv[1] + res[2] * pv[2] + res[3] * pv[3] - res[4];
+    double co1 = res[0] * aaa1 + d[D_RY1] * uuu + cross * res[6];
+    double co2 = res[0] * aaa2 + d[D_RT] * uuu + cross2 * res[6];
+    double co3 = res[0] * (ssh - ssy2 * pv[6]) + d[D_RHO] * rr * uuu + res[6] * ssy2;
+    double mm = 1.0 / (rb * (ss + c1 * cross1) - c2 * ssy1 + c3 * ss);
+    double r6 = res[6] - res[0] * pv[6];
+    x[0] = (c4 * cross1 * (-r6) - c2 * co1 + c3 * co2 + d[D_RHO] * b1 * (co2 + cross1 * res[5] - cross1 * pv[5] * res[0])) * mm * a;
+    x[1] = (res[1] - res[0] * pv[1]) * i_b1 * a;
+    x[2] = (res[2] - res[0] * pv[2]) * i_b1 * a;
+    x[3] = (res[3] - res[0] * pv[3]) * i_b1 * a;
+    x[4] = (c4 * i_b1 * ssy1 * r6 + c1 * co1 - c3 * i_b1 * co3 - (co3 - ssy1 * pv[5] * res[0] + res[5] * ssy1)) * mm * a;
+    x[5] = (ss * (res[5] - res[0] * pv[5]) - c4 * ss * i_b1 * r6 - c1 * co2 + c2 * i_b1 * co3) * mm * a;
+    x[6] = r6 * i_b1 * a;
+    if (npv > 7) {
+        double t1 = tt4[0] * va, t2 = tt4[1] * va, t3 = tt4[2] * va, t4 = tt4[3] * va;
+        double it = 1.0 / (t2 * t3 - (t1 + rb) * (t4 + rb));
+        double rk = res[7] - res[0] * pv[7], ro = res[8] - res[0] * pv[8];
+        x[7] = (ro * t2 - (rb + t4) * rk) * it * a;
+        x[8] = (rk * t3 - (rb + t1) * ro) * it * a;
+    }
+}
+
+// local pseudo time step (src/timestep_module.f90:156-198)
+HD double local_dt(const Par &P, const CellJ &c, const double f6[6][3])
+{
+    const double *pv = c.pv, *d = c.d;
+    double v2 = pv[1] * pv[1] + pv[2] * pv[2] + pv[3] * pv[3];
+    double sp = sndp2(P, d[D_C2], v2);
+    double q = sp / d[D_C2], omq = 1.0 - q;
+    double eig = 0.0, msum = 0.0;
+    for (int dd = 0; dd < 3; dd++) {
+        double o0 = 0.5 * (f6[2 * dd][0] + f6[2 * dd + 1][0]);
+        double o1 = 0.5 * (f6[2 * dd][1] + f6[2 * dd + 1][1]);
+        double o2 = 0.5 * (f6[2 * dd][2] + f6[2 * dd + 1][2]);
+        double s = o0 * o0 + o1 * o1 + o2 * o2;
+        double uc = fabs(o0 * pv[1] + o1 * pv[2] + o2 * pv[3]);
+        double e = sqrt((uc * uc) * (omq * omq) + 4.0 * sp * s);
+        eig += 0.5 * (uc * (1.0 + q) + e);
+        msum += s;
+    }
+    double ev = eigenvis(P, d, c.mut) * msum / c.vol;
+    return P.cfl * c.vol / (eig + 4.0 * ev);
+}
+
+// ------------------------------------------------------------ viscous flux
+// g3[q] = (normal difference, e-transverse, t-transverse) difference of
+// variable vars[q] = {u, v, w, T, k, omega} over the 18-point face stencil;
+// e4/t4: the four transverse face vectors of each family; gl/gr = {vol, ydns}
+// of the two cells; dl/dr compact props; mutl/mutr eddy viscosities;
+// cav = {uc, vc, wc, kc, oc} face averages.
+template <class R>
+HD void visc_flux(const Par &P, const R *nx, const R e4[4][3], const R t4[4][3], const R *gl, const R *gr,
+                  const R g3[6][3], const R *cav, const R *dl, const R *dr, R mutl, R mutr, R *fx)
+{
+    const R c23 = R(2.0 / 3.0);
+    const bool turb = (P.iturb == 0);
+    R ex = R(0.25) * (e4[0][0] + e4[1][0] + e4[2][0] + e4[3][0]);
+    R ey = R(0.25) * (e4[0][1] + e4[1][1] + e4[2][1] + e4[3][1]);
+    R ez = R(0.25) * (e4[0][2] + e4[1][2] + e4[2][2] + e4[3][2]);
+    R tx = R(0.25) * (t4[0][0] + t4[1][0] + t4[2][0] + t4[3][0]);
+    R ty = R(0.25) * (t4[0][1] + t4[1][1] + t4[2][1] + t4[3][1]);
+    R tz = R(0.25) * (t4[0][2] + t4[1][2] + t4[2][2] + t4[3][2]);
+    R vol = R(2) / (gl[0] + gr[0]);
+    R vis = R(0.5) * (dl[D_MU] + dr[D_MU]);
+    R con = R(0.5) * (dl[D_KAP] + dr[D_KAP]);
+    R emut = 0, tcon = 0;
+    if (turb) {
+        emut = R(0.5) * (mutl + mutr);
+        tcon = R(0.5) * (mutl * dl[D_HT] + mutr * dr[D_HT]) / R(0.9);
+    }
+    R g[6][3];
+    const int nv = turb ? 6 : 4;
+    for (int q = 0; q < nv; q++) {
+        R a = g3[q][0], b = g3[q][1], c = g3[q][2];
+        g[q][0] = (a * nx[0] + b * ex + c * tx) * vol;
+        g[q][1] = (a * nx[1] + b * ey + c * ty) * vol;
+        g[q][2] = (a * nx[2] + b * ez + c * tz) * vol;
+    }
+    R uc = cav[0], vc = cav[1], wc = cav[2];
+    R mu = turb ? vis + emut : vis, kap = turb ? con + tcon : con;
+    R div = g[0][0] + g[1][1] + g[2][2];
+    R txx = mu * (R(2) * g[0][0] - c23 * div), tyy = mu * (R(2) * g[1][1] - c23 * div), tzz = mu * (R(2) * g[2][2] - c23 * div);
+    R txy = mu * (g[0][1] + g[1][0]), txz = mu * (g[0][2] + g[2][0]), tyz = mu * (g[1][2] + g[2][1]);
+    fx[0] = 0;
+    fx[1] = txx * nx[0] + txy * nx[1] + txz * nx[2];
+    fx[2] = txy * nx[0] + tyy * nx[1] + tyz * nx[2];
+    fx[3] = txz * nx[0] + tyz * nx[1] + tzz * nx[2];
+    fx[4] = nx[0] * (uc * txx + vc * txy + wc * txz + kap * g[3][0]) + nx[1] * (uc * txy + vc * tyy + wc * tyz + kap * g[3][1])
+          + nx[2] * (uc * txz + vc * tyz + wc * tzz + kap * g[3][2]);
+    fx[5] = 0;
+    fx[6] = 0;
+    if (turb) {
+        R pkw = g[4][0] * g[5][0] + g[4][1] * g[5][1] + g[4][2] * g[5][2];
+        R rho = R(0.5) * (dl[D_RHO] + dr[D_RHO]);
+        R yd = R(0.5) * (gl[1] + gr[1]);
+        R kk = cav[3], oo = cav[4];
+        R cdkw = mx(R(1.712) * rho / oo * pkw, R(1.0e-10));
+        R t1 = sqrt(kk) / (R(0.09) * oo * yd);
+        R t2 = R(500) * vis / rho / (oo * (yd * yd));
+        R t3 = (R(3.424) * rho * kk) / (cdkw * (yd * yd));
+        R a1 = mn(mx(t1, t2), t3);
+        R a2 = a1 * a1;
+        R bf = tanh(a2 * a2);
+        R tdk = bf * R(0.85) + (R(1) - bf) * R(1);
+        R tdo = bf * R(0.5) + (R(1) - bf) * R(0.856);
+        fx[7] = (vis + emut * tdk) * (g[4][0] * nx[0] + g[4][1] * nx[1] + g[4][2] * nx[2]);
+        fx[8] = (vis + emut * tdo) * (g[5][0] * nx[0] + g[5][1] * nx[1] + g[5][2] * nx[2]);
+    }
+}
+
+// ------------------------------------------------------------ cell sources
+// x7[q][n]: cell (q=1), its i-, i+, j-, j+, k-, k+ neighbours (q = 0,2..6)
+HD void gg_grad(const double x7[7][NPV], int n, const double f6[6][3], double ivol, double *g)
+{
+    double a1 = 0.5 * (x7[0][n] + x7[1][n]), a2 = 0.5 * (x7[2][n] + x7[1][n]);
+    double a3 = 0.5 * (x7[3][n] + x7[1][n]), a4 = 0.5 * (x7[4][n] + x7[1][n]);
+    double a5 = 0.5 * (x7[5][n] + x7[1][n]), a6 = 0.5 * (x7[6][n] + x7[1][n]);
+    for (int m = 0; m < 3; m++)
+        g[m] = (a2 * f6[1][m] + a4 * f6[3][m] + a6 * f6[5][m] - a1 * f6[0][m] - a3 * f6[2][m] - a5 * f6[4][m]) * ivol;
+}
+
+// Merkle: returns r_v - r_c (volume-integrated) and the coupling icav[4]
+HD void merkle(const Par &P, const double *pc, const double *d, double vol, double &rc, double &rv, double *icav)
+{
+    rc = 0.0; rv = 0.0;
+    icav[0] = icav[1] = icav[2] = icav[3] = 0.0;
+    if (pc[4] >= P.t_crit) return;
+    double pww = sat_p(pc[4]);
+    double irho = 1.0 / d[D_RHO];
+    double pc_ = mx(pc[0] + P.pref - pww, 0.0), pv_ = mx(pww - pc[0] - P.pref, 0.0);
+    double r_c = P.c_c * d[D_RHO] * pc[5] * pc_ * P.dp_ref * P.t_ref;
+    double r_v = P.c_v * d[D_RHO] * (1.0 - pc[5] - pc[6]) * pv_ * P.dp_ref * P.t_ref;
+    if (P.cav_clip == 0) {   // as written in src/cav_module.f90:151-152 (pc[4] is T)
+        r_c = mn(r_c, d[D_RHO] * pc[4]);
+        r_v = mn(r_v, d[D_RHO] * (1.0 - pc[4] - pc[5]));
+    } else {                 // on the mass fractions
+        r_c = mn(r_c, d[D_RHO] * pc[5]);
+        r_v = mn(r_v, d[D_RHO] * (1.0 - pc[5] - pc[6]));
+    }
+    rc = r_c * vol;
+    rv = r_v * vol;
+    if (r_c != 0.0) {
+        icav[0] = -(r_c * d[D_RP] * irho) - P.c_c * d[D_RHO] * pc[5] * P.dp_ref * P.t_ref;
+        icav[1] = -(r_c * d[D_RT] * irho);
+        icav[2] = -(r_c * d[D_RY1] * irho) - P.c_c * d[D_RHO] * pc_ * P.dp_ref * P.t_ref;
+        icav[3] = -(r_c * d[D_RY2] * irho);
+    }
+    if (r_v != 0.0) {
+        icav[0] = r_v * d[D_RP] * irho - P.c_v * d[D_RHO] * (1.0 - pc[5] - pc[6]) * P.dp_ref * P.t_ref;
+        icav[1] = r_v * d[D_RT] * irho;
+        icav[2] = r_v * d[D_RY1] * irho - P.c_v * d[D_RHO] * pv_ * P.dp_ref * P.t_ref;
+        icav[3] = r_v * d[D_RY2] * irho - P.c_v * d[D_RHO] * pv_ * P.dp_ref * P.t_ref;
+    }
+    double s = (icav[2] < 0.0 || (icav[2] == 0.0 && signbit(icav[2]))) ? -1.0 : 1.0;
+    for (int q = 0; q < 4; q++) icav[q] *= s;
+}
+
+// k-omega SST source (ides = 0): src[2], itt[4], omega_cut
+HD void sst_source(const double x7[7][NPV], const double f6[6][3], double vol, double ydns, const double *d,
+                   double mut, double *src, double *itt, double &ocut)
+{
+    const double c23 = 2.0 / 3.0, c59 = 5.0 / 9.0;
+    double iv = 1.0 / vol;
+    double gu[3], gv[3], gw[3], gk[3], go[3];
+    gg_grad(x7, 7, f6, iv, gk);
+    gg_grad(x7, 8, f6, iv, go);
+    gg_grad(x7, 1, f6, iv, gu);
+    gg_grad(x7, 2, f6, iv, gv);
+    gg_grad(x7, 3, f6, iv, gw);
+    double div = gu[0] + gv[1] + gw[2];
+    double s2 = 2.0 * (gu[0] * gu[0] + gv[1] * gv[1] + gw[2] * gw[2]) + (gu[1] + gv[0]) * (gu[1] + gv[0])
+              + (gu[2] + gw[0]) * (gu[2] + gw[0]) + (gv[2] + gw[1]) * (gv[2] + gw[1]);
+    double k = x7[1][7], om = x7[1][8], rho = d[D_RHO];
+    double prod = mut * (s2 - c23 * (div * div)) - c23 * rho * k * div;
+    prod = mn(prod, 0.9 * rho * k * om);
+    ocut = sqrt(s2 - c23 * (div * div));
+    double pkw = gk[0] * go[0] + gk[1] * go[1] + gk[2] * go[2];
+    double cdkw = mx(1.712 * rho / om * pkw, 1.0e-10);
+    double t1 = sqrt(k) / (0.09 * om * ydns);
+    double t2 = 500.0 * d[D_MU] / rho / (om * (ydns * ydns));
+    double t3 = (3.424 * rho * k) / (cdkw * (ydns * ydns));
+    double a1 = mn(mx(t1, t2), t3);
+    double a2 = a1 * a1;
+    double bf = tanh(a2 * a2);
+    double ta = bf * c59 + (1.0 - bf) * 0.44;
+    double tb = bf * 0.075 + (1.0 - bf) * 0.0828;
+    double ki = 1.0 / k, oi = 1.0 / om, ei = 1.0 / mut;
+    double dest = rho * pow(k, 1.5) / (sqrt(k) / 9.0 / om * 100.0);
+    src[0] = (prod - dest) * vol;
+    src[1] = (ta * rho * ei * prod - tb * rho * (om * om) + 1.712 * (1.0 - bf) * rho * oi * pkw) * vol;
+    double dpdk = prod * ki;
+    double dpdo = -((prod + c23 * rho * k * div) * oi);
+    itt[0] = dpdk - dest * ki;
+    itt[1] = dpdo - dest * oi;
+    itt[2] = 0.0;
+    itt[3] = ta * rho * ei * dpdo + ta * rho * ei * prod * oi - 2.0 * tb * rho * om - 1.712 * (1.0 - bf) * rho * (oi * oi) * pkw;
+    double t = itt[0] + itt[3], det = itt[0] * itt[3] - itt[1] * itt[2];
+    double sq = sqrt(mx(0.25 * (t * t) - det, 0.0));
+    double l1 = 0.5 * t + sq, l2 = 0.5 * t - sq;
+    double s1 = signbit(l1) ? -1.0 : 1.0, s2s = signbit(l2) ? -1.0 : 1.0;
+    itt[0] *= s1; itt[1] *= s1; itt[2] *= s2s; itt[3] *= s2s;
+}
+
+// SST eddy viscosity mu_t = 0.31 rho k / max(0.31 omega, S F2)
+HD double sst_eddy(const double x7[7][NPV], const double f6[6][3], double vol, double ydns, const double *d)
+{
+    double iv = 1.0 / vol;
+    double gu[3], gv[3], gw[3];
+    gg_grad(x7, 1, f6, iv, gu);
+    gg_grad(x7, 2, f6, iv, gv);
+    gg_grad(x7, 3, f6, iv, gw);
+    double k = x7[1][7], om = x7[1][8];
+    double t1 = sqrt(k) / (0.09 * om * ydns);
+    double t2 = 500.0 * d[D_MU] / d[D_RHO] / (om * (ydns * ydns));
+    double a2 = mx(2.0 * t1, t2);
+    double bf2 = tanh(a2 * a2);
+    double sij = gu[0] * gu[0] + gv[1] * gv[1] + gw[2] * gw[2]
+               + 0.5 * ((gu[1] + gv[0]) * (gu[1] + gv[0]) + (gu[2] + gw[0]) * (gu[2] + gw[0]) + (gv[2] + gw[1]) * (gv[2] + gw[1]));
+    double sss = sqrt(2.0 * sij);
+    return 0.31 * d[D_RHO] * k / mx(0.31 * om, sss * bf2);
+}
+
+}  // namespace cav

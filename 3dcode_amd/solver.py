@@ -1,0 +1,222 @@
+"""ctypes binding of libcav3d.so (include/cav3d.h) — the product path.
+
+Mirrors the reference's operator interface for the pseudo-time step:
+``Solver.timeinteg`` is ``t_lusgs%timeinteg`` (src/update_module.f90:455),
+``calrhs`` is ``t_rhs%calrhs`` (src/rhs_module.f90:260), ``residual`` is
+``t_resi%setqres`` + ``t_resi%residual`` (src/residual_module.f90:94-186).
+Every call goes to the HIP library; there is no CPU fallback and a missing or
+failing library raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcav3d.so")
+_lib = None
+
+_D = C.POINTER(C.c_double)
+_I = C.POINTER(C.c_int)
+
+BC = {
+    "wall": 1, "inflow_subsonic": 2, "outflow_subsonic": 3, "farfield": 4, "symmetry": 5,
+    "inflow_supersonic": 6, "outflow_supersonic": 7, "degenerate": 8, "inflow": 9, "outflow": 10,
+}
+
+EXPORTS = [
+    "cav3d_create", "cav3d_set_metrics", "cav3d_put_state", "cav3d_get_state", "cav3d_timeinteg",
+    "cav3d_setbc", "cav3d_calrhs", "cav3d_timestep", "cav3d_residual", "cav3d_unsteady_shift",
+    "cav3d_eos", "cav3d_eos_host", "cav3d_profile", "cav3d_synchronize", "cav3d_last_error",
+    "cav3d_destroy",
+]
+
+
+class Params(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("iturb", "nscheme", "nmuscl", "nlim", "local", "prec", "ncav",
+                                       "nsteady", "precision", "cav_clip")] + \
+        [("reserved", C.c_int * 2)] + \
+        [(n, C.c_double) for n in ("cfl", "pref", "uref", "str", "dt_phy", "c_v", "c_c", "dp_ref",
+                                   "t_ref", "t_crit")] + \
+        [("bc_pv", C.c_double * 9), ("bc_pressure", C.c_double), ("emutref", C.c_double)]
+
+
+class BlockDesc(C.Structure):
+    _fields_ = [("imax", C.c_int), ("jmax", C.c_int), ("kmax", C.c_int), ("nbc", C.c_int),
+                ("bc", _I), ("ncon", C.c_int), ("con", _I)]
+
+
+class Cav3dError(RuntimeError):
+    pass
+
+
+def lib(build_if_missing: bool = True):
+    """Load libcav3d.so (building it with hipcc only if it is absent)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        if not build_if_missing:
+            raise Cav3dError(f"{LIB_PATH} is missing; run __graft_entry__.build()")
+        import importlib
+        importlib.import_module(__package__ + ".build").build()
+    L = C.CDLL(LIB_PATH)
+    L.cav3d_last_error.restype = C.c_char_p
+    L.cav3d_create.argtypes = [C.POINTER(Params), C.c_int, C.POINTER(BlockDesc), C.POINTER(C.c_void_p)]
+    L.cav3d_set_metrics.argtypes = [C.c_void_p, C.c_int, _D, _D, _D, _D]
+    L.cav3d_put_state.argtypes = [C.c_void_p, C.c_int, _D, _D, _D, _D]
+    L.cav3d_get_state.argtypes = [C.c_void_p, C.c_int, _D, _D, _D, _D]
+    L.cav3d_timeinteg.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, _D]
+    L.cav3d_setbc.argtypes = [C.c_void_p]
+    L.cav3d_calrhs.argtypes = [C.c_void_p, C.c_double, C.c_int, _D, _D, _D, _D]
+    L.cav3d_timestep.argtypes = [C.c_void_p, C.c_int, _D]
+    L.cav3d_residual.argtypes = [C.c_void_p, _D, _I, _D]
+    L.cav3d_unsteady_shift.argtypes = [C.c_void_p]
+    L.cav3d_eos.argtypes = [C.c_void_p, C.c_int] + [_D] * 6
+    L.cav3d_eos_host.argtypes = [C.c_int] + [_D] * 6
+    L.cav3d_profile.argtypes = [C.c_void_p, C.c_int, _D]
+    L.cav3d_synchronize.argtypes = [C.c_void_p]
+    L.cav3d_destroy.argtypes = [C.c_void_p]
+    L.cav3d_destroy.restype = None
+    _lib = L
+    return L
+
+
+def _p(a, t=C.c_double):
+    if a is None:
+        return None
+    if not a.flags["C_CONTIGUOUS"]:
+        raise ValueError("array must be C-contiguous")
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+def _check(rc: int):
+    if rc != 0:
+        msg = lib().cav3d_last_error().decode()
+        raise Cav3dError(f"cav3d error {rc}: {msg}")
+
+
+def eos_host(p, t, y1, y2):
+    """dv(18,n), tv(2,n) of absolute (p, T, y1, y2) on the host (set-up use)."""
+    p, t, y1, y2 = (np.ascontiguousarray(np.broadcast_to(a, np.broadcast(p, t, y1, y2).shape), dtype=np.float64).ravel()
+                    for a in (p, t, y1, y2))
+    n = p.size
+    dv = np.empty((n, 18))
+    tv = np.empty((n, 2))
+    _check(lib().cav3d_eos_host(n, _p(p), _p(t), _p(y1), _p(y2), _p(dv), _p(tv)))
+    return dv, tv
+
+
+def params_struct(case) -> Params:
+    P = Params()
+    for k, v in case.params.items():
+        if k == "bc_pv":
+            P.bc_pv = (C.c_double * 9)(*list(v))
+        elif hasattr(P, k):
+            setattr(P, k, v)
+    P.precision = case.params.get("precision", 0)
+    return P
+
+
+class Solver:
+    """One GPU context holding every block of a case."""
+
+    def __init__(self, case, precision: int = 0):
+        L = lib()
+        self.case = case
+        self.npv, self.ntv = case.npv, case.ntv
+        P = params_struct(case)
+        P.precision = precision
+        self._keep = []
+        descs = (BlockDesc * len(case.blocks))()
+        for ib, b in enumerate(case.blocks):
+            bc = np.ascontiguousarray(np.array(b.bc_desc, dtype=np.int32).reshape(-1))
+            con = np.ascontiguousarray(np.array(b.conn_desc, dtype=np.int32).reshape(-1)) if b.conn_desc \
+                else np.zeros(22, np.int32)
+            self._keep += [bc, con]
+            descs[ib] = BlockDesc(b.imax, b.jmax, b.kmax, len(b.bc_desc), _p(bc, C.c_int),
+                                  len(b.conn_desc), _p(con, C.c_int))
+        h = C.c_void_p()
+        _check(L.cav3d_create(C.byref(P), len(case.blocks), descs, C.byref(h)))
+        self.h = h
+        for ib, b in enumerate(case.blocks):
+            _check(L.cav3d_set_metrics(self.h, ib, _p(b.cx), _p(b.ex), _p(b.tx), _p(b.grd)))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().cav3d_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- state
+    def put_state(self, ib, pv, dv=None, tv=None, qq=None):
+        _check(lib().cav3d_put_state(self.h, ib, _p(pv), _p(dv), _p(tv), _p(qq)))
+
+    def get_state(self, ib):
+        b = self.case.blocks[ib]
+        ng = (b.imax + 5) * (b.jmax + 5) * (b.kmax + 5)
+        ni = (b.imax - 1) * (b.jmax - 1) * (b.kmax - 1)
+        pv = np.zeros(self.npv * ng)
+        dv = np.zeros(18 * ng)
+        tv = np.zeros(max(self.ntv, 1) * ng)
+        qq = np.zeros(2 * self.npv * ni)
+        _check(lib().cav3d_get_state(self.h, ib, _p(pv), _p(dv), _p(tv) if self.ntv else None, _p(qq)))
+        return pv, dv, (tv if self.ntv else None), qq
+
+    # ---- operators
+    def timeinteg(self, nt_phy=1, nt=1, timeprev=0.0, want_time=True):
+        t = C.c_double(0.0)
+        _check(lib().cav3d_timeinteg(self.h, nt_phy, nt, timeprev, C.byref(t) if want_time else None))
+        return t.value
+
+    def setbc(self):
+        _check(lib().cav3d_setbc(self.h))
+
+    def calrhs(self, ib, time=0.0):
+        b = self.case.blocks[ib]
+        ni = (b.imax - 1) * (b.jmax - 1) * (b.kmax - 1)
+        res = np.zeros(self.npv * ni)
+        icav = np.zeros(4 * ni)
+        itt = np.zeros(4 * ni)
+        oc = np.zeros(ni)
+        _check(lib().cav3d_calrhs(self.h, time, ib, _p(res), _p(icav), _p(itt), _p(oc)))
+        return res, icav, itt, oc
+
+    def timestep(self, ib):
+        b = self.case.blocks[ib]
+        dt = np.zeros((b.imax - 1) * (b.jmax - 1) * (b.kmax - 1))
+        _check(lib().cav3d_timestep(self.h, ib, _p(dt)))
+        return dt
+
+    def residual(self):
+        l2 = np.zeros(self.npv)
+        ijk = np.zeros(4, np.int32)
+        rm = C.c_double(0.0)
+        _check(lib().cav3d_residual(self.h, _p(l2), _p(ijk, C.c_int), C.byref(rm)))
+        return l2, ijk, rm.value
+
+    def unsteady_shift(self):
+        _check(lib().cav3d_unsteady_shift(self.h))
+
+    def eos(self, p, t, y1, y2):
+        p, t, y1, y2 = (np.ascontiguousarray(a, dtype=np.float64).ravel() for a in (p, t, y1, y2))
+        n = p.size
+        dv = np.empty((n, 18))
+        tv = np.empty((n, 2))
+        _check(lib().cav3d_eos(self.h, n, _p(p), _p(t), _p(y1), _p(y2), _p(dv), _p(tv)))
+        return dv, tv
+
+    def profile(self, enable=True):
+        out = np.zeros(6)
+        _check(lib().cav3d_profile(self.h, 1 if enable else 0, _p(out)))
+        return out
+
+    def synchronize(self):
+        _check(lib().cav3d_synchronize(self.h))

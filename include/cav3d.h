@@ -1,0 +1,158 @@
+/*
+ * cav3d — MI355X (gfx950) implementation of the pseudo-time iteration of the
+ * all-speed two-phase solver: RHS residual (MUSCL/MLP, IAPWS-IF97 EOS,
+ * preconditioned Roe / AUSM+-up, viscous flux, k-omega SST, Merkle mass
+ * transfer) plus the implicit LU-SGS update, with every grid block resident on
+ * one GPU and block-interface halos filled by device copy kernels.
+ *
+ * C ABI: plain pointers and sizes.  Every call returns 0 on success and a
+ * negative code on error; cav3d_last_error() gives the message.  Host arrays
+ * use the reference's Fortran layouts (component fastest, column-major,
+ * explicit lower bounds) and stay owned by the caller; device buffers are
+ * owned by the context.  A context is used from one host thread at a time.
+ *
+ * Reference interfaces replaced (kwonhwi1/3dcode):
+ *   cav3d_timeinteg      class(t_update)%timeinteg -> lusgs
+ *                        src/update_module.f90:37-52 (interface), :455-733
+ *                        (called at src/solve_module.f90:135)
+ *   cav3d_calrhs         t_rhs%calrhs + getres/geticav/getitt/getomega_cut
+ *                        src/rhs_module.f90:260-683
+ *   cav3d_setbc          t_bc%setbc src/bc_module.f90:1235-1396
+ *   cav3d_residual       t_resi%setqres + t_resi%residual
+ *                        src/residual_module.f90:94-186
+ *   cav3d_unsteady_shift t_solve%unsteadyupdate src/solve_module.f90:155-183
+ *   cav3d_put_state /    t_variable getpv/getdv/gettv/getqq + setters
+ *   cav3d_get_state      src/variable_module.f90:16-21, 69-77
+ *   cav3d_set_metrics    t_grid getcx/getex/gettx/getgrd src/grid_module.f90:37-39
+ *   cav3d_create         t_update%construct + t_bc%construct (BC faces, 12
+ *                        edges, 8 corners, interfaces)
+ *                        src/update_module.f90:77-178, src/bc_module.f90:71-1188
+ *   cav3d_eos            t_eos%deteos (deteos_prop / deteosonly)
+ *                        src/eos_module.f90:1120-1234, evaluated on the GPU
+ *   cav3d_eos_host       the same equation of state on the host (set-up use)
+ *   cav3d_rhs_vjp        no reference counterpart (discrete adjoint of calrhs)
+ */
+#ifndef CAV3D_H
+#define CAV3D_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* physical boundary kinds (reference bcname strings, src/bc_module.f90:221-335) */
+enum {
+    CAV3D_BC_WALL = 1,              /* 'BCWall'  (viscous / kw / inviscid by iturb) */
+    CAV3D_BC_INFLOW_SUBSONIC = 2,   /* 'BCInflowSubsonic' */
+    CAV3D_BC_OUTFLOW_SUBSONIC = 3,  /* 'BCOutflowSubsonic' */
+    CAV3D_BC_FARFIELD = 4,          /* 'BCFarfield' */
+    CAV3D_BC_SYMMETRY = 5,          /* 'BCSymmetryPlane' */
+    CAV3D_BC_INFLOW_SUPERSONIC = 6, /* 'BCInflowSupersonic' */
+    CAV3D_BC_OUTFLOW_SUPERSONIC = 7,/* 'BCOutflowSupersonic' / 'BCExtrapolate' */
+    CAV3D_BC_DEGENERATE = 8,        /* 'BCDegeneratePoint' / 'BCDegenerateLine' */
+    CAV3D_BC_INFLOW = 9,            /* 'BCInflow'  (no-op, as in the reference) */
+    CAV3D_BC_OUTFLOW = 10           /* 'BCOutflow' (no-op, as in the reference) */
+};
+
+enum { CAV3D_FP64 = 0, CAV3D_MIXED = 1 };
+
+/* Upper clip of the Merkle rates.  src/cav_module.f90:151-152 clips
+ * r_c <= rho*pv(5) and r_v <= rho*(1-pv(5)-pv(6)), where pv(5) is the
+ * temperature; with T > 1 K the second bound is negative, so every cell gets
+ * r_v = rho*(1-T-y1) and a cavitating run diverges within a few iterations.
+ * AS_WRITTEN reproduces that bit for bit; SPECIES clips on the mass
+ * fractions, r_c <= rho*y1 and r_v <= rho*(1-y1-y2). */
+enum { CAV3D_CLIP_AS_WRITTEN = 0, CAV3D_CLIP_SPECIES = 1 };
+
+/* Run parameters: the config_module fields the path reads
+ * (src/config_module.f90:7-28, 90-199), integer codes unchanged. */
+typedef struct {
+    int iturb;      /* -3 Euler, -2 laminar, 0 k-omega SST (ides = 0) */
+    int nscheme;    /* 1 preconditioned Roe, 4 AUSM+-up */
+    int nmuscl;     /* 0 first order, 1 MUSCL-TVD, 2 MLP */
+    int nlim;       /* limiter 0..10 */
+    int local;      /* -1 fixed, 0 global-min, 1 local time step */
+    int prec;       /* 0 none, 1 steady, 2 unsteady preconditioning */
+    int ncav;       /* 0 none, 1 Merkle */
+    int nsteady;    /* 0 steady, 1 BDF1, 2 BDF2 dual time */
+    int precision;  /* CAV3D_FP64 or CAV3D_MIXED */
+    int cav_clip;   /* CAV3D_CLIP_AS_WRITTEN or CAV3D_CLIP_SPECIES */
+    int reserved[2];
+    double cfl, pref, uref, str, dt_phy, c_v, c_c;
+    double dp_ref;  /* 2/(rho_ref*uref^2)  (src/cav_module.f90:81) */
+    double t_ref;   /* uref/l_chord        (src/cav_module.f90:82) */
+    double t_crit;  /* 647.096 for water   (src/cav_module.f90:86-95) */
+    double bc_pv[9];    /* boundary reference state bcinfo%pv, bc_pv[0] absolute */
+    double bc_pressure; /* bcinfo%pressure */
+    double emutref;     /* initial eddy viscosity */
+} cav3d_params;
+
+/* One grid block.  imax/jmax/kmax are vertex counts (CGNS convention,
+ * src/grid_module.f90:107-114); cells are 2..imax etc. */
+typedef struct {
+    int imax, jmax, kmax;
+    int nbc;
+    const int *bc;   /* nbc x 7: kind, istart[3], iend[3] as grid_module gives them */
+    int ncon;
+    const int *con;  /* ncon x 22: donor block, istart[3], iend[3], istart_donor[3],
+                        iend_donor[3], transmat[3][3] row-major: the connectinfo
+                        after src/bc_module.f90:355-488 */
+} cav3d_block;
+
+typedef struct cav3d_ctx cav3d_ctx;
+
+int cav3d_create(const cav3d_params *params, int nblocks, const cav3d_block *blocks, cav3d_ctx **out);
+
+/* cx(3,1:imax,1:jmax+1,1:kmax+1), ex(3,1:imax+1,1:jmax,1:kmax+1),
+ * tx(3,1:imax+1,1:jmax+1,1:kmax), grd(ngrd,1:imax+1,1:jmax+1,1:kmax+1),
+ * ngrd = 4 (laminar/Euler) or 5 (turbulent). */
+int cav3d_set_metrics(cav3d_ctx *ctx, int b, const double *cx, const double *ex,
+                      const double *tx, const double *grd);
+
+/* pv(npv,-1:imax+3,-1:jmax+3,-1:kmax+3), dv(18,...), tv(ntv,...),
+ * qq(npv,2,2:imax,2:jmax,2:kmax).  dv/tv may be NULL: then dv and tv(1:2)
+ * are evaluated from pv on the device and tv(3) is set to emutref. qq may be
+ * NULL: then both time levels are set from the state (initial_module set_others). */
+int cav3d_put_state(cav3d_ctx *ctx, int b, const double *pv, const double *dv,
+                    const double *tv, const double *qq);
+int cav3d_get_state(cav3d_ctx *ctx, int b, double *pv, double *dv, double *tv, double *qq);
+
+/* One LU-SGS pseudo-time step of every block.  time may be NULL (no host sync). */
+int cav3d_timeinteg(cav3d_ctx *ctx, int nt_phy, int nt, double timeprev, double *time);
+
+/* Fill all ghost cells (physical BCs, interfaces, edges, corners). */
+int cav3d_setbc(cav3d_ctx *ctx);
+
+/* RHS of block b from the current state (ghosts as they are): res(npv,2:imax,..),
+ * icav(4,..), itt(4,..), omega_cut(..); any output may be NULL. */
+int cav3d_calrhs(cav3d_ctx *ctx, double time, int b, double *res, double *icav,
+                 double *itt, double *omega_cut);
+
+/* Local time steps of block b (computed from the current state). */
+int cav3d_timestep(cav3d_ctx *ctx, int b, double *dt);
+
+/* Norms of the last cav3d_timeinteg: l2[npv] = sqrt(sum (dpv)^2) over all
+ * interior cells, ijk_max[4] = {block, i, j, k} of the first strict maximum
+ * of |dp| (> 1e-12) in k-j-i order, resmax its value. */
+int cav3d_residual(cav3d_ctx *ctx, double *l2, int *ijk_max, double *resmax);
+
+/* qq(2) = qq(1); qq(1) = conservative state (dual time). */
+int cav3d_unsteady_shift(cav3d_ctx *ctx);
+
+/* dv(18,n), tv(2,n) of n states (absolute p, T, y1, y2) on the device / host. */
+int cav3d_eos(cav3d_ctx *ctx, int n, const double *p, const double *t, const double *y1,
+              const double *y2, double *dv, double *tv);
+int cav3d_eos_host(int n, const double *p, const double *t, const double *y1, const double *y2,
+                   double *dv, double *tv);
+
+/* Device time, in milliseconds, of the last cav3d_timeinteg's kernels by phase:
+ * out[0] timestep, [1] boundary, [2] rhs, [3] lusgs, [4] update+eddy, [5] total. */
+int cav3d_profile(cav3d_ctx *ctx, int enable, double *out);
+
+int cav3d_synchronize(cav3d_ctx *ctx);
+const char *cav3d_last_error(void);
+void cav3d_destroy(cav3d_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

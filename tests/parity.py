@@ -1,0 +1,52 @@
+"""Shared helpers of the GPU parity tests: build a case, run the HIP
+library and the oracle side by side from the same state, and compare."""
+import numpy as np
+
+
+def small_cases(pk):
+    """Small bump-channel cases covering the scheme switches of the path."""
+    bc = pk.cases.bump_case
+    return {
+        "lam_roe_tvd": lambda: pk.cases.config1(12, 8, 6),
+        "lam_roe_mlp": lambda: bc(12, 8, 6, False, False, 1, 2, 8, cfl=5.0, seed=3),
+        "lam_ausm_tvd_vanleer": lambda: bc(12, 8, 6, False, False, 4, 1, 2, cfl=5.0, seed=4),
+        "sst_cav_ausm_mlp": lambda: _clip(pk.cases.config2(12, 8, 6), 1),
+        "sst_cav_roe_first": lambda: _clip(bc(12, 8, 6, True, True, 1, 0, 0, cfl=5.0, seed=5), 1),
+        "sst_cav_as_written": lambda: _clip(pk.cases.config2(12, 8, 6), 0),
+    }
+
+
+def _clip(case, mode):
+    case.params["cav_clip"] = mode
+    return case
+
+
+def pair(pk, orc, case):
+    s = pk.Solver(case)
+    o = orc.Oracle(case)
+    for ib, (pv, dv, tv, qq) in enumerate(case.init):
+        s.put_state(ib, pv, dv, tv, qq)
+        o.put_state(ib, pv, dv, tv, qq)
+    return s, o
+
+
+def interior(case, ib, a, nv):
+    b = case.blocks[ib]
+    A = a.reshape(b.kmax + 5, b.jmax + 5, b.imax + 5, nv)
+    return A[3:b.kmax + 2, 3:b.jmax + 2, 3:b.imax + 2].reshape(-1, nv)
+
+
+def rel_err(a, b, floor=1e-8):
+    """Largest per-element relative difference of a against b per column,
+    with elements below floor * (column max) compared against that floor."""
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    if a.ndim == 1:
+        a = a[:, None]
+        b = b[:, None]
+    scale = np.max(np.abs(b), axis=0)
+    den = np.maximum(np.abs(b), floor * np.maximum(scale, 1e-300))
+    e = np.abs(a - b) / den
+    e[np.isnan(a) != np.isnan(b)] = np.inf
+    e[np.isnan(a) & np.isnan(b)] = 0.0
+    return np.max(e, axis=0)

@@ -1,0 +1,119 @@
+"""HIP library against the oracle on the same inputs (FP64 path).
+
+Bar: 1e-11 relative per cell and equation (elements below 1e-8 of their
+field's largest magnitude are compared against that floor); the argmax
+cell of the residual is bit-exact."""
+import numpy as np
+import pytest
+
+from parity import interior, pair, rel_err, small_cases
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-11
+NAMES = ["lam_roe_tvd", "lam_roe_mlp", "lam_ausm_tvd_vanleer", "sst_cav_ausm_mlp", "sst_cav_roe_first",
+         "sst_cav_as_written"]
+
+
+def _case(pk, name):
+    print(f"case {name}", flush=True)
+    return small_cases(pk)[name]()
+
+
+def test_device_eos_matches_oracle(pk, orc):
+    case = pk.cases.config1(8, 6, 6)
+    s = pk.Solver(case)
+    rng = np.random.default_rng(7)
+    n = 4000
+    p = rng.uniform(2e3, 5e6, n)
+    t = rng.uniform(275.0, 500.0, n)
+    y1 = np.where(rng.random(n) < 0.5, 0.0, rng.uniform(0.0, 1.0, n))
+    y2 = np.where(rng.random(n) < 0.7, 0.0, rng.uniform(0.0, 1.0, n) * (1.0 - y1))
+    dv, tv = s.eos(p, t, y1, y2)
+    ref = np.array([orc.deteos_prop(*q) for q in zip(p, t, y1, y2)], dtype=object)
+    dref = np.stack([r[0] for r in ref])
+    tref = np.stack([r[1] for r in ref])
+    assert np.all(rel_err(dv, dref, floor=1e-14) < TOL)
+    assert np.all(rel_err(tv, tref, floor=1e-14) < TOL)
+    s.close()
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_setbc_fills_ghosts_like_oracle(pk, orc, name):
+    case = _case(pk, name)
+    s, o = pair(pk, orc, case)
+    s.setbc()
+    o.setbc()
+    npv = case.npv
+    for ib in range(len(case.blocks)):
+        pg, dg, tg, _ = s.get_state(ib)
+        po, do, to, _ = o.get_state(ib)
+        P_g, P_o = pg.reshape(-1, npv), po.reshape(-1, npv)
+        D_o = do.reshape(-1, 18)
+        set_cells = D_o[:, 0] != 0.0          # cells the reference fills (interior + ghosts)
+        assert set_cells.sum() > case.blocks[ib].ncell
+        assert np.all(rel_err(P_g[set_cells], P_o[set_cells]) < TOL), rel_err(P_g[set_cells], P_o[set_cells])
+        assert np.all(P_g[~set_cells] == 0.0)
+        D_g = dg.reshape(-1, 18)
+        e = rel_err(D_g[set_cells], D_o[set_cells])
+        assert np.all(e < TOL), e
+        if case.ntv:
+            T_g, T_o = tg.reshape(-1, case.ntv), to.reshape(-1, case.ntv)
+            e = rel_err(T_g[set_cells], T_o[set_cells])
+            assert np.all(e < TOL), e
+    s.close()
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_calrhs_and_timestep_match_oracle(pk, orc, name):
+    case = _case(pk, name)
+    s, o = pair(pk, orc, case)
+    s.setbc()
+    o.setbc()
+    for ib in range(len(case.blocks)):
+        rg, icg, itg, ocg = s.calrhs(ib)
+        ro, ico, ito, oco = o.calrhs(ib)
+        npv = case.npv
+        e = rel_err(rg.reshape(-1, npv), ro.reshape(-1, npv), floor=1e-6)
+        assert np.all(e < TOL), e
+        if case.params["ncav"]:
+            e = rel_err(icg.reshape(-1, 4), ico.reshape(-1, 4))
+            assert np.all(e < TOL), e
+        if case.npv == 9:
+            e = rel_err(itg.reshape(-1, 4), ito.reshape(-1, 4))
+            assert np.all(e < TOL), e
+            assert np.all(rel_err(ocg, oco) < TOL)
+        dg = s.timestep(ib)
+        o.timestep(1, 1, 0.0)
+        do = o.get_dt(ib)
+        assert np.all(rel_err(dg, do) < TOL), rel_err(dg, do)
+    s.close()
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_timeinteg_matches_oracle(pk, orc, name):
+    case = _case(pk, name)
+    s, o = pair(pk, orc, case)
+    npv = case.npv
+    nit = 1 if name == "sst_cav_as_written" else 3
+    for it in range(nit):
+        print(f"  iteration {it}", flush=True)
+        o.setqres()
+        o.timeinteg(1, it + 1, 0.0)
+        s.timeinteg(1, it + 1, 0.0)
+        for ib in range(len(case.blocks)):
+            pg, dg, tg, _ = s.get_state(ib)
+            po, do, to, _ = o.get_state(ib)
+            e = rel_err(interior(case, ib, pg, npv), interior(case, ib, po, npv))
+            assert np.all(e < TOL), (it, e)
+            e = rel_err(interior(case, ib, dg, 18), interior(case, ib, do, 18))
+            assert np.all(e < TOL), (it, e)
+            if case.ntv:
+                e = rel_err(interior(case, ib, tg, case.ntv), interior(case, ib, to, case.ntv))
+                assert np.all(e < TOL), (it, e)
+        l2g, ijkg, rmg = s.residual()
+        l2o, ijko, rmo = o.residual()
+        assert np.all(rel_err(l2g, l2o) < TOL)
+        assert list(ijkg) == list(ijko)
+        assert abs(rmg - rmo) <= TOL * abs(rmo)
+    s.close()
