@@ -22,6 +22,9 @@ HIPCC_FLAGS = [
     "-shared",
     # packed FP32 instructions stay off (known-bad forms on this pool)
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # no multiply-add contraction: the FP64 path reproduces the reference's
+    # (uncontracted) rounding operation for operation
+    "-ffp-contract=off",
     "-Wno-unused-result",
 ]
 

@@ -494,44 +494,18 @@ __global__ __launch_bounds__(128) void k_face(DS S)
     const bool visc = (P.iturb >= -2);
     double c9[NPV], c10[NPV], xl[NPV], xr[NPV];
     double g3[6][3] = {};
-    int vq = 0;
-    for (int n = 0; n < npv; n++) {
-        const double *f = S.pv + (long)n * S.NG + g9;
-        auto xs = [&](int ll) -> double { return f[off(ll)]; };
-        c9[n] = f[0];
-        c10[n] = f[sN];
-        if (P.nmuscl > 0) muscl_var<double>(P.nlim, P.nmuscl == 2, xs, xl[n], xr[n]);
-        else { xl[n] = c9[n]; xr[n] = c10[n]; }
-        if (visc && (n >= 1 && n <= 4 || n >= 7)) {
-            g3[vq][0] = xs(10) - xs(9);
-            g3[vq][1] = 0.0625 * (-xs(1) - xs(2) + xs(5) + xs(6) + 2.0 * (-xs(7) - xs(8) + xs(11) + xs(12))
-                                  - xs(13) - xs(14) + xs(17) + xs(18));
-            g3[vq][2] = 0.0625 * (-xs(1) - xs(2) + xs(13) + xs(14) + 2.0 * (-xs(3) - xs(4) + xs(15) + xs(16))
-                                  - xs(5) - xs(6) + xs(17) + xs(18));
-            vq++;
-        }
-    }
+    const double *f0 = S.pv + g9;
+    face_recon<double>(P, npv, [&](int ll, int n) -> double { return f0[(long)n * S.NG + off(ll)]; },
+                       c9, c10, xl, xr, g3);
     int ip = i, jp = j, kp = k;
     if (D == 0) ip++; else if (D == 1) jp++; else kp++;
     const long gq = g9 + sN;
-    FaceState<double> L, R;
     double n3[3];
     face_vec(S, B, D, i, j, k, n3);
-    if (P.nmuscl > 0) {
-        muscl_clamps<double>(npv, P.pref, c9, c10, xl, xr);
-        face_rho_h<double>(xl[0] + P.pref, xl[4], xl[5], xl[6], L.rho, L.h);
-        face_rho_h<double>(xr[0] + P.pref, xr[4], xr[5], xr[6], R.rho, R.h);
-    } else {
-        L.rho = S.dc[D_RHO * S.NG + g9]; L.h = S.dc[D_H * S.NG + g9];
-        R.rho = S.dc[D_RHO * S.NG + gq]; R.h = S.dc[D_H * S.NG + gq];
-    }
-    for (int n = 0; n < npv; n++) { L.p[n] = xl[n]; R.p[n] = xr[n]; }
-    double sl = sqrt(L.rho), sr = sqrt(R.rho), rd = 1.0 / (sl + sr);
-    double ya1 = (sl * xl[5] + sr * xr[5]) * rd, ya2 = (sl * xl[6] + sr * xr[6]) * rd;
-    double c2a = face_c2<double>(0.5 * (xl[0] + xr[0]) + P.pref, 0.5 * (xl[4] + xr[4]), ya1, ya2);
+    double rh9[2] = {S.dc[D_RHO * S.NG + g9], S.dc[D_H * S.NG + g9]};
+    double rh10[2] = {S.dc[D_RHO * S.NG + gq], S.dc[D_H * S.NG + gq]};
     double fx[NPV];
-    if (P.nscheme == 1) roe_flux<double>(P, npv, n3, L, R, c2a, fx);
-    else ausm_flux<double>(P, npv, n3, L, R, c2a, fx);
+    inviscid_face<double>(P, npv, n3, c9, c10, xl, xr, rh9, rh10, fx);
     double *out = S.ff[D];
     const long NF = S.NF[D];
     for (int n = 0; n < npv; n++) out[n * NF + t] = fx[n];

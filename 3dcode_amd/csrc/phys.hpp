@@ -41,24 +41,19 @@ struct Par {
 template <class R> HD R mx(R a, R b) { return (b > a) ? b : a; }
 template <class R> HD R mn(R a, R b) { return (b < a) ? b : a; }
 
-// x^n for a non-negative exponent that is a compile-time constant after
-// unrolling: folds into a multiplication chain shared by common sub-expressions.
-template <class R> HD R ipow(R x, int n)
-{
-    R y = R(1);
-    while (n) {
-        if (n & 1) y *= x;
-        x *= x;
-        n >>= 1;
-    }
-    return y;
-}
 
 // ------------------------------------------------------------ IF97 tables
 // IAPWS-IF97 (2007) region 1 (table 2), region 2 ideal/residual (tables 10,
 // 11) and metastable-vapour residual (table 16) coefficients.
 struct IJN { int i, j; double n; };
-__device__ __host__ constexpr IJN R1C[34] = {
+// Coefficient tables live in device memory in the device pass and as plain
+// constants in the host pass (the host never reads a device variable's shadow).
+#if defined(__HIP_DEVICE_COMPILE__)
+#define CAV_TABLE __device__ constexpr
+#else
+#define CAV_TABLE constexpr
+#endif
+CAV_TABLE IJN R1C[34] = {
     {0, -2, 0.14632971213167e0}, {0, -1, -0.84548187169114e0}, {0, 0, -0.37563603672040e1},
     {0, 1, 0.33855169168385e1}, {0, 2, -0.95791963387872e0}, {0, 3, 0.15772038513228e0},
     {0, 4, -0.16616417199501e-1}, {0, 5, 0.81214629983568e-3}, {1, -9, 0.28319080123804e-3},
@@ -71,12 +66,12 @@ __device__ __host__ constexpr IJN R1C[34] = {
     {8, -6, -0.17424871230634e-9}, {21, -29, -0.68762131295531e-18}, {23, -31, 0.14478307828521e-19},
     {29, -38, 0.26335781662795e-22}, {30, -39, -0.11947622640071e-22}, {31, -40, 0.18228094581404e-23},
     {32, -41, -0.93537087292458e-25}};
-__device__ __host__ constexpr int R2J0[9] = {0, 1, -5, -4, -3, -2, -1, 2, 3};
-__device__ __host__ constexpr double R2N0[9] = {-0.96927686500217e1, 0.10086655968018e2, -0.56087911283020e-2,
+CAV_TABLE int R2J0[9] = {0, 1, -5, -4, -3, -2, -1, 2, 3};
+CAV_TABLE double R2N0[9] = {-0.96927686500217e1, 0.10086655968018e2, -0.56087911283020e-2,
     0.71452738081455e-1, -0.40710498223928e0, 0.14240819171444e1, -0.43839511319450e1, -0.28408632460772e0,
     0.21268463753307e-1};
-__device__ __host__ constexpr double R2MN0[2] = {-0.96937268393049e1, 0.10087275970006e2};
-__device__ __host__ constexpr IJN R2C[43] = {
+CAV_TABLE double R2MN0[2] = {-0.96937268393049e1, 0.10087275970006e2};
+CAV_TABLE IJN R2C[43] = {
     {1, 0, -0.17731742473213e-2}, {1, 1, -0.17834862292358e-1}, {1, 2, -0.45996013696365e-1},
     {1, 3, -0.57581259083432e-1}, {1, 6, -0.50325278727930e-1}, {2, 1, -0.33032641670203e-4},
     {2, 2, -0.18948987516315e-3}, {2, 4, -0.39392777243355e-2}, {2, 7, -0.43797295650573e-1},
@@ -92,23 +87,23 @@ __device__ __host__ constexpr IJN R2C[43] = {
     {20, 48, -0.42002467698208e-5}, {21, 21, -0.59056029685639e-25}, {22, 53, 0.37826947613457e-5},
     {23, 39, -0.12768608934681e-14}, {24, 26, 0.73087610595061e-28}, {24, 40, 0.55414715350778e-16},
     {24, 58, -0.94369707241210e-6}};
-__device__ __host__ constexpr IJN R2MC[13] = {
+CAV_TABLE IJN R2MC[13] = {
     {1, 0, -0.73362260186506e-2}, {1, 2, -0.88223831943146e-1}, {1, 5, -0.72334555213245e-1},
     {1, 11, -0.40813178534455e-2}, {2, 1, 0.20097803380207e-2}, {2, 7, -0.53045921898642e-1},
     {2, 16, -0.76190409086970e-2}, {3, 4, -0.63498037657313e-2}, {3, 16, -0.86043093028588e-1},
     {4, 7, 0.75321581522770e-2}, {4, 10, -0.79238375446139e-2}, {5, 9, -0.22888160778447e-3},
     {5, 10, -0.26456501482810e-2}};
 // IAPS-85 viscosity / conductivity coefficients
-__device__ __host__ constexpr double TPH[4] = {1.0, 0.978197, 0.579829, -0.202354};
-__device__ __host__ constexpr double TPL[4] = {1.0, 6.978267, 2.599096, -0.998254};
-__device__ __host__ constexpr double TPHH[6][7] = {
+CAV_TABLE double TPH[4] = {1.0, 0.978197, 0.579829, -0.202354};
+CAV_TABLE double TPL[4] = {1.0, 6.978267, 2.599096, -0.998254};
+CAV_TABLE double TPHH[6][7] = {
     {0.5132047, 0.2151778, -0.2818107, 0.1778064, -0.0417661, 0.0, 0.0},
     {0.3205656, 0.7317883, -1.070786, 0.460504, 0.0, -0.01578386, 0.0},
     {0.0, 1.241044, -1.263184, 0.2340379, 0.0, 0.0, 0.0},
     {0.0, 1.476783, 0.0, -0.4924179, 0.1600435, 0.0, -0.003629481},
     {-0.7782567, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0},
     {0.1885447, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0}};
-__device__ __host__ constexpr double TPLL[5][6] = {
+CAV_TABLE double TPLL[5][6] = {
     {1.3293046, -0.40452437, 0.24409490, 0.018660751, -0.12961068, 0.044809953},
     {1.7018363, -2.2156845, 1.6511057, -0.76736002, 0.37283344, -0.11203160},
     {5.2246158, -10.124111, 4.9874687, -0.27297694, -0.43083393, 0.13333849},
@@ -138,80 +133,108 @@ HD double sat_p(double t)
     return 1.0e6 * (x2 * x2);
 }
 
-// Region 1: the five derivative sums of the Gibbs series are accumulated as
-// weighted sums of the terms and scaled once (sum_k J g / b etc.).
+// real**integer as gfortran evaluates it with a run-time exponent (libgcc
+// __powidf2: binary exponentiation, reciprocal for negative powers); the
+// exponents here are compile-time table entries, so the loop unrolls into
+// the same multiplication sequence.
+template <class R> HD R powi(R x, int m)
+{
+    unsigned n = m < 0 ? 0u - (unsigned)m : (unsigned)m;
+    R y = (n % 2u) ? x : R(1);
+    while (n >>= 1) {
+        x = x * x;
+        if (n % 2u) y = y * x;
+    }
+    return m < 0 ? R(1) / y : y;
+}
+
+// Region 1 (src/eos_module.f90:1331-1374), term by term in the reference's
+// operation order so the FP64 path reproduces its rounding.
 template <class R> HD E2<R> liquid(R p, R t)
 {
     R pi = p * R(0.00000006049607);
     R tau = R(1386.0) / t;
-    R a = R(7.1) - pi, ia = R(1) / a;
-    R b = tau - R(1.222), ib = R(1) / b;
-    R sj = 0, si = 0, sii = 0, sjj = 0, sij = 0;
+    R pi1 = R(1) / (R(7.1) - pi);
+    R tau1 = R(1) / (tau - R(1.222));
+    R gt = 0, gp = 0, gpp = 0, gtt = 0, gpt = 0;
 #pragma unroll
     for (int k = 0; k < 34; k++) {
         const int I = R1C[k].i, J = R1C[k].j;
-        R g = R(R1C[k].n) * ipow(a, I) * (J >= 0 ? ipow(b, J) : ipow(ib, -J));
-        sj += R(J) * g;
-        si += R(I) * g;
-        sii += R(I * (I - 1)) * g;
-        sjj += R(J * (J - 1)) * g;
-        sij += R(I * J) * g;
+        R g = R(R1C[k].n) * powi(R(7.1) - pi, I) * powi(tau - R(1.222), J);
+        R g1 = R(J) * g * tau1;
+        R g2 = -(R(I) * g * pi1);
+        R g3 = -(R(I - 1) * g2 * pi1);
+        R g4 = R(J - 1) * g1 * tau1;
+        R g5 = R(J) * g2 * tau1;
+        gt = gt + g1;
+        gp = gp + g2;
+        gpp = gpp + g3;
+        gtt = gtt + g4;
+        gpt = gpt + g5;
     }
-    R gt = sj * ib, gp = -si * ia, gpp = sii * (ia * ia), gtt = sjj * (ib * ib), gpt = -sij * (ia * ib);
     R gp1 = R(1) / gp;
+    R tau3 = tau * tau * tau;
     E2<R> e;
     e.rho = R(C_RHO1) * tau * gp1;
     e.h = R(639675.036) * gt;
     e.drdp = -(gpp * R(1.5632937721834122e-6) * tau * (gp1 * gp1));
-    e.drdt = -(e.rho * R(7.215007215007215e-4) * tau) + R(C_RT1) * gpt * (tau * tau * tau) * (gp1 * gp1);
+    e.drdt = -(e.rho * R(7.215007215007215e-4) * tau) + R(C_RT1) * gpt * tau3 * (gp1 * gp1);
     e.dhdp = R(C_HP1) * gpt;
     e.dhdt = -(R(461.526) * (tau * tau) * gtt);
     return e;
 }
 
-// Region 2 with the metastable-vapour branch for p > p_sat(T).
+// Region 2 (src/eos_module.f90:1376-1461) with the metastable-vapour
+// equation whenever p > p_sat(T), in the reference's operation order.
 template <class R> HD E2<R> vapour(R p, R t, double psat)
 {
     R pi = p * R(1.0e-6);
     R tau = R(540.0) / t;
-    R ipi = R(1.0e6) / p, itau = R(1) / tau;
-    R c = tau - R(0.5), ic = R(1) / c;
-    R s0j = 0, s0jj = 0;                // ideal part: sums of J g0, J(J-1) g0
-    R sj = 0, si = 0, sii = 0, sjj = 0, sij = 0;
+    R pi1 = R(1.0e6) / p;
+    R tau1 = R(1) / tau;
+    R tau2 = R(1) / (tau - R(0.5));
+    R g0t = 0, g0tt = 0;
+    R grt = 0, grp = 0, grpp = 0, grtt = 0, grpt = 0;
     const bool meta = (double)p > psat;
 #pragma unroll
     for (int k = 0; k < 9; k++) {
         const int J = R2J0[k];
         double n = R2N0[k];
         if (meta && k < 2) n = R2MN0[k];
-        R g = R(n) * (J >= 0 ? ipow(tau, J) : ipow(itau, -J));
-        s0j += R(J) * g;
-        s0jj += R(J * (J - 1)) * g;
+        R g0 = R(n) * powi(tau, J);
+        R g01 = R(J) * g0 * tau1;
+        R g02 = R(J - 1) * g01 * tau1;
+        g0t = g0t + g01;
+        g0tt = g0tt + g02;
     }
+    auto term = [&](const IJN c) {
+        R gr = R(c.n) * powi(pi, c.i) * powi(tau - R(0.5), c.j);
+        R gr1 = R(c.j) * gr * tau2;
+        R gr2 = R(c.i) * gr * pi1;
+        R gr3 = R(c.i - 1) * gr2 * pi1;
+        R gr4 = R(c.j - 1) * gr1 * tau2;
+        R gr5 = R(c.j) * gr2 * tau2;
+        grt = grt + gr1;
+        grp = grp + gr2;
+        grpp = grpp + gr3;
+        grtt = grtt + gr4;
+        grpt = grpt + gr5;
+    };
     if (meta) {
 #pragma unroll
-        for (int k = 0; k < 13; k++) {
-            const int I = R2MC[k].i, J = R2MC[k].j;
-            R g = R(R2MC[k].n) * ipow(pi, I) * ipow(c, J);
-            sj += R(J) * g; si += R(I) * g; sii += R(I * (I - 1)) * g; sjj += R(J * (J - 1)) * g; sij += R(I * J) * g;
-        }
+        for (int k = 0; k < 13; k++) term(R2MC[k]);
     } else {
 #pragma unroll
-        for (int k = 0; k < 43; k++) {
-            const int I = R2C[k].i, J = R2C[k].j;
-            R g = R(R2C[k].n) * ipow(pi, I) * ipow(c, J);
-            sj += R(J) * g; si += R(I) * g; sii += R(I * (I - 1)) * g; sjj += R(J * (J - 1)) * g; sij += R(I * J) * g;
-        }
+        for (int k = 0; k < 43; k++) term(R2C[k]);
     }
-    R g0t = s0j * itau, g0tt = s0jj * (itau * itau);
-    R grt = sj * ic, grp = si * ipi, grpp = sii * (ipi * ipi), grtt = sjj * (ic * ic), grpt = sij * (ipi * ic);
-    R gp1 = R(1) / (ipi + grp);
+    R gp1 = R(1) / (pi1 + grp);
+    R tau3 = tau * tau * tau;
     E2<R> e;
     e.rho = R(4.01245401527075799) * tau * gp1;
     e.h = R(249224.04) * (g0t + grt);
-    e.drdp = -((-(ipi * ipi) + grpp) * R(4.01245401527075799e-6) * tau * (gp1 * gp1));
-    e.drdt = -(e.rho * R(C_RT2A) * tau) + R(C_RT2B) * grpt * (tau * tau * tau) * (gp1 * gp1);
-    e.dhdp = R(0.24922404) * grpt;
+    e.drdp = -((-(pi1 * pi1) + grpp) * R(4.01245401527075799e-6) * tau * (gp1 * gp1));
+    e.drdt = -(e.rho * R(C_RT2A) * tau) + R(C_RT2B) * (R(0) + grpt) * tau3 * (gp1 * gp1);
+    e.dhdp = R(0.24922404) * (R(0) + grpt);
     e.dhdt = -(R(461.526) * (tau * tau) * (g0tt + grtt));
     return e;
 }
@@ -230,32 +253,31 @@ template <class R> HD E2<R> gas(R p, R t)
     return e;
 }
 
-// IAPS-85 mu, kappa of a water phase (no critical enhancement)
+// IAPS-85 mu, kappa of a water phase, no critical enhancement
+// (src/eos_module.f90:1508-1537), in the reference's operation order.
 template <class R> HD void iaps85(R t, R rho, R &mu, R &kap)
 {
     R tt = t * R(0.00154495032985);
     R rr = mx(R(1.0e-3), mn(R(1332.4), rho)) * R(0.00314699949333);
-    R t1 = R(647.27) / t;
-    R st = sqrt(tt);
-    R x = t1 - R(1), y = rr - R(1);
-    R s = 0;
+    R tt1 = R(647.27) / t;
+    R mu0 = sqrt(tt) / (R(TPH[0]) + R(TPH[1]) * tt1 + R(TPH[2]) * (tt1 * tt1) + R(TPH[3]) * (tt1 * tt1 * tt1));
+    R mu1 = 0;
 #pragma unroll
-    for (int i = 5; i >= 0; i--) {
-        R q = 0;
+    for (int i = 0; i <= 5; i++)
 #pragma unroll
-        for (int j = 6; j >= 0; j--) q = q * y + R(TPHH[i][j]);
-        s = s * x + q;
-    }
-    mu = R(55.071e-6) * (st / (R(TPH[0]) + t1 * (R(TPH[1]) + t1 * (R(TPH[2]) + t1 * R(TPH[3]))))) * exp(rr * s);
-    s = 0;
+        for (int j = 0; j <= 6; j++)
+            mu1 = mu1 + R(TPHH[i][j]) * powi(tt1 - R(1), i) * powi(rr - R(1), j);
+    mu1 = exp(rr * mu1);
+    mu = R(55.071e-6) * mu0 * mu1 * R(1);
+    R lam0 = sqrt(tt) / (R(TPL[0]) + R(TPL[1]) * tt1 + R(TPL[2]) * (tt1 * tt1) + R(TPL[3]) * (tt1 * tt1 * tt1));
+    R lam1 = 0;
 #pragma unroll
-    for (int i = 4; i >= 0; i--) {
-        R q = 0;
+    for (int i = 0; i <= 4; i++)
 #pragma unroll
-        for (int j = 5; j >= 0; j--) q = q * y + R(TPLL[i][j]);
-        s = s * x + q;
-    }
-    kap = R(0.4945) * (st / (R(TPL[0]) + t1 * (R(TPL[1]) + t1 * (R(TPL[2]) + t1 * R(TPL[3]))))) * exp(rr * s);
+        for (int j = 0; j <= 5; j++)
+            lam1 = lam1 + R(TPLL[i][j]) * powi(tt1 - R(1), i) * powi(rr - R(1), j);
+    lam1 = exp(rr * lam1);
+    kap = R(0.4945) * (lam0 * lam1 + R(0));
 }
 
 // Mixture of liquid, vapour and ideal gas (fluid = IF97, ngas = ideal).
@@ -551,6 +573,57 @@ template <class R> HD void ausm_flux(const Par &P, int npv, const R *n3, const F
     for (int k = 5; k < NPV; k++) if (k < npv) fx[k] = (ml * pl[k] + mr * pr[k]) * dl;
 }
 
+// ------------------------------------------------------------ one face
+// Reconstruction of every variable at a face from its 38-point stencil,
+// xs(ll, n) = stencil value at Fortran index ll of variable n
+// (src/rhs_module.f90:288-312), plus the normal / transverse differences of
+// u, v, w, T, k, omega the viscous flux uses (src/vsflux_module.f90:156-180).
+template <class R, class X>
+HD void face_recon(const Par &P, int npv, X xs, R *c9, R *c10, R *xl, R *xr, R g3[6][3])
+{
+    const bool visc = (P.iturb >= -2);
+    int vq = 0;
+    for (int n = 0; n < npv; n++) {
+        auto xn = [&](int ll) -> R { return xs(ll, n); };
+        c9[n] = xn(9);
+        c10[n] = xn(10);
+        if (P.nmuscl > 0) muscl_var<R>(P.nlim, P.nmuscl == 2, xn, xl[n], xr[n]);
+        else { xl[n] = c9[n]; xr[n] = c10[n]; }
+        if (visc && ((n >= 1 && n <= 4) || n >= 7)) {
+            g3[vq][0] = xn(10) - xn(9);
+            g3[vq][1] = R(0.0625) * (-xn(1) - xn(2) + xn(5) + xn(6) + R(2) * (-xn(7) - xn(8) + xn(11) + xn(12))
+                                     - xn(13) - xn(14) + xn(17) + xn(18));
+            g3[vq][2] = R(0.0625) * (-xn(1) - xn(2) + xn(13) + xn(14) + R(2) * (-xn(3) - xn(4) + xn(15) + xn(16))
+                                     - xn(5) - xn(6) + xn(17) + xn(18));
+            vq++;
+        }
+    }
+}
+
+// Inviscid flux of a face from the reconstructed states: clamps, face EOS,
+// Roe-averaged c^2 and the Roe / AUSM+-up flux (src/rhs_module.f90:314-337).
+// rh9/rh10 = {rho, h} of the two cells (first-order faces read the cells').
+template <class R>
+HD void inviscid_face(const Par &P, int npv, const R *n3, const R *c9, const R *c10, R *xl, R *xr,
+                      const R *rh9, const R *rh10, R *fx)
+{
+    FaceState<R> L, Rt;
+    if (P.nmuscl > 0) {
+        muscl_clamps<R>(npv, R(P.pref), c9, c10, xl, xr);
+        face_rho_h<R>(xl[0] + R(P.pref), xl[4], xl[5], xl[6], L.rho, L.h);
+        face_rho_h<R>(xr[0] + R(P.pref), xr[4], xr[5], xr[6], Rt.rho, Rt.h);
+    } else {
+        L.rho = rh9[0]; L.h = rh9[1];
+        Rt.rho = rh10[0]; Rt.h = rh10[1];
+    }
+    for (int n = 0; n < npv; n++) { L.p[n] = xl[n]; Rt.p[n] = xr[n]; }
+    R sl = sqrt(L.rho), sr = sqrt(Rt.rho), rd = R(1) / (sl + sr);
+    R ya1 = (sl * xl[5] + sr * xr[5]) * rd, ya2 = (sl * xl[6] + sr * xr[6]) * rd;
+    R c2a = face_c2<R>(R(0.5) * (xl[0] + xr[0]) + R(P.pref), R(0.5) * (xl[4] + xr[4]), ya1, ya2);
+    if (P.nscheme == 1) roe_flux<R>(P, npv, n3, L, Rt, c2a, fx);
+    else ausm_flux<R>(P, npv, n3, L, Rt, c2a, fx);
+}
+
 // ------------------------------------------------------------ Jacobian / getx
 // Cell data the split Jacobian and the diagonal inverse read.
 struct CellJ {
@@ -567,39 +640,60 @@ HD double eigenvis(const Par &P, const double *d, double mut)
     return mx(d[D_RP] * (d[D_KAP] + d[D_HT] * mut / 0.9) / den, 4.0 / 3.0 * (d[D_MU] + mut) / d[D_RHO]);
 }
 
-// y += sgn * A(cell, n3, sign) x, with A the split flux Jacobian of the
-// reference assembled implicitly (row 1 vector plus rank-one rows).
+// y = y + sgn * A x, A = the split flux Jacobian a(n,l) of the given sign
+// (src/jacobian_module.f90:312-402), accumulated row by row over l in the
+// reference's order (src/update_module.f90:507-544).  The entries that are
+// identically zero are skipped: adding a zero product leaves y unchanged.
 HD void jac_apply(const Par &P, int npv, const CellJ &c, const double *n3, int sign, const double *x, double *y, double sgn)
 {
     const double *pv = c.pv, *d = c.d;
+    const double cappa = 1.05;
     double ds = n3[0] * n3[0] + n3[1] * n3[1] + n3[2] * n3[2];
     double u = n3[0] * pv[1] + n3[1] * pv[2] + n3[2] * pv[3];
-    double v2 = pv[1] * pv[1] + pv[2] * pv[2] + pv[3] * pv[3];
-    double sp = sndp2(P, d[D_C2], v2);
-    double q = sp / d[D_C2], omq = 1.0 - q;
-    double up = 0.5 * (1.0 + q) * u;
+    double sp = sndp2(P, d[D_C2], pv[1] * pv[1] + pv[2] * pv[2] + pv[3] * pv[3]);
+    double up = 0.5 * (1.0 + sp / d[D_C2]) * u;
+    double omq = 1.0 - sp / d[D_C2];
     double dd = 0.5 * sqrt((u * u) * (omq * omq) + 4.0 * sp * ds);
-    double lam = (double)sign * 1.05 * (fabs(up) + dd);
+    double lam = (double)sign * cappa * (fabs(up) + dd);
     double vis = (double)sign * 2.0 * eigenvis(P, d, c.mut) * ds / c.vol;
     double beta = 1.0 / sp - 1.0 / d[D_C2] + d[D_RP];
-    double lv = lam + vis;
-    double r0 = u * d[D_RP] + lam * beta + vis * d[D_RP];
-    double r4 = (u + lv) * d[D_RT], r5 = (u + lv) * d[D_RY1], r6 = (u + lv) * d[D_RY2];
-    double s = r0 * x[0] + d[D_RHO] * (n3[0] * x[1] + n3[1] * x[2] + n3[2] * x[3]) + r4 * x[4] + r5 * x[5] + r6 * x[6];
-    double ru = d[D_RHO] * (u + lv);
-    double hh = d[D_H] + 0.5 * v2;
-    double w = 0.5 * sgn;
-    y[0] += w * s;
-    y[1] += w * (pv[1] * s + n3[0] * x[0] + ru * x[1]);
-    y[2] += w * (pv[2] * s + n3[1] * x[0] + ru * x[2]);
-    y[3] += w * (pv[3] * s + n3[2] * x[0] + ru * x[3]);
-    y[4] += w * (hh * s + ru * (d[D_HP] * x[0] + pv[1] * x[1] + pv[2] * x[2] + pv[3] * x[3] + d[D_HT] * x[4]
-                                + d[D_HY1] * x[5] + d[D_HY2] * x[6]) - lv * x[0]);
-    y[5] += w * (pv[5] * s + ru * x[5]);
-    y[6] += w * (pv[6] * s + ru * x[6]);
-    if (npv > 7) {
-        y[7] += w * (pv[7] * s + ru * x[7]);
-        y[8] += w * (pv[8] * s + ru * x[8]);
+    double uuu = u + lam + vis;
+    double h = d[D_H] + 0.5 * (pv[1] * pv[1] + pv[2] * pv[2] + pv[3] * pv[3]);
+    double r1[7];
+    r1[0] = u * d[D_RP] + lam * beta + vis * d[D_RP];
+    r1[1] = n3[0] * d[D_RHO];
+    r1[2] = n3[1] * d[D_RHO];
+    r1[3] = n3[2] * d[D_RHO];
+    r1[4] = u * d[D_RT] + lam * d[D_RT] + vis * d[D_RT];
+    r1[5] = u * d[D_RY1] + lam * d[D_RY1] + vis * d[D_RY1];
+    r1[6] = u * d[D_RY2] + lam * d[D_RY2] + vis * d[D_RY2];
+    const double ru = d[D_RHO] * uuu;
+    auto acc = [&](int n, double a, int l) {
+        double t = (0.5 * a) * x[l];
+        y[n] = (sgn > 0.0) ? y[n] + t : y[n] - t;
+    };
+    for (int l = 0; l < 7; l++) acc(0, r1[l], l);
+    for (int n = 1; n <= 3; n++) {
+        for (int l = 0; l < 7; l++) {
+            double a = pv[n] * r1[l];
+            if (l == 0) a = pv[n] * r1[0] + n3[n - 1];
+            if (l == n) a = pv[n] * r1[n] + ru;
+            acc(n, a, l);
+        }
+    }
+    const double hy[7] = {d[D_HP], pv[1], pv[2], pv[3], d[D_HT], d[D_HY1], d[D_HY2]};
+    for (int l = 0; l < 7; l++) {
+        double a = ru * hy[l] + h * r1[l];
+        if (l == 0) a = a - lam - vis;
+        acc(4, a, l);
+    }
+    for (int n = 5; n < npv; n++) {
+        for (int l = 0; l < 7; l++) {
+            double a = pv[n] * r1[l];
+            if (l == n) a = pv[n] * r1[n] + ru;
+            acc(n, a, l);
+        }
+        if (n >= 7) acc(n, uuu * d[D_RHO], n);
     }
 }
 
@@ -622,13 +716,15 @@ HD void getx(const Par &P, int npv, const CellJ &c, const double f6[6][3], doubl
         double up = 0.5 * (1.0 + q) * u;
         double e = 0.5 * sqrt((u * u) * (omq * omq) + 4.0 * sp * ds);
         eig += 1.05 * (fabs(up) + e);
-        msum += ds;
+        for (int m = 0; m < 3; m++) {   // src/lhs_module.f90:525-527: nine terms in sequence
+            double qq = f6[2 * dd][m] + f6[2 * dd + 1][m];
+            msum = (dd == 0 && m == 0) ? 0.25 * (qq * qq) : msum + 0.25 * (qq * qq);
+        }
     }
-    double ev = eigenvis(P, d, c.mut) * msum;
+    double ev = (P.iturb == -3) ? 0.0 : eigenvis(P, d, c.mut) * msum;
     double a = 1.0 / (c.vol / dt + eig);
     double b = ((double)P.nsteady * P.bdf_b * c.vol / P.dt_phy + 2.0 * ev / c.vol) * a;
-    double va = c.vol * a;
-    double c1 = cc[0] * va, c2 = cc[1] * va, c3 = cc[2] * va, c4 = cc[3] * va;
+    double c1 = cc[0] * c.vol * a, c2 = cc[1] * c.vol * a, c3 = cc[2] * c.vol * a, c4 = cc[3] * c.vol * a;
     double b1 = 1.0 + b;
     double rb = b1 * d[D_RHO];
     double i_b1 = 1.0 / rb;
@@ -658,7 +754,7 @@ HD void getx(const Par &P, int npv, const CellJ &c, const double f6[6][3], doubl
     x[5] = (ss * (res[5] - res[0] * pv[5]) - c4 * ss * i_b1 * r6 - c1 * co2 + c2 * i_b1 * co3) * mm * a;
     x[6] = r6 * i_b1 * a;
     if (npv > 7) {
-        double t1 = tt4[0] * va, t2 = tt4[1] * va, t3 = tt4[2] * va, t4 = tt4[3] * va;
+        double t1 = tt4[0] * c.vol * a, t2 = tt4[1] * c.vol * a, t3 = tt4[2] * c.vol * a, t4 = tt4[3] * c.vol * a;
         double it = 1.0 / (t2 * t3 - (t1 + rb) * (t4 + rb));
         double rk = res[7] - res[0] * pv[7], ro = res[8] - res[0] * pv[8];
         x[7] = (ro * t2 - (rb + t4) * rk) * it * a;

@@ -15,14 +15,14 @@ NAMES = ["lam_roe_tvd", "lam_roe_mlp", "lam_ausm_tvd_vanleer", "sst_cav_ausm_mlp
          "sst_cav_as_written"]
 
 
-def _case(pk, name):
+def _case(pkg, name):
     print(f"case {name}", flush=True)
-    return small_cases(pk)[name]()
+    return small_cases(pkg)[name]()
 
 
-def test_device_eos_matches_oracle(pk, orc):
-    case = pk.cases.config1(8, 6, 6)
-    s = pk.Solver(case)
+def test_device_eos_matches_oracle(pkg, orc):
+    case = pkg.cases.config1(8, 6, 6)
+    s = pkg.Solver(case)
     rng = np.random.default_rng(7)
     n = 4000
     p = rng.uniform(2e3, 5e6, n)
@@ -39,9 +39,9 @@ def test_device_eos_matches_oracle(pk, orc):
 
 
 @pytest.mark.parametrize("name", NAMES)
-def test_setbc_fills_ghosts_like_oracle(pk, orc, name):
-    case = _case(pk, name)
-    s, o = pair(pk, orc, case)
+def test_setbc_fills_ghosts_like_oracle(pkg, orc, name):
+    case = _case(pkg, name)
+    s, o = pair(pkg, orc, case)
     s.setbc()
     o.setbc()
     npv = case.npv
@@ -65,9 +65,9 @@ def test_setbc_fills_ghosts_like_oracle(pk, orc, name):
 
 
 @pytest.mark.parametrize("name", NAMES)
-def test_calrhs_and_timestep_match_oracle(pk, orc, name):
-    case = _case(pk, name)
-    s, o = pair(pk, orc, case)
+def test_calrhs_and_timestep_match_oracle(pkg, orc, name):
+    case = _case(pkg, name)
+    s, o = pair(pkg, orc, case)
     s.setbc()
     o.setbc()
     for ib in range(len(case.blocks)):
@@ -91,9 +91,9 @@ def test_calrhs_and_timestep_match_oracle(pk, orc, name):
 
 
 @pytest.mark.parametrize("name", NAMES)
-def test_timeinteg_matches_oracle(pk, orc, name):
-    case = _case(pk, name)
-    s, o = pair(pk, orc, case)
+def test_timeinteg_matches_oracle(pkg, orc, name):
+    case = _case(pkg, name)
+    s, o = pair(pkg, orc, case)
     npv = case.npv
     nit = 1 if name == "sst_cav_as_written" else 3
     for it in range(nit):

@@ -1,0 +1,227 @@
+"""The library's pointwise physics (3dcode_amd/csrc/phys.hpp, built for the
+host with contraction off) against the oracle on identical inputs.
+
+Every routine is required to agree bit for bit: the kernels follow the
+reference's operation order, so the only device-side differences left are
+the device math library's exp/pow/tanh roundings."""
+import ctypes as C
+import importlib
+
+import numpy as np
+import pytest
+
+import physhost as H
+
+D = C.POINTER(C.c_double)
+PP = C.POINTER(D)
+
+
+def dp(a):
+    return a.ctypes.data_as(D)
+
+
+def ptrs(arrs):
+    return (D * len(arrs))(*[a.ctypes.data_as(D) for a in arrs])
+
+
+@pytest.fixture(scope="module")
+def L(tmp_path_factory):
+    return H.build(str(tmp_path_factory.mktemp("physcheck")))
+
+
+@pytest.fixture(scope="module")
+def OL(orc):
+    L = orc.lib()
+    L.orc_tvd.argtypes = [C.c_void_p, D, D, D]
+    L.orc_mlp.argtypes = [C.c_void_p, D, D, D, C.c_void_p]
+    L.orc_roe.argtypes = [C.c_void_p, D, D, D, D, D, D]
+    L.orc_ausmpup.argtypes = [C.c_void_p, D, D, D, D, D, D, C.c_void_p]
+    L.orc_vsflux.argtypes = [C.c_void_p, D, PP, PP, D, D, D, D, D, D, D, D]
+    L.orc_localdt.argtypes = [C.c_void_p, PP, D, D, D, D]
+    L.orc_localdt.restype = C.c_double
+    L.orc_getx.argtypes = [C.c_void_p, PP, D, D, D, D, C.c_double, D, D, D, D]
+    L.orc_caljac.argtypes = [C.c_void_p, D, D, D, D, D, C.c_int, D]
+    L.orc_merkle.argtypes = [C.c_void_p, D, D, D, D, D, D]
+    L.orc_kwsst_source.argtypes = [PP, D, D, D, D, D, D, D]
+    L.orc_kwsst_eddy.argtypes = [PP, D, D, D, D]
+    L.orc_kwsst_eddy.restype = C.c_double
+    return L
+
+
+def cfg_struct(orc, cfg):
+    cf = orc.Cfg()
+    for k, v in cfg.items():
+        if k == "bc_pv":
+            cf.bc_pv = (C.c_double * 9)(*v)
+        else:
+            setattr(cf, k, v)
+    return cf
+
+
+def test_phase_eos_bit_exact(L, orc):
+    rng = np.random.default_rng(1)
+    for _ in range(1000):
+        p, t = rng.uniform(2e3, 5e6), rng.uniform(275.0, 500.0)
+        for fn, ref in ((L.pc_liquid, orc.iapws97_l), (L.pc_vapour, orc.iapws97_v)):
+            o = np.zeros(6)
+            fn(p, t, dp(o))
+            e = ref(p, t)
+            assert np.array_equal(o, [e.rho, e.h, e.drdp, e.drdt, e.dhdp, e.dhdt]), (fn, p, t)
+
+
+def test_mixture_eos_bit_exact(L, orc):
+    rng = np.random.default_rng(2)
+    for _ in range(1000):
+        p, t = rng.uniform(2e3, 5e6), rng.uniform(275.0, 500.0)
+        y1 = rng.choice([0.0, rng.uniform(0, 1e-3), rng.uniform(0, 1)])
+        y2 = rng.choice([0.0, rng.uniform(0, 1 - y1)])
+        full, tv = np.zeros(18), np.zeros(2)
+        L.pc_mixture(p, t, y1, y2, dp(full), dp(tv))
+        dref, tref = orc.deteos_prop(p, t, y1, y2)
+        assert np.array_equal(full, dref) and np.array_equal(tv, tref)
+
+
+def _stencil(rng, P4, npv, amp):
+    k, j, i = rng.integers(3, P4.shape[0] - 3), rng.integers(3, P4.shape[1] - 3), rng.integers(4, P4.shape[2] - 4)
+    rows = [P4[k + kk, j + jj, i + ii] for kk in (-1, 0, 1) for jj in (-1, 0, 1) for ii in (0, 1)]
+    rows += [P4[k + kk, j + jj, i - 1] for kk in (-1, 0, 1) for jj in (-1, 0, 1)]
+    rows += [P4[k + kk, j + jj, i + 2] for kk in (-1, 0, 1) for jj in (-1, 0, 1)]
+    rows += [P4[k, j, i - 2], P4[k, j, i + 3]]
+    x = np.zeros((38, 9))
+    x[:, :npv] = np.array(rows) * (1 + amp * rng.standard_normal((38, npv)))
+    return x
+
+
+SCHEMES = [(1, 0, 0), (1, 1, 1), (1, 2, 8), (4, 1, 2), (4, 2, 8), (4, 1, 5), (1, 2, 3)]
+
+
+@pytest.mark.parametrize("cfgname", ["config1", "config2"])
+@pytest.mark.parametrize("nscheme,nmuscl,nlim", SCHEMES)
+def test_inviscid_face_bit_exact(L, OL, orc, pkg, cfgname, nscheme, nmuscl, nlim):
+    """Reconstruction + clamps + face EOS + Roe/AUSM+-up (src/rhs_module.f90:314-337)."""
+    case = getattr(pkg.cases, cfgname)(10, 8, 8)
+    case.params.update(nscheme=nscheme, nmuscl=nmuscl, nlim=nlim)
+    cfg = case.cfg_dict()
+    iv, dvp = H.par_arrays(cfg)
+    cf = cfg_struct(orc, cfg)
+    b = case.blocks[0]
+    P4 = case.init[0][0].reshape(b.kmax + 5, b.jmax + 5, b.imax + 5, case.npv)
+    rng = np.random.default_rng(nscheme * 100 + nmuscl * 10 + nlim)
+    for trial in range(150):
+        x = _stencil(rng, P4, case.npv, 1e-2 if trial % 2 else 1e-6)
+        n3 = rng.standard_normal(3) * 1e-3
+        d9 = orc.deteos_simple(x[8, 0] + cfg["pref"], x[8, 4], x[8, 5], x[8, 6])
+        d10 = orc.deteos_simple(x[9, 0] + cfg["pref"], x[9, 4], x[9, 5], x[9, 6])
+        fx, xl, xr, g3 = np.zeros(9), np.zeros(9), np.zeros(9), np.zeros(18)
+        L.pc_face(H.p(iv, C.c_int), dp(dvp), H.p(x), dp(n3), H.p(d9[:2]), H.p(d10[:2]), dp(fx), dp(xl), dp(xr), dp(g3))
+        ol, orr = np.zeros(9), np.zeros(9)
+        if nmuscl == 1:
+            OL.orc_tvd(C.byref(cf), dp(x), dp(ol), dp(orr))
+        elif nmuscl == 2:
+            OL.orc_mlp(C.byref(cf), dp(x), dp(ol), dp(orr), None)
+        else:
+            ol[:], orr[:] = x[8], x[9]
+        if nmuscl:
+            dl = orc.deteos_simple(ol[0] + cfg["pref"], ol[4], ol[5], ol[6])
+            dr = orc.deteos_simple(orr[0] + cfg["pref"], orr[4], orr[5], orr[6])
+        else:
+            dl, dr = d9, d10
+        fo = np.zeros(9)
+        if nscheme == 1:
+            OL.orc_roe(C.byref(cf), dp(n3), dp(ol), dp(orr), dp(dl), dp(dr), dp(fo))
+        else:
+            OL.orc_ausmpup(C.byref(cf), dp(n3), dp(ol), dp(orr), dp(dl), dp(dr), dp(fo), None)
+        n = case.npv
+        assert np.array_equal(xl[:n], ol[:n]) and np.array_equal(xr[:n], orr[:n]), trial
+        assert np.array_equal(fx[:n], fo[:n]), (trial, fx[:n], fo[:n])
+
+
+@pytest.mark.parametrize("turb", [True, False])
+def test_viscous_face_bit_exact(L, OL, orc, pkg, turb):
+    case = pkg.cases.config2(8, 6, 6) if turb else pkg.cases.config1(8, 6, 6)
+    cfg = case.cfg_dict()
+    iv, dvp = H.par_arrays(cfg)
+    cf = cfg_struct(orc, cfg)
+    rng = np.random.default_rng(5)
+    for _ in range(200):
+        base = np.array([rng.uniform(-3e4, 3e4), rng.uniform(-15, 15), rng.uniform(-3, 3), rng.uniform(-3, 3),
+                         rng.uniform(290, 310), 0.0, 0.0, 0.01, 1000.0])
+        x = np.ascontiguousarray(np.tile(base, (38, 1)) * (1 + 1e-2 * rng.standard_normal((38, 9))))
+        x[:, 5] = rng.choice([0.0, 1e-4])
+        x[:, 6] = 0.0
+        n3 = rng.standard_normal(3) * 1e-4
+        e4 = [rng.standard_normal(3) * 1e-4 for _ in range(4)]
+        t4 = [rng.standard_normal(3) * 1e-4 for _ in range(4)]
+        grdl = np.array([rng.uniform(1e-9, 1e-6), 0, 0, 0, rng.uniform(1e-4, 1e-2)])
+        grdr = np.array([rng.uniform(1e-9, 1e-6), 0, 0, 0, rng.uniform(1e-4, 1e-2)])
+        dl, tl2 = orc.deteos_prop(x[8, 0] + cfg["pref"], x[8, 4], x[8, 5], x[8, 6])
+        dr, tr2 = orc.deteos_prop(x[9, 0] + cfg["pref"], x[9, 4], x[9, 5], x[9, 6])
+        ml, mr = rng.uniform(1e-5, 1e-2, 2)
+        tvl, tvr = np.array([tl2[0], tl2[1], ml]), np.array([tr2[0], tr2[1], mr])
+        fo, fg = np.zeros(9), np.zeros(9)
+        OL.orc_vsflux(C.byref(cf), dp(n3), ptrs(e4), ptrs(t4), dp(grdl), dp(grdr), dp(x), dp(dl), dp(dr),
+                      dp(tvl), dp(tvr), dp(fo))
+        L.pc_visc(H.p(iv, C.c_int), dp(dvp), dp(x), dp(n3), H.p(np.concatenate(e4)), H.p(np.concatenate(t4)),
+                  H.p(np.array([grdl[0], grdl[4]])), H.p(np.array([grdr[0], grdr[4]])),
+                  H.p(H.compact(dl, tl2)), H.p(H.compact(dr, tr2)), ml if turb else 0.0, mr if turb else 0.0, dp(fg))
+        assert np.array_equal(fg[:case.npv], fo[:case.npv])
+
+
+@pytest.mark.parametrize("clip,nsteady", [(0, 0), (1, 0), (1, 2), (0, 1)])
+def test_cell_routines_bit_exact(L, OL, orc, pkg, clip, nsteady):
+    """local dt, split Jacobian product, getx, Merkle, SST source and eddy viscosity."""
+    case = pkg.cases.config2(8, 6, 6)
+    case.params.update(cav_clip=clip, nsteady=nsteady)
+    cfg = case.cfg_dict()
+    iv, dvp = H.par_arrays(cfg)
+    cf = cfg_struct(orc, cfg)
+    ivp = H.p(iv, C.c_int)
+    rng = np.random.default_rng(10 + clip + 3 * nsteady)
+    for _ in range(200):
+        pv = np.array([rng.uniform(-3e4, 3e4), rng.uniform(-15, 15), rng.uniform(-3, 3), rng.uniform(-3, 3),
+                       rng.uniform(290, 310), rng.choice([0.0, rng.uniform(0, 1e-3), rng.uniform(0, 0.5)]), 0.0,
+                       rng.uniform(1e-4, 0.1), rng.uniform(10, 1e4)])
+        dvv, tv2 = orc.deteos_prop(pv[0] + cfg["pref"], pv[4], pv[5], pv[6])
+        mut = rng.uniform(1e-5, 1e-2)
+        tv = np.array([tv2[0], tv2[1], mut])
+        d13 = H.compact(dvv, tv2)
+        vol = rng.uniform(1e-9, 1e-6)
+        grd = np.array([vol, 0.1, 0.05, 0.02, rng.uniform(1e-4, 1e-2)])
+        f6 = [rng.standard_normal(3) * 1e-4 for _ in range(6)]
+        f6a = np.concatenate(f6)
+        dt = rng.uniform(1e-5, 1e-2)
+        assert L.pc_localdt(ivp, dp(dvp), dp(pv), dp(d13), mut, vol, dp(f6a)) == \
+            OL.orc_localdt(C.byref(cf), ptrs(f6), dp(grd), dp(pv), dp(dvv), dp(tv))
+        cc = rng.standard_normal(4) * 1e5
+        tt = rng.standard_normal(4) * 1e2
+        res = rng.standard_normal(9)
+        xg, xo = np.zeros(9), np.zeros(9)
+        L.pc_getx(ivp, dp(dvp), dp(pv), dp(d13), mut, vol, dp(f6a), dt, dp(cc), dp(tt), dp(res), dp(xg))
+        OL.orc_getx(C.byref(cf), ptrs(f6), dp(grd), dp(pv), dp(dvv), dp(tv), dt, dp(cc), dp(tt), dp(res), dp(xo))
+        assert np.array_equal(xg, xo)
+        n3 = rng.standard_normal(3) * 1e-4
+        for sg in (1, -1):
+            x = rng.standard_normal(9)
+            y = np.zeros(9)
+            L.pc_jac(ivp, dp(dvp), dp(pv), dp(d13), mut, vol, dp(n3), sg, dp(x), dp(y))
+            A = np.zeros(81)
+            OL.orc_caljac(C.byref(cf), dp(n3), dp(grd), dp(pv), dp(dvv), dp(tv), sg, dp(A))
+            Am = A.reshape(9, 9).T
+            yo = np.zeros(9)
+            for n in range(9):            # the sweep's sequential row sums
+                for l in range(9):
+                    yo[n] = yo[n] + Am[n, l] * x[l]
+            assert np.array_equal(y, yo)
+        x7 = np.ascontiguousarray(np.tile(pv, (7, 1)) * (1 + 1e-2 * rng.standard_normal((7, 9))))
+        mo = np.zeros(6)
+        L.pc_merkle(ivp, dp(dvp), H.p(x7[1]), dp(d13), vol, dp(mo))
+        rc, rv, ic = C.c_double(), C.c_double(), np.zeros(4)
+        OL.orc_merkle(C.byref(cf), dp(x7), dp(dvv), dp(grd), C.byref(rc), C.byref(rv), dp(ic))
+        assert np.array_equal(mo, [rc.value, rv.value, *ic])
+        so = np.zeros(7)
+        L.pc_sst_source(dp(x7), dp(f6a), vol, grd[4], dp(d13), mut, dp(so))
+        src, itt, oc = np.zeros(2), np.zeros(4), C.c_double()
+        OL.orc_kwsst_source(ptrs(f6), dp(grd), dp(x7), dp(dvv), dp(tv), dp(src), dp(itt), C.byref(oc))
+        assert np.array_equal(so, [*src, *itt, oc.value])
+        assert L.pc_sst_eddy(dp(x7), dp(f6a), vol, grd[4], dp(d13)) == \
+            OL.orc_kwsst_eddy(ptrs(f6), dp(grd), dp(x7), dp(dvv), dp(tv))
