@@ -44,7 +44,7 @@ int fail(const std::string &m, int code = -1) { g_err = m; return code; }
 
 struct BlkD {
     int I, J, K, nx, ny;
-    long g0, c0, w0, f0[3], cx0, ex0, tx0;
+    long g0, c0, w0, f0[3], cx0, ex0, tx0, p0;   // p0: first (j,k) slot of the block's sweep plane
     long ng, nc, nw, nf[3], ncx, nex, ntx;
 };
 
@@ -66,6 +66,10 @@ struct DS {           // device state, passed by value
     double *pv, *dc, *mut, *vol, *ydn, *cx, *ex, *tx;
     double *res, *dt, *icav, *itt, *ocut, *rcav, *qq;
     double *dqf, *dqb, *dcv;
+    double *gx;       // diagonal-inverse terms [NGX][NC]
+    double *jc;       // Jacobian cell terms [2][NC]: beta, h
+    double *jf;       // Jacobian face terms [18][NC]: faces -i,+i,-j,+j,-k,+k x (u, lam, vis)
+    long NPLANE;      // sum over blocks of (J-1)(K-1)
     double *ff[3];    // face fluxes [2*npv][NF]
     double *red;      // norm partials
     double *scal;     // [0] dt min bits, [1] accumulated time, [2..] norms
@@ -623,52 +627,188 @@ __global__ void k_combine(DS S)
 
 // ------------------------------------------------------------ LU-SGS sweeps
 // One hyperplane i+j+k = s of block b.  Thread -> (j,k); i = s - j - k.
-template <bool FWD>
-__global__ __launch_bounds__(64) void k_sweep(DS S, int b, int s)
+// Per-cell terms of the LU-SGS operator that do not depend on the sweep:
+// the diagonal-inverse coefficients and the Jacobian's cell and face terms
+// (src/lhs_module.f90:346-457, src/jacobian_module.f90:182-403).
+__global__ __launch_bounds__(128) void k_lhs_prep(DS S)
 {
-    const BlkD &B = S.b[b];
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= S.NC) return;
+    const BlkD &B = S.b[find_blk(S, t, 0)];
     const Par &P = S.P;
-    const int npv = S.npv;
-    int nj = B.J - 1;
-    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    int j = 2 + (int)(q % nj), k = 2 + (int)(q / nj);
-    if (k > B.K) return;
-    int i = s - j - k;
-    if (i < 2 || i > B.I) return;
-    long c = cidx(B, i, j, k);
-    long w = widx(B, i, j, k);
-    const double *dq = FWD ? S.dqf : S.dqb;
-    double y[NPV];
-    if (FWD) for (int n = 0; n < npv; n++) y[n] = S.res[n * S.NC + c];
-    else for (int n = 0; n < npv; n++) y[n] = S.dcv[n * S.NW + w];
-    const int sg = FWD ? 1 : -1;
-    for (int ax = 0; ax < 3; ax++) {
-        int a[3] = {i, j, k};
-        a[ax] -= sg;
-        const int lim = (ax == 0) ? B.I : (ax == 1 ? B.J : B.K);
-        if (a[ax] < 2 || a[ax] > lim) continue;       // ghost neighbour: its dq is zero
-        CellJ nb;
-        load_cell(S, B, a[0], a[1], a[2], nb);
-        double n3[3];
-        if (FWD) face_vec(S, B, ax, a[0], a[1], a[2], n3);
-        else face_vec(S, B, ax, i, j, k, n3);
-        long wn = widx(B, a[0], a[1], a[2]);
-        double x[NPV];
-        for (int n = 0; n < npv; n++) x[n] = dq[n * S.NW + wn];
-        jac_apply(P, npv, nb, n3, sg, x, y, FWD ? 1.0 : -1.0);
-    }
-    if (FWD) for (int n = 0; n < npv; n++) S.dcv[n * S.NW + w] = y[n];
+    int i, j, k;
+    cell_of(B, t - B.c0, i, j, k);
     CellJ cc;
     load_cell(S, B, i, j, k, cc);
     double f6[6][3];
     cell_faces(S, B, i, j, k, f6);
     double ic[4] = {0, 0, 0, 0}, it[4] = {0, 0, 0, 0};
-    if (P.ncav) for (int m = 0; m < 4; m++) ic[m] = S.icav[m * S.NC + c];
-    if (P.iturb == 0) for (int m = 0; m < 4; m++) it[m] = S.itt[m * S.NC + c];
-    double x[NPV];
-    getx(P, npv, cc, f6, S.dt[c], ic, it, y, x);
-    double *out = FWD ? S.dqf : S.dqb;
-    for (int n = 0; n < npv; n++) out[n * S.NW + w] = x[n];
+    if (P.ncav) for (int m = 0; m < 4; m++) ic[m] = S.icav[m * S.NC + t];
+    if (P.iturb == 0) for (int m = 0; m < 4; m++) it[m] = S.itt[m * S.NC + t];
+    double g[NGX];
+    getx_prep(P, S.npv, cc, f6, S.dt[t], ic, it, g);
+    for (int q = 0; q < NGX; q++) S.gx[q * S.NC + t] = g[q];
+    JacCell jc = jac_cell(P, cc);
+    S.jc[t] = jc.beta;
+    S.jc[S.NC + t] = jc.h;
+    for (int q = 0; q < 6; q++) {
+        JacFace f = jac_face(P, cc, f6[q]);
+        S.jf[(3 * q) * S.NC + t] = f.u;
+        S.jf[(3 * q + 1) * S.NC + t] = f.lam;
+        S.jf[(3 * q + 2) * S.NC + t] = f.vis;
+    }
+}
+
+// ---------------------------------------------------------------- LU-SGS sweeps
+// The forward (backward) sweep, src/update_module.f90:486-670:
+//   dcv = res + A+(i-1) dq(i-1) + A+(j-1) dq(j-1) + A+(k-1) dq(k-1),  dq = D^-1 dcv
+//   (dcv - A-(i+1) dq(i+1) - ... backward), ghost dq = 0.
+// Work is split into pencils along i: a 16-lane group owns pencil (j,k) and
+// walks i in sweep order, keeping its previous cell's dq in registers.  The
+// two cross-pencil inputs dq(i, j-1, k) and dq(i, j, k-1) are published by
+// their owners with device-coherent stores and read with device-coherent
+// loads; an entry still holding the "not yet written" bit pattern is polled
+// again.  Lane n accumulates row n of each Jacobian product in the
+// reference's order, then the rows are exchanged and lane n evaluates
+// component n of D^-1.
+//
+// Progress: the four pencils of a wavefront lie on one diagonal j+k = t and
+// depend only on pencils of diagonal t-1, which other wavefronts own; each
+// wavefront takes its pencils in increasing t, and the cooperative launch
+// keeps every wavefront resident, so the dependency chain always advances.
+// Every poll is bounded: past the limit the kernel records an error and
+// stops waiting, so each wavefront finishes.
+constexpr unsigned long long DQ_UNSET = 0x7FF4A5A5A5A5A5A5ull;   // a signalling NaN no arithmetic produces
+constexpr long POLL_LIMIT = 1L << 19;
+
+__device__ __forceinline__ double ld_dev(const double *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_dev(double *p, double v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// dq of cell wm, waiting (bounded) until its owner has written it
+__device__ __forceinline__ void wait_dq(const double *dq, long NW, long wm, int npv, double *x, int *err)
+{
+    long polls = 0;
+    bool ready;
+    do {
+        ready = true;
+        for (int q = 0; q < npv; q++) {
+            x[q] = ld_dev(dq + q * NW + wm);
+            if ((unsigned long long)__double_as_longlong(x[q]) == DQ_UNSET) ready = false;
+        }
+        if (!ready) {
+            if (++polls > POLL_LIMIT || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                atomicOr(err, 1);
+                for (int q = 0; q < npv; q++) x[q] = 0.0;
+                return;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+    } while (!ready);
+}
+
+struct NbData {               // what the Jacobian of one neighbour reads
+    double pv[NPV], d[ND], n3[3];
+    JacCell jc;
+    JacFace jf;
+};
+
+__device__ __forceinline__ void load_nb(const DS &S, const BlkD &B, int ax, int face_side, const int *m,
+                                        const int *fpos, NbData &nb)
+{
+    const long gm = gidx(B, m[0], m[1], m[2]), cm = cidx(B, m[0], m[1], m[2]);
+    for (int q = 0; q < S.npv; q++) nb.pv[q] = S.pv[q * S.NG + gm];
+    for (int q = 0; q < ND; q++) nb.d[q] = S.dc[q * S.NG + gm];
+    face_vec(S, B, ax, fpos[0], fpos[1], fpos[2], nb.n3);
+    const int fq = 2 * ax + face_side;
+    nb.jc = JacCell{S.jc[cm], S.jc[S.NC + cm]};
+    nb.jf = JacFace{S.jf[(3 * fq) * S.NC + cm], S.jf[(3 * fq + 1) * S.NC + cm], S.jf[(3 * fq + 2) * S.NC + cm]};
+}
+
+template <bool FWD>
+__device__ void sweep_pencil(const DS &S, const BlkD &B, int j, int k, int n, int *err)
+{
+    const int npv = S.npv;
+    const int sg = FWD ? 1 : -1;
+    const double sgn = FWD ? 1.0 : -1.0;
+    double *dq = FWD ? S.dqf : S.dqb;
+    const int side = FWD ? 1 : 0;                    // the neighbour's face shared with this cell
+    const int i0 = FWD ? 2 : B.I, i1 = FWD ? B.I : 2;
+    NbData prev;                                     // the previous cell of this pencil
+    double xprev = 0.0;
+    for (int i = i0;; i += sg) {
+        const long c = cidx(B, i, j, k), w = widx(B, i, j, k), g = gidx(B, i, j, k);
+        // static data of the j and k neighbours first (independent of the polls)
+        NbData nbj, nbk;
+        const int mj[3] = {i, j - sg, k}, mk[3] = {i, j, k - sg};
+        const bool hj = (mj[1] >= 2 && mj[1] <= B.J), hk = (mk[2] >= 2 && mk[2] <= B.K);
+        const int self[3] = {i, j, k};
+        if (hj) load_nb(S, B, 1, side, mj, FWD ? mj : self, nbj);
+        if (hk) load_nb(S, B, 2, side, mk, FWD ? mk : self, nbk);
+        double y = 0.0;
+        if (n < npv) y = FWD ? S.res[n * S.NC + c] : S.dcv[n * S.NW + w];
+        if (i != i0) {
+            double xp[NPV];
+#pragma unroll
+            for (int q = 0; q < NPV; q++) xp[q] = __shfl(xprev, q, 16);
+            jac_row(npv, prev.pv, prev.d, prev.jc, prev.jf, prev.n3, sg, n, xp, y, sgn);
+        }
+        if (hj) {
+            double xj[NPV];
+            wait_dq(dq, S.NW, widx(B, mj[0], mj[1], mj[2]), npv, xj, err);
+            jac_row(npv, nbj.pv, nbj.d, nbj.jc, nbj.jf, nbj.n3, sg, n, xj, y, sgn);
+        }
+        if (hk) {
+            double xk[NPV];
+            wait_dq(dq, S.NW, widx(B, mk[0], mk[1], mk[2]), npv, xk, err);
+            jac_row(npv, nbk.pv, nbk.d, nbk.jc, nbk.jf, nbk.n3, sg, n, xk, y, sgn);
+        }
+        if (FWD && n < npv) S.dcv[n * S.NW + w] = y;
+        double yy[NPV];
+#pragma unroll
+        for (int q = 0; q < NPV; q++) yy[q] = __shfl(y, q, 16);
+        // this cell: data for D^-1 now and for the next cell's i-neighbour term
+        for (int q = 0; q < npv; q++) prev.pv[q] = S.pv[q * S.NG + g];
+        for (int q = 0; q < ND; q++) prev.d[q] = S.dc[q * S.NG + g];
+        const int fq = side;                         // face along i shared with the next cell
+        prev.jc = JacCell{S.jc[c], S.jc[S.NC + c]};
+        prev.jf = JacFace{S.jf[(3 * fq) * S.NC + c], S.jf[(3 * fq + 1) * S.NC + c], S.jf[(3 * fq + 2) * S.NC + c]};
+        face_vec(S, B, 0, FWD ? i : i - 1, j, k, prev.n3);
+        double xn = 0.0;
+        if (n < npv) {
+            const double *gxc = S.gx + c;
+            const long NC = S.NC;
+            xn = getx_apply([&](int q) { return gxc[q * NC]; }, prev.pv, prev.d[D_RY1], prev.d[D_RT], yy, n);
+            st_dev(dq + n * S.NW + w, xn);
+        }
+        xprev = xn;
+        if (i == i1) break;
+    }
+}
+
+// slots: four (block, j, k) entries per wavefront-quad, all on one diagonal
+// j+k (block = -1 pads); quads sorted by diagonal.
+template <bool FWD>
+__global__ __launch_bounds__(256) void k_sweep(DS S, const int *slots, long nquad, int *err)
+{
+    const int lane = threadIdx.x & 15, grp = (threadIdx.x >> 4) & 3;
+    const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const long nwave = ((long)gridDim.x * blockDim.x) >> 6;
+    for (long qd = wave; qd < nquad; qd += nwave) {
+        const int *sl = slots + 3 * (4 * qd + grp);
+        if (sl[0] >= 0) sweep_pencil<FWD>(S, S.b[sl[0]], sl[1], sl[2], lane, err);
+    }
+}
+
+__global__ void k_fill_bits(double *p, long n, unsigned long long bits)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = __longlong_as_double((long long)bits);
 }
 
 // ------------------------------------------------------------ update + norms
@@ -880,6 +1020,10 @@ struct cav3d_ctx {
     double *stage = nullptr;
     size_t stage_n = 0;
     int nwg_update = 0;
+    int coop_blocks = 0;      // grid of the cooperative sweep kernels
+    int *slots = nullptr;     // pencil table of the sweeps (device)
+    long nquad = 0;
+    int *sweep_err = nullptr; // set when a sweep wait gave up
     bool profile = false;
     hipEvent_t ev[8] = {};
     double prof_ms[6] = {};
@@ -1083,15 +1227,19 @@ int launch_timestep(cav3d_ctx *c)
 int launch_sweeps(cav3d_ctx *c)
 {
     DS &S = c->S;
-    for (int b = 0; b < c->nb; b++) {
-        const BlkD &B = c->blk[b];
-        long nt = (long)(B.J - 1) * (B.K - 1);
-        unsigned g = nblk(nt, 64);
-        for (int s = 6; s <= B.I + B.J + B.K; s++)
-            hipLaunchKernelGGL(k_sweep<true>, dim3(g), dim3(64), 0, c->st, S, b, s);
-        for (int s = B.I + B.J + B.K; s >= 6; s--)
-            hipLaunchKernelGGL(k_sweep<false>, dim3(g), dim3(64), 0, c->st, S, b, s);
-    }
+    hipLaunchKernelGGL(k_lhs_prep, dim3(nblk(S.NC, 128)), dim3(128), 0, c->st, S);
+    const long nq = (long)S.npv * S.NW;
+    hipLaunchKernelGGL(k_fill_bits, dim3(nblk(nq, 256)), dim3(256), 0, c->st, S.dqf, nq, DQ_UNSET);
+    hipLaunchKernelGGL(k_fill_bits, dim3(nblk(nq, 256)), dim3(256), 0, c->st, S.dqb, nq, DQ_UNSET);
+    const int *slots = c->slots;
+    long nquad = c->nquad;
+    int *err = c->sweep_err;
+    void *args[4] = {&S, &slots, &nquad, &err};
+    dim3 grid(c->coop_blocks), block(256);
+    hipError_t e = hipLaunchCooperativeKernel((const void *)k_sweep<true>, grid, block, args, 0, c->st);
+    if (e != hipSuccess) return fail(std::string("forward sweep launch: ") + hipGetErrorString(e), -3);
+    e = hipLaunchCooperativeKernel((const void *)k_sweep<false>, grid, block, args, 0, c->st);
+    if (e != hipSuccess) return fail(std::string("backward sweep launch: ") + hipGetErrorString(e), -3);
     return 0;
 }
 
@@ -1101,6 +1249,15 @@ int launch_update(cav3d_ctx *c)
     hipLaunchKernelGGL(k_update, dim3(c->nwg_update), dim3(256), 0, c->st, S);
     hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(256), 0, c->st, S, c->nwg_update);
     if (c->prm.iturb == 0) hipLaunchKernelGGL(k_eddy, dim3(nblk(S.NC, 128)), dim3(128), 0, c->st, S);
+    return 0;
+}
+
+// a sweep whose bounded wait gave up leaves a flag; report it (stream synchronised by the caller)
+int sweep_status(cav3d_ctx *c)
+{
+    int e = 0;
+    HIPCHK(hipMemcpy(&e, c->sweep_err, sizeof(int), hipMemcpyDeviceToHost));
+    if (e) return fail("LU-SGS sweep: a dependency wait reached its limit; results of that iteration are invalid", -5);
     return 0;
 }
 
@@ -1157,7 +1314,7 @@ int cav3d_create(const cav3d_params *prm, int nblocks, const cav3d_block *blocks
     S.nb = nblocks;
     S.npv = c->npv;
     S.P = P;
-    long g0 = 0, c0 = 0, w0 = 0, f0[3] = {0, 0, 0}, x0 = 0, e0 = 0, t0 = 0;
+    long g0 = 0, c0 = 0, w0 = 0, f0[3] = {0, 0, 0}, x0 = 0, e0 = 0, t0 = 0, p0 = 0;
     c->blk.resize(nblocks);
     c->bcdesc.resize(nblocks);
     for (int b = 0; b < nblocks; b++) {
@@ -1177,12 +1334,14 @@ int cav3d_create(const cav3d_params *prm, int nblocks, const cav3d_block *blocks
         B.ntx = (long)(B.I + 1) * (B.J + 1) * B.K;
         B.g0 = g0; B.c0 = c0; B.w0 = w0; B.cx0 = x0; B.ex0 = e0; B.tx0 = t0;
         for (int d = 0; d < 3; d++) { B.f0[d] = f0[d]; f0[d] += B.nf[d]; }
+        B.p0 = p0;
+        p0 += (long)(B.J - 1) * (B.K - 1);
         g0 += B.ng; c0 += B.nc; w0 += B.nw; x0 += B.ncx; e0 += B.nex; t0 += B.ntx;
         S.b[b] = B;
         if (cb.nbc < 0 || (cb.nbc > 0 && !cb.bc)) { delete c; return fail("bad bc list", -1); }
         c->bcdesc[b].assign(cb.bc, cb.bc + 7 * cb.nbc);
     }
-    S.NG = g0; S.NC = c0; S.NW = w0; S.NCX = x0; S.NEX = e0; S.NTX = t0;
+    S.NG = g0; S.NC = c0; S.NW = w0; S.NCX = x0; S.NEX = e0; S.NTX = t0; S.NPLANE = p0;
     for (int d = 0; d < 3; d++) S.NF[d] = f0[d];
     // interfaces
     long hs = 0;
@@ -1225,6 +1384,43 @@ int cav3d_create(const cav3d_params *prm, int nblocks, const cav3d_block *blocks
     }
     if (prm->nsteady > 0) {
         if ((rc = dalloc(c, &S.qq, 2 * npv * S.NC))) { cav3d_destroy(c); return rc; }
+    }
+    if ((rc = dalloc(c, &S.gx, (long)NGX * S.NC)) || (rc = dalloc(c, &S.jc, 2 * S.NC)) ||
+        (rc = dalloc(c, &S.jf, 18 * S.NC))) { cav3d_destroy(c); return rc; }
+    {
+        // pencil table: per diagonal t = j + k of each block, pencils in groups of
+        // four (one wavefront), diagonals in increasing order over all blocks
+        std::vector<int> tab;
+        int tmax = 0;
+        for (int b = 0; b < nblocks; b++) tmax = std::max(tmax, c->blk[b].J + c->blk[b].K);
+        for (int t = 4; t <= tmax; t++) {
+            std::vector<int> diag;
+            for (int b = 0; b < nblocks; b++)
+                for (int k = 2; k <= c->blk[b].K; k++) {
+                    int j = t - k;
+                    if (j >= 2 && j <= c->blk[b].J) { diag.push_back(b); diag.push_back(j); diag.push_back(k); }
+                }
+            while ((diag.size() / 3) % 4) { diag.push_back(-1); diag.push_back(0); diag.push_back(0); }
+            tab.insert(tab.end(), diag.begin(), diag.end());
+        }
+        c->nquad = (long)tab.size() / 12;
+        if ((rc = dalloc(c, &c->slots, tab.size())) || (rc = dalloc(c, &c->sweep_err, 1))) {
+            cav3d_destroy(c); return rc;
+        }
+        if (hipMemcpyAsync(c->slots, tab.data(), tab.size() * sizeof(int), hipMemcpyHostToDevice, c->st) != hipSuccess) {
+            cav3d_destroy(c); return fail("pencil table upload failed", -3);
+        }
+        // cooperative grid: no more wavefronts than quads, no more blocks than are co-resident
+        int dev = 0, ncu = 0, per = 0, per2 = 0;
+        hipGetDevice(&dev);
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void *)k_sweep<true>, 256, 0);
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per2, (const void *)k_sweep<false>, 256, 0);
+        per = std::min(per, per2);
+        long need = (c->nquad + 3) / 4;
+        long cap = (long)ncu * per;
+        if (cap < 1) { cav3d_destroy(c); return fail("sweep kernel cannot be made resident", -3); }
+        c->coop_blocks = (int)std::max(1L, std::min(need, cap));
     }
     c->nwg_update = (int)nblk(S.NC, 256);
     if ((rc = dalloc(c, &S.red, (long)c->nwg_update * (NPV + 2)))) { cav3d_destroy(c); return rc; }
@@ -1412,7 +1608,8 @@ int cav3d_timeinteg(cav3d_ctx *c, int nt_phy, int nt, double timeprev, double *t
     if (c->profile) hipEventRecord(c->ev[2], c->st);
     launch_rhs(c);
     if (c->profile) hipEventRecord(c->ev[3], c->st);
-    launch_sweeps(c);
+    int lrc = launch_sweeps(c);
+    if (lrc) return lrc;
     if (c->profile) hipEventRecord(c->ev[4], c->st);
     launch_update(c);
     if (c->profile) {
@@ -1429,6 +1626,7 @@ int cav3d_timeinteg(cav3d_ctx *c, int nt_phy, int nt, double timeprev, double *t
         double tacc;
         HIPCHK(hipMemcpyAsync(&tacc, c->S.scal + 1, sizeof(double), hipMemcpyDeviceToHost, c->st));
         HIPCHK(hipStreamSynchronize(c->st));
+        if ((rc = sweep_status(c))) return rc;
         *time = (c->prm.nsteady == 0) ? tacc + timeprev : c->prm.dt_phy * (double)nt_phy;
     }
     return 0;
@@ -1440,6 +1638,8 @@ int cav3d_residual(cav3d_ctx *c, double *l2, int *ijk, double *resmax)
     double h[32];
     HIPCHK(hipMemcpyAsync(h, c->S.scal, sizeof(h), hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
+    int rc = sweep_status(c);
+    if (rc) return rc;
     if (l2) for (int n = 0; n < c->npv; n++) l2[n] = h[4 + n];
     long bi;
     memcpy(&bi, &h[5 + NPV], sizeof(long));

@@ -640,11 +640,27 @@ HD double eigenvis(const Par &P, const double *d, double mut)
     return mx(d[D_RP] * (d[D_KAP] + d[D_HT] * mut / 0.9) / den, 4.0 / 3.0 * (d[D_MU] + mut) / d[D_RHO]);
 }
 
-// y = y + sgn * A x, A = the split flux Jacobian a(n,l) of the given sign
-// (src/jacobian_module.f90:312-402), accumulated row by row over l in the
-// reference's order (src/update_module.f90:507-544).  The entries that are
-// identically zero are skipped: adding a zero product leaves y unchanged.
-HD void jac_apply(const Par &P, int npv, const CellJ &c, const double *n3, int sign, const double *x, double *y, double sgn)
+// The split flux Jacobian a(n,l) of a cell through one of its faces
+// (src/jacobian_module.f90:312-402), applied as y = y + sgn * A x row by row
+// over l in the reference's order (src/update_module.f90:507-544).  The
+// cell-only terms (JacCell) and the face terms (JacFace, for sign +1; the
+// sign flips lam and vis exactly) are computed once per iteration; entries
+// that are identically zero are skipped (adding a zero product leaves y
+// unchanged).
+struct JacCell { double beta, h; };
+struct JacFace { double u, lam, vis; };
+
+HD JacCell jac_cell(const Par &P, const CellJ &c)
+{
+    const double *pv = c.pv, *d = c.d;
+    double sp = sndp2(P, d[D_C2], pv[1] * pv[1] + pv[2] * pv[2] + pv[3] * pv[3]);
+    JacCell j;
+    j.beta = 1.0 / sp - 1.0 / d[D_C2] + d[D_RP];
+    j.h = d[D_H] + 0.5 * (pv[1] * pv[1] + pv[2] * pv[2] + pv[3] * pv[3]);
+    return j;
+}
+
+HD JacFace jac_face(const Par &P, const CellJ &c, const double *n3)
 {
     const double *pv = c.pv, *d = c.d;
     const double cappa = 1.05;
@@ -654,53 +670,80 @@ HD void jac_apply(const Par &P, int npv, const CellJ &c, const double *n3, int s
     double up = 0.5 * (1.0 + sp / d[D_C2]) * u;
     double omq = 1.0 - sp / d[D_C2];
     double dd = 0.5 * sqrt((u * u) * (omq * omq) + 4.0 * sp * ds);
-    double lam = (double)sign * cappa * (fabs(up) + dd);
-    double vis = (double)sign * 2.0 * eigenvis(P, d, c.mut) * ds / c.vol;
-    double beta = 1.0 / sp - 1.0 / d[D_C2] + d[D_RP];
-    double uuu = u + lam + vis;
-    double h = d[D_H] + 0.5 * (pv[1] * pv[1] + pv[2] * pv[2] + pv[3] * pv[3]);
+    JacFace f;
+    f.u = u;
+    f.lam = cappa * (fabs(up) + dd);
+    f.vis = 2.0 * eigenvis(P, d, c.mut) * ds / c.vol;
+    return f;
+}
+
+// row n of y += sgn * A x; pv/d of the cell the Jacobian belongs to
+HD void jac_row(int npv, const double *pv, const double *d, const JacCell &jc, const JacFace &jf, const double *n3,
+                int sign, int n, const double *x, double &y, double sgn)
+{
+    const double lam = sign > 0 ? jf.lam : -jf.lam, vis = sign > 0 ? jf.vis : -jf.vis;
+    const double u = jf.u;
     double r1[7];
-    r1[0] = u * d[D_RP] + lam * beta + vis * d[D_RP];
+    r1[0] = u * d[D_RP] + lam * jc.beta + vis * d[D_RP];
     r1[1] = n3[0] * d[D_RHO];
     r1[2] = n3[1] * d[D_RHO];
     r1[3] = n3[2] * d[D_RHO];
     r1[4] = u * d[D_RT] + lam * d[D_RT] + vis * d[D_RT];
     r1[5] = u * d[D_RY1] + lam * d[D_RY1] + vis * d[D_RY1];
     r1[6] = u * d[D_RY2] + lam * d[D_RY2] + vis * d[D_RY2];
+    const double uuu = u + lam + vis;
     const double ru = d[D_RHO] * uuu;
-    auto acc = [&](int n, double a, int l) {
+    auto acc = [&](double a, int l) {
         double t = (0.5 * a) * x[l];
-        y[n] = (sgn > 0.0) ? y[n] + t : y[n] - t;
+        y = (sgn > 0.0) ? y + t : y - t;
     };
-    for (int l = 0; l < 7; l++) acc(0, r1[l], l);
-    for (int n = 1; n <= 3; n++) {
+    if (n == 0) {
+        for (int l = 0; l < 7; l++) acc(r1[l], l);
+    } else if (n <= 3) {
         for (int l = 0; l < 7; l++) {
             double a = pv[n] * r1[l];
             if (l == 0) a = pv[n] * r1[0] + n3[n - 1];
             if (l == n) a = pv[n] * r1[n] + ru;
-            acc(n, a, l);
+            acc(a, l);
         }
-    }
-    const double hy[7] = {d[D_HP], pv[1], pv[2], pv[3], d[D_HT], d[D_HY1], d[D_HY2]};
-    for (int l = 0; l < 7; l++) {
-        double a = ru * hy[l] + h * r1[l];
-        if (l == 0) a = a - lam - vis;
-        acc(4, a, l);
-    }
-    for (int n = 5; n < npv; n++) {
+    } else if (n == 4) {
+        const double hy[7] = {d[D_HP], pv[1], pv[2], pv[3], d[D_HT], d[D_HY1], d[D_HY2]};
+        for (int l = 0; l < 7; l++) {
+            double a = ru * hy[l] + jc.h * r1[l];
+            if (l == 0) a = a - lam - vis;
+            acc(a, l);
+        }
+    } else if (n < npv) {
         for (int l = 0; l < 7; l++) {
             double a = pv[n] * r1[l];
             if (l == n) a = pv[n] * r1[n] + ru;
-            acc(n, a, l);
+            acc(a, l);
         }
-        if (n >= 7) acc(n, uuu * d[D_RHO], n);
+        if (n >= 7) acc(uuu * d[D_RHO], n);
     }
 }
 
+HD void jac_apply(const Par &P, int npv, const CellJ &c, const double *n3, int sign, const double *x, double *y, double sgn)
+{
+    JacCell jc = jac_cell(P, c);
+    JacFace jf = jac_face(P, c, n3);
+    for (int n = 0; n < npv; n++) jac_row(npv, c.pv, c.d, jc, jf, n3, sign, n, x, y[n], sgn);
+}
+
 // Closed-form inverse of the preconditioned diagonal block with the Merkle
-// (icav) and SST (itt) couplings.  f6 = the cell's six face vectors.
-HD void getx(const Par &P, int npv, const CellJ &c, const double f6[6][3], double dt,
-             const double *cc, const double *tt4, const double *res, double *x)
+// (icav) and SST (itt) couplings (src/lhs_module.f90:346-457), split into the
+// part that depends only on the cell (GxPre, computed once per iteration in a
+// parallel pass) and the part linear in the right-hand side (getx_apply, run
+// inside the sweeps).  Products are grouped exactly as the reference groups
+// them, so the split changes no rounding.
+enum {
+    GX_A = 0, GX_IB1, GX_MM, GX_SS, GX_AAA1, GX_AAA2, GX_CROSS, GX_CROSS2, GX_CROSS1, GX_SSHM, GX_RRR,
+    GX_SSY1, GX_SSY2, GX_C1, GX_C2, GX_C3, GX_C4C1, GX_RB1, GX_CP5, GX_C4IB1S1, GX_C3IB1, GX_S1P5,
+    GX_C4SSIB1, GX_C2IB1, GX_T2, GX_T3, GX_RBT4, GX_RBT1, GX_IT, NGX
+};
+
+HD void getx_prep(const Par &P, int npv, const CellJ &c, const double f6[6][3], double dt,
+                  const double *cc, const double *tt4, double *g)
 {
     const double *pv = c.pv, *d = c.d;
     double v2 = pv[1] * pv[1] + pv[2] * pv[2] + pv[3] * pv[3];
@@ -738,28 +781,75 @@ HD void getx(const Par &P, int npv, const CellJ &c, const double f6[6][3], doubl
     double ssy1 = d[D_HY1] * d[D_RHO] * rr + b1 * d[D_RY1] * dhdp1;
     double ssy2 = d[D_HY2] * d[D_RHO] * rr + b1 * d[D_RY2] * dhdp1;
     double ssh = d[D_RHO] * rr * hst - b1 * dhdp1 * d[D_RHO];
-    double aaa1 = d[D_RY1] * hst + d[D_HY1] * d[D_RHO] - cross * pv[6];
-    double aaa2 = d[D_RT] * hst + d[D_HT] * d[D_RHO] - cross2 * pv[6];
-    double uuu = res[1] * pv[1] + res[2] * pv[2] + res[3] * pv[3] - res[4];
-    double co1 = res[0] * aaa1 + d[D_RY1] * uuu + cross * res[6];
-    double co2 = res[0] * aaa2 + d[D_RT] * uuu + cross2 * res[6];
-    double co3 = res[0] * (ssh - ssy2 * pv[6]) + d[D_RHO] * rr * uuu + res[6] * ssy2;
-    double mm = 1.0 / (rb * (ss + c1 * cross1) - c2 * ssy1 + c3 * ss);
-    double r6 = res[6] - res[0] * pv[6];
-    x[0] = (c4 * cross1 * (-r6) - c2 * co1 + c3 * co2 + d[D_RHO] * b1 * (co2 + cross1 * res[5] - cross1 * pv[5] * res[0])) * mm * a;
-    x[1] = (res[1] - res[0] * pv[1]) * i_b1 * a;
-    x[2] = (res[2] - res[0] * pv[2]) * i_b1 * a;
-    x[3] = (res[3] - res[0] * pv[3]) * i_b1 * a;
-    x[4] = (c4 * i_b1 * ssy1 * r6 + c1 * co1 - c3 * i_b1 * co3 - (co3 - ssy1 * pv[5] * res[0] + res[5] * ssy1)) * mm * a;
-    x[5] = (ss * (res[5] - res[0] * pv[5]) - c4 * ss * i_b1 * r6 - c1 * co2 + c2 * i_b1 * co3) * mm * a;
-    x[6] = r6 * i_b1 * a;
+    g[GX_A] = a;
+    g[GX_IB1] = i_b1;
+    g[GX_MM] = 1.0 / (rb * (ss + c1 * cross1) - c2 * ssy1 + c3 * ss);
+    g[GX_SS] = ss;
+    g[GX_AAA1] = d[D_RY1] * hst + d[D_HY1] * d[D_RHO] - cross * pv[6];
+    g[GX_AAA2] = d[D_RT] * hst + d[D_HT] * d[D_RHO] - cross2 * pv[6];
+    g[GX_CROSS] = cross;
+    g[GX_CROSS2] = cross2;
+    g[GX_CROSS1] = cross1;
+    g[GX_SSHM] = ssh - ssy2 * pv[6];
+    g[GX_RRR] = d[D_RHO] * rr;
+    g[GX_SSY1] = ssy1;
+    g[GX_SSY2] = ssy2;
+    g[GX_C1] = c1;
+    g[GX_C2] = c2;
+    g[GX_C3] = c3;
+    g[GX_C4C1] = c4 * cross1;
+    g[GX_RB1] = d[D_RHO] * b1;
+    g[GX_CP5] = cross1 * pv[5];
+    g[GX_C4IB1S1] = c4 * i_b1 * ssy1;
+    g[GX_C3IB1] = c3 * i_b1;
+    g[GX_S1P5] = ssy1 * pv[5];
+    g[GX_C4SSIB1] = c4 * ss * i_b1;
+    g[GX_C2IB1] = c2 * i_b1;
     if (npv > 7) {
         double t1 = tt4[0] * c.vol * a, t2 = tt4[1] * c.vol * a, t3 = tt4[2] * c.vol * a, t4 = tt4[3] * c.vol * a;
-        double it = 1.0 / (t2 * t3 - (t1 + rb) * (t4 + rb));
-        double rk = res[7] - res[0] * pv[7], ro = res[8] - res[0] * pv[8];
-        x[7] = (ro * t2 - (rb + t4) * rk) * it * a;
-        x[8] = (rk * t3 - (rb + t1) * ro) * it * a;
+        g[GX_T2] = t2;
+        g[GX_T3] = t3;
+        g[GX_RBT4] = rb + t4;
+        g[GX_RBT1] = rb + t1;
+        g[GX_IT] = 1.0 / (t2 * t3 - (t1 + rb) * (t4 + rb));
+    } else {
+        g[GX_T2] = g[GX_T3] = g[GX_RBT4] = g[GX_RBT1] = g[GX_IT] = 0.0;
     }
+}
+
+// Component n of x = D^-1 res; G(q) returns the GxPre entry q.
+template <class F>
+HD double getx_apply(F G, const double *pv, double d_ry1, double d_rt, const double *res, int n)
+{
+    const double a = G(GX_A);
+    if (n >= 1 && n <= 3) return (res[n] - res[0] * pv[n]) * G(GX_IB1) * a;
+    double r6 = res[6] - res[0] * pv[6];
+    if (n == 6) return r6 * G(GX_IB1) * a;
+    if (n >= 7) {
+        double rk = res[7] - res[0] * pv[7], ro = res[8] - res[0] * pv[8];
+        if (n == 7) return (ro * G(GX_T2) - G(GX_RBT4) * rk) * G(GX_IT) * a;
+        return (rk * G(GX_T3) - G(GX_RBT1) * ro) * G(GX_IT) * a;
+    }
+    double uuu = res[1] * pv[1] + res[2] * pv[2] + res[3] * pv[3] - res[4];
+    double co1 = res[0] * G(GX_AAA1) + d_ry1 * uuu + G(GX_CROSS) * res[6];
+    double co2 = res[0] * G(GX_AAA2) + d_rt * uuu + G(GX_CROSS2) * res[6];
+    double co3 = res[0] * G(GX_SSHM) + G(GX_RRR) * uuu + res[6] * G(GX_SSY2);
+    double mm = G(GX_MM);
+    if (n == 0)
+        return (G(GX_C4C1) * (-r6) - G(GX_C2) * co1 + G(GX_C3) * co2
+                + G(GX_RB1) * (co2 + G(GX_CROSS1) * res[5] - G(GX_CP5) * res[0])) * mm * a;
+    if (n == 4)
+        return (G(GX_C4IB1S1) * r6 + G(GX_C1) * co1 - G(GX_C3IB1) * co3
+                - (co3 - G(GX_S1P5) * res[0] + res[5] * G(GX_SSY1))) * mm * a;
+    return (G(GX_SS) * (res[5] - res[0] * pv[5]) - G(GX_C4SSIB1) * r6 - G(GX_C1) * co2 + G(GX_C2IB1) * co3) * mm * a;
+}
+
+HD void getx(const Par &P, int npv, const CellJ &c, const double f6[6][3], double dt,
+             const double *cc, const double *tt4, const double *res, double *x)
+{
+    double g[NGX];
+    getx_prep(P, npv, c, f6, dt, cc, tt4, g);
+    for (int n = 0; n < npv; n++) x[n] = getx_apply([&](int q) { return g[q]; }, c.pv, c.d[D_RY1], c.d[D_RT], res, n);
 }
 
 // local pseudo time step (src/timestep_module.f90:156-198)

@@ -1,0 +1,319 @@
+"""Properties of the oracle fixed by the mathematics rather than by its own
+formulas: flux consistency, antisymmetry and the supersonic limit,
+reconstruction of linear data, free-stream preservation of the whole step,
+the closed-form time step, and the LU-SGS factorisation the two sweeps
+implement (src/update_module.f90:486-670) checked by brute force."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+D = C.POINTER(C.c_double)
+PP = C.POINTER(D)
+
+
+def dp(a):
+    return a.ctypes.data_as(D)
+
+
+def ptrs(arrs):
+    return (D * len(arrs))(*[np.ascontiguousarray(a).ctypes.data_as(D) for a in arrs])
+
+
+@pytest.fixture(scope="module")
+def OL(orc):
+    L = orc.lib()
+    L.orc_tvd.argtypes = [C.c_void_p, D, D, D]
+    L.orc_mlp.argtypes = [C.c_void_p, D, D, D, C.c_void_p]
+    L.orc_roe.argtypes = [C.c_void_p, D, D, D, D, D, D]
+    L.orc_ausmpup.argtypes = [C.c_void_p, D, D, D, D, D, D, C.c_void_p]
+    L.orc_localdt.argtypes = [C.c_void_p, PP, D, D, D, D]
+    L.orc_localdt.restype = C.c_double
+    L.orc_getx.argtypes = [C.c_void_p, PP, D, D, D, D, C.c_double, D, D, D, D]
+    L.orc_caljac.argtypes = [C.c_void_p, D, D, D, D, D, C.c_int, D]
+    return L
+
+
+def cfg_of(orc, pkg, **over):
+    case = pkg.cases.config1(6, 5, 4)
+    case.params.update(over)
+    cfg = case.cfg_dict()
+    cf = orc.Cfg()
+    for k, v in cfg.items():
+        if k == "bc_pv":
+            cf.bc_pv = (C.c_double * 9)(*v)
+        else:
+            setattr(cf, k, v)
+    return case, cfg, cf
+
+
+def phys_flux(orc, pref, q, n3, npv):
+    """Physical flux of primitive state q through area vector n3 (absolute pressure)."""
+    dv = orc.deteos_simple(q[0] + pref, q[4], q[5], q[6])
+    rho, h = dv[0], dv[1]
+    U = q[1] * n3[0] + q[2] * n3[1] + q[3] * n3[2]
+    H = h + 0.5 * (q[1] ** 2 + q[2] ** 2 + q[3] ** 2)
+    f = np.zeros(npv)
+    f[0] = rho * U
+    f[1:4] = rho * q[1:4] * U + (q[0] + pref) * n3
+    f[4] = rho * H * U
+    f[5:npv] = rho * q[5:npv] * U
+    return f
+
+
+def flux(OL, cf, orc, cfg, nscheme, ql, qr, n3):
+    dl = orc.deteos_simple(ql[0] + cfg["pref"], ql[4], ql[5], ql[6])
+    dr = orc.deteos_simple(qr[0] + cfg["pref"], qr[4], qr[5], qr[6])
+    f = np.zeros(9)
+    ql, qr, n3 = (np.ascontiguousarray(a, dtype=np.float64) for a in (ql, qr, n3))
+    if nscheme == 1:
+        OL.orc_roe(C.byref(cf), dp(n3), dp(ql), dp(qr), dp(dl), dp(dr), dp(f))
+    else:
+        OL.orc_ausmpup(C.byref(cf), dp(n3), dp(ql), dp(qr), dp(dl), dp(dr), dp(f), None)
+    return f
+
+
+def rand_state(rng, npv, vapour=False):
+    q = np.zeros(9)
+    q[0] = rng.uniform(-2e4, 2e4)
+    q[1:4] = rng.uniform(-15, 15, 3)
+    q[4] = rng.uniform(290, 320)
+    q[5] = rng.choice([0.0, rng.uniform(0, 1e-3)])
+    q[7:9] = rng.uniform(1e-3, 1), rng.uniform(10, 1e3)
+    return q
+
+
+@pytest.mark.parametrize("nscheme", [1, 4])
+@pytest.mark.parametrize("prec", [0, 1])
+def test_flux_consistency(orc, pkg, OL, nscheme, prec):
+    """F(q, q, n) is the physical flux (the dissipation vanishes)."""
+    case, cfg, cf = cfg_of(orc, pkg, prec=prec)
+    rng = np.random.default_rng(nscheme + 10 * prec)
+    for _ in range(100):
+        q = rand_state(rng, 7)
+        n3 = rng.standard_normal(3) * 1e-3
+        f = flux(OL, cf, orc, cfg, nscheme, q, q, n3)[:7]
+        ref = phys_flux(orc, cfg["pref"], q, n3, 7)
+        scale = np.abs(ref) + abs(q[0] + cfg["pref"]) * np.linalg.norm(n3) * 1e-3
+        # AUSM+-up builds the mass flux from split-Mach polynomials whose terms are
+        # O(1) while M ~ 1e-2: rounding is amplified by 1/M
+        tol = 1e-12 if nscheme == 1 else 1e-10
+        assert np.all(np.abs(f - ref) <= tol * scale + 1e-300), (f, ref)
+
+
+@pytest.mark.parametrize("nscheme", [1, 4])
+def test_flux_antisymmetry(orc, pkg, OL, nscheme):
+    """F(L, R, n) = -F(R, L, -n)."""
+    case, cfg, cf = cfg_of(orc, pkg)
+    rng = np.random.default_rng(20 + nscheme)
+    for _ in range(100):
+        ql, qr = rand_state(rng, 7), rand_state(rng, 7)
+        qr[5] = ql[5]
+        n3 = rng.standard_normal(3) * 1e-3
+        f1 = flux(OL, cf, orc, cfg, nscheme, ql, qr, n3)[:7]
+        f2 = flux(OL, cf, orc, cfg, nscheme, qr, ql, -n3)[:7]
+        scale = np.abs(f1) + (abs(ql[0]) + abs(qr[0]) + cfg["pref"]) * np.linalg.norm(n3)
+        assert np.all(np.abs(f1 + f2) <= 1e-12 * scale), (f1, f2)
+
+
+def test_ausm_supersonic_upwind(orc, pkg, OL):
+    """Both sides supersonic toward +n: AUSM+-up returns the left physical flux."""
+    case, cfg, cf = cfg_of(orc, pkg, nscheme=4)
+    rng = np.random.default_rng(3)
+    for _ in range(50):
+        # pure vapour at low pressure, u well above its sound speed (~430 m/s)
+        ql = np.array([3000.0 - cfg["pref"], 900.0, 0, 0, 350.0, 1.0, 0, 0, 0])
+        qr = ql.copy()
+        qr[1] += rng.uniform(0, 100)
+        qr[0] += rng.uniform(-100, 100)
+        n3 = np.array([1e-3, 0, 0])
+        f = flux(OL, cf, orc, cfg, 4, ql, qr, n3)[:7]
+        ref = phys_flux(orc, cfg["pref"], ql, n3, 7)
+        assert np.allclose(f, ref, rtol=1e-12, atol=1e-14 * np.max(np.abs(ref)))
+
+
+def stencil_linear(a, b, npv):
+    """38-point stencil of data a + b*s along the face normal (s = 0 at cell i)."""
+    s = np.zeros(38)
+    for ll in range(18):
+        s[ll] = ll % 2                          # ii = 0, 1
+    s[18:27] = -1.0
+    s[27:36] = 2.0
+    s[36], s[37] = -2.0, 3.0
+    x = np.zeros((38, 9))
+    x[:, :npv] = a[None, :npv] + b[None, :npv] * s[:, None]
+    return x
+
+
+@pytest.mark.parametrize("nmuscl,nlim", [(1, n) for n in range(11)] + [(2, n) for n in (1, 2, 3, 5, 8)])
+def test_reconstruction_exact_on_linear_data(orc, pkg, OL, nmuscl, nlim):
+    case, cfg, cf = cfg_of(orc, pkg, nmuscl=nmuscl, nlim=nlim)
+    rng = np.random.default_rng(nlim)
+    fn = OL.orc_tvd if nmuscl == 1 else OL.orc_mlp
+    for _ in range(50):
+        a = np.array([rng.uniform(-1e3, 1e3), rng.uniform(-10, 10), 1.0, -2.0, 300.0, 0.0, 0.0, 0.1, 100.0])
+        b = np.array([rng.uniform(-50, 50), rng.uniform(-1, 1), 0.1, 0.2, rng.uniform(-1, 1), 0, 0, 1e-3, 1.0])
+        x = stencil_linear(a, b, 7)
+        xl, xr = np.zeros(9), np.zeros(9)
+        args = [C.byref(cf), dp(x), dp(xl), dp(xr)] + ([None] if nmuscl == 2 else [])
+        fn(*args)
+        face = a[:7] + 0.5 * b[:7]
+        assert np.allclose(xl[:7], face, rtol=1e-13, atol=1e-12), (xl[:7], face)
+        assert np.allclose(xr[:7], face, rtol=1e-13, atol=1e-12), (xr[:7], face)
+        # constant data are reproduced exactly
+        xc = stencil_linear(a, 0 * b, 7)
+        fn(*([C.byref(cf), dp(xc), dp(xl), dp(xr)] + ([None] if nmuscl == 2 else [])))
+        assert np.array_equal(xl[:7], a[:7]) and np.array_equal(xr[:7], a[:7])
+
+
+def test_pressure_clamp(orc, pkg, OL):
+    """Reconstructed absolute pressure stays >= 10 Pa (src/muscl_module.f90:266)."""
+    case, cfg, cf = cfg_of(orc, pkg, nmuscl=1, nlim=0)
+    a = np.array([-cfg["pref"] + 50.0, 10, 0, 0, 300, 0, 0, 0, 0])
+    b = np.array([-200.0, 0, 0, 0, 0, 0, 0, 0, 0])
+    x = stencil_linear(a, b, 7)
+    xl, xr = np.zeros(9), np.zeros(9)
+    OL.orc_tvd(C.byref(cf), dp(x), dp(xl), dp(xr))
+    # left state: 50 - 0.5*200 = -50 Pa absolute -> clamped; right: -150 - (-100) ... both clamp
+    assert xl[0] == -cfg["pref"] + 10.0
+    assert xr[0] == -cfg["pref"] + 10.0
+
+
+def test_local_dt_closed_form(orc, pkg, OL):
+    """Euler, no preconditioning: dt = CFL vol / sum_d (|U_d| + c |S_d|), S_d the mean face vector."""
+    case, cfg, cf = cfg_of(orc, pkg, prec=0, iturb=-3, npv=7, ntv=0)
+    rng = np.random.default_rng(4)
+    for _ in range(50):
+        q = rand_state(rng, 7)
+        dv = orc.deteos_simple(q[0] + cfg["pref"], q[4], q[5], q[6])
+        f6 = [rng.standard_normal(3) * 1e-4 for _ in range(6)]
+        vol = rng.uniform(1e-9, 1e-7)
+        grd = np.array([vol, 0, 0, 0, 0.0])
+        dt = OL.orc_localdt(C.byref(cf), ptrs(f6), dp(grd), dp(q), dp(dv), dp(np.zeros(3)))
+        c = np.sqrt(dv[5])
+        lam = 0.0
+        for d in range(3):
+            S = 0.5 * (f6[2 * d] + f6[2 * d + 1])
+            lam += abs(S @ q[1:4]) + c * np.linalg.norm(S)
+        assert abs(dt / (cfg["cfl"] * vol / lam) - 1.0) < 1e-13
+
+
+def test_free_stream_preserved(orc, pkg):
+    """A uniform state on a stretched straight channel with far-field sides
+    gives a zero residual (closed-cell metric identity) and no update.
+    (Symmetry planes would not do: the reference fills edges between two
+    symmetry planes with the wall rule u = -(sum of three neighbours),
+    src/bc_module.f90:1806-1862.)"""
+    cs = pkg.cases
+    b = cs.bump_channel(8, 6, 5, bump=0.0)
+    I, J, K = b.imax, b.jmax, b.kmax
+    b.bc_desc = [cs.face_bc("inflow_subsonic", "imin", I, J, K), cs.face_bc("outflow_subsonic", "imax", I, J, K),
+                 cs.face_bc("farfield", "jmin", I, J, K), cs.face_bc("farfield", "jmax", I, J, K),
+                 cs.face_bc("farfield", "kmin", I, J, K), cs.face_bc("farfield", "kmax", I, J, K)]
+    p = cs.make_params(-2, 1, 2, 8, 1, 0, 0, 5.0, 101325.0, 10.0)
+    cs.finish_block(b, 4)
+    case = cs.Case("freestream", p, [b], 7, 2, 4)
+    pv = np.zeros((K + 5, J + 5, I + 5, 7))
+    pv[..., 1] = 10.0
+    pv[..., 4] = 300.0
+    dv, tv = cs.eos_host(101325.0, 300.0, 0.0, 0.0)
+    dva = np.broadcast_to(dv[0], (K + 5, J + 5, I + 5, 18)).copy()
+    tva = np.broadcast_to(tv[0], (K + 5, J + 5, I + 5, 2)).copy()
+    o = orc.Oracle(case)
+    o.put_state(0, pv.ravel(), dva.ravel(), tva.ravel())
+    o.setbc()
+    res = o.calrhs(0)[0].reshape(-1, 7)
+    flux_scale = (996.5 * 10.0 ** 2 + 101325.0) * 1e-3   # (rho u^2 + p) S, faces ~1e-3 m^2
+    assert np.max(np.abs(res)) < 1e-12 * flux_scale, np.max(np.abs(res), axis=0)
+    o2 = orc.Oracle(case)
+    o2.put_state(0, pv.ravel(), dva.ravel(), tva.ravel())
+    o2.timeinteg(1, 1, 0.0)
+    pv1 = o2.get_state(0)[0].reshape(K + 5, J + 5, I + 5, 7)[3:K + 2, 3:J + 2, 3:I + 2]
+    assert np.allclose(pv1[..., 1], 10.0, rtol=0, atol=1e-9)
+    assert np.allclose(pv1[..., 0], 0.0, rtol=0, atol=1e-6)
+    assert np.allclose(pv1[..., 4], 300.0, rtol=0, atol=1e-9)
+
+
+def test_lusgs_sweeps_solve_the_factored_system(orc, pkg, OL):
+    """Brute force on a 5x4x3 block: the oracle's update dq satisfies
+    (D - L+) D^-1 (D + U-) dq = res, with D^-1 = getx, L+ / U- the split
+    Jacobians of the lower / upper neighbours through their shared faces
+    (ghost neighbours excluded)."""
+    case = pkg.cases.bump_case(5, 4, 3, False, False, 1, 1, 1, cfl=3.0, seed=9)
+    cfg = case.cfg_dict()
+    cf = orc.Cfg()
+    for k, v in cfg.items():
+        if k == "bc_pv":
+            cf.bc_pv = (C.c_double * 9)(*v)
+        else:
+            setattr(cf, k, v)
+    b = case.blocks[0]
+    I, J, K, n = b.imax, b.jmax, b.kmax, case.npv
+    o = orc.Oracle(case)
+    o.put_state(0, *case.init[0])
+    o.timestep(1, 1, 0.0)
+    o.setbc()
+    res = o.calrhs(0)[0].reshape(K - 1, J - 1, I - 1, n)
+    dt = o.get_dt(0).reshape(K - 1, J - 1, I - 1)
+    pv, dv, tv, _ = o.get_state(0)
+    PV = pv.reshape(K + 5, J + 5, I + 5, n)
+    DV = dv.reshape(K + 5, J + 5, I + 5, 18)
+    TV = tv.reshape(K + 5, J + 5, I + 5, 2)
+    o2 = orc.Oracle(case)
+    o2.put_state(0, *case.init[0])
+    o2.timeinteg(1, 1, 0.0)
+    PV1 = o2.get_state(0)[0].reshape(K + 5, J + 5, I + 5, n)
+    cx = b.cx.reshape(K + 1, J + 1, I, 3)
+    ex = b.ex.reshape(K + 1, J, I + 1, 3)
+    tx = b.tx.reshape(K, J + 1, I + 1, 3)
+    grd = b.grd.reshape(K + 1, J + 1, I + 1, case.ngrd)
+    cells = [(i, j, k) for k in range(2, K + 1) for j in range(2, J + 1) for i in range(2, I + 1)]
+    P = lambda a, i, j, k: np.ascontiguousarray(a[k + 1, j + 1, i + 1])      # ghosted arrays
+    G = lambda i, j, k: np.ascontiguousarray(grd[k - 1, j - 1, i - 1])
+    face = {0: lambda i, j, k: cx[k - 1, j - 1, i - 1], 1: lambda i, j, k: ex[k - 1, j - 1, i - 1],
+            2: lambda i, j, k: tx[k - 1, j - 1, i - 1]}
+    dq = {c: PV1[c[2] + 1, c[1] + 1, c[0] + 1] - PV[c[2] + 1, c[1] + 1, c[0] + 1] for c in cells}
+
+    def dinv(c):
+        i, j, k = c
+        f6 = [face[0](i - 1, j, k), face[0](i, j, k), face[1](i, j - 1, k), face[1](i, j, k),
+              face[2](i, j, k - 1), face[2](i, j, k)]
+        M = np.zeros((n, n))
+        for l in range(n):
+            e = np.zeros(9)
+            e[l] = 1.0
+            x = np.zeros(9)
+            OL.orc_getx(C.byref(cf), ptrs(f6), dp(G(i, j, k)), dp(P(PV, i, j, k)), dp(P(DV, i, j, k)),
+                        dp(np.concatenate([P(TV, i, j, k), [0.0]])), dt[k - 2, j - 2, i - 2],
+                        dp(np.zeros(4)), dp(np.zeros(4)), dp(e), dp(x))
+            M[:, l] = x[:n]
+        return M
+
+    def jac(m, n3, sign):
+        i, j, k = m
+        a = np.zeros(81)
+        OL.orc_caljac(C.byref(cf), dp(np.ascontiguousarray(n3)), dp(G(i, j, k)), dp(P(PV, i, j, k)),
+                      dp(P(DV, i, j, k)), dp(np.concatenate([P(TV, i, j, k), [0.0]])), sign, dp(a))
+        return a[:n * n].reshape(n, n).T
+
+    Dinv = {c: dinv(c) for c in cells}
+    lim = (I, J, K)
+
+    def neighbour_sum(vec, c, upper):
+        i, j, k = c
+        acc = np.zeros(n)
+        for ax in range(3):
+            m = [i, j, k]
+            m[ax] += 1 if upper else -1
+            if not (2 <= m[ax] <= lim[ax]):
+                continue
+            n3 = face[ax](*(c if upper else m))
+            acc += jac(tuple(m), n3, -1 if upper else 1) @ vec[tuple(m)]
+        return acc
+
+    dqs = {c: dq[c] + Dinv[c] @ neighbour_sum(dq, c, True) for c in cells}          # backward: dq* = dq + D^-1 U- dq
+    for c in cells:
+        lhs = dqs[c] - Dinv[c] @ neighbour_sum(dqs, c, False)                          # forward: dq* - D^-1 L+ dq*
+        rhs = Dinv[c] @ res[c[2] - 2, c[1] - 2, c[0] - 2]
+        scale = np.abs(rhs) + np.abs(dqs[c]) + 1e-300
+        assert np.all(np.abs(lhs - rhs) <= 1e-9 * scale + 1e-13 * np.max(np.abs(dqs[c]))), (c, lhs, rhs)
