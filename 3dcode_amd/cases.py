@@ -426,3 +426,96 @@ def config2(ni=128, nj=64, nk=64, seed=2):
     """Cavitating bump: AUSM+-up, MLP (5th-order limiter), k-omega SST, Merkle, sigma = 0.8."""
     return bump_case(ni, nj, nk, turbulent=True, cavitating=True, nscheme=4, nmuscl=2, nlim=8, cfl=5.0,
                      sigma=0.8, seed=seed, name=f"config2_{ni}x{nj}x{nk}")
+
+
+# ------------------------------------------------------------ multiblock
+def connection(ipnts, ipnts_donor, transvec, donor):
+    """Interface descriptor (22 ints) from a CGNS 1-to-1 connection: the
+    point ranges as grid_module converts them (src/grid_module.f90:222-300)
+    and the three-layer extension of bc_module (src/bc_module.f90:355-488)."""
+    T = np.zeros((3, 3), dtype=int)
+    for c in range(3):
+        T[abs(transvec[c]) - 1, c] = int(np.sign(transvec[c]))
+    ist, ien = list(ipnts[:3]), list(ipnts[3:])
+    isd, ied = list(ipnts_donor[:3]), list(ipnts_donor[3:])
+    for m in range(3):
+        if ipnts[m] == ipnts[3 + m]:
+            if ipnts[m] == 1:
+                ist, ien = list(ipnts[:3]), list(ipnts[3:])
+                ist[m] += 1
+                ien[m] += 1
+        if ipnts_donor[m] == ipnts_donor[3 + m] and ipnts_donor[m] != 1:
+            isd, ied = list(ipnts_donor[:3]), list(ipnts_donor[3:])
+            isd[m] += 1
+            ied[m] += 1
+    for m in range(3):                          # the face direction of this block
+        if ist[m] == ien[m]:
+            # the donor index that this block's normal direction maps onto
+            col = T[:, m]
+            q = int(np.nonzero(col)[0][0])
+            if ist[m] == 2:
+                ien[m] += 2
+                if col[q] == 1:
+                    ied[q] += 2
+                else:
+                    ied[q] -= 2
+            else:
+                ist[m] -= 2
+                if col[q] == 1:
+                    isd[q] -= 2
+                else:
+                    isd[q] += 2
+            break
+    return [donor, *ist, *ien, *isd, *ied, *T.reshape(-1).tolist()]
+
+
+def split_i(case, cuts):
+    """The same case cut into blocks along i at the given vertex indices
+    (1-based), with interfaces between consecutive blocks.  Metrics are
+    recomputed per block (wall distance over all walls), the state is the
+    single block's, cut."""
+    (b0,) = case.blocks
+    I, J, K = b0.imax, b0.jmax, b0.kmax
+    bounds = [1] + list(cuts) + [I]
+    walls = wall_face_centres(b0.X, b0.bc_desc)
+    kinds = {}
+    for d in b0.bc_desc:
+        s, e = d[1:4], d[4:7]
+        face = next(f for f, (ax, v) in {"imin": (0, 1), "imax": (0, I + 1), "jmin": (1, 1), "jmax": (1, J + 1),
+                                          "kmin": (2, 1), "kmax": (2, K + 1)}.items() if s[ax] == e[ax] == v)
+        kinds[face] = d[0]
+    inv = {v: k for k, v in BC.items()}
+    blocks, init = [], []
+    n = len(bounds) - 1
+    for q in range(n):
+        lo, hi = bounds[q], bounds[q + 1]
+        X = b0.X[lo - 1:hi]
+        Ib = hi - lo + 1
+        nb = Block(Ib, J, K, X)
+        for face, kind in kinds.items():
+            if face == "imin" and q > 0:
+                continue
+            if face == "imax" and q < n - 1:
+                continue
+            nb.bc_desc.append(face_bc(inv[kind], face, Ib, J, K))
+        if q > 0:      # imin face joins the previous block's imax face
+            prevI = bounds[q] - bounds[q - 1] + 1
+            nb.conn_desc.append(connection([1, 1, 1, 1, J, K], [prevI, 1, 1, prevI, J, K], [1, 2, 3], q - 1))
+        if q < n - 1:  # imax face joins the next block's imin face
+            nb.conn_desc.append(connection([Ib, 1, 1, Ib, J, K], [1, 1, 1, 1, J, K], [1, 2, 3], q + 1))
+        finish_block(nb, case.ngrd, walls)
+        blocks.append(nb)
+        sl = slice(lo - 1, hi + 5)          # ghosted i range of the sub-block: lo-2 .. hi+3 (Fortran)
+        pv, dv, tv, qq = case.init[0]
+        P = pv.reshape(K + 5, J + 5, I + 5, case.npv)[:, :, sl]
+        Dv = dv.reshape(K + 5, J + 5, I + 5, 18)[:, :, sl]
+        Tv = tv.reshape(K + 5, J + 5, I + 5, -1)[:, :, sl] if tv is not None else None
+        Q = None
+        if qq is not None:
+            Q = qq.reshape(K - 1, J - 1, I - 1, 2, case.npv)[:, :, lo - 1:hi - 1]
+            Q = np.ascontiguousarray(Q).ravel()
+        init.append((np.ascontiguousarray(P).ravel(), np.ascontiguousarray(Dv).ravel(),
+                     None if Tv is None else np.ascontiguousarray(Tv).ravel(), Q))
+    out = Case(case.name + f"_split{n}", dict(case.params), blocks, case.npv, case.ntv, case.ngrd)
+    out.init = init
+    return out

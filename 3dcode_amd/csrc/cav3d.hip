@@ -201,7 +201,7 @@ __global__ void k_reset_scal(DS S)
     }
 }
 
-__global__ void k_timestep(DS S)
+__global__ __launch_bounds__(256) void k_timestep(DS S)
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     double dt = 1.0e300;
@@ -216,10 +216,18 @@ __global__ void k_timestep(DS S)
         dt = (S.P.local == -1) ? S.P.cfl : local_dt(S.P, c, f6);
         S.dt[t] = dt;
     }
-    // wavefront min, then one atomic per wavefront (positive doubles order as integers)
+    // wavefront min, workgroup min, then one atomic per workgroup
+    // (positive doubles order as their bit patterns)
+    __shared__ double wmin[4];
     for (int o = 32; o > 0; o >>= 1) dt = fmin(dt, __shfl_xor(dt, o));
-    if ((threadIdx.x & 63) == 0 && dt < 1.0e299)
-        atomicMin(reinterpret_cast<unsigned long long *>(S.scal), (unsigned long long)__double_as_longlong(dt));
+    if ((threadIdx.x & 63) == 0) wmin[threadIdx.x >> 6] = dt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = wmin[0];
+        for (int q = 1; q < (int)(blockDim.x >> 6); q++) m = fmin(m, wmin[q]);
+        if (m < 1.0e299)
+            atomicMin(reinterpret_cast<unsigned long long *>(S.scal), (unsigned long long)__double_as_longlong(m));
+    }
 }
 
 __global__ void k_time_acc(DS S)
@@ -901,14 +909,21 @@ __global__ void k_norm_final(DS S, int nwg)
     pm[tid] = bm;
     pi[tid] = bi;
     __syncthreads();
-    if (tid == 0) {
-        for (int q = 1; q < 256; q++) {
-            for (int n = 0; n < S.npv; n++) acc[n] += part[q][n];
-            if (pm[q] > bm || (pm[q] == bm && pi[q] < bi)) { bm = pm[q]; bi = pi[q]; }
+    // pairwise tree in a fixed pairing: the same inputs always give the same bits
+    for (int h = 128; h > 0; h >>= 1) {
+        if (tid < h) {
+            for (int n = 0; n < NPV; n++) part[tid][n] += part[tid + h][n];
+            if (pm[tid + h] > pm[tid] || (pm[tid + h] == pm[tid] && pi[tid + h] < pi[tid])) {
+                pm[tid] = pm[tid + h];
+                pi[tid] = pi[tid + h];
+            }
         }
-        for (int n = 0; n < S.npv; n++) S.scal[4 + n] = sqrt(acc[n]);
-        S.scal[4 + NPV] = bm;
-        reinterpret_cast<long *>(S.scal)[5 + NPV] = bi;
+        __syncthreads();
+    }
+    if (tid == 0) {
+        for (int n = 0; n < S.npv; n++) S.scal[4 + n] = sqrt(part[0][n]);
+        S.scal[4 + NPV] = pm[0];
+        reinterpret_cast<long *>(S.scal)[5 + NPV] = pi[0];
     }
 }
 
@@ -1095,9 +1110,15 @@ int build_bcs(cav3d_ctx *c)
             default: return fail("unknown boundary kind " + std::to_string(kind), -2);
             }
             s.pad = kind;
-            for (int d = 0; d < 3; d++) {
-                int lo = (d == 0 ? -1 : -1), hi = (d == 0 ? B.I + 3 : (d == 1 ? B.J + 3 : B.K + 3));
-                if (s.is[d] < lo || s.ie[d] > hi) return fail("boundary range out of the block", -2);
+            {
+                // the face's own axis spans its three ghost layers; the others stay on interior cells
+                const int ext[3] = {B.I, B.J, B.K};
+                const int ax = (s.face == F_IMIN || s.face == F_IMAX) ? 0 : ((s.face == F_JMIN || s.face == F_JMAX) ? 1 : 2);
+                for (int d = 0; d < 3; d++) {
+                    const int lo = (d == ax) ? -1 : 2, hi = (d == ax) ? ext[d] + 3 : ext[d];
+                    if (s.is[d] < lo || s.ie[d] > hi || s.is[d] > s.ie[d])
+                        return fail("boundary range of block " + std::to_string(b) + " lies outside its face", -2);
+                }
             }
             faces.push_back(s);
         }
@@ -1358,6 +1379,25 @@ int cav3d_create(const cav3d_params *prm, int nblocks, const cav3d_block *blocks
             long cnt = 1;
             for (int q = 0; q < 3; q++) cnt *= (cn.ie[q] >= cn.is[q]) ? (cn.ie[q] - cn.is[q] + 1) : 0;
             if (cnt == 0) continue;
+            // both the source box and its image must lie inside the ghosted arrays
+            const int ext[2][3] = {{blocks[b].imax, blocks[b].jmax, blocks[b].kmax},
+                                   {blocks[cn.dst].imax, blocks[cn.dst].jmax, blocks[cn.dst].kmax}};
+            bool ok = true;
+            for (int q = 0; q < 3; q++) {
+                if (cn.is[q] < -1 || cn.ie[q] > ext[0][q] + 3) ok = false;
+                for (int r = 0; r < 3; r++)
+                    if (cn.tm[3 * q + r] < -1 || cn.tm[3 * q + r] > 1) ok = false;
+            }
+            for (int corner = 0; ok && corner < 8; corner++) {
+                int del[3];
+                for (int q = 0; q < 3; q++) del[q] = ((corner >> q) & 1) ? cn.ie[q] - cn.is[q] : 0;
+                for (int q = 0; q < 3; q++) {
+                    int v = cn.tm[3 * q] * del[0] + cn.tm[3 * q + 1] * del[1] + cn.tm[3 * q + 2] * del[2] + cn.isd[q];
+                    if (v < -1 || v > ext[1][q] + 3) ok = false;
+                }
+            }
+            if (!ok) { delete c; return fail("interface " + std::to_string(n) + " of block " + std::to_string(b) +
+                                             " maps outside the ghosted arrays", -1); }
             cn.start = hs;
             hs += cnt;
             c->cons.push_back(cn);

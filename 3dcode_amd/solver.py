@@ -157,6 +157,13 @@ class Solver:
 
     # ---- state
     def put_state(self, ib, pv, dv=None, tv=None, qq=None):
+        b = self.case.blocks[ib]
+        ng = (b.imax + 5) * (b.jmax + 5) * (b.kmax + 5)
+        ni = (b.imax - 1) * (b.jmax - 1) * (b.kmax - 1)
+        for name, a, size in (("pv", pv, self.npv * ng), ("dv", dv, 18 * ng), ("tv", tv, self.ntv * ng),
+                              ("qq", qq, 2 * self.npv * ni)):
+            if a is not None and (a.dtype != np.float64 or a.size != size):
+                raise ValueError(f"{name}: expected {size} float64 values, got {a.size} {a.dtype}")
         _check(lib().cav3d_put_state(self.h, ib, _p(pv), _p(dv), _p(tv), _p(qq)))
 
     def get_state(self, ib):

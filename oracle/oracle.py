@@ -185,6 +185,12 @@ class Oracle:
             pass
 
     def put_state(self, ib, pv, dv, tv, qq=None):
+        b = self.case.blocks[ib]
+        ng = (b.imax + 5) * (b.jmax + 5) * (b.kmax + 5)
+        ni = (b.imax - 1) * (b.jmax - 1) * (b.kmax - 1)
+        for a, size in ((pv, self.case.npv * ng), (dv, 18 * ng), (tv, self.case.ntv * ng), (qq, 2 * self.case.npv * ni)):
+            if a is not None and a.size != size:
+                raise ValueError(f"state array has {a.size} values, expected {size}")
         lib().orc_put_state(self.h, ib, _p(pv), _p(dv), _p(tv), _p(qq))
 
     def get_state(self, ib):

@@ -13,6 +13,8 @@ def small_cases(pk):
         "sst_cav_ausm_mlp": lambda: _clip(pk.cases.config2(12, 8, 6), 1),
         "sst_cav_roe_first": lambda: _clip(bc(12, 8, 6, True, True, 1, 0, 0, cfl=5.0, seed=5), 1),
         "sst_cav_as_written": lambda: _clip(pk.cases.config2(12, 8, 6), 0),
+        "lam_roe_2block": lambda: pk.cases.split_i(pk.cases.config1(12, 8, 6), [7]),
+        "sst_cav_3block": lambda: _clip(pk.cases.split_i(pk.cases.config2(14, 8, 6), [5, 10]), 1),
     }
 
 

@@ -12,7 +12,7 @@ pytestmark = pytest.mark.gpu
 
 TOL = 1e-11
 NAMES = ["lam_roe_tvd", "lam_roe_mlp", "lam_ausm_tvd_vanleer", "sst_cav_ausm_mlp", "sst_cav_roe_first",
-         "sst_cav_as_written"]
+         "sst_cav_as_written", "lam_roe_2block", "sst_cav_3block"]
 
 
 def _case(pkg, name):
