@@ -14,6 +14,9 @@ def small_cases(pk):
         "sst_cav_roe_first": lambda: _clip(bc(12, 8, 6, True, True, 1, 0, 0, cfl=5.0, seed=5), 1),
         "sst_cav_as_written": lambda: _clip(pk.cases.config2(12, 8, 6), 0),
         "lam_roe_2block": lambda: pk.cases.split_i(pk.cases.config1(12, 8, 6), [7]),
+        "lam_roe_bdf2": lambda: bc(12, 8, 6, False, False, 1, 1, 1, cfl=5.0, nsteady=2, dt_phy=2e-4, prec=2, seed=6),
+        "sst_cav_bdf1": lambda: _clip(bc(12, 8, 6, True, True, 4, 2, 8, cfl=5.0, nsteady=1, dt_phy=2e-4, prec=2,
+                                         seed=8), 1),
         "sst_cav_3block": lambda: _clip(pk.cases.split_i(pk.cases.config2(14, 8, 6), [5, 10]), 1),
     }
 

@@ -12,7 +12,7 @@ pytestmark = pytest.mark.gpu
 
 TOL = 1e-11
 NAMES = ["lam_roe_tvd", "lam_roe_mlp", "lam_ausm_tvd_vanleer", "sst_cav_ausm_mlp", "sst_cav_roe_first",
-         "sst_cav_as_written", "lam_roe_2block", "sst_cav_3block"]
+         "sst_cav_as_written", "lam_roe_2block", "sst_cav_3block", "lam_roe_bdf2", "sst_cav_bdf1"]
 
 
 def _case(pkg, name):
@@ -116,4 +116,27 @@ def test_timeinteg_matches_oracle(pkg, orc, name):
         assert np.all(rel_err(l2g, l2o) < TOL)
         assert list(ijkg) == list(ijko)
         assert abs(rmg - rmo) <= TOL * abs(rmo)
+    s.close()
+
+
+@pytest.mark.parametrize("name", ["lam_roe_bdf2", "sst_cav_bdf1"])
+def test_dual_time_shift_matches_oracle(pkg, orc, name):
+    """Two physical steps of dual time: inner iterations, the qq shift
+    (src/solve_module.f90:155-183), more inner iterations."""
+    case = _case(pkg, name)
+    s, o = pair(pkg, orc, case)
+    npv = case.npv
+    for step in (1, 2):
+        for it in range(2):
+            tg = s.timeinteg(step, it + 1, 0.0)
+            o.setqres()
+            to = o.timeinteg(step, it + 1, 0.0)
+            assert tg == to
+        for ib in range(len(case.blocks)):
+            pg, _, _, qg = s.get_state(ib)
+            po, _, _, qo = o.get_state(ib)
+            assert np.all(rel_err(interior(case, ib, pg, npv), interior(case, ib, po, npv)) < TOL)
+            assert np.all(rel_err(qg.reshape(-1, npv), qo.reshape(-1, npv)) < TOL)
+        s.unsteady_shift()
+        o.unsteady_shift()
     s.close()
