@@ -14,13 +14,16 @@
 //   k_face<D>             reconstruction + EOS + inviscid/viscous face flux
 //   k_combine             flux differences in the reference's i,j,k order
 //                         + Merkle / SST / BDF sources   (rhs_module calrhs)
-//   k_sweep<fwd>          one hyperplane i+j+k=s of the L / U sweep
+//   k_lhs_prep            sweep-independent terms of D^-1 and of the Jacobians
+//   k_sweep_tile<fwd>     the L (U) sweep: tiles of i-pencils, hyperplane order,
+//                         LDS hand-over inside a tile, coherent polls across tiles
 //   k_update, k_eddy      pv += dq, clip, EOS, norms; SST eddy viscosity
 #include "phys.hpp"
 #include "../../include/cav3d.h"
 
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <array>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -705,14 +708,16 @@ __device__ __forceinline__ void wait_dq(const double *dq, long NW, long wm, int 
     bool ready;
     do {
         ready = true;
-        for (int q = 0; q < npv; q++) {
-            x[q] = ld_dev(dq + q * NW + wm);
+#pragma unroll
+        for (int q = 0; q < NPV; q++) {
+            x[q] = (q < npv) ? ld_dev(dq + q * NW + wm) : 0.0;
             if ((unsigned long long)__double_as_longlong(x[q]) == DQ_UNSET) ready = false;
         }
         if (!ready) {
             if (++polls > POLL_LIMIT || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
                 atomicOr(err, 1);
-                for (int q = 0; q < npv; q++) x[q] = 0.0;
+#pragma unroll
+                for (int q = 0; q < NPV; q++) x[q] = 0.0;
                 return;
             }
             __builtin_amdgcn_s_sleep(1);
@@ -720,96 +725,138 @@ __device__ __forceinline__ void wait_dq(const double *dq, long NW, long wm, int 
     } while (!ready);
 }
 
-struct NbData {               // what the Jacobian of one neighbour reads
-    double pv[NPV], d[ND], n3[3];
-    JacCell jc;
-    JacFace jf;
-};
-
-__device__ __forceinline__ void load_nb(const DS &S, const BlkD &B, int ax, int face_side, const int *m,
-                                        const int *fpos, NbData &nb)
+// Row `lane` of the Jacobian of cell m through its face on axis ax (side 0:
+// minus face, 1: plus face); fpos = the face's index position.
+__device__ __forceinline__ JacNb load_jnb(const DS &S, const BlkD &B, int ax, int side, const int *m,
+                                          const int *fpos, int lane)
 {
     const long gm = gidx(B, m[0], m[1], m[2]), cm = cidx(B, m[0], m[1], m[2]);
-    for (int q = 0; q < S.npv; q++) nb.pv[q] = S.pv[q * S.NG + gm];
-    for (int q = 0; q < ND; q++) nb.d[q] = S.dc[q * S.NG + gm];
-    face_vec(S, B, ax, fpos[0], fpos[1], fpos[2], nb.n3);
-    const int fq = 2 * ax + face_side;
-    nb.jc = JacCell{S.jc[cm], S.jc[S.NC + cm]};
-    nb.jf = JacFace{S.jf[(3 * fq) * S.NC + cm], S.jf[(3 * fq + 1) * S.NC + cm], S.jf[(3 * fq + 2) * S.NC + cm]};
+    JacNb q;
+    q.pvn = (lane < S.npv) ? S.pv[lane * S.NG + gm] : 0.0;
+    q.pv1 = S.pv[1 * S.NG + gm]; q.pv2 = S.pv[2 * S.NG + gm]; q.pv3 = S.pv[3 * S.NG + gm];
+    q.rho = S.dc[D_RHO * S.NG + gm]; q.rp = S.dc[D_RP * S.NG + gm]; q.rt = S.dc[D_RT * S.NG + gm];
+    q.ry1 = S.dc[D_RY1 * S.NG + gm]; q.ry2 = S.dc[D_RY2 * S.NG + gm]; q.hp = S.dc[D_HP * S.NG + gm];
+    q.ht = S.dc[D_HT * S.NG + gm]; q.hy1 = S.dc[D_HY1 * S.NG + gm]; q.hy2 = S.dc[D_HY2 * S.NG + gm];
+    face_vec(S, B, ax, fpos[0], fpos[1], fpos[2], q.n3);
+    q.n3n = 0.0;
+    if (lane >= 1 && lane <= 3) {       // the lane's own component, read with a lane-dependent address
+        const double *a;
+        long nn, o;
+        if (ax == 0) { a = S.cx; nn = S.NCX; o = B.cx0 + (fpos[0] - 1) + (long)B.I * ((fpos[1] - 1) + (long)(B.J + 1) * (fpos[2] - 1)); }
+        else if (ax == 1) { a = S.ex; nn = S.NEX; o = B.ex0 + (fpos[0] - 1) + (long)(B.I + 1) * ((fpos[1] - 1) + (long)B.J * (fpos[2] - 1)); }
+        else { a = S.tx; nn = S.NTX; o = B.tx0 + (fpos[0] - 1) + (long)(B.I + 1) * ((fpos[1] - 1) + (long)(B.J + 1) * (fpos[2] - 1)); }
+        q.n3n = a[(lane - 1) * nn + o];
+    }
+    const int fq = 2 * ax + side;
+    q.jc = JacCell{S.jc[cm], S.jc[S.NC + cm]};
+    q.jf = JacFace{S.jf[(3 * fq) * S.NC + cm], S.jf[(3 * fq + 1) * S.NC + cm], S.jf[(3 * fq + 2) * S.NC + cm]};
+    return q;
 }
 
+// Tiled sweep: a workgroup owns a tile of TJ x TK pencils of one block and
+// walks its hyperplanes s = i + j + k in sweep order with one barrier per
+// hyperplane.  A pencil's neighbour inside the tile hands over its dq through
+// LDS (double-buffered by the parity of s); a neighbour in another tile
+// through the device-coherent dq array, waited for with a bounded poll.
+// Tiles wait only on tiles upstream in j and k, every tile is resident
+// (cooperative launch) and a workgroup takes its tiles in topological order,
+// so the chain always advances.
+constexpr int TJ = 4, TK = 4;
+
 template <bool FWD>
-__device__ void sweep_pencil(const DS &S, const BlkD &B, int j, int k, int n, int *err)
+__global__ __launch_bounds__(256) void k_sweep_tile(DS S, const int *tiles, int ntiles, int *err)
 {
+    __shared__ double lds[2][TJ * TK][NPV];
     const int npv = S.npv;
+    const int lane = threadIdx.x & 15, grp = threadIdx.x >> 4;     // 16 groups of 16 lanes
+    const int gj = grp % TJ, gk = grp / TJ;
     const int sg = FWD ? 1 : -1;
     const double sgn = FWD ? 1.0 : -1.0;
     double *dq = FWD ? S.dqf : S.dqb;
-    const int side = FWD ? 1 : 0;                    // the neighbour's face shared with this cell
-    const int i0 = FWD ? 2 : B.I, i1 = FWD ? B.I : 2;
-    NbData prev;                                     // the previous cell of this pencil
-    double xprev = 0.0;
-    for (int i = i0;; i += sg) {
-        const long c = cidx(B, i, j, k), w = widx(B, i, j, k), g = gidx(B, i, j, k);
-        // static data of the j and k neighbours first (independent of the polls)
-        NbData nbj, nbk;
-        const int mj[3] = {i, j - sg, k}, mk[3] = {i, j, k - sg};
-        const bool hj = (mj[1] >= 2 && mj[1] <= B.J), hk = (mk[2] >= 2 && mk[2] <= B.K);
-        const int self[3] = {i, j, k};
-        if (hj) load_nb(S, B, 1, side, mj, FWD ? mj : self, nbj);
-        if (hk) load_nb(S, B, 2, side, mk, FWD ? mk : self, nbk);
-        double y = 0.0;
-        if (n < npv) y = FWD ? S.res[n * S.NC + c] : S.dcv[n * S.NW + w];
-        if (i != i0) {
-            double xp[NPV];
+    const int side = FWD ? 1 : 0;
+    for (int tq = blockIdx.x; tq < ntiles; tq += gridDim.x) {
+        const int *td = tiles + 4 * (FWD ? tq : ntiles - 1 - tq);
+        const BlkD &B = S.b[td[0]];
+        const int j0 = td[1], k0 = td[2];
+        const int jend = min(j0 + TJ - 1, B.J), kend = min(k0 + TK - 1, B.K);
+        const int j = j0 + gj, k = k0 + gk;
+        const bool live = (j <= jend && k <= kend);
+        // the tile's hyperplane range in sweep order
+        const int sa = FWD ? 2 + j0 + k0 : B.I + jend + kend;
+        const int sb = FWD ? B.I + jend + kend : 2 + j0 + k0;
+        JacNb prev{};
+        double xprev = 0.0;
+        bool have_prev = false;
+        for (int s = sa;; s += sg) {
+            const int i = s - j - k;
+            if (live && i >= 2 && i <= B.I) {
+                const long c = cidx(B, i, j, k), w = widx(B, i, j, k), g = gidx(B, i, j, k);
+                const int mj[3] = {i, j - sg, k}, mk[3] = {i, j, k - sg};
+                const bool hj = (mj[1] >= 2 && mj[1] <= B.J), hk = (mk[2] >= 2 && mk[2] <= B.K);
+                const bool lj = hj && mj[1] >= j0 && mj[1] <= jend, lk = hk && mk[2] >= k0 && mk[2] <= kend;
+                const int self[3] = {i, j, k};
+                double y = 0.0;
+                if (lane < npv) y = FWD ? S.res[lane * S.NC + c] : S.dcv[lane * S.NW + w];
+                double xv[NPV];
+                if (have_prev) {
 #pragma unroll
-            for (int q = 0; q < NPV; q++) xp[q] = __shfl(xprev, q, 16);
-            jac_row(npv, prev.pv, prev.d, prev.jc, prev.jf, prev.n3, sg, n, xp, y, sgn);
-        }
-        if (hj) {
-            double xj[NPV];
-            wait_dq(dq, S.NW, widx(B, mj[0], mj[1], mj[2]), npv, xj, err);
-            jac_row(npv, nbj.pv, nbj.d, nbj.jc, nbj.jf, nbj.n3, sg, n, xj, y, sgn);
-        }
-        if (hk) {
-            double xk[NPV];
-            wait_dq(dq, S.NW, widx(B, mk[0], mk[1], mk[2]), npv, xk, err);
-            jac_row(npv, nbk.pv, nbk.d, nbk.jc, nbk.jf, nbk.n3, sg, n, xk, y, sgn);
-        }
-        if (FWD && n < npv) S.dcv[n * S.NW + w] = y;
-        double yy[NPV];
+                    for (int q = 0; q < NPV; q++) xv[q] = __shfl(xprev, q, 16);
+                    jac_row(npv, prev, sg, lane, xv, xprev, y, sgn);
+                }
+                const int pb = (s - sg) & 1;                 // buffer the previous hyperplane wrote
+                if (hj) {
+                    const JacNb nbj = load_jnb(S, B, 1, side, mj, FWD ? mj : self, lane);
+                    double xo;
+                    if (lj) {
 #pragma unroll
-        for (int q = 0; q < NPV; q++) yy[q] = __shfl(y, q, 16);
-        // this cell: data for D^-1 now and for the next cell's i-neighbour term
-        for (int q = 0; q < npv; q++) prev.pv[q] = S.pv[q * S.NG + g];
-        for (int q = 0; q < ND; q++) prev.d[q] = S.dc[q * S.NG + g];
-        const int fq = side;                         // face along i shared with the next cell
-        prev.jc = JacCell{S.jc[c], S.jc[S.NC + c]};
-        prev.jf = JacFace{S.jf[(3 * fq) * S.NC + c], S.jf[(3 * fq + 1) * S.NC + c], S.jf[(3 * fq + 2) * S.NC + c]};
-        face_vec(S, B, 0, FWD ? i : i - 1, j, k, prev.n3);
-        double xn = 0.0;
-        if (n < npv) {
-            const double *gxc = S.gx + c;
-            const long NC = S.NC;
-            xn = getx_apply([&](int q) { return gxc[q * NC]; }, prev.pv, prev.d[D_RY1], prev.d[D_RT], yy, n);
-            st_dev(dq + n * S.NW + w, xn);
+                        for (int q = 0; q < NPV; q++) xv[q] = lds[pb][grp - sg][q];
+                        xo = lds[pb][grp - sg][lane < NPV ? lane : 0];
+                    } else {
+                        const long wm = widx(B, mj[0], mj[1], mj[2]);
+                        wait_dq(dq, S.NW, wm, npv, xv, err);
+                        xo = (lane < npv) ? ld_dev(dq + lane * S.NW + wm) : 0.0;
+                    }
+                    jac_row(npv, nbj, sg, lane, xv, xo, y, sgn);
+                }
+                if (hk) {
+                    const JacNb nbk = load_jnb(S, B, 2, side, mk, FWD ? mk : self, lane);
+                    double xo;
+                    if (lk) {
+#pragma unroll
+                        for (int q = 0; q < NPV; q++) xv[q] = lds[pb][grp - sg * TJ][q];
+                        xo = lds[pb][grp - sg * TJ][lane < NPV ? lane : 0];
+                    } else {
+                        const long wm = widx(B, mk[0], mk[1], mk[2]);
+                        wait_dq(dq, S.NW, wm, npv, xv, err);
+                        xo = (lane < npv) ? ld_dev(dq + lane * S.NW + wm) : 0.0;
+                    }
+                    jac_row(npv, nbk, sg, lane, xv, xo, y, sgn);
+                }
+                if (FWD && lane < npv) S.dcv[lane * S.NW + w] = y;
+#pragma unroll
+                for (int q = 0; q < NPV; q++) xv[q] = __shfl(y, q, 16);
+                // this cell's row of its Jacobian through the face it shares with the next cell
+                const int nxt[3] = {FWD ? i : i - 1, j, k};
+                prev = load_jnb(S, B, 0, side, self, nxt, lane);
+                double xn = 0.0;
+                if (lane < npv) {
+                    double pc[NPV];
+#pragma unroll
+                    for (int q = 0; q < NPV; q++) pc[q] = (q < npv) ? S.pv[q * S.NG + g] : 0.0;
+                    const double *gxc = S.gx + c;
+                    const long NC = S.NC;
+                    xn = getx_apply([&](int q) { return gxc[q * NC]; }, pc, S.dc[D_RY1 * S.NG + g], S.dc[D_RT * S.NG + g],
+                                    xv, lane, y, S.pv[lane * S.NG + g]);
+                    st_dev(dq + lane * S.NW + w, xn);
+                }
+                if (lane < NPV) lds[s & 1][grp][lane] = xn;
+                xprev = xn;
+                have_prev = true;
+            }
+            __syncthreads();
+            if (s == sb) break;
         }
-        xprev = xn;
-        if (i == i1) break;
-    }
-}
-
-// slots: four (block, j, k) entries per wavefront-quad, all on one diagonal
-// j+k (block = -1 pads); quads sorted by diagonal.
-template <bool FWD>
-__global__ __launch_bounds__(256) void k_sweep(DS S, const int *slots, long nquad, int *err)
-{
-    const int lane = threadIdx.x & 15, grp = (threadIdx.x >> 4) & 3;
-    const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const long nwave = ((long)gridDim.x * blockDim.x) >> 6;
-    for (long qd = wave; qd < nquad; qd += nwave) {
-        const int *sl = slots + 3 * (4 * qd + grp);
-        if (sl[0] >= 0) sweep_pencil<FWD>(S, S.b[sl[0]], sl[1], sl[2], lane, err);
+        __syncthreads();
     }
 }
 
@@ -1036,8 +1083,8 @@ struct cav3d_ctx {
     size_t stage_n = 0;
     int nwg_update = 0;
     int coop_blocks = 0;      // grid of the cooperative sweep kernels
-    int *slots = nullptr;     // pencil table of the sweeps (device)
-    long nquad = 0;
+    int *tiles = nullptr;     // tile table of the sweeps (device): block, j0, k0, diagonal
+    int ntiles = 0;
     int *sweep_err = nullptr; // set when a sweep wait gave up
     bool profile = false;
     hipEvent_t ev[8] = {};
@@ -1252,14 +1299,14 @@ int launch_sweeps(cav3d_ctx *c)
     const long nq = (long)S.npv * S.NW;
     hipLaunchKernelGGL(k_fill_bits, dim3(nblk(nq, 256)), dim3(256), 0, c->st, S.dqf, nq, DQ_UNSET);
     hipLaunchKernelGGL(k_fill_bits, dim3(nblk(nq, 256)), dim3(256), 0, c->st, S.dqb, nq, DQ_UNSET);
-    const int *slots = c->slots;
-    long nquad = c->nquad;
+    const int *tiles = c->tiles;
+    int ntiles = c->ntiles;
     int *err = c->sweep_err;
-    void *args[4] = {&S, &slots, &nquad, &err};
+    void *args[4] = {&S, &tiles, &ntiles, &err};
     dim3 grid(c->coop_blocks), block(256);
-    hipError_t e = hipLaunchCooperativeKernel((const void *)k_sweep<true>, grid, block, args, 0, c->st);
+    hipError_t e = hipLaunchCooperativeKernel((const void *)k_sweep_tile<true>, grid, block, args, 0, c->st);
     if (e != hipSuccess) return fail(std::string("forward sweep launch: ") + hipGetErrorString(e), -3);
-    e = hipLaunchCooperativeKernel((const void *)k_sweep<false>, grid, block, args, 0, c->st);
+    e = hipLaunchCooperativeKernel((const void *)k_sweep_tile<false>, grid, block, args, 0, c->st);
     if (e != hipSuccess) return fail(std::string("backward sweep launch: ") + hipGetErrorString(e), -3);
     return 0;
 }
@@ -1428,39 +1475,36 @@ int cav3d_create(const cav3d_params *prm, int nblocks, const cav3d_block *blocks
     if ((rc = dalloc(c, &S.gx, (long)NGX * S.NC)) || (rc = dalloc(c, &S.jc, 2 * S.NC)) ||
         (rc = dalloc(c, &S.jf, 18 * S.NC))) { cav3d_destroy(c); return rc; }
     {
-        // pencil table: per diagonal t = j + k of each block, pencils in groups of
-        // four (one wavefront), diagonals in increasing order over all blocks
-        std::vector<int> tab;
-        int tmax = 0;
-        for (int b = 0; b < nblocks; b++) tmax = std::max(tmax, c->blk[b].J + c->blk[b].K);
-        for (int t = 4; t <= tmax; t++) {
-            std::vector<int> diag;
-            for (int b = 0; b < nblocks; b++)
-                for (int k = 2; k <= c->blk[b].K; k++) {
-                    int j = t - k;
-                    if (j >= 2 && j <= c->blk[b].J) { diag.push_back(b); diag.push_back(j); diag.push_back(k); }
-                }
-            while ((diag.size() / 3) % 4) { diag.push_back(-1); diag.push_back(0); diag.push_back(0); }
-            tab.insert(tab.end(), diag.begin(), diag.end());
+        // tile table: TJ x TK pencil tiles of every block, ordered by the tile
+        // diagonal tj + tk (the forward sweep takes them in this order, the
+        // backward sweep in reverse)
+        std::vector<std::array<int, 4>> tl;
+        for (int b = 0; b < nblocks; b++) {
+            const int ntj = (c->blk[b].J - 1 + TJ - 1) / TJ, ntk = (c->blk[b].K - 1 + TK - 1) / TK;
+            for (int tk = 0; tk < ntk; tk++)
+                for (int tj = 0; tj < ntj; tj++) tl.push_back({b, 2 + TJ * tj, 2 + TK * tk, tj + tk});
         }
-        c->nquad = (long)tab.size() / 12;
-        if ((rc = dalloc(c, &c->slots, tab.size())) || (rc = dalloc(c, &c->sweep_err, 1))) {
+        std::stable_sort(tl.begin(), tl.end(), [](const std::array<int, 4> &x, const std::array<int, 4> &y) {
+            return x[3] < y[3];
+        });
+        std::vector<int> tab;
+        for (auto &t : tl) tab.insert(tab.end(), t.begin(), t.end());
+        c->ntiles = (int)tl.size();
+        if ((rc = dalloc(c, &c->tiles, tab.size())) || (rc = dalloc(c, &c->sweep_err, 1))) {
             cav3d_destroy(c); return rc;
         }
-        if (hipMemcpyAsync(c->slots, tab.data(), tab.size() * sizeof(int), hipMemcpyHostToDevice, c->st) != hipSuccess) {
-            cav3d_destroy(c); return fail("pencil table upload failed", -3);
+        if (hipMemcpyAsync(c->tiles, tab.data(), tab.size() * sizeof(int), hipMemcpyHostToDevice, c->st) != hipSuccess) {
+            cav3d_destroy(c); return fail("tile table upload failed", -3);
         }
-        // cooperative grid: no more wavefronts than quads, no more blocks than are co-resident
         int dev = 0, ncu = 0, per = 0, per2 = 0;
         hipGetDevice(&dev);
         hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void *)k_sweep<true>, 256, 0);
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per2, (const void *)k_sweep<false>, 256, 0);
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void *)k_sweep_tile<true>, 256, 0);
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per2, (const void *)k_sweep_tile<false>, 256, 0);
         per = std::min(per, per2);
-        long need = (c->nquad + 3) / 4;
         long cap = (long)ncu * per;
         if (cap < 1) { cav3d_destroy(c); return fail("sweep kernel cannot be made resident", -3); }
-        c->coop_blocks = (int)std::max(1L, std::min(need, cap));
+        c->coop_blocks = (int)std::max(1L, std::min((long)c->ntiles, cap));
     }
     c->nwg_update = (int)nblk(S.NC, 256);
     if ((rc = dalloc(c, &S.red, (long)c->nwg_update * (NPV + 2)))) { cav3d_destroy(c); return rc; }

@@ -677,49 +677,81 @@ HD JacFace jac_face(const Par &P, const CellJ &c, const double *n3)
     return f;
 }
 
-// row n of y += sgn * A x; pv/d of the cell the Jacobian belongs to
-HD void jac_row(int npv, const double *pv, const double *d, const JacCell &jc, const JacFace &jf, const double *n3,
-                int sign, int n, const double *x, double &y, double sgn)
+// What row n of a Jacobian reads: the cell's pv(n), velocity, density
+// derivatives and enthalpy derivatives, the face vector and the
+// precomputed cell/face terms.
+struct JacNb {
+    double pvn, pv1, pv2, pv3;
+    double rho, rp, rt, ry1, ry2, hp, ht, hy1, hy2;
+    double n3[3];
+    double n3n;       // n3[n-1] for rows 1..3 (rows are lanes: no run-time register indexing)
+    JacCell jc;
+    JacFace jf;
+};
+
+HD JacNb jac_nb(const double *pv, const double *d, const double *n3, const JacCell &jc, const JacFace &jf, int n, int npv)
 {
-    const double lam = sign > 0 ? jf.lam : -jf.lam, vis = sign > 0 ? jf.vis : -jf.vis;
-    const double u = jf.u;
+    JacNb q;
+    q.pvn = (n < npv) ? pv[n] : 0.0;
+    q.pv1 = pv[1]; q.pv2 = pv[2]; q.pv3 = pv[3];
+    q.rho = d[D_RHO]; q.rp = d[D_RP]; q.rt = d[D_RT]; q.ry1 = d[D_RY1]; q.ry2 = d[D_RY2];
+    q.hp = d[D_HP]; q.ht = d[D_HT]; q.hy1 = d[D_HY1]; q.hy2 = d[D_HY2];
+    q.n3[0] = n3[0]; q.n3[1] = n3[1]; q.n3[2] = n3[2];
+    q.n3n = (n >= 1 && n <= 3) ? n3[n - 1] : 0.0;
+    q.jc = jc;
+    q.jf = jf;
+    return q;
+}
+
+// row n of y += sgn * A x; xn = x[n]
+HD void jac_row(int npv, const JacNb &q, int sign, int n, const double *x, double xn, double &y, double sgn)
+{
+    const double lam = sign > 0 ? q.jf.lam : -q.jf.lam, vis = sign > 0 ? q.jf.vis : -q.jf.vis;
+    const double u = q.jf.u;
     double r1[7];
-    r1[0] = u * d[D_RP] + lam * jc.beta + vis * d[D_RP];
-    r1[1] = n3[0] * d[D_RHO];
-    r1[2] = n3[1] * d[D_RHO];
-    r1[3] = n3[2] * d[D_RHO];
-    r1[4] = u * d[D_RT] + lam * d[D_RT] + vis * d[D_RT];
-    r1[5] = u * d[D_RY1] + lam * d[D_RY1] + vis * d[D_RY1];
-    r1[6] = u * d[D_RY2] + lam * d[D_RY2] + vis * d[D_RY2];
+    r1[0] = u * q.rp + lam * q.jc.beta + vis * q.rp;
+    r1[1] = q.n3[0] * q.rho;
+    r1[2] = q.n3[1] * q.rho;
+    r1[3] = q.n3[2] * q.rho;
+    r1[4] = u * q.rt + lam * q.rt + vis * q.rt;
+    r1[5] = u * q.ry1 + lam * q.ry1 + vis * q.ry1;
+    r1[6] = u * q.ry2 + lam * q.ry2 + vis * q.ry2;
     const double uuu = u + lam + vis;
-    const double ru = d[D_RHO] * uuu;
+    const double ru = q.rho * uuu;
     auto acc = [&](double a, int l) {
         double t = (0.5 * a) * x[l];
         y = (sgn > 0.0) ? y + t : y - t;
     };
     if (n == 0) {
+#pragma unroll
         for (int l = 0; l < 7; l++) acc(r1[l], l);
     } else if (n <= 3) {
+#pragma unroll
         for (int l = 0; l < 7; l++) {
-            double a = pv[n] * r1[l];
-            if (l == 0) a = pv[n] * r1[0] + n3[n - 1];
-            if (l == n) a = pv[n] * r1[n] + ru;
+            double a = q.pvn * r1[l];
+            if (l == 0) a = q.pvn * r1[0] + q.n3n;
+            if (l == n) a = q.pvn * r1[n] + ru;
             acc(a, l);
         }
     } else if (n == 4) {
-        const double hy[7] = {d[D_HP], pv[1], pv[2], pv[3], d[D_HT], d[D_HY1], d[D_HY2]};
+        const double hy[7] = {q.hp, q.pv1, q.pv2, q.pv3, q.ht, q.hy1, q.hy2};
+#pragma unroll
         for (int l = 0; l < 7; l++) {
-            double a = ru * hy[l] + jc.h * r1[l];
+            double a = ru * hy[l] + q.jc.h * r1[l];
             if (l == 0) a = a - lam - vis;
             acc(a, l);
         }
     } else if (n < npv) {
+#pragma unroll
         for (int l = 0; l < 7; l++) {
-            double a = pv[n] * r1[l];
-            if (l == n) a = pv[n] * r1[n] + ru;
+            double a = q.pvn * r1[l];
+            if (l == n) a = q.pvn * r1[n] + ru;
             acc(a, l);
         }
-        if (n >= 7) acc(uuu * d[D_RHO], n);
+        if (n >= 7) {
+            double t = (0.5 * (uuu * q.rho)) * xn;
+            y = (sgn > 0.0) ? y + t : y - t;
+        }
     }
 }
 
@@ -727,7 +759,7 @@ HD void jac_apply(const Par &P, int npv, const CellJ &c, const double *n3, int s
 {
     JacCell jc = jac_cell(P, c);
     JacFace jf = jac_face(P, c, n3);
-    for (int n = 0; n < npv; n++) jac_row(npv, c.pv, c.d, jc, jf, n3, sign, n, x, y[n], sgn);
+    for (int n = 0; n < npv; n++) jac_row(npv, jac_nb(c.pv, c.d, n3, jc, jf, n, npv), sign, n, x, x[n], y[n], sgn);
 }
 
 // Closed-form inverse of the preconditioned diagonal block with the Merkle
@@ -819,10 +851,10 @@ HD void getx_prep(const Par &P, int npv, const CellJ &c, const double f6[6][3], 
 
 // Component n of x = D^-1 res; G(q) returns the GxPre entry q.
 template <class F>
-HD double getx_apply(F G, const double *pv, double d_ry1, double d_rt, const double *res, int n)
+HD double getx_apply(F G, const double *pv, double d_ry1, double d_rt, const double *res, int n, double resn, double pvn)
 {
     const double a = G(GX_A);
-    if (n >= 1 && n <= 3) return (res[n] - res[0] * pv[n]) * G(GX_IB1) * a;
+    if (n >= 1 && n <= 3) return (resn - res[0] * pvn) * G(GX_IB1) * a;
     double r6 = res[6] - res[0] * pv[6];
     if (n == 6) return r6 * G(GX_IB1) * a;
     if (n >= 7) {
@@ -849,7 +881,8 @@ HD void getx(const Par &P, int npv, const CellJ &c, const double f6[6][3], doubl
 {
     double g[NGX];
     getx_prep(P, npv, c, f6, dt, cc, tt4, g);
-    for (int n = 0; n < npv; n++) x[n] = getx_apply([&](int q) { return g[q]; }, c.pv, c.d[D_RY1], c.d[D_RT], res, n);
+    for (int n = 0; n < npv; n++)
+        x[n] = getx_apply([&](int q) { return g[q]; }, c.pv, c.d[D_RY1], c.d[D_RT], res, n, res[n], c.pv[n]);
 }
 
 // local pseudo time step (src/timestep_module.f90:156-198)
