@@ -124,7 +124,7 @@ __device__ __forceinline__ int find_blk(const DS &S, long t, int which)
 __device__ __forceinline__ void load_cell(const DS &S, const BlkD &B, int i, int j, int k, CellJ &c)
 {
     long g = gidx(B, i, j, k);
-    for (int n = 0; n < S.npv; n++) c.pv[n] = S.pv[n * S.NG + g];
+    for (int n = 0; n < NPV; n++) if (n < S.npv) c.pv[n] = S.pv[n * S.NG + g];
     for (int m = 0; m < ND; m++) c.d[m] = S.dc[m * S.NG + g];
     c.mut = S.mut ? S.mut[g] : 0.0;
     c.vol = S.vol[widx(B, i, j, k)];
@@ -150,7 +150,7 @@ __device__ __forceinline__ void cell_of(const BlkD &B, long r, int &i, int &j, i
 
 // ------------------------------------------------------------ EOS kernels
 // dc (and mut = emutref) of interior cells from pv
-__global__ void k_eos_cells(DS S, int interior_only, int set_mut)
+__global__ __launch_bounds__(256) void k_eos_cells(DS S, int interior_only, int set_mut)
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     long n = interior_only ? S.NC : S.NG;
@@ -170,7 +170,7 @@ __global__ void k_eos_cells(DS S, int interior_only, int set_mut)
 }
 
 // full dv(18), tv(1:2) of every ghosted cell, AoS for the host
-__global__ void k_eos_full(DS S, long g0, long n, double *dv, double *tv2)
+__global__ __launch_bounds__(256) void k_eos_full(DS S, long g0, long n, double *dv, double *tv2)
 {
     long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= n) return;
@@ -183,7 +183,7 @@ __global__ void k_eos_full(DS S, long g0, long n, double *dv, double *tv2)
     tv2[2 * q + 1] = d[D_KAP];
 }
 
-__global__ void k_eos_points(Par P, int n, const double *p, const double *t, const double *y1, const double *y2,
+__global__ __launch_bounds__(256) void k_eos_points(Par P, int n, const double *p, const double *t, const double *y1, const double *y2,
                              double *dv, double *tv)
 {
     int q = blockIdx.x * blockDim.x + threadIdx.x;
@@ -196,7 +196,7 @@ __global__ void k_eos_points(Par P, int n, const double *p, const double *t, con
 }
 
 // ------------------------------------------------------------ time step
-__global__ void k_reset_scal(DS S)
+__global__ __launch_bounds__(256) void k_reset_scal(DS S)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         unsigned long long big = 0x7ff0000000000000ull;   // +inf
@@ -233,7 +233,7 @@ __global__ __launch_bounds__(256) void k_timestep(DS S)
     }
 }
 
-__global__ void k_time_acc(DS S)
+__global__ __launch_bounds__(256) void k_time_acc(DS S)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         double m = __longlong_as_double((long long)reinterpret_cast<unsigned long long *>(S.scal)[0]);
@@ -243,7 +243,7 @@ __global__ void k_time_acc(DS S)
     }
 }
 
-__global__ void k_dt_fill(DS S)
+__global__ __launch_bounds__(256) void k_dt_fill(DS S)
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < S.NC) S.dt[t] = __longlong_as_double((long long)reinterpret_cast<unsigned long long *>(S.scal)[0]);
@@ -252,7 +252,7 @@ __global__ void k_dt_fill(DS S)
 // ------------------------------------------------------------ boundary conditions
 __device__ __forceinline__ void put_cell(const DS &S, long g, const double *pv, const double *d, double mut)
 {
-    for (int n = 0; n < S.npv; n++) S.pv[n * S.NG + g] = pv[n];
+    for (int n = 0; n < NPV; n++) if (n < S.npv) S.pv[n * S.NG + g] = pv[n];
     for (int m = 0; m < ND; m++) S.dc[m * S.NG + g] = d[m];
     if (S.mut) S.mut[g] = mut;
 }
@@ -298,7 +298,7 @@ __device__ void apply_bc_cell(const DS &S, const Seg &s, int i, int j, int k)
     long gm = gidx(B, m[0], m[1], m[2]);
     long gg = gidx(B, i, j, k);
     double pv[NPV], d[ND], mut = 0.0;
-    for (int n = 0; n < npv; n++) pv[n] = S.pv[n * S.NG + gm];
+    for (int n = 0; n < NPV; n++) if (n < npv) pv[n] = S.pv[n * S.NG + gm];
     for (int q = 0; q < ND; q++) d[q] = S.dc[q * S.NG + gm];
     if (S.mut) mut = S.mut[gm];
     switch (s.fn) {
@@ -337,7 +337,7 @@ __device__ void apply_bc_cell(const DS &S, const Seg &s, int i, int j, int k)
         double press = P.bc_pressure;
         if (s.fn == FN_INSUPER) {
             pv[0] = 2.0 * (press - rp[0]) - pv[0];
-            for (int q = 1; q < npv; q++) pv[q] = 2.0 * rp[q] - pv[q];
+            for (int q = 1; q < NPV; q++) if (q < npv) pv[q] = 2.0 * rp[q] - pv[q];
             pv[5] = mn(mx(pv[5], 0.0), 1.0);
             pv[6] = mn(mx(pv[6], 0.0), 1.0);
         } else {
@@ -346,7 +346,7 @@ __device__ void apply_bc_cell(const DS &S, const Seg &s, int i, int j, int k)
             surface_of(s, i, j, k, o);
             outward_of(S, B, s.face, o, n);
             long gs = gidx(B, o[0], o[1], o[2]);
-            for (int q = 0; q < npv; q++) ps[q] = S.pv[q * S.NG + gs];
+            for (int q = 0; q < NPV; q++) if (q < npv) ps[q] = S.pv[q * S.NG + gs];
             double rs = S.dc[D_RHO * S.NG + gs], cs = S.dc[D_C2 * S.NG + gs];
             double dl = 1.0 / sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
             double vel = (ps[1] * n[0] + ps[2] * n[1] + ps[3] * n[2]) * dl;
@@ -364,7 +364,7 @@ __device__ void apply_bc_cell(const DS &S, const Seg &s, int i, int j, int k)
                 pv[1] = 2.0 * (rp[1] + n[0] * 0.5 * var / rs / a * dl) - pv[1];
                 pv[2] = 2.0 * (rp[2] + n[1] * 0.5 * var / rs / a * dl) - pv[2];
                 pv[3] = 2.0 * (rp[3] + n[2] * 0.5 * var / rs / a * dl) - pv[3];
-                for (int q = 4; q < npv; q++) pv[q] = 2.0 * rp[q] - pv[q];
+                for (int q = 4; q < NPV; q++) if (q < npv) pv[q] = 2.0 * rp[q] - pv[q];
                 pv[5] = mn(mx(pv[5], 0.0), 1.0);
                 pv[6] = mn(mx(pv[6], 0.0), 1.0);
             } else if (mode == 1) {
@@ -375,7 +375,7 @@ __device__ void apply_bc_cell(const DS &S, const Seg &s, int i, int j, int k)
                 pv[3] = 2.0 * (ps[3] + n[2] * dp / rs / a * dl) - pv[3];
             } else if (mode == 3) {
                 pv[0] = 2.0 * (press - rp[0]) - pv[0];
-                for (int q = 1; q < npv; q++) pv[q] = 2.0 * rp[q] - pv[q];
+                for (int q = 1; q < NPV; q++) if (q < npv) pv[q] = 2.0 * rp[q] - pv[q];
                 pv[5] = mn(mx(pv[5], 0.0), 1.0);
                 pv[6] = mn(mx(pv[6], 0.0), 1.0);
             }
@@ -393,12 +393,12 @@ __device__ void apply_bc_cell(const DS &S, const Seg &s, int i, int j, int k)
         const bool corner = (s.fn == FN_CORNERWW || s.fn == FN_CORNERWWKW);
         const int nn = corner ? 6 : 3;
         double pp[NPV];
-        for (int q = 0; q < npv; q++) pp[q] = -pv[q];
+        for (int q = 0; q < NPV; q++) if (q < npv) pp[q] = -pv[q];
         for (int t = 0; t < nn; t++) {
             const int *nb = s.nb[t];
             if (!corner && nb[0] == 0 && nb[1] == 0 && nb[2] == 0) continue;
             long gn = gidx(B, nb[0] + s.dir[0] * i, nb[1] + s.dir[1] * j, nb[2] + s.dir[2] * k);
-            for (int q = 0; q < npv; q++) pp[q] = pp[q] - S.pv[q * S.NG + gn];
+            for (int q = 0; q < NPV; q++) if (q < npv) pp[q] = pp[q] - S.pv[q * S.NG + gn];
         }
         pv[1] = pp[1]; pv[2] = pp[2]; pv[3] = pp[3];
         if (s.fn == FN_EDGEWW || s.fn == FN_CORNERWW) {
@@ -420,11 +420,11 @@ __device__ void apply_bc_cell(const DS &S, const Seg &s, int i, int j, int k)
             const int *nb = s.nb[t];
             if (!corner && nb[0] == 0 && nb[1] == 0 && nb[2] == 0) continue;
             long gn = gidx(B, nb[0] + s.dir[0] * i, nb[1] + s.dir[1] * j, nb[2] + s.dir[2] * k);
-            for (int q = 0; q < npv; q++) pv[q] = pv[q] + S.pv[q * S.NG + gn];
+            for (int q = 0; q < NPV; q++) if (q < npv) pv[q] = pv[q] + S.pv[q * S.NG + gn];
             if (S.mut) mt = mt + S.mut[gn];
         }
         double f = corner ? 1.0 / 7.0 : 1.0 / 3.0;
-        for (int q = 0; q < npv; q++) pv[q] = f * pv[q];
+        for (int q = 0; q < NPV; q++) if (q < npv) pv[q] = f * pv[q];
         mt = f * mt;
         mixture_eos<double>(pv[0] + rp[0], pv[4], pv[5], pv[6], d, P.iturb != -3);
         put_cell(S, gg, pv, d, mt);
@@ -434,7 +434,7 @@ __device__ void apply_bc_cell(const DS &S, const Seg &s, int i, int j, int k)
     }
 }
 
-__global__ void k_bc(DS S, int which)
+__global__ __launch_bounds__(256) void k_bc(DS S, int which)
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= S.NBC[which]) return;
@@ -454,7 +454,7 @@ __global__ void k_bc(DS S, int which)
 }
 
 // interface copy: the sending block's cells to the donor block's ghost cells
-__global__ void k_halo(DS S)
+__global__ __launch_bounds__(256) void k_halo(DS S)
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= S.NHALO) return;
@@ -472,7 +472,7 @@ __global__ void k_halo(DS S)
     int kk = c.tm[6] * di + c.tm[7] * dj + c.tm[8] * dk + c.isd[2];
     long gs = gidx(S.b[c.src], c.is[0] + di, c.is[1] + dj, c.is[2] + dk);
     long gd = gidx(S.b[c.dst], ii, jj, kk);
-    for (int n = 0; n < S.npv; n++) S.pv[n * S.NG + gd] = S.pv[n * S.NG + gs];
+    for (int n = 0; n < NPV; n++) if (n < S.npv) S.pv[n * S.NG + gd] = S.pv[n * S.NG + gs];
     for (int m = 0; m < ND; m++) S.dc[m * S.NG + gd] = S.dc[m * S.NG + gs];
     if (S.mut) S.mut[gd] = S.mut[gs];
 }
@@ -481,39 +481,53 @@ __global__ void k_halo(DS S)
 // Direction D: normal axis D, first transverse axis E=(D+1)%3, second T=(D+2)%3,
 // matching the reference's stencil orderings (src/rhs_module.f90:288-312,
 // 374-398, 460-484): ll = 1 + dn + 2(de+1) + 6(dt+1) for dn in {0,1}.
+// face of direction D handled by thread t: its (i,j,k) (the left cell) and block
+template <int D>
+__device__ __forceinline__ const BlkD &face_of(const DS &S, long t, int &i, int &j, int &k)
+{
+    int b = 0;
+    for (int q = 1; q < S.nb; q++) if (t >= S.b[q].f0[D]) b = q;
+    const BlkD &B = S.b[b];
+    long r = t - B.f0[D];
+    if (D == 0) { i = 1 + (int)(r % B.I); j = 2 + (int)((r / B.I) % (B.J - 1)); k = 2 + (int)(r / ((long)B.I * (B.J - 1))); }
+    else if (D == 1) { i = 2 + (int)(r % (B.I - 1)); j = 1 + (int)((r / (B.I - 1)) % B.J); k = 2 + (int)(r / ((long)(B.I - 1) * B.J)); }
+    else { i = 2 + (int)(r % (B.I - 1)); j = 2 + (int)((r / (B.I - 1)) % (B.J - 1)); k = 1 + (int)(r / ((long)(B.I - 1) * (B.J - 1))); }
+    return B;
+}
+
+// offset of stencil point ll (Fortran numbering) from the face's left cell
+template <int D>
+__device__ __forceinline__ long stencil_off(const BlkD &B, int ll)
+{
+    const long st[3] = {1, (long)B.nx, (long)B.nx * B.ny};
+    const long sN = st[D], sE = st[(D + 1) % 3], sT = st[(D + 2) % 3];
+    int q = ll - 1;
+    if (q < 18) return (long)(q & 1) * sN + (long)((q >> 1) % 3 - 1) * sE + (long)(q / 6 - 1) * sT;
+    if (q < 27) { q -= 18; return -sN + (long)(q % 3 - 1) * sE + (long)(q / 3 - 1) * sT; }
+    if (q < 36) { q -= 27; return 2 * sN + (long)(q % 3 - 1) * sE + (long)(q / 3 - 1) * sT; }
+    return q == 36 ? -2 * sN : 3 * sN;
+}
+
+// Inviscid flux of every face of direction D (src/rhs_module.f90:282-337 and
+// the j/k analogues): reconstruction, clamps, face EOS, Roe / AUSM+-up.
 template <int D>
 __global__ __launch_bounds__(128) void k_face(DS S)
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= S.NF[D]) return;
-    int b = 0;
-    for (int q = 1; q < S.nb; q++) if (t >= S.b[q].f0[D]) b = q;
-    const BlkD &B = S.b[b];
+    int i, j, k;
+    const BlkD &B = face_of<D>(S, t, i, j, k);
     const Par &P = S.P;
     const int npv = S.npv;
-    long r = t - B.f0[D];
-    int i, j, k;
-    if (D == 0) { i = 1 + (int)(r % B.I); j = 2 + (int)((r / B.I) % (B.J - 1)); k = 2 + (int)(r / ((long)B.I * (B.J - 1))); }
-    else if (D == 1) { i = 2 + (int)(r % (B.I - 1)); j = 1 + (int)((r / (B.I - 1)) % B.J); k = 2 + (int)(r / ((long)(B.I - 1) * B.J)); }
-    else { i = 2 + (int)(r % (B.I - 1)); j = 2 + (int)((r / (B.I - 1)) % (B.J - 1)); k = 1 + (int)(r / ((long)(B.I - 1) * (B.J - 1))); }
-    const long st[3] = {1, (long)B.nx, (long)B.nx * B.ny};
-    const long sN = st[D], sE = st[(D + 1) % 3], sT = st[(D + 2) % 3];
     const long g9 = gidx(B, i, j, k);
-    auto off = [&](int ll) -> long {
-        int q = ll - 1;
-        if (q < 18) return (long)(q & 1) * sN + (long)((q >> 1) % 3 - 1) * sE + (long)(q / 6 - 1) * sT;
-        if (q < 27) { q -= 18; return -sN + (long)(q % 3 - 1) * sE + (long)(q / 3 - 1) * sT; }
-        if (q < 36) { q -= 27; return 2 * sN + (long)(q % 3 - 1) * sE + (long)(q / 3 - 1) * sT; }
-        return q == 36 ? -2 * sN : 3 * sN;
-    };
-    const bool visc = (P.iturb >= -2);
+    const long sN = D == 0 ? 1 : (D == 1 ? (long)B.nx : (long)B.nx * B.ny);
     double c9[NPV], c10[NPV], xl[NPV], xr[NPV];
-    double g3[6][3] = {};
+    double g3[6][3];
     const double *f0 = S.pv + g9;
-    face_recon<double>(P, npv, [&](int ll, int n) -> double { return f0[(long)n * S.NG + off(ll)]; },
+    Par Pi = P;
+    Pi.iturb = -3;          // no viscous differences here
+    face_recon<double>(Pi, npv, [&](int ll, int n) -> double { return f0[(long)n * S.NG + stencil_off<D>(B, ll)]; },
                        c9, c10, xl, xr, g3);
-    int ip = i, jp = j, kp = k;
-    if (D == 0) ip++; else if (D == 1) jp++; else kp++;
     const long gq = g9 + sN;
     double n3[3];
     face_vec(S, B, D, i, j, k, n3);
@@ -523,43 +537,82 @@ __global__ __launch_bounds__(128) void k_face(DS S)
     inviscid_face<double>(P, npv, n3, c9, c10, xl, xr, rh9, rh10, fx);
     double *out = S.ff[D];
     const long NF = S.NF[D];
-    for (int n = 0; n < npv; n++) out[n * NF + t] = fx[n];
-    if (visc) {
-        double e4[4][3], t4[4][3];
-        const int E = (D + 1) % 3, T = (D + 2) % 3;
-        int p0[3] = {i, j, k}, p1[3] = {ip, jp, kp};
-        int a0[3], a1[3];
-        // e-family: (P,e-1), (P+1,e-1), (P,e), (P+1,e)
-        for (int q = 0; q < 3; q++) { a0[q] = p0[q]; a1[q] = p1[q]; }
-        a0[E]--; a1[E]--;
-        face_vec(S, B, E, a0[0], a0[1], a0[2], e4[0]);
-        face_vec(S, B, E, a1[0], a1[1], a1[2], e4[1]);
-        face_vec(S, B, E, p0[0], p0[1], p0[2], e4[2]);
-        face_vec(S, B, E, p1[0], p1[1], p1[2], e4[3]);
-        // t-family: i faces (P,t-1),(P+1,t-1),(P,t),(P+1,t); j and k faces (P,t-1),(P,t),(P+1,t-1),(P+1,t)
-        for (int q = 0; q < 3; q++) { a0[q] = p0[q]; a1[q] = p1[q]; }
-        a0[T]--; a1[T]--;
-        face_vec(S, B, T, a0[0], a0[1], a0[2], t4[0]);
-        if (D == 0) {
-            face_vec(S, B, T, a1[0], a1[1], a1[2], t4[1]);
-            face_vec(S, B, T, p0[0], p0[1], p0[2], t4[2]);
-        } else {
-            face_vec(S, B, T, p0[0], p0[1], p0[2], t4[1]);
-            face_vec(S, B, T, a1[0], a1[1], a1[2], t4[2]);
+    for (int n = 0; n < NPV; n++) if (n < npv) out[n * NF + t] = fx[n];
+}
+
+// Viscous flux of every face of direction D (src/vsflux_module.f90:182-398),
+// with the metric averaging of src/rhs_module.f90:340-357.
+template <int D>
+__global__ __launch_bounds__(128) void k_vface(DS S)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= S.NF[D]) return;
+    int i, j, k;
+    const BlkD &B = face_of<D>(S, t, i, j, k);
+    const Par &P = S.P;
+    const int npv = S.npv;
+    const long g9 = gidx(B, i, j, k);
+    const long sN = D == 0 ? 1 : (D == 1 ? (long)B.nx : (long)B.nx * B.ny);
+    double g3[6][3] = {};
+    double c9[NPV], c10[NPV];
+#pragma unroll
+    for (int n = 0; n < NPV; n++) {
+        c9[n] = c10[n] = 0.0;
+        if (n >= npv) continue;
+        const double *f = S.pv + (long)n * S.NG + g9;
+        auto xs = [&](int ll) -> double { return f[stencil_off<D>(B, ll)]; };
+        c9[n] = f[0];
+        c10[n] = f[sN];
+        if ((n >= 1 && n <= 4) || n >= 7) {
+            const int q = (n <= 4) ? n - 1 : n - 3;     // u, v, w, T, k, omega
+            g3[q][0] = xs(10) - xs(9);
+            g3[q][1] = 0.0625 * (-xs(1) - xs(2) + xs(5) + xs(6) + 2.0 * (-xs(7) - xs(8) + xs(11) + xs(12))
+                                 - xs(13) - xs(14) + xs(17) + xs(18));
+            g3[q][2] = 0.0625 * (-xs(1) - xs(2) + xs(13) + xs(14) + 2.0 * (-xs(3) - xs(4) + xs(15) + xs(16))
+                                 - xs(5) - xs(6) + xs(17) + xs(18));
         }
-        face_vec(S, B, T, p1[0], p1[1], p1[2], t4[3]);
-        double gl[2], gr[2], dl[ND], dr[ND];
-        long wl = widx(B, i, j, k), wr = widx(B, ip, jp, kp);
-        gl[0] = S.vol[wl]; gr[0] = S.vol[wr];
-        gl[1] = S.ydn ? S.ydn[wl] : 1.0; gr[1] = S.ydn ? S.ydn[wr] : 1.0;
-        for (int m = 0; m < ND; m++) { dl[m] = S.dc[m * S.NG + g9]; dr[m] = S.dc[m * S.NG + gq]; }
-        double ml = S.mut ? S.mut[g9] : 0.0, mr = S.mut ? S.mut[gq] : 0.0;
-        double cav[5] = {0.5 * (c9[1] + c10[1]), 0.5 * (c9[2] + c10[2]), 0.5 * (c9[3] + c10[3]), 0.0, 0.0};
-        if (npv > 7) { cav[3] = 0.5 * (c9[7] + c10[7]); cav[4] = 0.5 * (c9[8] + c10[8]); }
-        double fv[NPV];
-        visc_flux<double>(P, n3, e4, t4, gl, gr, g3, cav, dl, dr, ml, mr, fv);
-        for (int n = 0; n < npv; n++) out[(npv + n) * NF + t] = fv[n];
     }
+    int ip = i, jp = j, kp = k;
+    if (D == 0) ip++; else if (D == 1) jp++; else kp++;
+    const long gq = g9 + sN;
+    double n3[3];
+    face_vec(S, B, D, i, j, k, n3);
+    double e4[4][3], t4[4][3];
+    const int E = (D + 1) % 3, T = (D + 2) % 3;
+    int p0[3] = {i, j, k}, p1[3] = {ip, jp, kp};
+    int a0[3], a1[3];
+    // e-family: (P,e-1), (P+1,e-1), (P,e), (P+1,e)
+    for (int q = 0; q < 3; q++) { a0[q] = p0[q]; a1[q] = p1[q]; }
+    a0[E]--; a1[E]--;
+    face_vec(S, B, E, a0[0], a0[1], a0[2], e4[0]);
+    face_vec(S, B, E, a1[0], a1[1], a1[2], e4[1]);
+    face_vec(S, B, E, p0[0], p0[1], p0[2], e4[2]);
+    face_vec(S, B, E, p1[0], p1[1], p1[2], e4[3]);
+    // t-family: i faces (P,t-1),(P+1,t-1),(P,t),(P+1,t); j and k faces (P,t-1),(P,t),(P+1,t-1),(P+1,t)
+    for (int q = 0; q < 3; q++) { a0[q] = p0[q]; a1[q] = p1[q]; }
+    a0[T]--; a1[T]--;
+    face_vec(S, B, T, a0[0], a0[1], a0[2], t4[0]);
+    if (D == 0) {
+        face_vec(S, B, T, a1[0], a1[1], a1[2], t4[1]);
+        face_vec(S, B, T, p0[0], p0[1], p0[2], t4[2]);
+    } else {
+        face_vec(S, B, T, p0[0], p0[1], p0[2], t4[1]);
+        face_vec(S, B, T, a1[0], a1[1], a1[2], t4[2]);
+    }
+    face_vec(S, B, T, p1[0], p1[1], p1[2], t4[3]);
+    double gl[2], gr[2], dl[ND], dr[ND];
+    long wl = widx(B, i, j, k), wr = widx(B, ip, jp, kp);
+    gl[0] = S.vol[wl]; gr[0] = S.vol[wr];
+    gl[1] = S.ydn ? S.ydn[wl] : 1.0; gr[1] = S.ydn ? S.ydn[wr] : 1.0;
+    for (int m = 0; m < ND; m++) { dl[m] = S.dc[m * S.NG + g9]; dr[m] = S.dc[m * S.NG + gq]; }
+    double ml = S.mut ? S.mut[g9] : 0.0, mr = S.mut ? S.mut[gq] : 0.0;
+    double cav[5] = {0.5 * (c9[1] + c10[1]), 0.5 * (c9[2] + c10[2]), 0.5 * (c9[3] + c10[3]), 0.0, 0.0};
+    if (npv > 7) { cav[3] = 0.5 * (c9[7] + c10[7]); cav[4] = 0.5 * (c9[8] + c10[8]); }
+    double fv[NPV];
+    visc_flux<double>(P, n3, e4, t4, gl, gr, g3, cav, dl, dr, ml, mr, fv);
+    double *out = S.ff[D];
+    const long NF = S.NF[D];
+    for (int n = 0; n < NPV; n++) if (n < npv) out[(npv + n) * NF + t] = fv[n];
 }
 
 __device__ __forceinline__ long fidx(const BlkD &B, int D, int i, int j, int k)
@@ -570,7 +623,7 @@ __device__ __forceinline__ long fidx(const BlkD &B, int D, int i, int j, int k)
 }
 
 // flux differences in the reference's order, then the cell sources
-__global__ void k_combine(DS S)
+__global__ __launch_bounds__(256) void k_combine(DS S)
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= S.NC) return;
@@ -586,7 +639,7 @@ __global__ void k_combine(DS S)
         long f0 = fidx(B, D, i - (D == 0), j - (D == 1), k - (D == 2));
         const double *F = S.ff[D];
         long NF = S.NF[D];
-        for (int n = 0; n < npv; n++) {
+        for (int n = 0; n < NPV; n++) if (n < npv) {
             double a = -(F[n * NF + f1] - F[n * NF + f0]);
             r[n] = (D == 0) ? a : r[n] + a;
             if (visc) r[n] = r[n] + (F[(npv + n) * NF + f1] - F[(npv + n) * NF + f0]);
@@ -595,7 +648,7 @@ __global__ void k_combine(DS S)
     long g = gidx(B, i, j, k);
     double d[ND], pc[NPV];
     for (int m = 0; m < ND; m++) d[m] = S.dc[m * S.NG + g];
-    for (int n = 0; n < npv; n++) pc[n] = S.pv[n * S.NG + g];
+    for (int n = 0; n < NPV; n++) if (n < npv) pc[n] = S.pv[n * S.NG + g];
     double vol = S.vol[widx(B, i, j, k)];
     if (P.ncav) {
         double rc, rv, ic[4];
@@ -610,7 +663,7 @@ __global__ void k_combine(DS S)
         const int di[7] = {-1, 0, 1, 0, 0, 0, 0}, dj[7] = {0, 0, 0, -1, 1, 0, 0}, dk[7] = {0, 0, 0, 0, 0, -1, 1};
         for (int q = 0; q < 7; q++) {
             long gn = gidx(B, i + di[q], j + dj[q], k + dk[q]);
-            for (int n = 0; n < npv; n++) x7[q][n] = S.pv[n * S.NG + gn];
+            for (int n = 0; n < NPV; n++) if (n < npv) x7[q][n] = S.pv[n * S.NG + gn];
         }
         cell_faces(S, B, i, j, k, f6);
         sst_source(x7, f6, vol, S.ydn[widx(B, i, j, k)], d, S.mut[g], src, it, oc);
@@ -626,14 +679,14 @@ __global__ void k_combine(DS S)
         fx[2] = d[D_RHO] * pc[2];
         fx[3] = d[D_RHO] * pc[3];
         fx[4] = d[D_RHO] * (d[D_H] + 0.5 * (pc[1] * pc[1] + pc[2] * pc[2] + pc[3] * pc[3])) - (pc[0] + P.pref);
-        for (int n = 5; n < npv; n++) fx[n] = d[D_RHO] * pc[n];
-        for (int n = 0; n < npv; n++) {
+        for (int n = 5; n < NPV; n++) if (n < npv) fx[n] = d[D_RHO] * pc[n];
+        for (int n = 0; n < NPV; n++) if (n < npv) {
             double q1 = S.qq[(long)n * S.NC + t], q2 = S.qq[(long)(npv + n) * S.NC + t];
             double v = (double)P.nsteady * (P.bdf_b * fx[n] - q1) + 0.5 * ((double)P.nsteady - 1.0) * q2;
             r[n] = r[n] - v * vol / P.dt_phy;
         }
     }
-    for (int n = 0; n < npv; n++) S.res[n * S.NC + t] = r[n];
+    for (int n = 0; n < NPV; n++) if (n < npv) S.res[n * S.NC + t] = r[n];
 }
 
 // ------------------------------------------------------------ LU-SGS sweeps
@@ -860,7 +913,7 @@ __global__ __launch_bounds__(256) void k_sweep_tile(DS S, const int *tiles, int 
     }
 }
 
-__global__ void k_fill_bits(double *p, long n, unsigned long long bits)
+__global__ __launch_bounds__(256) void k_fill_bits(double *p, long n, unsigned long long bits)
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < n) p[t] = __longlong_as_double((long long)bits);
@@ -885,7 +938,7 @@ __global__ __launch_bounds__(256) void k_update(DS S)
         cell_of(B, t - B.c0, i, j, k);
         long g = gidx(B, i, j, k), w = widx(B, i, j, k);
         double p[NPV], old[NPV];
-        for (int n = 0; n < npv; n++) { old[n] = S.pv[n * S.NG + g]; p[n] = old[n] + S.dqb[n * S.NW + w]; }
+        for (int n = 0; n < NPV; n++) if (n < npv) { old[n] = S.pv[n * S.NG + g]; p[n] = old[n] + S.dqb[n * S.NW + w]; }
         p[0] = mx(-P.pref + 1.0e1, p[0]);
         p[5] = mn(1.0, mx(0.0, p[5]));
         p[6] = mn(1.0, mx(0.0, p[6]));
@@ -893,11 +946,11 @@ __global__ __launch_bounds__(256) void k_update(DS S)
             p[7] = mx(p[7], 1.0e-16);
             p[8] = mx(mx(p[8], 1.0e-16), S.ocut[t]);
         }
-        for (int n = 0; n < npv; n++) S.pv[n * S.NG + g] = p[n];
+        for (int n = 0; n < NPV; n++) if (n < npv) S.pv[n * S.NG + g] = p[n];
         double d[ND];
         mixture_eos<double>(p[0] + P.pref, p[4], p[5], p[6], d, P.iturb != -3);
         for (int m = 0; m < ND; m++) S.dc[m * S.NG + g] = d[m];
-        for (int n = 0; n < npv; n++) { double q = fabs(old[n] - p[n]); dsq[n] = q * q; }
+        for (int n = 0; n < NPV; n++) if (n < npv) { double q = fabs(old[n] - p[n]); dsq[n] = q * q; }
         double q0 = fabs(old[0] - p[0]);
         if (q0 > 1.0e-12) {
             amax = q0;
@@ -907,14 +960,14 @@ __global__ __launch_bounds__(256) void k_update(DS S)
     }
     // wavefront reduction
     for (int o = 32; o > 0; o >>= 1) {
-        for (int n = 0; n < npv; n++) dsq[n] += __shfl_xor(dsq[n], o);
+        for (int n = 0; n < NPV; n++) if (n < npv) dsq[n] += __shfl_xor(dsq[n], o);
         double om = __shfl_xor(amax, o);
         long oi = __shfl_xor(aidx, o);
         if (om > amax || (om == amax && oi < aidx)) { amax = om; aidx = oi; }
     }
     int wv = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
-        for (int n = 0; n < npv; n++) sred[wv][n] = dsq[n];
+        for (int n = 0; n < NPV; n++) if (n < npv) sred[wv][n] = dsq[n];
         sred[wv][NPV] = amax;
         sidx[wv] = aidx;
     }
@@ -923,20 +976,20 @@ __global__ __launch_bounds__(256) void k_update(DS S)
         double acc[NPV + 1];
         double bm = sred[0][NPV];
         long bi = sidx[0];
-        for (int n = 0; n < npv; n++) acc[n] = sred[0][n];
+        for (int n = 0; n < NPV; n++) if (n < npv) acc[n] = sred[0][n];
         for (int q = 1; q < (int)(blockDim.x >> 6); q++) {
-            for (int n = 0; n < npv; n++) acc[n] += sred[q][n];
+            for (int n = 0; n < NPV; n++) if (n < npv) acc[n] += sred[q][n];
             if (sred[q][NPV] > bm || (sred[q][NPV] == bm && sidx[q] < bi)) { bm = sred[q][NPV]; bi = sidx[q]; }
         }
         double *o = S.red + (long)blockIdx.x * (NPV + 2);
-        for (int n = 0; n < npv; n++) o[n] = acc[n];
+        for (int n = 0; n < NPV; n++) if (n < npv) o[n] = acc[n];
         o[NPV] = bm;
         reinterpret_cast<long *>(o)[NPV + 1] = bi;
     }
 }
 
 // fixed-order final reduction of the per-workgroup partials (deterministic)
-__global__ void k_norm_final(DS S, int nwg)
+__global__ __launch_bounds__(256) void k_norm_final(DS S, int nwg)
 {
     __shared__ double part[256][NPV];
     __shared__ double pm[256];
@@ -948,7 +1001,7 @@ __global__ void k_norm_final(DS S, int nwg)
     long bi = 0x7fffffffffffffffL;
     for (int q = tid; q < nwg; q += 256) {
         const double *o = S.red + (long)q * (NPV + 2);
-        for (int n = 0; n < S.npv; n++) acc[n] += o[n];
+        for (int n = 0; n < NPV; n++) if (n < S.npv) acc[n] += o[n];
         long oi = reinterpret_cast<const long *>(o)[NPV + 1];
         if (o[NPV] > bm || (o[NPV] == bm && oi < bi)) { bm = o[NPV]; bi = oi; }
     }
@@ -968,13 +1021,13 @@ __global__ void k_norm_final(DS S, int nwg)
         __syncthreads();
     }
     if (tid == 0) {
-        for (int n = 0; n < S.npv; n++) S.scal[4 + n] = sqrt(part[0][n]);
+        for (int n = 0; n < NPV; n++) if (n < S.npv) S.scal[4 + n] = sqrt(part[0][n]);
         S.scal[4 + NPV] = pm[0];
         reinterpret_cast<long *>(S.scal)[5 + NPV] = pi[0];
     }
 }
 
-__global__ void k_eddy(DS S)
+__global__ __launch_bounds__(256) void k_eddy(DS S)
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= S.NC) return;
@@ -985,7 +1038,7 @@ __global__ void k_eddy(DS S)
     const int di[7] = {-1, 0, 1, 0, 0, 0, 0}, dj[7] = {0, 0, 0, -1, 1, 0, 0}, dk[7] = {0, 0, 0, 0, 0, -1, 1};
     for (int q = 0; q < 7; q++) {
         long gn = gidx(B, i + di[q], j + dj[q], k + dk[q]);
-        for (int n = 0; n < S.npv; n++) x7[q][n] = S.pv[n * S.NG + gn];
+        for (int n = 0; n < NPV; n++) if (n < S.npv) x7[q][n] = S.pv[n * S.NG + gn];
     }
     long g = gidx(B, i, j, k);
     for (int m = 0; m < ND; m++) d[m] = S.dc[m * S.NG + g];
@@ -995,7 +1048,7 @@ __global__ void k_eddy(DS S)
 }
 
 // dual time: qq(2) = qq(1), qq(1) = Q(pv, dv)
-__global__ void k_unsteady_shift(DS S)
+__global__ __launch_bounds__(256) void k_unsteady_shift(DS S)
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= S.NC) return;
@@ -1005,35 +1058,35 @@ __global__ void k_unsteady_shift(DS S)
     long g = gidx(B, i, j, k);
     const int npv = S.npv;
     double pc[NPV];
-    for (int n = 0; n < npv; n++) pc[n] = S.pv[n * S.NG + g];
+    for (int n = 0; n < NPV; n++) if (n < npv) pc[n] = S.pv[n * S.NG + g];
     double rho = S.dc[D_RHO * S.NG + g], h = S.dc[D_H * S.NG + g];
-    for (int n = 0; n < npv; n++) S.qq[(long)(npv + n) * S.NC + t] = S.qq[(long)n * S.NC + t];
+    for (int n = 0; n < NPV; n++) if (n < npv) S.qq[(long)(npv + n) * S.NC + t] = S.qq[(long)n * S.NC + t];
     double q[NPV];
     q[0] = rho;
     q[1] = rho * pc[1];
     q[2] = rho * pc[2];
     q[3] = rho * pc[3];
     q[4] = rho * (h + 0.5 * (pc[1] * pc[1] + pc[2] * pc[2] + pc[3] * pc[3])) - pc[0] - S.P.pref;
-    for (int n = 5; n < npv; n++) q[n] = rho * pc[n];
-    for (int n = 0; n < npv; n++) S.qq[(long)n * S.NC + t] = q[n];
+    for (int n = 5; n < NPV; n++) if (n < npv) q[n] = rho * pc[n];
+    for (int n = 0; n < NPV; n++) if (n < npv) S.qq[(long)n * S.NC + t] = q[n];
 }
 
 // ------------------------------------------------------------ layout transposes
 // host AoS (m components fastest) <-> device SoA for one block's ghosted cells
-__global__ void k_aos_to_soa(const double *src, int m, int msrc, int moff, double *dst, long nstride, long g0, long n)
+__global__ __launch_bounds__(256) void k_aos_to_soa(const double *src, int m, int msrc, int moff, double *dst, long nstride, long g0, long n)
 {
     long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= n) return;
     for (int q = 0; q < m; q++) dst[q * nstride + g0 + g] = src[g * msrc + moff + q];
 }
-__global__ void k_soa_to_aos(const double *src, int m, long nstride, long g0, double *dst, long n)
+__global__ __launch_bounds__(256) void k_soa_to_aos(const double *src, int m, long nstride, long g0, double *dst, long n)
 {
     long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= n) return;
     for (int q = 0; q < m; q++) dst[g * m + q] = src[q * nstride + g0 + g];
 }
 // dv(18) AoS -> compact dc, tv(ntv) AoS -> mu, kappa, mut
-__global__ void k_dv_to_dc(const double *dv, const double *tv, int ntv, double *dc, double *mut, long nstride, long g0, long n)
+__global__ __launch_bounds__(256) void k_dv_to_dc(const double *dv, const double *tv, int ntv, double *dc, double *mut, long nstride, long g0, long n)
 {
     long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= n) return;
@@ -1046,19 +1099,19 @@ __global__ void k_dv_to_dc(const double *dv, const double *tv, int ntv, double *
     if (ntv == 3 && mut) mut[g0 + g] = tv[ntv * g + 2];
 }
 // interior-cell arrays (c-indexed SoA, m components) -> reference AoS (m,2:imax,...)
-__global__ void k_cell_to_aos(const double *src, int m, long nstride, long c0, double *dst, long n)
+__global__ __launch_bounds__(256) void k_cell_to_aos(const double *src, int m, long nstride, long c0, double *dst, long n)
 {
     long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= n) return;
     for (int a = 0; a < m; a++) dst[q * m + a] = src[a * nstride + c0 + q];
 }
-__global__ void k_cell_from_aos(const double *src, int m, long nstride, long c0, double *dst, long n)
+__global__ __launch_bounds__(256) void k_cell_from_aos(const double *src, int m, long nstride, long c0, double *dst, long n)
 {
     long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= n) return;
     for (int a = 0; a < m; a++) dst[a * nstride + c0 + q] = src[q * m + a];
 }
-__global__ void k_fill(double *p, double v, long n)
+__global__ __launch_bounds__(256) void k_fill(double *p, double v, long n)
 {
     long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (q < n) p[q] = v;
@@ -1278,6 +1331,11 @@ int launch_rhs(cav3d_ctx *c)
     hipLaunchKernelGGL(k_face<0>, dim3(nblk(S.NF[0], 128)), dim3(128), 0, c->st, S);
     hipLaunchKernelGGL(k_face<1>, dim3(nblk(S.NF[1], 128)), dim3(128), 0, c->st, S);
     hipLaunchKernelGGL(k_face<2>, dim3(nblk(S.NF[2], 128)), dim3(128), 0, c->st, S);
+    if (c->prm.iturb >= -2) {
+        hipLaunchKernelGGL(k_vface<0>, dim3(nblk(S.NF[0], 128)), dim3(128), 0, c->st, S);
+        hipLaunchKernelGGL(k_vface<1>, dim3(nblk(S.NF[1], 128)), dim3(128), 0, c->st, S);
+        hipLaunchKernelGGL(k_vface<2>, dim3(nblk(S.NF[2], 128)), dim3(128), 0, c->st, S);
+    }
     hipLaunchKernelGGL(k_combine, dim3(nblk(S.NC, 128)), dim3(128), 0, c->st, S);
     return 0;
 }

@@ -481,7 +481,7 @@ template <class R> HD void roe_flux(const Par &P, int npv, const R *n3, const Fa
                                     const FaceState<R> &Rt, R c2avg, R *fx)
 {
     R dl = sqrt(n3[0] * n3[0] + n3[1] * n3[1] + n3[2] * n3[2]);
-    if (dl == R(0)) { for (int k = 0; k < npv; k++) fx[k] = 0; return; }
+    if (dl == R(0)) { for (int k = 0; k < NPV; k++) if (k < npv) fx[k] = 0; return; }
     R nx = n3[0] / dl, ny = n3[1] / dl, nz = n3[2] / dl;
     const R *pl = L.p, *pr = Rt.p;
     R uull = nx * pl[1] + ny * pl[2] + nz * pl[3];
@@ -524,7 +524,7 @@ template <class R> HD void ausm_flux(const Par &P, int npv, const R *n3, const F
 {
     const R beta = R(0.125), kp = R(0.25), ku = R(1);
     R dl = sqrt(n3[0] * n3[0] + n3[1] * n3[1] + n3[2] * n3[2]);
-    if (dl == R(0)) { for (int k = 0; k < npv; k++) fx[k] = 0; return; }
+    if (dl == R(0)) { for (int k = 0; k < NPV; k++) if (k < npv) fx[k] = 0; return; }
     R nx = n3[0] / dl, ny = n3[1] / dl, nz = n3[2] / dl;
     const R *pl = L.p, *pr = Rt.p;
     R uull = nx * pl[1] + ny * pl[2] + nz * pl[3];
@@ -583,7 +583,7 @@ HD void face_recon(const Par &P, int npv, X xs, R *c9, R *c10, R *xl, R *xr, R g
 {
     const bool visc = (P.iturb >= -2);
     int vq = 0;
-    for (int n = 0; n < npv; n++) {
+    for (int n = 0; n < NPV; n++) if (n < npv) {
         auto xn = [&](int ll) -> R { return xs(ll, n); };
         c9[n] = xn(9);
         c10[n] = xn(10);
@@ -616,7 +616,7 @@ HD void inviscid_face(const Par &P, int npv, const R *n3, const R *c9, const R *
         L.rho = rh9[0]; L.h = rh9[1];
         Rt.rho = rh10[0]; Rt.h = rh10[1];
     }
-    for (int n = 0; n < npv; n++) { L.p[n] = xl[n]; Rt.p[n] = xr[n]; }
+    for (int n = 0; n < NPV; n++) if (n < npv) { L.p[n] = xl[n]; Rt.p[n] = xr[n]; }
     R sl = sqrt(L.rho), sr = sqrt(Rt.rho), rd = R(1) / (sl + sr);
     R ya1 = (sl * xl[5] + sr * xr[5]) * rd, ya2 = (sl * xl[6] + sr * xr[6]) * rd;
     R c2a = face_c2<R>(R(0.5) * (xl[0] + xr[0]) + R(P.pref), R(0.5) * (xl[4] + xr[4]), ya1, ya2);
@@ -759,7 +759,7 @@ HD void jac_apply(const Par &P, int npv, const CellJ &c, const double *n3, int s
 {
     JacCell jc = jac_cell(P, c);
     JacFace jf = jac_face(P, c, n3);
-    for (int n = 0; n < npv; n++) jac_row(npv, jac_nb(c.pv, c.d, n3, jc, jf, n, npv), sign, n, x, x[n], y[n], sgn);
+    for (int n = 0; n < NPV; n++) if (n < npv) jac_row(npv, jac_nb(c.pv, c.d, n3, jc, jf, n, npv), sign, n, x, x[n], y[n], sgn);
 }
 
 // Closed-form inverse of the preconditioned diagonal block with the Merkle
@@ -881,7 +881,7 @@ HD void getx(const Par &P, int npv, const CellJ &c, const double f6[6][3], doubl
 {
     double g[NGX];
     getx_prep(P, npv, c, f6, dt, cc, tt4, g);
-    for (int n = 0; n < npv; n++)
+    for (int n = 0; n < NPV; n++) if (n < npv)
         x[n] = getx_apply([&](int q) { return g[q]; }, c.pv, c.d[D_RY1], c.d[D_RT], res, n, res[n], c.pv[n]);
 }
 
@@ -935,7 +935,9 @@ HD void visc_flux(const Par &P, const R *nx, const R e4[4][3], const R t4[4][3],
     }
     R g[6][3];
     const int nv = turb ? 6 : 4;
-    for (int q = 0; q < nv; q++) {
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+        if (q >= nv) { g[q][0] = g[q][1] = g[q][2] = R(0); continue; }
         R a = g3[q][0], b = g3[q][1], c = g3[q][2];
         g[q][0] = (a * nx[0] + b * ex + c * tx) * vol;
         g[q][1] = (a * nx[1] + b * ey + c * ty) * vol;
