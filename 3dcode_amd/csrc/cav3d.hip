@@ -811,22 +811,95 @@ __device__ __forceinline__ JacNb load_jnb(const DS &S, const BlkD &B, int ax, in
 // hyperplane.  A pencil's neighbour inside the tile hands over its dq through
 // LDS (double-buffered by the parity of s); a neighbour in another tile
 // through the device-coherent dq array, waited for with a bounded poll.
+// Everything a cell reads that does not depend on the sweep (its residual,
+// state, properties, diagonal-inverse terms, and the Jacobian operands of its
+// j and k neighbours) is fetched one hyperplane ahead into an LDS stage, so
+// a step's critical path is the dq hand-over and the arithmetic.
 // Tiles wait only on tiles upstream in j and k, every tile is resident
 // (cooperative launch) and a workgroup takes its tiles in topological order,
 // so the chain always advances.
 constexpr int TJ = 4, TK = 4;
+// LDS stage of one pencil-step (doubles)
+enum {
+    ST_RES = 0, ST_PV = 9, ST_DC = 18, ST_N3 = 31, ST_JC = 34, ST_JF = 36, ST_GX = 39,
+    ST_J = 68, ST_K = 98,                  // neighbour blocks: pv 9, dc 13, n3 3, jc 2, jf 3
+    NB_PV = 0, NB_DC = 9, NB_N3 = 22, NB_JC = 25, NB_JF = 27, ST_N = 128
+};
+static_assert(ST_GX + NGX == ST_J, "stage layout");
+
+__device__ __forceinline__ double face_comp(const DS &S, const BlkD &B, int ax, const int *p, int comp)
+{
+    if (ax == 0) return S.cx[comp * S.NCX + B.cx0 + (p[0] - 1) + (long)B.I * ((p[1] - 1) + (long)(B.J + 1) * (p[2] - 1))];
+    if (ax == 1) return S.ex[comp * S.NEX + B.ex0 + (p[0] - 1) + (long)(B.I + 1) * ((p[1] - 1) + (long)B.J * (p[2] - 1))];
+    return S.tx[comp * S.NTX + B.tx0 + (p[0] - 1) + (long)(B.I + 1) * ((p[1] - 1) + (long)(B.J + 1) * (p[2] - 1))];
+}
+
+// item e of the stage of cell (i,j,k) of a sweep; 0 where nothing is read
+template <bool FWD>
+__device__ __forceinline__ double stage_item(const DS &S, const BlkD &B, int i, int j, int k, int e)
+{
+    const int sg = FWD ? 1 : -1, side = FWD ? 1 : 0, npv = S.npv;
+    const long c = cidx(B, i, j, k), g = gidx(B, i, j, k);
+    if (e < ST_PV) return e < npv ? (FWD ? S.res[e * S.NC + c] : S.dcv[e * S.NW + widx(B, i, j, k)]) : 0.0;
+    if (e < ST_DC) return (e - ST_PV) < npv ? S.pv[(e - ST_PV) * S.NG + g] : 0.0;
+    if (e < ST_N3) return S.dc[(e - ST_DC) * S.NG + g];
+    if (e < ST_JC) { const int p[3] = {FWD ? i : i - 1, j, k}; return face_comp(S, B, 0, p, e - ST_N3); }
+    if (e < ST_JF) return S.jc[(e - ST_JC) * S.NC + c];
+    if (e < ST_GX) return S.jf[(3 * side + e - ST_JF) * S.NC + c];
+    if (e < ST_J) return S.gx[(e - ST_GX) * S.NC + c];
+    const int ax = e < ST_K ? 1 : 2;
+    const int r = e - (ax == 1 ? ST_J : ST_K);
+    int m[3] = {i, j, k};
+    m[ax] -= sg;
+    const int lim = ax == 1 ? B.J : B.K;
+    if (m[ax] < 2 || m[ax] > lim) return 0.0;
+    const long gm = gidx(B, m[0], m[1], m[2]), cm = cidx(B, m[0], m[1], m[2]);
+    if (r < NB_DC) return r < npv ? S.pv[r * S.NG + gm] : 0.0;
+    if (r < NB_N3) return S.dc[(r - NB_DC) * S.NG + gm];
+    if (r < NB_JC) { const int self[3] = {i, j, k}; return face_comp(S, B, ax, FWD ? m : self, r - NB_N3); }
+    if (r < NB_JF) return S.jc[(r - NB_JC) * S.NC + cm];
+    return S.jf[(3 * (2 * ax + side) + r - NB_JF) * S.NC + cm];
+}
+
+// row `lane` Jacobian operands of a staged block (pv at b + NB_PV, ...)
+__device__ __forceinline__ JacNb stage_jnb(const double *st, int b, int lane, int npv)
+{
+    JacNb q;
+    q.pvn = lane < npv ? st[b + NB_PV + lane] : 0.0;
+    q.pv1 = st[b + NB_PV + 1]; q.pv2 = st[b + NB_PV + 2]; q.pv3 = st[b + NB_PV + 3];
+    const double *d = st + b + NB_DC;
+    q.rho = d[D_RHO]; q.rp = d[D_RP]; q.rt = d[D_RT]; q.ry1 = d[D_RY1]; q.ry2 = d[D_RY2];
+    q.hp = d[D_HP]; q.ht = d[D_HT]; q.hy1 = d[D_HY1]; q.hy2 = d[D_HY2];
+    q.n3[0] = st[b + NB_N3]; q.n3[1] = st[b + NB_N3 + 1]; q.n3[2] = st[b + NB_N3 + 2];
+    q.n3n = (lane >= 1 && lane <= 3) ? st[b + NB_N3 + lane - 1] : 0.0;
+    q.jc = JacCell{st[b + NB_JC], st[b + NB_JC + 1]};
+    q.jf = JacFace{st[b + NB_JF], st[b + NB_JF + 1], st[b + NB_JF + 2]};
+    return q;
+}
+static_assert(ST_PV + 9 == ST_DC && ST_DC + ND == ST_N3 && ST_N3 + 3 == ST_JC && ST_JC + 2 == ST_JF,
+              "the own-cell part of the stage has the neighbour-block layout");
+
+// workgroup barrier that orders LDS only: outstanding global loads stay in
+// flight across it (the stage prefetch lands after the barrier)
+__device__ __forceinline__ void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
 
 template <bool FWD>
 __global__ __launch_bounds__(256) void k_sweep_tile(DS S, const int *tiles, int ntiles, int *err)
 {
     __shared__ double lds[2][TJ * TK][NPV];
+    __shared__ double stg[2][TJ * TK][ST_N];
     const int npv = S.npv;
     const int lane = threadIdx.x & 15, grp = threadIdx.x >> 4;     // 16 groups of 16 lanes
     const int gj = grp % TJ, gk = grp / TJ;
     const int sg = FWD ? 1 : -1;
     const double sgn = FWD ? 1.0 : -1.0;
     double *dq = FWD ? S.dqf : S.dqb;
-    const int side = FWD ? 1 : 0;
+    constexpr int PER = ST_N / 16;                                  // stage items per lane
     for (int tq = blockIdx.x; tq < ntiles; tq += gridDim.x) {
         const int *td = tiles + 4 * (FWD ? tq : ntiles - 1 - tq);
         const BlkD &B = S.b[td[0]];
@@ -837,28 +910,53 @@ __global__ __launch_bounds__(256) void k_sweep_tile(DS S, const int *tiles, int 
         // the tile's hyperplane range in sweep order
         const int sa = FWD ? 2 + j0 + k0 : B.I + jend + kend;
         const int sb = FWD ? B.I + jend + kend : 2 + j0 + k0;
+        auto in_range = [&](int ii) { return live && ii >= 2 && ii <= B.I; };
+        auto ahead = [&](int s, int d) { return FWD ? (s + d <= sb) : (s - d >= sb); };
+        // prologue: stage of the first hyperplane; the second one's loads in flight
+        {
+            const int i = sa - j - k;
+            if (in_range(i))
+#pragma unroll
+                for (int r = 0; r < PER; r++) stg[sa & 1][grp][lane + 16 * r] = stage_item<FWD>(S, B, i, j, k, lane + 16 * r);
+        }
+        double nxt[PER];
+        bool hn = ahead(sa, 1) && in_range(sa + sg - j - k);
+        if (hn) {
+#pragma unroll
+            for (int r = 0; r < PER; r++) nxt[r] = stage_item<FWD>(S, B, sa + sg - j - k, j, k, lane + 16 * r);
+        }
+        __syncthreads();
         JacNb prev{};
         double xprev = 0.0;
         bool have_prev = false;
         for (int s = sa;; s += sg) {
             const int i = s - j - k;
-            if (live && i >= 2 && i <= B.I) {
-                const long c = cidx(B, i, j, k), w = widx(B, i, j, k), g = gidx(B, i, j, k);
+            // stage of hyperplane s+1 (loaded during the previous step), then the loads for s+2
+            if (hn) {
+#pragma unroll
+                for (int r = 0; r < PER; r++) stg[(s + sg) & 1][grp][lane + 16 * r] = nxt[r];
+            }
+            hn = ahead(s, 2) && in_range(i + 2 * sg);
+            if (hn) {
+#pragma unroll
+                for (int r = 0; r < PER; r++) nxt[r] = stage_item<FWD>(S, B, i + 2 * sg, j, k, lane + 16 * r);
+            }
+            if (in_range(i)) {
+                const double *st = stg[s & 1][grp];
+                const long w = widx(B, i, j, k);
                 const int mj[3] = {i, j - sg, k}, mk[3] = {i, j, k - sg};
                 const bool hj = (mj[1] >= 2 && mj[1] <= B.J), hk = (mk[2] >= 2 && mk[2] <= B.K);
                 const bool lj = hj && mj[1] >= j0 && mj[1] <= jend, lk = hk && mk[2] >= k0 && mk[2] <= kend;
-                const int self[3] = {i, j, k};
-                double y = 0.0;
-                if (lane < npv) y = FWD ? S.res[lane * S.NC + c] : S.dcv[lane * S.NW + w];
+                double y = lane < npv ? st[ST_RES + lane] : 0.0;
                 double xv[NPV];
                 if (have_prev) {
 #pragma unroll
                     for (int q = 0; q < NPV; q++) xv[q] = __shfl(xprev, q, 16);
                     jac_row(npv, prev, sg, lane, xv, xprev, y, sgn);
                 }
-                const int pb = (s - sg) & 1;                 // buffer the previous hyperplane wrote
+                const int pb = (s - sg) & 1;                 // dq buffer the previous hyperplane wrote
                 if (hj) {
-                    const JacNb nbj = load_jnb(S, B, 1, side, mj, FWD ? mj : self, lane);
+                    const JacNb nbj = stage_jnb(st, ST_J, lane, npv);
                     double xo;
                     if (lj) {
 #pragma unroll
@@ -872,7 +970,7 @@ __global__ __launch_bounds__(256) void k_sweep_tile(DS S, const int *tiles, int 
                     jac_row(npv, nbj, sg, lane, xv, xo, y, sgn);
                 }
                 if (hk) {
-                    const JacNb nbk = load_jnb(S, B, 2, side, mk, FWD ? mk : self, lane);
+                    const JacNb nbk = stage_jnb(st, ST_K, lane, npv);
                     double xo;
                     if (lk) {
 #pragma unroll
@@ -888,25 +986,22 @@ __global__ __launch_bounds__(256) void k_sweep_tile(DS S, const int *tiles, int 
                 if (FWD && lane < npv) S.dcv[lane * S.NW + w] = y;
 #pragma unroll
                 for (int q = 0; q < NPV; q++) xv[q] = __shfl(y, q, 16);
-                // this cell's row of its Jacobian through the face it shares with the next cell
-                const int nxt[3] = {FWD ? i : i - 1, j, k};
-                prev = load_jnb(S, B, 0, side, self, nxt, lane);
+                // this cell's Jacobian row through the face it shares with the next cell
+                prev = stage_jnb(st, ST_PV, lane, npv);
                 double xn = 0.0;
                 if (lane < npv) {
                     double pc[NPV];
 #pragma unroll
-                    for (int q = 0; q < NPV; q++) pc[q] = (q < npv) ? S.pv[q * S.NG + g] : 0.0;
-                    const double *gxc = S.gx + c;
-                    const long NC = S.NC;
-                    xn = getx_apply([&](int q) { return gxc[q * NC]; }, pc, S.dc[D_RY1 * S.NG + g], S.dc[D_RT * S.NG + g],
-                                    xv, lane, y, S.pv[lane * S.NG + g]);
+                    for (int q = 0; q < NPV; q++) pc[q] = st[ST_PV + q];
+                    xn = getx_apply([&](int q) { return st[ST_GX + q]; }, pc, st[ST_DC + D_RY1], st[ST_DC + D_RT],
+                                    xv, lane, y, st[ST_PV + lane]);
                     st_dev(dq + lane * S.NW + w, xn);
                 }
                 if (lane < NPV) lds[s & 1][grp][lane] = xn;
                 xprev = xn;
                 have_prev = true;
             }
-            __syncthreads();
+            lds_barrier();
             if (s == sb) break;
         }
         __syncthreads();
@@ -1140,8 +1235,8 @@ struct cav3d_ctx {
     int ntiles = 0;
     int *sweep_err = nullptr; // set when a sweep wait gave up
     bool profile = false;
-    hipEvent_t ev[8] = {};
-    double prof_ms[6] = {};
+    hipEvent_t ev[10] = {};
+    double prof_ms[8] = {};
     std::vector<void *> allocs;
 };
 
@@ -1362,10 +1457,13 @@ int launch_sweeps(cav3d_ctx *c)
     int *err = c->sweep_err;
     void *args[4] = {&S, &tiles, &ntiles, &err};
     dim3 grid(c->coop_blocks), block(256);
+    if (c->profile) hipEventRecord(c->ev[6], c->st);
     hipError_t e = hipLaunchCooperativeKernel((const void *)k_sweep_tile<true>, grid, block, args, 0, c->st);
     if (e != hipSuccess) return fail(std::string("forward sweep launch: ") + hipGetErrorString(e), -3);
+    if (c->profile) hipEventRecord(c->ev[7], c->st);
     e = hipLaunchCooperativeKernel((const void *)k_sweep_tile<false>, grid, block, args, 0, c->st);
     if (e != hipSuccess) return fail(std::string("backward sweep launch: ") + hipGetErrorString(e), -3);
+    if (c->profile) hipEventRecord(c->ev[8], c->st);
     return 0;
 }
 
@@ -1578,7 +1676,7 @@ int cav3d_create(const cav3d_params *prm, int nblocks, const cav3d_block *blocks
         hipMemcpyAsync(S.con, c->cons.data(), c->cons.size() * sizeof(Con), hipMemcpyHostToDevice, c->st) != hipSuccess) {
         cav3d_destroy(c); return fail("interface upload failed", -3);
     }
-    for (int q = 0; q < 8; q++) hipEventCreate(&c->ev[q]);
+    for (int q = 0; q < 10; q++) hipEventCreate(&c->ev[q]);
     if (hipStreamSynchronize(c->st) != hipSuccess) { cav3d_destroy(c); return fail("device set-up failed", -3); }
     *out = c;
     return 0;
@@ -1590,7 +1688,7 @@ void cav3d_destroy(cav3d_ctx *c)
     if (c->st) hipStreamSynchronize(c->st);
     for (void *p : c->allocs) hipFree(p);
     if (c->stage) hipFree(c->stage);
-    for (int q = 0; q < 8; q++) if (c->ev[q]) hipEventDestroy(c->ev[q]);
+    for (int q = 0; q < 10; q++) if (c->ev[q]) hipEventDestroy(c->ev[q]);
     if (c->st) hipStreamDestroy(c->st);
     delete c;
 }
@@ -1761,6 +1859,10 @@ int cav3d_timeinteg(cav3d_ctx *c, int nt_phy, int nt, double timeprev, double *t
         for (int q = 0; q < 5; q++) { hipEventElapsedTime(&ms, c->ev[q], c->ev[q + 1]); c->prof_ms[q] = ms; }
         hipEventElapsedTime(&ms, c->ev[0], c->ev[5]);
         c->prof_ms[5] = ms;
+        hipEventElapsedTime(&ms, c->ev[6], c->ev[7]);
+        c->prof_ms[6] = ms;
+        hipEventElapsedTime(&ms, c->ev[7], c->ev[8]);
+        c->prof_ms[7] = ms;
     }
     int rc = check_launch("cav3d_timeinteg");
     if (rc) return rc;
@@ -1846,7 +1948,7 @@ int cav3d_profile(cav3d_ctx *c, int enable, double *out)
 {
     if (!c) return fail("null context", -1);
     c->profile = enable != 0;
-    if (out) for (int q = 0; q < 6; q++) out[q] = c->prof_ms[q];
+    if (out) for (int q = 0; q < 8; q++) out[q] = c->prof_ms[q];
     return 0;
 }
 

@@ -221,7 +221,7 @@ class Solver:
         return dv, tv
 
     def profile(self, enable=True):
-        out = np.zeros(6)
+        out = np.zeros(8)
         _check(lib().cav3d_profile(self.h, 1 if enable else 0, _p(out)))
         return out
 

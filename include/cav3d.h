@@ -144,8 +144,10 @@ int cav3d_eos(cav3d_ctx *ctx, int n, const double *p, const double *t, const dou
 int cav3d_eos_host(int n, const double *p, const double *t, const double *y1, const double *y2,
                    double *dv, double *tv);
 
-/* Device time, in milliseconds, of the last cav3d_timeinteg's kernels by phase:
- * out[0] timestep, [1] boundary, [2] rhs, [3] lusgs, [4] update+eddy, [5] total. */
+/* Device time, in milliseconds, of the last cav3d_timeinteg's kernels by phase
+ * (HIP events on the context's stream, recorded while enabled):
+ * out[0] timestep, [1] boundary, [2] rhs, [3] lusgs, [4] update+eddy, [5] total,
+ * [6] forward sweep kernel, [7] backward sweep kernel.  out holds 8 doubles. */
 int cav3d_profile(cav3d_ctx *ctx, int enable, double *out);
 
 int cav3d_synchronize(cav3d_ctx *ctx);
