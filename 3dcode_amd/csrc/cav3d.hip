@@ -728,20 +728,13 @@ __global__ __launch_bounds__(128) void k_lhs_prep(DS S)
 //   dcv = res + A+(i-1) dq(i-1) + A+(j-1) dq(j-1) + A+(k-1) dq(k-1),  dq = D^-1 dcv
 //   (dcv - A-(i+1) dq(i+1) - ... backward), ghost dq = 0.
 // Work is split into pencils along i: a 16-lane group owns pencil (j,k) and
-// walks i in sweep order, keeping its previous cell's dq in registers.  The
-// two cross-pencil inputs dq(i, j-1, k) and dq(i, j, k-1) are published by
-// their owners with device-coherent stores and read with device-coherent
-// loads; an entry still holding the "not yet written" bit pattern is polled
-// again.  Lane n accumulates row n of each Jacobian product in the
-// reference's order, then the rows are exchanged and lane n evaluates
-// component n of D^-1.
-//
-// Progress: the four pencils of a wavefront lie on one diagonal j+k = t and
-// depend only on pencils of diagonal t-1, which other wavefronts own; each
-// wavefront takes its pencils in increasing t, and the cooperative launch
-// keeps every wavefront resident, so the dependency chain always advances.
-// Every poll is bounded: past the limit the kernel records an error and
-// stops waiting, so each wavefront finishes.
+// walks i in sweep order.  Lane n accumulates row n of each Jacobian product
+// in the reference's order, then the rows are exchanged and lane n evaluates
+// component n of D^-1.  dq of a neighbour in another workgroup is published
+// with a device-coherent store and read with device-coherent loads; an entry
+// still holding the "not yet written" bit pattern is polled again.  Every
+// poll is bounded: past the limit the kernel records an error and stops
+// waiting, so each wavefront finishes.
 constexpr unsigned long long DQ_UNSET = 0x7FF4A5A5A5A5A5A5ull;   // a signalling NaN no arithmetic produces
 constexpr long POLL_LIMIT = 1L << 19;
 
@@ -752,58 +745,6 @@ __device__ __forceinline__ double ld_dev(const double *p)
 __device__ __forceinline__ void st_dev(double *p, double v)
 {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// dq of cell wm, waiting (bounded) until its owner has written it
-__device__ __forceinline__ void wait_dq(const double *dq, long NW, long wm, int npv, double *x, int *err)
-{
-    long polls = 0;
-    bool ready;
-    do {
-        ready = true;
-#pragma unroll
-        for (int q = 0; q < NPV; q++) {
-            x[q] = (q < npv) ? ld_dev(dq + q * NW + wm) : 0.0;
-            if ((unsigned long long)__double_as_longlong(x[q]) == DQ_UNSET) ready = false;
-        }
-        if (!ready) {
-            if (++polls > POLL_LIMIT || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-                atomicOr(err, 1);
-#pragma unroll
-                for (int q = 0; q < NPV; q++) x[q] = 0.0;
-                return;
-            }
-            __builtin_amdgcn_s_sleep(1);
-        }
-    } while (!ready);
-}
-
-// Row `lane` of the Jacobian of cell m through its face on axis ax (side 0:
-// minus face, 1: plus face); fpos = the face's index position.
-__device__ __forceinline__ JacNb load_jnb(const DS &S, const BlkD &B, int ax, int side, const int *m,
-                                          const int *fpos, int lane)
-{
-    const long gm = gidx(B, m[0], m[1], m[2]), cm = cidx(B, m[0], m[1], m[2]);
-    JacNb q;
-    q.pvn = (lane < S.npv) ? S.pv[lane * S.NG + gm] : 0.0;
-    q.pv1 = S.pv[1 * S.NG + gm]; q.pv2 = S.pv[2 * S.NG + gm]; q.pv3 = S.pv[3 * S.NG + gm];
-    q.rho = S.dc[D_RHO * S.NG + gm]; q.rp = S.dc[D_RP * S.NG + gm]; q.rt = S.dc[D_RT * S.NG + gm];
-    q.ry1 = S.dc[D_RY1 * S.NG + gm]; q.ry2 = S.dc[D_RY2 * S.NG + gm]; q.hp = S.dc[D_HP * S.NG + gm];
-    q.ht = S.dc[D_HT * S.NG + gm]; q.hy1 = S.dc[D_HY1 * S.NG + gm]; q.hy2 = S.dc[D_HY2 * S.NG + gm];
-    face_vec(S, B, ax, fpos[0], fpos[1], fpos[2], q.n3);
-    q.n3n = 0.0;
-    if (lane >= 1 && lane <= 3) {       // the lane's own component, read with a lane-dependent address
-        const double *a;
-        long nn, o;
-        if (ax == 0) { a = S.cx; nn = S.NCX; o = B.cx0 + (fpos[0] - 1) + (long)B.I * ((fpos[1] - 1) + (long)(B.J + 1) * (fpos[2] - 1)); }
-        else if (ax == 1) { a = S.ex; nn = S.NEX; o = B.ex0 + (fpos[0] - 1) + (long)(B.I + 1) * ((fpos[1] - 1) + (long)B.J * (fpos[2] - 1)); }
-        else { a = S.tx; nn = S.NTX; o = B.tx0 + (fpos[0] - 1) + (long)(B.I + 1) * ((fpos[1] - 1) + (long)(B.J + 1) * (fpos[2] - 1)); }
-        q.n3n = a[(lane - 1) * nn + o];
-    }
-    const int fq = 2 * ax + side;
-    q.jc = JacCell{S.jc[cm], S.jc[S.NC + cm]};
-    q.jf = JacFace{S.jf[(3 * fq) * S.NC + cm], S.jf[(3 * fq + 1) * S.NC + cm], S.jf[(3 * fq + 2) * S.NC + cm]};
-    return q;
 }
 
 // Tiled sweep: a workgroup owns a tile of TJ x TK pencils of one block and
@@ -827,38 +768,46 @@ enum {
 };
 static_assert(ST_GX + NGX == ST_J, "stage layout");
 
-__device__ __forceinline__ double face_comp(const DS &S, const BlkD &B, int ax, const int *p, int comp)
+__device__ __forceinline__ long face_off(const BlkD &B, int ax, const int *p)
 {
-    if (ax == 0) return S.cx[comp * S.NCX + B.cx0 + (p[0] - 1) + (long)B.I * ((p[1] - 1) + (long)(B.J + 1) * (p[2] - 1))];
-    if (ax == 1) return S.ex[comp * S.NEX + B.ex0 + (p[0] - 1) + (long)(B.I + 1) * ((p[1] - 1) + (long)B.J * (p[2] - 1))];
-    return S.tx[comp * S.NTX + B.tx0 + (p[0] - 1) + (long)(B.I + 1) * ((p[1] - 1) + (long)(B.J + 1) * (p[2] - 1))];
+    if (ax == 0) return B.cx0 + (p[0] - 1) + (long)B.I * ((p[1] - 1) + (long)(B.J + 1) * (p[2] - 1));
+    if (ax == 1) return B.ex0 + (p[0] - 1) + (long)(B.I + 1) * ((p[1] - 1) + (long)B.J * (p[2] - 1));
+    return B.tx0 + (p[0] - 1) + (long)(B.I + 1) * ((p[1] - 1) + (long)(B.J + 1) * (p[2] - 1));
+}
+__device__ __forceinline__ const double *face_base(const DS &S, const BlkD &B, int ax, const int *p, int comp)
+{
+    const long o = face_off(B, ax, p);
+    if (ax == 0) return S.cx + comp * S.NCX + o;
+    if (ax == 1) return S.ex + comp * S.NEX + o;
+    return S.tx + comp * S.NTX + o;
 }
 
-// item e of the stage of cell (i,j,k) of a sweep; 0 where nothing is read
+// Stage item e of pencil (j,k) of a sweep: every field is stored i fastest,
+// so the item of cell i is base[i]; nullptr where the item is not read.
 template <bool FWD>
-__device__ __forceinline__ double stage_item(const DS &S, const BlkD &B, int i, int j, int k, int e)
+__device__ __forceinline__ const double *stage_base(const DS &S, const BlkD &B, int j, int k, int e)
 {
     const int sg = FWD ? 1 : -1, side = FWD ? 1 : 0, npv = S.npv;
-    const long c = cidx(B, i, j, k), g = gidx(B, i, j, k);
-    if (e < ST_PV) return e < npv ? (FWD ? S.res[e * S.NC + c] : S.dcv[e * S.NW + widx(B, i, j, k)]) : 0.0;
-    if (e < ST_DC) return (e - ST_PV) < npv ? S.pv[(e - ST_PV) * S.NG + g] : 0.0;
-    if (e < ST_N3) return S.dc[(e - ST_DC) * S.NG + g];
-    if (e < ST_JC) { const int p[3] = {FWD ? i : i - 1, j, k}; return face_comp(S, B, 0, p, e - ST_N3); }
-    if (e < ST_JF) return S.jc[(e - ST_JC) * S.NC + c];
-    if (e < ST_GX) return S.jf[(3 * side + e - ST_JF) * S.NC + c];
-    if (e < ST_J) return S.gx[(e - ST_GX) * S.NC + c];
+    const long c = cidx(B, 0, j, k), g = gidx(B, 0, j, k);
+    if (e < ST_PV) return e < npv ? (FWD ? S.res + e * S.NC + c : S.dcv + e * S.NW + widx(B, 0, j, k)) : nullptr;
+    if (e < ST_DC) return (e - ST_PV) < npv ? S.pv + (e - ST_PV) * S.NG + g : nullptr;
+    if (e < ST_N3) return S.dc + (e - ST_DC) * S.NG + g;
+    if (e < ST_JC) { const int p[3] = {FWD ? 0 : -1, j, k}; return face_base(S, B, 0, p, e - ST_N3); }
+    if (e < ST_JF) return S.jc + (e - ST_JC) * S.NC + c;
+    if (e < ST_GX) return S.jf + (3 * side + e - ST_JF) * S.NC + c;
+    if (e < ST_J) return S.gx + (e - ST_GX) * S.NC + c;
     const int ax = e < ST_K ? 1 : 2;
     const int r = e - (ax == 1 ? ST_J : ST_K);
-    int m[3] = {i, j, k};
+    int m[3] = {0, j, k};
     m[ax] -= sg;
     const int lim = ax == 1 ? B.J : B.K;
-    if (m[ax] < 2 || m[ax] > lim) return 0.0;
-    const long gm = gidx(B, m[0], m[1], m[2]), cm = cidx(B, m[0], m[1], m[2]);
-    if (r < NB_DC) return r < npv ? S.pv[r * S.NG + gm] : 0.0;
-    if (r < NB_N3) return S.dc[(r - NB_DC) * S.NG + gm];
-    if (r < NB_JC) { const int self[3] = {i, j, k}; return face_comp(S, B, ax, FWD ? m : self, r - NB_N3); }
-    if (r < NB_JF) return S.jc[(r - NB_JC) * S.NC + cm];
-    return S.jf[(3 * (2 * ax + side) + r - NB_JF) * S.NC + cm];
+    if (m[ax] < 2 || m[ax] > lim) return nullptr;
+    const long gm = gidx(B, 0, m[1], m[2]), cm = cidx(B, 0, m[1], m[2]);
+    if (r < NB_DC) return r < npv ? S.pv + r * S.NG + gm : nullptr;
+    if (r < NB_N3) return S.dc + (r - NB_DC) * S.NG + gm;
+    if (r < NB_JC) { const int self[3] = {0, j, k}; return face_base(S, B, ax, FWD ? m : self, r - NB_N3); }
+    if (r < NB_JF) return S.jc + (r - NB_JC) * S.NC + cm;
+    return S.jf + (3 * (2 * ax + side) + r - NB_JF) * S.NC + cm;
 }
 
 // row `lane` Jacobian operands of a staged block (pv at b + NB_PV, ...)
@@ -887,17 +836,47 @@ __device__ __forceinline__ void lds_barrier()
     __builtin_amdgcn_s_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
+// LDS written by some lanes of a wavefront, then read by others
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+}
+
+// Row `lane` of the dq of a cell in another tile, starting from a load
+// already issued (x) and polled, bounded, until every row of the lane's
+// 16-lane group holds its owner's value.
+__device__ __forceinline__ double settle_dq(double x, const double *p, bool mine, int *err)
+{
+    const unsigned long long gm = 0xFFFFull << (threadIdx.x & 48);
+    long polls = 0;
+    while (__ballot(mine && (unsigned long long)__double_as_longlong(x) == DQ_UNSET) & gm) {
+        if (++polls > POLL_LIMIT || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+            atomicOr(err, 1);
+            return 0.0;
+        }
+        __builtin_amdgcn_s_sleep(1);
+        if (mine) x = ld_dev(p);
+    }
+    return x;
+}
+
+constexpr int ST_ROW = ST_N + 2;     // LDS stage row, padded so the four groups of a wavefront hit distinct banks
+__device__ double g_nil[1] = {0.0};  // what a stage slot without an item reads
 
 template <bool FWD>
 __global__ __launch_bounds__(256) void k_sweep_tile(DS S, const int *tiles, int ntiles, int *err)
 {
-    __shared__ double lds[2][TJ * TK][NPV];
-    __shared__ double stg[2][TJ * TK][ST_N];
+    __shared__ double lds[2][TJ * TK][NPV];      // in-tile dq hand-over, by the parity of s
+    __shared__ double xch[2][TJ * TK][NPV];      // dq of the j / k neighbour in another tile
+    __shared__ double stg[2][TJ * TK][ST_ROW];
     const int npv = S.npv;
     const int lane = threadIdx.x & 15, grp = threadIdx.x >> 4;     // 16 groups of 16 lanes
     const int gj = grp % TJ, gk = grp / TJ;
     const int sg = FWD ? 1 : -1;
     const double sgn = FWD ? 1.0 : -1.0;
+    const bool mine = lane < npv;
     double *dq = FWD ? S.dqf : S.dqb;
     constexpr int PER = ST_N / 16;                                  // stage items per lane
     for (int tq = blockIdx.x; tq < ntiles; tq += gridDim.x) {
@@ -910,96 +889,100 @@ __global__ __launch_bounds__(256) void k_sweep_tile(DS S, const int *tiles, int 
         // the tile's hyperplane range in sweep order
         const int sa = FWD ? 2 + j0 + k0 : B.I + jend + kend;
         const int sb = FWD ? B.I + jend + kend : 2 + j0 + k0;
-        auto in_range = [&](int ii) { return live && ii >= 2 && ii <= B.I; };
-        auto ahead = [&](int s, int d) { return FWD ? (s + d <= sb) : (s - d >= sb); };
-        // prologue: stage of the first hyperplane; the second one's loads in flight
-        {
-            const int i = sa - j - k;
-            if (in_range(i))
+        const int ifirst = FWD ? 2 : B.I;
+        // where this lane's stage items come from: item r of cell i is bp[r][i]
+        const double *bp[PER];
+        unsigned msk = 0;
 #pragma unroll
-                for (int r = 0; r < PER; r++) stg[sa & 1][grp][lane + 16 * r] = stage_item<FWD>(S, B, i, j, k, lane + 16 * r);
+        for (int r = 0; r < PER; r++) {
+            const double *q = live ? stage_base<FWD>(S, B, j, k, lane + 16 * r) : nullptr;
+            if (q) msk |= 1u << r;
+            bp[r] = q ? q : g_nil;
         }
+        // branch-free stage loads of cell ii (clamped into the pencil: the
+        // stage of a cell outside it is loaded but never read)
+        auto fetch = [&](int ii, double *v) {
+            const int ic = min(max(ii, 2), B.I);
+#pragma unroll
+            for (int r = 0; r < PER; r++) v[r] = bp[r][((msk >> r) & 1) ? ic : 0];
+        };
         double nxt[PER];
-        bool hn = ahead(sa, 1) && in_range(sa + sg - j - k);
-        if (hn) {
+        fetch(sa - j - k, nxt);
 #pragma unroll
-            for (int r = 0; r < PER; r++) nxt[r] = stage_item<FWD>(S, B, sa + sg - j - k, j, k, lane + 16 * r);
-        }
+        for (int r = 0; r < PER; r++) stg[sa & 1][grp][lane + 16 * r] = nxt[r];
+        if (sa != sb) fetch(sa + sg - j - k, nxt);
         __syncthreads();
         JacNb prev{};
-        double xprev = 0.0;
-        bool have_prev = false;
         for (int s = sa;; s += sg) {
             const int i = s - j - k;
-            // stage of hyperplane s+1 (loaded during the previous step), then the loads for s+2
-            if (hn) {
+            // stage of hyperplane s+1, loaded during the previous step
+            if (s != sb) {
 #pragma unroll
                 for (int r = 0; r < PER; r++) stg[(s + sg) & 1][grp][lane + 16 * r] = nxt[r];
             }
-            hn = ahead(s, 2) && in_range(i + 2 * sg);
-            if (hn) {
-#pragma unroll
-                for (int r = 0; r < PER; r++) nxt[r] = stage_item<FWD>(S, B, i + 2 * sg, j, k, lane + 16 * r);
-            }
-            if (in_range(i)) {
+            const bool act = live && i >= 2 && i <= B.I;
+            const int jm = j - sg, km = k - sg;
+            const bool hj = act && jm >= 2 && jm <= B.J, hk = act && km >= 2 && km <= B.K;
+            const bool lj = jm >= j0 && jm <= jend, lk = km >= k0 && km <= kend;
+            // neighbours in other tiles: their loads go out first, then the
+            // stage loads of hyperplane s+2
+            const bool gj_ = hj && !lj && mine, gk_ = hk && !lk && mine;
+            const double *pj = dq + lane * S.NW + widx(B, i, jm, k), *pk = dq + lane * S.NW + widx(B, i, j, km);
+            double xj, xk;
+            if (gj_) xj = ld_dev(pj);
+            if (gk_) xk = ld_dev(pk);
+            if (FWD ? s + 2 <= sb : s - 2 >= sb) fetch(i + 2 * sg, nxt);
+            double y = 0.0, xn = 0.0;
+            if (act) {
                 const double *st = stg[s & 1][grp];
-                const long w = widx(B, i, j, k);
-                const int mj[3] = {i, j - sg, k}, mk[3] = {i, j, k - sg};
-                const bool hj = (mj[1] >= 2 && mj[1] <= B.J), hk = (mk[2] >= 2 && mk[2] <= B.K);
-                const bool lj = hj && mj[1] >= j0 && mj[1] <= jend, lk = hk && mk[2] >= k0 && mk[2] <= kend;
-                double y = lane < npv ? st[ST_RES + lane] : 0.0;
-                double xv[NPV];
-                if (have_prev) {
-#pragma unroll
-                    for (int q = 0; q < NPV; q++) xv[q] = __shfl(xprev, q, 16);
-                    jac_row(npv, prev, sg, lane, xv, xprev, y, sgn);
-                }
                 const int pb = (s - sg) & 1;                 // dq buffer the previous hyperplane wrote
-                if (hj) {
-                    const JacNb nbj = stage_jnb(st, ST_J, lane, npv);
-                    double xo;
-                    if (lj) {
+                const int ln = lane < NPV ? lane : 0;
+                y = mine ? st[ST_RES + lane] : 0.0;
+                double xv[NPV];
+                if (i != ifirst) {
 #pragma unroll
-                        for (int q = 0; q < NPV; q++) xv[q] = lds[pb][grp - sg][q];
-                        xo = lds[pb][grp - sg][lane < NPV ? lane : 0];
-                    } else {
-                        const long wm = widx(B, mj[0], mj[1], mj[2]);
-                        wait_dq(dq, S.NW, wm, npv, xv, err);
-                        xo = (lane < npv) ? ld_dev(dq + lane * S.NW + wm) : 0.0;
+                    for (int q = 0; q < NPV; q++) xv[q] = lds[pb][grp][q];
+                    jac_row(npv, prev, sg, lane, xv, lds[pb][grp][ln], y, sgn);
+                }
+                if (hj) {
+                    if (!lj) {
+                        xj = settle_dq(xj, pj, gj_, err);
+                        if (lane < NPV) xch[0][grp][lane] = xj;
+                        wave_lds_sync();
                     }
-                    jac_row(npv, nbj, sg, lane, xv, xo, y, sgn);
+                    const double *row = lj ? lds[pb][grp - sg] : xch[0][grp];
+#pragma unroll
+                    for (int q = 0; q < NPV; q++) xv[q] = row[q];
+                    jac_row(npv, stage_jnb(st, ST_J, lane, npv), sg, lane, xv, row[ln], y, sgn);
                 }
                 if (hk) {
-                    const JacNb nbk = stage_jnb(st, ST_K, lane, npv);
-                    double xo;
-                    if (lk) {
-#pragma unroll
-                        for (int q = 0; q < NPV; q++) xv[q] = lds[pb][grp - sg * TJ][q];
-                        xo = lds[pb][grp - sg * TJ][lane < NPV ? lane : 0];
-                    } else {
-                        const long wm = widx(B, mk[0], mk[1], mk[2]);
-                        wait_dq(dq, S.NW, wm, npv, xv, err);
-                        xo = (lane < npv) ? ld_dev(dq + lane * S.NW + wm) : 0.0;
+                    if (!lk) {
+                        xk = settle_dq(xk, pk, gk_, err);
+                        if (lane < NPV) xch[1][grp][lane] = xk;
+                        wave_lds_sync();
                     }
-                    jac_row(npv, nbk, sg, lane, xv, xo, y, sgn);
+                    const double *row = lk ? lds[pb][grp - sg * TJ] : xch[1][grp];
+#pragma unroll
+                    for (int q = 0; q < NPV; q++) xv[q] = row[q];
+                    jac_row(npv, stage_jnb(st, ST_K, lane, npv), sg, lane, xv, row[ln], y, sgn);
                 }
-                if (FWD && lane < npv) S.dcv[lane * S.NW + w] = y;
 #pragma unroll
                 for (int q = 0; q < NPV; q++) xv[q] = __shfl(y, q, 16);
                 // this cell's Jacobian row through the face it shares with the next cell
                 prev = stage_jnb(st, ST_PV, lane, npv);
-                double xn = 0.0;
-                if (lane < npv) {
+                if (mine) {
                     double pc[NPV];
 #pragma unroll
                     for (int q = 0; q < NPV; q++) pc[q] = st[ST_PV + q];
                     xn = getx_apply([&](int q) { return st[ST_GX + q]; }, pc, st[ST_DC + D_RY1], st[ST_DC + D_RT],
                                     xv, lane, y, st[ST_PV + lane]);
-                    st_dev(dq + lane * S.NW + w, xn);
                 }
                 if (lane < NPV) lds[s & 1][grp][lane] = xn;
-                xprev = xn;
-                have_prev = true;
+            }
+            if (act && mine) {
+                const long w = lane * S.NW + widx(B, i, j, k);
+                if (FWD) S.dcv[w] = y;
+                st_dev(dq + w, xn);
             }
             lds_barrier();
             if (s == sb) break;

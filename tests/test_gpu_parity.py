@@ -140,3 +140,29 @@ def test_dual_time_shift_matches_oracle(pkg, orc, name):
         s.unsteady_shift()
         o.unsteady_shift()
     s.close()
+
+
+@pytest.mark.parametrize("name", ["lam_roe_tvd", "sst_cav_ausm_mlp"])
+def test_timeinteg_many_tiles_matches_oracle(pkg, orc, name):
+    """A cross-section of 43 x 43 cells: 121 sweep tiles in flight at once,
+    most of them waiting on two upstream tiles."""
+    case = pkg.cases.config1(20, 44, 44) if name == "lam_roe_tvd" else pkg.cases.config2(20, 44, 44)
+    if name != "lam_roe_tvd":
+        case.params["cav_clip"] = 1
+    s, o = pair(pkg, orc, case)
+    npv = case.npv
+    for it in range(2):
+        print(f"  iteration {it}", flush=True)
+        o.setqres()
+        o.timeinteg(1, it + 1, 0.0)
+        s.timeinteg(1, it + 1, 0.0)
+        pg = interior(case, 0, s.get_state(0)[0], npv)
+        po = interior(case, 0, o.get_state(0)[0], npv)
+        e = rel_err(pg, po)
+        if not np.all(e < TOL):
+            bad = np.argmax(np.max(np.abs(pg - po) / np.maximum(np.abs(po), 1e-300), axis=1))
+            b = case.blocks[0]
+            ni, nj = b.imax - 1, b.jmax - 1
+            raise AssertionError(f"iteration {it}: rel err {e}; worst cell i={2 + bad % ni} "
+                                 f"j={2 + (bad // ni) % nj} k={2 + bad // (ni * nj)}")
+    s.close()
