@@ -70,6 +70,7 @@ struct DS {           // device state, passed by value
     double *res, *dt, *icav, *itt, *ocut, *rcav, *qq;
     double *dqf, *dqb, *dcv;
     double *gx;       // diagonal-inverse terms [NGX][NC]
+    double *zero;     // zeros, one more than the longest pencil: the sweeps' source for absent items
     double *jc;       // Jacobian cell terms [2][NC]: beta, h
     double *jf;       // Jacobian face terms [18][NC]: faces -i,+i,-j,+j,-k,+k x (u, lam, vis)
     long NPLANE;      // sum over blocks of (J-1)(K-1)
@@ -811,14 +812,15 @@ __device__ __forceinline__ const double *stage_base(const DS &S, const BlkD &B, 
 }
 
 // row `lane` Jacobian operands of a staged block (pv at b + NB_PV, ...)
-__device__ __forceinline__ JacNb stage_jnb(const double *st, int b, int lane, int npv)
+template <class A>
+__device__ __forceinline__ JacNb stage_jnb(const A &st, int b, int lane, int npv)
 {
     JacNb q;
     q.pvn = lane < npv ? st[b + NB_PV + lane] : 0.0;
     q.pv1 = st[b + NB_PV + 1]; q.pv2 = st[b + NB_PV + 2]; q.pv3 = st[b + NB_PV + 3];
-    const double *d = st + b + NB_DC;
-    q.rho = d[D_RHO]; q.rp = d[D_RP]; q.rt = d[D_RT]; q.ry1 = d[D_RY1]; q.ry2 = d[D_RY2];
-    q.hp = d[D_HP]; q.ht = d[D_HT]; q.hy1 = d[D_HY1]; q.hy2 = d[D_HY2];
+    const int d = b + NB_DC;
+    q.rho = st[d + D_RHO]; q.rp = st[d + D_RP]; q.rt = st[d + D_RT]; q.ry1 = st[d + D_RY1]; q.ry2 = st[d + D_RY2];
+    q.hp = st[d + D_HP]; q.ht = st[d + D_HT]; q.hy1 = st[d + D_HY1]; q.hy2 = st[d + D_HY2];
     q.n3[0] = st[b + NB_N3]; q.n3[1] = st[b + NB_N3 + 1]; q.n3[2] = st[b + NB_N3 + 2];
     q.n3n = (lane >= 1 && lane <= 3) ? st[b + NB_N3 + lane - 1] : 0.0;
     q.jc = JacCell{st[b + NB_JC], st[b + NB_JC + 1]};
@@ -836,154 +838,253 @@ __device__ __forceinline__ void lds_barrier()
     __builtin_amdgcn_s_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
-// LDS written by some lanes of a wavefront, then read by others
-__device__ __forceinline__ void wave_lds_sync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
-}
+// global address space: a pointer read back from LDS would otherwise be
+// generic, and generic loads also count against the LDS wait counter
+#define GLOBAL __attribute__((address_space(1)))
+constexpr int WIN = 4;               // cells of a stage window
+// one cell of a stage window: item e at stride WIN
+struct WinRow {
+    const double *p;
+    __device__ double operator[](int e) const { return p[e * WIN]; }
+};
+constexpr int SWEEP_THREADS = 256 + 64 + 64;   // 16 pencil groups of 16 lanes, a store and a poll wavefront
+constexpr int NEDGE = 2 * 4 * NPV;   // cross-tile dq values a tile reads per step: two edges of four pencils
+static_assert(TJ == 4 && TK == 4, "the poll wavefront maps an edge of four pencils");
 
-// Row `lane` of the dq of a cell in another tile, starting from a load
-// already issued (x) and polled, bounded, until every row of the lane's
-// 16-lane group holds its owner's value.
-__device__ __forceinline__ double settle_dq(double x, const double *p, bool mine, int *err)
-{
-    const unsigned long long gm = 0xFFFFull << (threadIdx.x & 48);
-    long polls = 0;
-    while (__ballot(mine && (unsigned long long)__double_as_longlong(x) == DQ_UNSET) & gm) {
-        if (++polls > POLL_LIMIT || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-            atomicOr(err, 1);
-            return 0.0;
-        }
-        __builtin_amdgcn_s_sleep(1);
-        if (mine) x = ld_dev(p);
-    }
-    return x;
-}
-
-constexpr int ST_ROW = ST_N + 2;     // LDS stage row, padded so the four groups of a wavefront hit distinct banks
-__device__ double g_nil[1] = {0.0};  // what a stage slot without an item reads
-
+// Division of labour inside a tile's workgroup:
+//  - four pencil wavefronts (16 groups of 16 lanes, lane = row) run the
+//    hyperplane steps; they read only LDS and the stage prefetch (plain
+//    loads), so their waits cover nothing slower;
+//  - a poll wavefront reads, with device-coherent loads, the dq of the
+//    upstream cells in other tiles one step ahead of use and hands it over
+//    through LDS (it is the only wavefront that waits on other tiles);
+//  - a store wavefront writes each hyperplane's dq (and dcv) one step later.
+// A step's Jacobian coefficients are formed during the step before from its
+// stage without branching on the row, so the dependent chain of a step is
+// three row products, the exchange of y and D^-1.
 template <bool FWD>
-__global__ __launch_bounds__(256) void k_sweep_tile(DS S, const int *tiles, int ntiles, int *err)
+__global__ __launch_bounds__(SWEEP_THREADS) void k_sweep_tile(DS S, const int *tiles, int ntiles, int *err)
 {
-    __shared__ double lds[2][TJ * TK][NPV];      // in-tile dq hand-over, by the parity of s
-    __shared__ double xch[2][TJ * TK][NPV];      // dq of the j / k neighbour in another tile
-    __shared__ double stg[2][TJ * TK][ST_ROW];
+    __shared__ double lds[2][TJ * TK][NPV];      // dq of the hyperplane, by the parity of s
+    __shared__ double ylds[2][TJ * TK][NPV];     // dcv of the hyperplane (forward sweep)
+    __shared__ double xch[2][2][TJ * TK][NPV];   // [parity of s][j / k] dq of the neighbour in another tile
+    __shared__ double win[2][TJ * TK][ST_N][WIN];   // stage windows (see the pencil wavefronts)
+    __shared__ const double *src[TJ * TK][ST_N];
+    __shared__ double zrow[NPV];                 // dq of a ghost neighbour
+    __shared__ int gave_up;
     const int npv = S.npv;
-    const int lane = threadIdx.x & 15, grp = threadIdx.x >> 4;     // 16 groups of 16 lanes
-    const int gj = grp % TJ, gk = grp / TJ;
     const int sg = FWD ? 1 : -1;
     const double sgn = FWD ? 1.0 : -1.0;
-    const bool mine = lane < npv;
     double *dq = FWD ? S.dqf : S.dqb;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 15, grp = (threadIdx.x >> 4) & 15;
     constexpr int PER = ST_N / 16;                                  // stage items per lane
+    if (threadIdx.x == 0) gave_up = 0;
+    if (threadIdx.x < NPV) zrow[threadIdx.x] = 0.0;
     for (int tq = blockIdx.x; tq < ntiles; tq += gridDim.x) {
         const int *td = tiles + 4 * (FWD ? tq : ntiles - 1 - tq);
-        const BlkD &B = S.b[td[0]];
+        const BlkD B = S.b[td[0]];           // a copy: the loops read the block's extents from registers
+        const long NW = S.NW;
         const int j0 = td[1], k0 = td[2];
         const int jend = min(j0 + TJ - 1, B.J), kend = min(k0 + TK - 1, B.K);
-        const int j = j0 + gj, k = k0 + gk;
-        const bool live = (j <= jend && k <= kend);
         // the tile's hyperplane range in sweep order
         const int sa = FWD ? 2 + j0 + k0 : B.I + jend + kend;
         const int sb = FWD ? B.I + jend + kend : 2 + j0 + k0;
-        const int ifirst = FWD ? 2 : B.I;
-        // where this lane's stage items come from: item r of cell i is bp[r][i]
-        const double *bp[PER];
-        unsigned msk = 0;
+        auto active = [&](int g, int ii) {
+            const int jj = j0 + g % TJ, kk = k0 + g / TJ;
+            return jj <= jend && kk <= kend && ii - jj - kk >= 2 && ii - jj - kk <= B.I;
+        };
+        if (wave == 4) {
+            // store wavefront: hyperplane s is published during step s+1
+            auto publish = [&](int s) {
+                const int pb = s & 1;
+                for (int e = threadIdx.x - 256; e < TJ * TK * NPV; e += 64) {
+                    const int g = e / NPV, n = e - g * NPV;
+                    if (n < npv && active(g, s)) {
+                        const int jj = j0 + g % TJ, kk = k0 + g / TJ;
+                        const long w = n * NW + widx(B, s - jj - kk, jj, kk);
+                        if (FWD) S.dcv[w] = ylds[pb][g][n];
+                        st_dev(dq + w, lds[pb][g][n]);
+                    }
+                }
+            };
+            __syncthreads();
+            for (int s = sa;; s += sg) {
+                if (s != sa) publish(s - sg);
+                lds_barrier();
+                if (s == sb) break;
+            }
+            publish(sb);
+            if (gave_up && threadIdx.x == 256) atomicOr(err, 1);
+            __syncthreads();
+            continue;
+        }
+        if (wave == 5) {
+            // poll wavefront: item q = lane, lane + 64 of the 72 edge values
+            // (edge a = q / 36: the j (a = 0) or k (a = 1) neighbours; pencil
+            // e = q / 9 % 4 along the edge; row n = q % 9)
+            const int l64 = threadIdx.x - 320;
+            const double *dummy = dq + widx(B, min(max(sa - j0 - k0, 2), B.I), j0, k0);
+            auto item = [&](int q, int s, bool &need) -> const double * {
+                const int a = q / (4 * NPV), e = (q / NPV) % 4, n = q % NPV;
+                const int g = a == 0 ? e * TJ + (FWD ? 0 : TJ - 1) : (FWD ? 0 : TK - 1) * TJ + e;
+                const int jj = j0 + g % TJ, kk = k0 + g / TJ;
+                const int jn = a == 0 ? jj - sg : jj, kn = a == 0 ? kk : kk - sg;
+                need = q < NEDGE && n < npv && active(g, s) && jn >= 2 && jn <= B.J && kn >= 2 && kn <= B.K;
+                return need ? dq + n * NW + widx(B, s - jj - kk, jn, kn) : dummy;
+            };
+            bool n0, n1;
+            const double *p0, *p1;
+            double v0, v1;
+            auto issue = [&](int s) {            // loads for step s (unconditional: the count of loads never varies)
+                p0 = item(l64, s, n0);
+                p1 = item(l64 + 64, s, n1);
+                v0 = ld_dev(p0);
+                v1 = ld_dev(p1);
+            };
+            auto deliver = [&](int s) {          // wait (bounded) for step s's values, then hand them over
+                auto unset = [](bool nd, double v) { return nd && (unsigned long long)__double_as_longlong(v) == DQ_UNSET; };
+                if (__ballot(unset(n0, v0) || unset(n1, v1))) {
+                    long polls = 0;
+                    do {
+                        if (++polls > POLL_LIMIT || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                            gave_up = 1;
+                            v0 = v1 = 0.0;
+                            break;
+                        }
+                        __builtin_amdgcn_s_sleep(1);
+                        if (unset(n0, v0)) v0 = ld_dev(p0);
+                        if (unset(n1, v1)) v1 = ld_dev(p1);
+                    } while (__ballot(unset(n0, v0) || unset(n1, v1)));
+                }
+                const int par = s & 1;
+                for (int t = 0; t < 2; t++) {
+                    const int q = l64 + 64 * t;
+                    if (q < NEDGE) {
+                        const int a = q / (4 * NPV), e = (q / NPV) % 4, n = q % NPV;
+                        const int g = a == 0 ? e * TJ + (FWD ? 0 : TJ - 1) : (FWD ? 0 : TK - 1) * TJ + e;
+                        xch[par][a][g][n] = (t == 0 ? n0 : n1) ? (t == 0 ? v0 : v1) : 0.0;
+                    }
+                }
+            };
+            issue(sa);
+            deliver(sa);
+            issue(sa + sg);
+            __syncthreads();
+            for (int s = sa;; s += sg) {
+                if (s != sb) {
+                    deliver(s + sg);
+                    issue(s + 2 * sg);
+                }
+                lds_barrier();
+                if (s == sb) break;
+            }
+            __syncthreads();
+            continue;
+        }
+        // pencil wavefronts
+        const int gj = grp % TJ, gk = grp / TJ;
+        const bool mine = lane < npv;
+        const int ln = lane < NPV ? lane : 0;
+        const int j = j0 + gj, k = k0 + gk;
+        const bool live = (j <= jend && k <= kend);
+        const int BI = B.I;
+        const int jm = j - sg, km = k - sg;
+        const bool lj = jm >= j0 && jm <= jend, lk = km >= k0 && km <= kend;
+        const bool nj = jm >= 2 && jm <= B.J, nk = km >= 2 && km <= B.K;   // neighbour inside the block
+        // where the stage items come from: item e of cell i is src[grp][e][i]
+        // (the zero array for items a pencil does not read)
 #pragma unroll
         for (int r = 0; r < PER; r++) {
             const double *q = live ? stage_base<FWD>(S, B, j, k, lane + 16 * r) : nullptr;
-            if (q) msk |= 1u << r;
-            bp[r] = q ? q : g_nil;
+            src[grp][lane + 16 * r] = q ? q : S.zero;
         }
-        // branch-free stage loads of cell ii (clamped into the pencil: the
-        // stage of a cell outside it is loaded but never read)
-        auto fetch = [&](int ii, double *v) {
-            const int ic = min(max(ii, 2), B.I);
+        // Stage windows: window m holds cells at sweep distance 4m..4m+3 of
+        // the pencil (distance d = i - 2 forward, I - i backward), item-major
+        // with the four cells adjacent, in LDS buffer m & 1.  Four adjacent
+        // lanes load one item's four cells (one cache line between them);
+        // quarter q of a window (items 32q..32q+31) is loaded in one step and
+        // written to LDS at the top of the next.  During the step at distance
+        // d the quarter (d+1) & 3 of window ((d+1) >> 2) + 1 is loaded, so a
+        // window is complete one step before its first cell is the step's
+        // lookahead, and it reuses the buffer of the window two back only
+        // after that window's last read.
+        const int l4 = lane >> 2, c4 = lane & 3;
+        auto wcell = [&](int m, int c) { return FWD ? 2 + 4 * m + c : BI - 4 * m - c; };
+        auto fetchq = [&](int m, int qq, double *v) {
+            const int ic = min(max(wcell(m, c4), 2), BI);
 #pragma unroll
-            for (int r = 0; r < PER; r++) v[r] = bp[r][((msk >> r) & 1) ? ic : 0];
+            for (int r = 0; r < 8; r++) v[r] = ((const GLOBAL double *)src[grp][32 * qq + 4 * r + l4])[ic];
         };
-        double nxt[PER];
-        fetch(sa - j - k, nxt);
+        auto putq = [&](int m, int qq, const double *v) {
 #pragma unroll
-        for (int r = 0; r < PER; r++) stg[sa & 1][grp][lane + 16 * r] = nxt[r];
-        if (sa != sb) fetch(sa + sg - j - k, nxt);
+            for (int r = 0; r < 8; r++) win[m & 1][grp][32 * qq + 4 * r + l4][c4] = v[r];
+        };
+        // prologue: window 0 complete, window -1's buffer zeroed (steps before
+        // the pencil's first cell read it), the first pending quarter in flight
+        for (int qq = 0; qq < 4; qq++) {
+            double v[8];
+            fetchq(0, qq, v);
+            putq(0, qq, v);
+        }
+        for (int e = lane; e < ST_N * WIN; e += 16) (&win[1][grp][0][0])[e] = 0.0;
+        const int d0 = FWD ? sa - j - k - 2 : BI - (sa - j - k);
+        int pm = (d0 >> 2) + 1, pq = d0 & 3;                          // the quarter in flight
+        double nx[8];
+        fetchq(pm, pq, nx);
+        if (lane < NPV) lds[(sa - sg) & 1][grp][lane] = 0.0;       // the first step's upstream reads zeros
+        double ci[8], cj[8], ck[8];                                  // coefficients of the coming step
+#pragma unroll
+        for (int q = 0; q < 8; q++) ci[q] = 0.0;
+        {
+            const WinRow s0{&win[(d0 >> 2) & 1][grp][0][d0 & 3]};
+            jac_row_coef(npv, stage_jnb(s0, ST_J, lane, npv), sg, lane, sgn, cj);
+            jac_row_coef(npv, stage_jnb(s0, ST_K, lane, npv), sg, lane, sgn, ck);
+        }
         __syncthreads();
-        JacNb prev{};
         for (int s = sa;; s += sg) {
             const int i = s - j - k;
-            // stage of hyperplane s+1, loaded during the previous step
-            if (s != sb) {
+            const int d = FWD ? i - 2 : BI - i;
+            const bool act = live && i >= 2 && i <= BI;
+            // the quarter loaded during the previous step, then the next one
+            putq(pm, pq, nx);
+            pm = ((d + 1) >> 2) + 1;
+            pq = (d + 1) & 3;
+            fetchq(pm, pq, nx);
+            // dq of the three upstream neighbours: in the tile from the
+            // previous hyperplane, from another tile through the poll wavefront
+            const WinRow st{&win[(d >> 2) & 1][grp][0][d & 3]};
+            const int pb = (s - sg) & 1;
+            const double *ri = lds[pb][grp];
+            const double *rj = lj ? lds[pb][grp - sg] : (nj ? xch[s & 1][0][grp] : zrow);
+            const double *rk = lk ? lds[pb][grp - sg * TJ] : (nk ? xch[s & 1][1][grp] : zrow);
+            double xi[NPV], xj[NPV], xk[NPV];
 #pragma unroll
-                for (int r = 0; r < PER; r++) stg[(s + sg) & 1][grp][lane + 16 * r] = nxt[r];
+            for (int q = 0; q < NPV; q++) { xi[q] = ri[q]; xj[q] = rj[q]; xk[q] = rk[q]; }
+            // dcv = res + the three upstream products, in the reference's i, j, k order
+            double y = mine ? st[ST_RES + lane] : 0.0;
+            jac_row_apply(ci, xi, ri[ln], y);
+            jac_row_apply(cj, xj, rj[ln], y);
+            jac_row_apply(ck, xk, rk[ln], y);
+            __builtin_amdgcn_sched_barrier(0);
+            // dq = D^-1 dcv
+            double yv[NPV], pc[NPV];
+#pragma unroll
+            for (int q = 0; q < NPV; q++) { yv[q] = __shfl(y, q, 16); pc[q] = st[ST_PV + q]; }
+            const double xn = getx_apply_sel([&](int q) { return st[ST_GX + q]; }, pc, st[ST_DC + D_RY1],
+                                             st[ST_DC + D_RT], yv, lane, y, st[ST_PV + ln]);
+            if (lane < NPV) {
+                lds[s & 1][grp][lane] = act ? xn : 0.0;
+                if (FWD) ylds[s & 1][grp][lane] = y;
             }
-            const bool act = live && i >= 2 && i <= B.I;
-            const int jm = j - sg, km = k - sg;
-            const bool hj = act && jm >= 2 && jm <= B.J, hk = act && km >= 2 && km <= B.K;
-            const bool lj = jm >= j0 && jm <= jend, lk = km >= k0 && km <= kend;
-            // neighbours in other tiles: their loads go out first, then the
-            // stage loads of hyperplane s+2
-            const bool gj_ = hj && !lj && mine, gk_ = hk && !lk && mine;
-            const double *pj = dq + lane * S.NW + widx(B, i, jm, k), *pk = dq + lane * S.NW + widx(B, i, j, km);
-            double xj, xk;
-            if (gj_) xj = ld_dev(pj);
-            if (gk_) xk = ld_dev(pk);
-            if (FWD ? s + 2 <= sb : s - 2 >= sb) fetch(i + 2 * sg, nxt);
-            double y = 0.0, xn = 0.0;
-            if (act) {
-                const double *st = stg[s & 1][grp];
-                const int pb = (s - sg) & 1;                 // dq buffer the previous hyperplane wrote
-                const int ln = lane < NPV ? lane : 0;
-                y = mine ? st[ST_RES + lane] : 0.0;
-                double xv[NPV];
-                if (i != ifirst) {
-#pragma unroll
-                    for (int q = 0; q < NPV; q++) xv[q] = lds[pb][grp][q];
-                    jac_row(npv, prev, sg, lane, xv, lds[pb][grp][ln], y, sgn);
-                }
-                if (hj) {
-                    if (!lj) {
-                        xj = settle_dq(xj, pj, gj_, err);
-                        if (lane < NPV) xch[0][grp][lane] = xj;
-                        wave_lds_sync();
-                    }
-                    const double *row = lj ? lds[pb][grp - sg] : xch[0][grp];
-#pragma unroll
-                    for (int q = 0; q < NPV; q++) xv[q] = row[q];
-                    jac_row(npv, stage_jnb(st, ST_J, lane, npv), sg, lane, xv, row[ln], y, sgn);
-                }
-                if (hk) {
-                    if (!lk) {
-                        xk = settle_dq(xk, pk, gk_, err);
-                        if (lane < NPV) xch[1][grp][lane] = xk;
-                        wave_lds_sync();
-                    }
-                    const double *row = lk ? lds[pb][grp - sg * TJ] : xch[1][grp];
-#pragma unroll
-                    for (int q = 0; q < NPV; q++) xv[q] = row[q];
-                    jac_row(npv, stage_jnb(st, ST_K, lane, npv), sg, lane, xv, row[ln], y, sgn);
-                }
-#pragma unroll
-                for (int q = 0; q < NPV; q++) xv[q] = __shfl(y, q, 16);
-                // this cell's Jacobian row through the face it shares with the next cell
-                prev = stage_jnb(st, ST_PV, lane, npv);
-                if (mine) {
-                    double pc[NPV];
-#pragma unroll
-                    for (int q = 0; q < NPV; q++) pc[q] = st[ST_PV + q];
-                    xn = getx_apply([&](int q) { return st[ST_GX + q]; }, pc, st[ST_DC + D_RY1], st[ST_DC + D_RT],
-                                    xv, lane, y, st[ST_PV + lane]);
-                }
-                if (lane < NPV) lds[s & 1][grp][lane] = xn;
-            }
-            if (act && mine) {
-                const long w = lane * S.NW + widx(B, i, j, k);
-                if (FWD) S.dcv[w] = y;
-                st_dev(dq + w, xn);
-            }
+            __builtin_amdgcn_sched_barrier(0);
+            // coefficients of the next step (independent of dq): this cell's
+            // row through its face to the next cell, and the j / k neighbours
+            // of the next cell
+            jac_row_coef(npv, stage_jnb(st, ST_PV, lane, npv), sg, lane, sgn, ci);
+            const WinRow s1{&win[((d + 1) >> 2) & 1][grp][0][(d + 1) & 3]};
+            jac_row_coef(npv, stage_jnb(s1, ST_J, lane, npv), sg, lane, sgn, cj);
+            jac_row_coef(npv, stage_jnb(s1, ST_K, lane, npv), sg, lane, sgn, ck);
             lds_barrier();
             if (s == sb) break;
         }
@@ -1439,7 +1540,7 @@ int launch_sweeps(cav3d_ctx *c)
     int ntiles = c->ntiles;
     int *err = c->sweep_err;
     void *args[4] = {&S, &tiles, &ntiles, &err};
-    dim3 grid(c->coop_blocks), block(256);
+    dim3 grid(c->coop_blocks), block(SWEEP_THREADS);
     if (c->profile) hipEventRecord(c->ev[6], c->st);
     hipError_t e = hipLaunchCooperativeKernel((const void *)k_sweep_tile<true>, grid, block, args, 0, c->st);
     if (e != hipSuccess) return fail(std::string("forward sweep launch: ") + hipGetErrorString(e), -3);
@@ -1611,8 +1712,10 @@ int cav3d_create(const cav3d_params *prm, int nblocks, const cav3d_block *blocks
     if (prm->nsteady > 0) {
         if ((rc = dalloc(c, &S.qq, 2 * npv * S.NC))) { cav3d_destroy(c); return rc; }
     }
+    int maxi = 0;
+    for (int b = 0; b < nblocks; b++) maxi = std::max(maxi, c->blk[b].I);
     if ((rc = dalloc(c, &S.gx, (long)NGX * S.NC)) || (rc = dalloc(c, &S.jc, 2 * S.NC)) ||
-        (rc = dalloc(c, &S.jf, 18 * S.NC))) { cav3d_destroy(c); return rc; }
+        (rc = dalloc(c, &S.jf, 18 * S.NC)) || (rc = dalloc(c, &S.zero, maxi + 8))) { cav3d_destroy(c); return rc; }
     {
         // tile table: TJ x TK pencil tiles of every block, ordered by the tile
         // diagonal tj + tk (the forward sweep takes them in this order, the
@@ -1638,8 +1741,8 @@ int cav3d_create(const cav3d_params *prm, int nblocks, const cav3d_block *blocks
         int dev = 0, ncu = 0, per = 0, per2 = 0;
         hipGetDevice(&dev);
         hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void *)k_sweep_tile<true>, 256, 0);
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per2, (const void *)k_sweep_tile<false>, 256, 0);
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void *)k_sweep_tile<true>, SWEEP_THREADS, 0);
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per2, (const void *)k_sweep_tile<false>, SWEEP_THREADS, 0);
         per = std::min(per, per2);
         long cap = (long)ncu * per;
         if (cap < 1) { cav3d_destroy(c); return fail("sweep kernel cannot be made resident", -3); }

@@ -755,6 +755,51 @@ HD void jac_row(int npv, const JacNb &q, int sign, int n, const double *x, doubl
     }
 }
 
+// Row n of the same product as coefficients, formed without branching on n
+// (the rows of a cell sit on neighbouring lanes): y = y + c[l] x[l] over
+// l = 0..6 in order, then y = y + c[7] x[n], repeats jac_row's roundings.
+// Every a(n,l) is built from jac_row's products in jac_row's order; the
+// terms a row does not have enter as exact zeros (P r1 + 0 hy, a + 0), the
+// factor 0.5 and the sweep's sign are folded in exactly (c = +-0.5 a), and
+// rows n >= npv get zero coefficients.  Only the sign of a zero sum can
+// differ from jac_row.
+HD void jac_row_coef(int npv, const JacNb &q, int sign, int n, double sgn, double *c)
+{
+    const double lam = sign > 0 ? q.jf.lam : -q.jf.lam, vis = sign > 0 ? q.jf.vis : -q.jf.vis;
+    const double u = q.jf.u;
+    double r1[7];
+    r1[0] = u * q.rp + lam * q.jc.beta + vis * q.rp;
+    r1[1] = q.n3[0] * q.rho;
+    r1[2] = q.n3[1] * q.rho;
+    r1[3] = q.n3[2] * q.rho;
+    r1[4] = u * q.rt + lam * q.rt + vis * q.rt;
+    r1[5] = u * q.ry1 + lam * q.ry1 + vis * q.ry1;
+    r1[6] = u * q.ry2 + lam * q.ry2 + vis * q.ry2;
+    const double uuu = u + lam + vis;
+    const double ru = q.rho * uuu;
+    const double hy[7] = {q.hp, q.pv1, q.pv2, q.pv3, q.ht, q.hy1, q.hy2};
+    const bool mom = n >= 1 && n <= 3, en = n == 4, row = n < npv;
+    const double P = n == 0 ? 1.0 : (en ? q.jc.h : q.pvn);
+    const double Q = en ? ru : 0.0;
+    const double hs = sgn > 0.0 ? 0.5 : -0.5;
+#pragma unroll
+    for (int l = 0; l < 7; l++) {
+        double a = P * r1[l] + Q * hy[l];
+        const double e1 = l == 0 ? (mom ? q.n3n : (en ? -lam : 0.0)) : ((l == n && !en) ? ru : 0.0);
+        a = a + e1;
+        a = a + ((en && l == 0) ? -vis : 0.0);
+        c[l] = row ? hs * a : 0.0;
+    }
+    c[7] = (row && n >= 7) ? hs * (uuu * q.rho) : 0.0;
+}
+
+HD void jac_row_apply(const double *c, const double *x, double xn, double &y)
+{
+#pragma unroll
+    for (int l = 0; l < 7; l++) y = y + c[l] * x[l];
+    y = y + c[7] * xn;
+}
+
 HD void jac_apply(const Par &P, int npv, const CellJ &c, const double *n3, int sign, const double *x, double *y, double sgn)
 {
     JacCell jc = jac_cell(P, c);
@@ -874,6 +919,31 @@ HD double getx_apply(F G, const double *pv, double d_ry1, double d_rt, const dou
         return (G(GX_C4IB1S1) * r6 + G(GX_C1) * co1 - G(GX_C3IB1) * co3
                 - (co3 - G(GX_S1P5) * res[0] + res[5] * G(GX_SSY1))) * mm * a;
     return (G(GX_SS) * (res[5] - res[0] * pv[5]) - G(GX_C4SSIB1) * r6 - G(GX_C1) * co2 + G(GX_C2IB1) * co3) * mm * a;
+}
+
+// getx_apply for any row without branching on n: every row's value is
+// formed with getx_apply's operations in getx_apply's order, then row n's is
+// selected (the rows of a cell sit on neighbouring lanes).
+template <class F>
+HD double getx_apply_sel(F G, const double *pv, double d_ry1, double d_rt, const double *res, int n, double resn, double pvn)
+{
+    const double a = G(GX_A);
+    const double v1 = (resn - res[0] * pvn) * G(GX_IB1) * a;          // rows 1..3, and row 6 (resn = res[6], pvn = pv[6])
+    const double r6 = res[6] - res[0] * pv[6];
+    const double rk = res[7] - res[0] * pv[7], ro = res[8] - res[0] * pv[8];
+    const double v7 = (ro * G(GX_T2) - G(GX_RBT4) * rk) * G(GX_IT) * a;
+    const double v8 = (rk * G(GX_T3) - G(GX_RBT1) * ro) * G(GX_IT) * a;
+    const double uuu = res[1] * pv[1] + res[2] * pv[2] + res[3] * pv[3] - res[4];
+    const double co1 = res[0] * G(GX_AAA1) + d_ry1 * uuu + G(GX_CROSS) * res[6];
+    const double co2 = res[0] * G(GX_AAA2) + d_rt * uuu + G(GX_CROSS2) * res[6];
+    const double co3 = res[0] * G(GX_SSHM) + G(GX_RRR) * uuu + res[6] * G(GX_SSY2);
+    const double mm = G(GX_MM);
+    const double v0 = (G(GX_C4C1) * (-r6) - G(GX_C2) * co1 + G(GX_C3) * co2
+                       + G(GX_RB1) * (co2 + G(GX_CROSS1) * res[5] - G(GX_CP5) * res[0])) * mm * a;
+    const double v4 = (G(GX_C4IB1S1) * r6 + G(GX_C1) * co1 - G(GX_C3IB1) * co3
+                       - (co3 - G(GX_S1P5) * res[0] + res[5] * G(GX_SSY1))) * mm * a;
+    const double v5 = (G(GX_SS) * (res[5] - res[0] * pv[5]) - G(GX_C4SSIB1) * r6 - G(GX_C1) * co2 + G(GX_C2IB1) * co3) * mm * a;
+    return n == 0 ? v0 : n == 4 ? v4 : n == 5 ? v5 : n == 7 ? v7 : n == 8 ? v8 : v1;
 }
 
 HD void getx(const Par &P, int npv, const CellJ &c, const double f6[6][3], double dt,

@@ -99,6 +99,41 @@ void pc_jac(const int *iv, const double *dv, const double *pv, const double *d, 
     jac_apply(P, P.npv, mkcell(pv, d, mut, vol), n3, sign, x, y, 1.0);
 }
 
+// the same product through the branch-free row coefficients the sweeps use
+void pc_jac_coef(const int *iv, const double *dv, const double *pv, const double *d, double mut, double vol,
+                 const double *n3, int sign, double sgn, const double *x, double *y)
+{
+    Par P = mkpar(iv, dv);
+    CellJ c = mkcell(pv, d, mut, vol);
+    JacCell jc = jac_cell(P, c);
+    JacFace jf = jac_face(P, c, n3);
+    for (int n = 0; n < NPV; n++) {
+        double cf[8];
+        jac_row_coef(P.npv, jac_nb(c.pv, c.d, n3, jc, jf, n, P.npv), sign, n, sgn, cf);
+        jac_row_apply(cf, x, x[n], y[n]);
+    }
+}
+
+void pc_jac_rows(const int *iv, const double *dv, const double *pv, const double *d, double mut, double vol,
+                 const double *n3, int sign, double sgn, const double *x, double *y)
+{
+    Par P = mkpar(iv, dv);
+    jac_apply(P, P.npv, mkcell(pv, d, mut, vol), n3, sign, x, y, sgn);
+}
+
+// getx through the row-selecting form the sweeps use
+void pc_getx_sel(const int *iv, const double *dv, const double *pv, const double *d, double mut, double vol,
+                 const double *f6, double dt, const double *cc, const double *tt, const double *res, double *x)
+{
+    Par P = mkpar(iv, dv);
+    double f[6][3], g[NGX];
+    for (int q = 0; q < 18; q++) f[q / 3][q % 3] = f6[q];
+    CellJ c = mkcell(pv, d, mut, vol);
+    getx_prep(P, P.npv, c, f, dt, cc, tt, g);
+    for (int n = 0; n < NPV; n++) if (n < P.npv)
+        x[n] = getx_apply_sel([&](int q) { return g[q]; }, c.pv, c.d[D_RY1], c.d[D_RT], res, n, res[n], c.pv[n]);
+}
+
 void pc_merkle(const int *iv, const double *dv, const double *pc, const double *d, double vol, double *out)
 {
     Par P = mkpar(iv, dv);

@@ -24,6 +24,9 @@ def build(outdir):
     L.pc_localdt.argtypes = [_I, _D, _D, _D, C.c_double, C.c_double, _D]
     L.pc_getx.argtypes = [_I, _D, _D, _D, C.c_double, C.c_double, _D, C.c_double, _D, _D, _D, _D]
     L.pc_jac.argtypes = [_I, _D, _D, _D, C.c_double, C.c_double, _D, C.c_int, _D, _D]
+    L.pc_jac_coef.argtypes = [_I, _D, _D, _D, C.c_double, C.c_double, _D, C.c_int, C.c_double, _D, _D]
+    L.pc_jac_rows.argtypes = [_I, _D, _D, _D, C.c_double, C.c_double, _D, C.c_int, C.c_double, _D, _D]
+    L.pc_getx_sel.argtypes = [_I, _D, _D, _D, C.c_double, C.c_double, _D, C.c_double, _D, _D, _D, _D]
     L.pc_merkle.argtypes = [_I, _D, _D, _D, C.c_double, _D]
     L.pc_sst_source.argtypes = [_D, _D, C.c_double, C.c_double, _D, C.c_double, _D]
     L.pc_sst_eddy.argtypes = [_D, _D, C.c_double, C.c_double, _D]

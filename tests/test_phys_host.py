@@ -199,7 +199,17 @@ def test_cell_routines_bit_exact(L, OL, orc, pkg, clip, nsteady):
         L.pc_getx(ivp, dp(dvp), dp(pv), dp(d13), mut, vol, dp(f6a), dt, dp(cc), dp(tt), dp(res), dp(xg))
         OL.orc_getx(C.byref(cf), ptrs(f6), dp(grd), dp(pv), dp(dvv), dp(tv), dt, dp(cc), dp(tt), dp(res), dp(xo))
         assert np.array_equal(xg, xo)
+        xs = np.zeros(9)
+        L.pc_getx_sel(ivp, dp(dvp), dp(pv), dp(d13), mut, vol, dp(f6a), dt, dp(cc), dp(tt), dp(res), dp(xs))
+        assert np.array_equal(xs, xo)
         n3 = rng.standard_normal(3) * 1e-4
+        for sg, sgn in ((1, 1.0), (1, -1.0), (-1, 1.0), (-1, -1.0)):
+            x = rng.standard_normal(9)
+            y0 = rng.standard_normal(9)
+            yr, yc = y0.copy(), y0.copy()
+            L.pc_jac_rows(ivp, dp(dvp), dp(pv), dp(d13), mut, vol, dp(n3), sg, sgn, dp(x), dp(yr))
+            L.pc_jac_coef(ivp, dp(dvp), dp(pv), dp(d13), mut, vol, dp(n3), sg, sgn, dp(x), dp(yc))
+            assert np.array_equal(yr[:case.npv], yc[:case.npv])
         for sg in (1, -1):
             x = rng.standard_normal(9)
             y = np.zeros(9)
@@ -225,3 +235,38 @@ def test_cell_routines_bit_exact(L, OL, orc, pkg, clip, nsteady):
         assert np.array_equal(so, [*src, *itt, oc.value])
         assert L.pc_sst_eddy(dp(x7), dp(f6a), vol, grd[4], dp(d13)) == \
             OL.orc_kwsst_eddy(ptrs(f6), dp(grd), dp(x7), dp(dvv), dp(tv))
+
+
+def test_sweep_row_forms_laminar(L, pkg):
+    """The branch-free Jacobian rows and the row-selecting getx of the sweeps
+    against the row-by-row forms, for the 7-variable laminar state."""
+    case = pkg.cases.config1(8, 6, 6)
+    cfg = case.cfg_dict()
+    iv, dvp = H.par_arrays(cfg)
+    ivp = H.p(iv, C.c_int)
+    rng = np.random.default_rng(77)
+    for _ in range(200):
+        pv = np.array([rng.uniform(-3e4, 3e4), rng.uniform(-15, 15), rng.uniform(-3, 3), rng.uniform(-3, 3),
+                       rng.uniform(290, 310), 0.0, 0.0, 0.0, 0.0])
+        full = np.zeros(18)
+        tv2 = np.zeros(2)
+        L.pc_mixture(pv[0] + cfg["pref"], pv[4], 0.0, 0.0, dp(full), dp(tv2))
+        d13 = H.compact(full, tv2)
+        vol = rng.uniform(1e-9, 1e-6)
+        f6a = rng.standard_normal(18) * 1e-4
+        xg, xs = np.zeros(9), np.zeros(9)
+        cc = rng.standard_normal(4) * 1e5
+        tt = rng.standard_normal(4) * 1e2
+        res = rng.standard_normal(9)
+        dt = rng.uniform(1e-5, 1e-2)
+        L.pc_getx(ivp, dp(dvp), dp(pv), dp(d13), 0.0, vol, dp(f6a), dt, dp(cc), dp(tt), dp(res), dp(xg))
+        L.pc_getx_sel(ivp, dp(dvp), dp(pv), dp(d13), 0.0, vol, dp(f6a), dt, dp(cc), dp(tt), dp(res), dp(xs))
+        assert np.array_equal(xg[:7], xs[:7])
+        n3 = rng.standard_normal(3) * 1e-4
+        for sg, sgn in ((1, 1.0), (-1, -1.0)):
+            x = rng.standard_normal(9)
+            y0 = rng.standard_normal(9)
+            yr, yc = y0.copy(), y0.copy()
+            L.pc_jac_rows(ivp, dp(dvp), dp(pv), dp(d13), 0.0, vol, dp(n3), sg, sgn, dp(x), dp(yr))
+            L.pc_jac_coef(ivp, dp(dvp), dp(pv), dp(d13), 0.0, vol, dp(n3), sg, sgn, dp(x), dp(yc))
+            assert np.array_equal(yr[:7], yc[:7])
