@@ -895,7 +895,8 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_sweep_tile(DS S, const int *t
             return jj <= jend && kk <= kend && ii - jj - kk >= 2 && ii - jj - kk <= B.I;
         };
         if (wave == 4) {
-            // store wavefront: hyperplane s is published during step s+1
+            // store wavefront: hyperplane s is published during step s+1,
+            // while the pencil wavefronts compute
             auto publish = [&](int s) {
                 const int pb = s & 1;
                 for (int e = threadIdx.x - 256; e < TJ * TK * NPV; e += 64) {
@@ -947,12 +948,13 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_sweep_tile(DS S, const int *t
                 if (__ballot(unset(n0, v0) || unset(n1, v1))) {
                     long polls = 0;
                     do {
-                        if (++polls > POLL_LIMIT || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                        // another tile's give-up is read now and then, not on every poll
+                        if (++polls > POLL_LIMIT ||
+                            ((polls & 255) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
                             gave_up = 1;
                             v0 = v1 = 0.0;
                             break;
                         }
-                        __builtin_amdgcn_s_sleep(1);
                         if (unset(n0, v0)) v0 = ld_dev(p0);
                         if (unset(n1, v1)) v1 = ld_dev(p1);
                     } while (__ballot(unset(n0, v0) || unset(n1, v1)));

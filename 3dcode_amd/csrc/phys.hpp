@@ -921,6 +921,19 @@ HD double getx_apply(F G, const double *pv, double d_ry1, double d_rt, const dou
     return (G(GX_SS) * (res[5] - res[0] * pv[5]) - G(GX_C4SSIB1) * r6 - G(GX_C1) * co2 + G(GX_C2IB1) * co3) * mm * a;
 }
 
+// a where c holds, else b, bit for bit and without a branch: the mask is
+// opaque to the compiler on the device, so it cannot sink the work behind a
+// or b into a branch on c (the rows of a cell sit on neighbouring lanes).
+HD double bsel(bool c, double a, double b)
+{
+    unsigned long long m = c ? ~0ull : 0ull;
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm volatile("" : "+v"(m));
+#endif
+    return __builtin_bit_cast(double, (__builtin_bit_cast(unsigned long long, a) & m) |
+                                      (__builtin_bit_cast(unsigned long long, b) & ~m));
+}
+
 // getx_apply for any row without branching on n: every row's value is
 // formed with getx_apply's operations in getx_apply's order, then row n's is
 // selected (the rows of a cell sit on neighbouring lanes).
@@ -943,7 +956,7 @@ HD double getx_apply_sel(F G, const double *pv, double d_ry1, double d_rt, const
     const double v4 = (G(GX_C4IB1S1) * r6 + G(GX_C1) * co1 - G(GX_C3IB1) * co3
                        - (co3 - G(GX_S1P5) * res[0] + res[5] * G(GX_SSY1))) * mm * a;
     const double v5 = (G(GX_SS) * (res[5] - res[0] * pv[5]) - G(GX_C4SSIB1) * r6 - G(GX_C1) * co2 + G(GX_C2IB1) * co3) * mm * a;
-    return n == 0 ? v0 : n == 4 ? v4 : n == 5 ? v5 : n == 7 ? v7 : n == 8 ? v8 : v1;
+    return bsel(n == 0, v0, bsel(n == 4, v4, bsel(n == 5, v5, bsel(n == 7, v7, bsel(n == 8, v8, v1)))));
 }
 
 HD void getx(const Par &P, int npv, const CellJ &c, const double f6[6][3], double dt,
