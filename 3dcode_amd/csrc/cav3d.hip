@@ -830,13 +830,33 @@ __device__ __forceinline__ JacNb stage_jnb(const A &st, int b, int lane, int npv
 static_assert(ST_PV + 9 == ST_DC && ST_DC + ND == ST_N3 && ST_N3 + 3 == ST_JC && ST_JC + 2 == ST_JF,
               "the own-cell part of the stage has the neighbour-block layout");
 
-// workgroup barrier that orders LDS only: outstanding global loads stay in
-// flight across it (the stage prefetch lands after the barrier)
+// lane q of the lane's 16-lane row, by a DPP row broadcast (no LDS traffic)
+template <int Q>
+__device__ __forceinline__ double row_lane(double v)
+{
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x150 + Q, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x150 + Q, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+// yv[q] = v of lane q of the row, q < 9 (every lane of the wavefront active)
+__device__ __forceinline__ void bcast_rows(double v, double *yv)
+{
+    static_assert(NPV == 9, "one broadcast per row of the system");
+    yv[0] = row_lane<0>(v); yv[1] = row_lane<1>(v); yv[2] = row_lane<2>(v);
+    yv[3] = row_lane<3>(v); yv[4] = row_lane<4>(v); yv[5] = row_lane<5>(v);
+    yv[6] = row_lane<6>(v); yv[7] = row_lane<7>(v); yv[8] = row_lane<8>(v);
+}
+
+// Workgroup barrier for LDS hand-over: this wavefront's LDS accesses are
+// complete (lgkmcnt), nothing else is waited for, so global loads and the
+// stage LDS-DMA stay in flight across it (a release fence on LDS would also
+// wait for every LDS-DMA).  The asm clobbers keep the compiler from moving
+// memory accesses across it.
 __device__ __forceinline__ void lds_barrier()
 {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
     __builtin_amdgcn_s_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+    asm volatile("" ::: "memory");
 }
 // global address space: a pointer read back from LDS would otherwise be
 // generic, and generic loads also count against the LDS wait counter
@@ -865,12 +885,20 @@ static_assert(TJ == 4 && TK == 4, "the poll wavefront maps an edge of four penci
 template <bool FWD>
 __global__ __launch_bounds__(SWEEP_THREADS) void k_sweep_tile(DS S, const int *tiles, int ntiles, int *err)
 {
-    __shared__ double lds[2][TJ * TK][NPV];      // dq of the hyperplane, by the parity of s
+    // dq hand-over, one LDS object (so that the compiler can tell its reads
+    // from the stage LDS-DMA): [0, HX) the dq of the hyperplane by the parity
+    // of s, [HX, HZ) the dq of the j / k neighbour in another tile by the
+    // parity of s, [HZ, HZ + NPV) zeros for a ghost neighbour
+    constexpr int HX = 2 * TJ * TK * NPV, HZ = HX + 2 * 2 * TJ * TK * NPV;
+    __shared__ double hand[HZ + NPV];
+    auto lds_at = [](int par, int g) { return (par * TJ * TK + g) * NPV; };
+    auto xch_at = [](int par, int a, int g) { return HX + ((par * 2 + a) * TJ * TK + g) * NPV; };
+    auto lds = [&](int par, int g) { return hand + lds_at(par, g); };
+    auto xch = [&](int par, int a, int g) { return hand + xch_at(par, a, g); };
+    double *const zrow = hand + HZ;
     __shared__ double ylds[2][TJ * TK][NPV];     // dcv of the hyperplane (forward sweep)
-    __shared__ double xch[2][2][TJ * TK][NPV];   // [parity of s][j / k] dq of the neighbour in another tile
     __shared__ double win[2][TJ * TK][ST_N][WIN];   // stage windows (see the pencil wavefronts)
     __shared__ const double *src[TJ * TK][ST_N];
-    __shared__ double zrow[NPV];                 // dq of a ghost neighbour
     __shared__ int gave_up;
     const int npv = S.npv;
     const int sg = FWD ? 1 : -1;
@@ -905,7 +933,7 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_sweep_tile(DS S, const int *t
                         const int jj = j0 + g % TJ, kk = k0 + g / TJ;
                         const long w = n * NW + widx(B, s - jj - kk, jj, kk);
                         if (FWD) S.dcv[w] = ylds[pb][g][n];
-                        st_dev(dq + w, lds[pb][g][n]);
+                        st_dev(dq + w, lds(pb, g)[n]);
                     }
                 }
             };
@@ -965,7 +993,7 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_sweep_tile(DS S, const int *t
                     if (q < NEDGE) {
                         const int a = q / (4 * NPV), e = (q / NPV) % 4, n = q % NPV;
                         const int g = a == 0 ? e * TJ + (FWD ? 0 : TJ - 1) : (FWD ? 0 : TK - 1) * TJ + e;
-                        xch[par][a][g][n] = (t == 0 ? n0 : n1) ? (t == 0 ? v0 : v1) : 0.0;
+                        xch(par, a, g)[n] = (t == 0 ? n0 : n1) ? (t == 0 ? v0 : v1) : 0.0;
                     }
                 }
             };
@@ -1034,7 +1062,7 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_sweep_tile(DS S, const int *t
         int pm = (d0 >> 2) + 1, pq = d0 & 3;                          // the quarter in flight
         double nx[8];
         fetchq(pm, pq, nx);
-        if (lane < NPV) lds[(sa - sg) & 1][grp][lane] = 0.0;       // the first step's upstream reads zeros
+        if (lane < NPV) lds((sa - sg) & 1, grp)[lane] = 0.0;       // the first step's upstream reads zeros
         double ci[8], cj[8], ck[8];                                  // coefficients of the coming step
 #pragma unroll
         for (int q = 0; q < 8; q++) ci[q] = 0.0;
@@ -1057,26 +1085,29 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_sweep_tile(DS S, const int *t
             // previous hyperplane, from another tile through the poll wavefront
             const WinRow st{&win[(d >> 2) & 1][grp][0][d & 3]};
             const int pb = (s - sg) & 1;
-            const double *ri = lds[pb][grp];
-            const double *rj = lj ? lds[pb][grp - sg] : (nj ? xch[s & 1][0][grp] : zrow);
-            const double *rk = lk ? lds[pb][grp - sg * TJ] : (nk ? xch[s & 1][1][grp] : zrow);
+            // (indices into the hand-over object, not pointers chosen among
+            // several, so the compiler sees these reads cannot touch the stage)
+            const int oi = lds_at(pb, grp);
+            const int oj = lj ? lds_at(pb, grp - sg) : (nj ? xch_at(s & 1, 0, grp) : HZ);
+            const int ok = lk ? lds_at(pb, grp - sg * TJ) : (nk ? xch_at(s & 1, 1, grp) : HZ);
             double xi[NPV], xj[NPV], xk[NPV];
 #pragma unroll
-            for (int q = 0; q < NPV; q++) { xi[q] = ri[q]; xj[q] = rj[q]; xk[q] = rk[q]; }
+            for (int q = 0; q < NPV; q++) { xi[q] = hand[oi + q]; xj[q] = hand[oj + q]; xk[q] = hand[ok + q]; }
             // dcv = res + the three upstream products, in the reference's i, j, k order
             double y = mine ? st[ST_RES + lane] : 0.0;
-            jac_row_apply(ci, xi, ri[ln], y);
-            jac_row_apply(cj, xj, rj[ln], y);
-            jac_row_apply(ck, xk, rk[ln], y);
+            jac_row_apply(ci, xi, hand[oi + ln], y);
+            jac_row_apply(cj, xj, hand[oj + ln], y);
+            jac_row_apply(ck, xk, hand[ok + ln], y);
             __builtin_amdgcn_sched_barrier(0);
             // dq = D^-1 dcv
             double yv[NPV], pc[NPV];
+            bcast_rows(y, yv);
 #pragma unroll
-            for (int q = 0; q < NPV; q++) { yv[q] = __shfl(y, q, 16); pc[q] = st[ST_PV + q]; }
+            for (int q = 0; q < NPV; q++) pc[q] = st[ST_PV + q];
             const double xn = getx_apply_sel([&](int q) { return st[ST_GX + q]; }, pc, st[ST_DC + D_RY1],
                                              st[ST_DC + D_RT], yv, lane, y, st[ST_PV + ln]);
             if (lane < NPV) {
-                lds[s & 1][grp][lane] = act ? xn : 0.0;
+                lds(s & 1, grp)[lane] = act ? xn : 0.0;
                 if (FWD) ylds[s & 1][grp][lane] = y;
             }
             __builtin_amdgcn_sched_barrier(0);

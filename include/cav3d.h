@@ -100,6 +100,7 @@ typedef struct {
 
 typedef struct cav3d_ctx cav3d_ctx;
 
+/* Blocks need imax, jmax, kmax >= 4 (3 cells per direction). */
 int cav3d_create(const cav3d_params *params, int nblocks, const cav3d_block *blocks, cav3d_ctx **out);
 
 /* cx(3,1:imax,1:jmax+1,1:kmax+1), ex(3,1:imax+1,1:jmax,1:kmax+1),

@@ -142,11 +142,12 @@ def test_dual_time_shift_matches_oracle(pkg, orc, name):
     s.close()
 
 
-@pytest.mark.parametrize("name", ["lam_roe_tvd", "sst_cav_ausm_mlp"])
-def test_timeinteg_many_tiles_matches_oracle(pkg, orc, name):
+@pytest.mark.parametrize("name,nj", [("lam_roe_tvd", 44), ("sst_cav_ausm_mlp", 44), ("lam_roe_tvd", 72)])
+def test_timeinteg_many_tiles_matches_oracle(pkg, orc, name, nj):
     """A cross-section of 43 x 43 cells: 121 sweep tiles in flight at once,
-    most of them waiting on two upstream tiles."""
-    case = pkg.cases.config1(20, 44, 44) if name == "lam_roe_tvd" else pkg.cases.config2(20, 44, 44)
+    most of them waiting on two upstream tiles; 71 x 71 cells: 324 tiles, more
+    than one per workgroup of the sweep's grid on a 256-CU device."""
+    case = pkg.cases.config1(20, nj, nj) if name == "lam_roe_tvd" else pkg.cases.config2(20, nj, nj)
     if name != "lam_roe_tvd":
         case.params["cav_clip"] = 1
     s, o = pair(pkg, orc, case)
