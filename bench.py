@@ -23,6 +23,86 @@ HBM_PEAK_GBS = 8000.0
 B_ALG_FP64_SST = 720.0
 
 
+TRAFFIC_FILE = "profiles/hbm_traffic_bump128x64x64_fp64.txt"
+TRAFFIC_SIZE = "128x64x64"
+
+
+TRAFFIC_ITERS = 5          # iterations in the counter runs: 1 warm-up + 2 timed + 2 profiled
+SETUP_KERNELS = ("k_dv_to_dc", "k_aos_to_soa", "k_cell_from_aos", "k_eos_cells")
+
+
+def measured_traffic(size, kernel=None):
+    """HBM bytes per launch of `kernel`, or per iteration over all kernels when
+    kernel is None (FETCH_SIZE + WRITE_SIZE, each scaled by the copy-kernel
+    calibration), from the committed rocprofv3 summary of the default
+    workload; None for other workloads or without the summary."""
+    path = os.path.join(ROOT, TRAFFIC_FILE)
+    if size != TRAFFIC_SIZE or not os.path.exists(path):
+        return None
+    total = 0.0
+    for line in open(path):
+        f = line.split()
+        if len(f) >= 8 and f[0] == "kernel":
+            b = float(f[5]) + float(f[7])
+            if kernel is not None and f[1] == kernel:
+                return int(b)
+            if f[1] not in SETUP_KERNELS:
+                total += b * int(f[3]) / TRAFFIC_ITERS
+    return None if kernel is not None else int(total)
+
+
+def pmc_summary(args, dfetch, dwrite, dcfetch, dcwrite):
+    """Per-kernel HBM traffic per dispatch from rocprofv3 --pmc databases."""
+    import glob
+    import re
+    import sqlite3
+    from collections import defaultdict
+
+    def rows(folder, counter):
+        db = glob.glob(os.path.join(folder, "**", "*results.db"), recursive=True)[0]
+        c = sqlite3.connect(db)
+        out = defaultdict(list)
+        for name, value in c.execute("select kernel_name, value from counters_collection where counter_name = ?",
+                                     (counter,)):
+            name = re.sub(r"\(.*", "", name.replace("(anonymous namespace)::", "")).replace("void ", "")
+            out[name].append(value)
+        return out
+
+    cf, cw = rows(dcfetch, "FETCH_SIZE"), rows(dcwrite, "WRITE_SIZE")
+    copy_bytes = 256 * (1 << 20)
+    kf = np.mean(cf["__amd_rocclr_copyBuffer"]) * 1024.0
+    kw = np.mean(cw["__amd_rocclr_copyBuffer"]) * 1024.0
+    ff, fw = copy_bytes / kf, copy_bytes / kw
+    bf, bw = rows(dfetch, "FETCH_SIZE"), rows(dwrite, "WRITE_SIZE")
+    cmd = "rocprofv3 --pmc {c} -d <dir> -o run -- python3 bench.py {a}"
+    lines = [
+        "# HBM traffic per dispatch, FP64 config-2 workload (cavitating bump 128x64x64, SST, AUSM+-up, MLP, Merkle)",
+        "# from four rocprofv3 counter runs on one MI355X (run from /tmp with TMPDIR=/tmp, R = repository root):",
+        "#   " + cmd.format(c="FETCH_SIZE", a="--steps 2 --warmup 1 --cpu-baseline 0"),
+        "#   " + cmd.format(c="WRITE_SIZE", a="--steps 2 --warmup 1 --cpu-baseline 0"),
+        "#   " + cmd.format(c="FETCH_SIZE", a="--copy-probe 256"),
+        "#   " + cmd.format(c="WRITE_SIZE", a="--copy-probe 256"),
+        "# summarised by: python3 bench.py --pmc-summary <fetch dir> <write dir> <copy fetch dir> <copy write dir>",
+        "# the bench runs make %d iterations: per-iteration traffic = sum over kernels of bytes x dispatches / %d"
+        % (TRAFFIC_ITERS, TRAFFIC_ITERS),
+        "# calibration: a plain copy of %d B reads FETCH_SIZE %.1f KiB and WRITE_SIZE %.1f KiB per dispatch;"
+        % (copy_bytes, kf / 1024, kw / 1024),
+        "# each counter below is scaled by (bytes copied) / (counter bytes): fetch x %.4f, write x %.4f" % (ff, fw),
+        "# kernel <name> dispatches <n> fetch_bytes <B> write_bytes <B>  (mean per dispatch, scaled)",
+    ]
+    names = sorted(set(bf) | set(bw), key=lambda k: -(np.mean(bf.get(k, [0])) * ff + np.mean(bw.get(k, [0])) * fw))
+    for k in names:
+        if k.startswith("__amd") or k.startswith("elementwise"):
+            continue
+        vf = np.mean(bf.get(k, [0.0])) * 1024.0 * ff
+        vw = np.mean(bw.get(k, [0.0])) * 1024.0 * fw
+        lines.append("kernel %s dispatches %d fetch_bytes %.0f write_bytes %.0f"
+                     % (k, max(len(bf.get(k, [])), len(bw.get(k, []))), vf, vw))
+    out = os.path.join(ROOT, TRAFFIC_FILE)
+    open(out, "w").write("\n".join(lines) + "\n")
+    print("\n".join(lines))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -30,7 +110,27 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--size", default="128x64x64")
     ap.add_argument("--cpu-baseline", type=int, default=1, help="1: time the oracle on the host")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="run only a plain device-to-device copy of MIB MiB (5 times): the calibration "
+                         "kernel for the FETCH_SIZE / WRITE_SIZE counters")
+    ap.add_argument("--pmc-summary", nargs=4, metavar=("FETCH", "WRITE", "COPY_FETCH", "COPY_WRITE"),
+                    help="write the HBM-traffic summary from four rocprofv3 --pmc output folders "
+                         "(bench FETCH_SIZE, bench WRITE_SIZE, copy probe FETCH_SIZE, copy probe WRITE_SIZE)")
     args = ap.parse_args()
+    if args.pmc_summary:
+        pmc_summary(args, *args.pmc_summary)
+        return
+    if args.copy_probe:
+        import torch
+        n = args.copy_probe * (1 << 20) // 8
+        a = torch.arange(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_mib": args.copy_probe, "bytes_each_way": n * 8}), flush=True)
+        return
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the library keeps every block of a case on a single device "
               "and has no multi-GPU path", file=sys.stderr)
@@ -81,12 +181,18 @@ def main():
     t_fwd = phase[6] * 1e-3
     achieved = b_fwd * ncell / t_fwd / 1e9
     it_achieved = B_ALG_FP64_SST * ncell / (phase[5] * 1e-3) / 1e9
+    traffic = measured_traffic(args.size, "k_sweep_tile<true>")
+    it_traffic = measured_traffic(args.size)
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                "traffic_source": TRAFFIC_FILE if traffic is not None else None,
                 "kernel": "k_sweep_tile<forward> (LU-SGS forward sweep, one launch over all cells)",
                 "bytes_per_cell": b_fwd, "launch_ms": round(phase[6], 4),
                 "iteration": {"bytes_per_cell": B_ALG_FP64_SST, "achieved": round(it_achieved, 2),
-                              "frac": round(it_achieved / HBM_PEAK_GBS, 5)},
+                              "frac": round(it_achieved / HBM_PEAK_GBS, 5),
+                              "traffic": it_traffic,
+                              "measured_gbs": (round(it_traffic / (phase[5] * 1e-3) / 1e9, 1)
+                                               if it_traffic is not None else None)},
                 "phase_ms": {k: round(v, 4) for k, v in zip(
                     ["timestep", "boundary", "rhs", "lusgs", "update_eddy", "total", "sweep_fwd", "sweep_bwd"], phase)}}
 
