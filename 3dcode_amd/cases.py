@@ -451,6 +451,15 @@ def connection(ipnts, ipnts_donor, transvec, donor):
             isd, ied = list(ipnts_donor[:3]), list(ipnts_donor[3:])
             isd[m] += 1
             ied[m] += 1
+    # calnormal's closing re-ordering moves the tangential ranges from vertex
+    # to cell indices (src/grid_module.f90:924-938)
+    for m in range(3):
+        if ist[m] != ien[m]:
+            ist[m] += 1
+        if isd[m] < ied[m]:
+            isd[m] += 1
+        elif isd[m] > ied[m]:
+            ied[m] += 1
     for m in range(3):                          # the face direction of this block
         if ist[m] == ien[m]:
             # the donor index that this block's normal direction maps onto
@@ -472,11 +481,30 @@ def connection(ipnts, ipnts_donor, transvec, donor):
     return [donor, *ist, *ien, *isd, *ied, *T.reshape(-1).tolist()]
 
 
+def exchange_i_metrics(L, R, ngrd):
+    """The metric exchange across an i-cut with aligned indices, L's imax face
+    joined to R's imin face: each block's ghost layer takes the other's first
+    interior layer, as calnormal (tangential face vectors ex, tx over vertex
+    ranges j = 1..J, k = 1..K; src/grid_module.f90:734-840), calvolume (grd(1)
+    over cells 2..J, 2..K; :1127-1190) and calydns (grd(5); :509-560) do."""
+    IL, IR, J, K = L.imax, R.imax, L.jmax, L.kmax
+    exL, exR = L.ex.reshape(K + 1, J, IL + 1, 3), R.ex.reshape(K + 1, J, IR + 1, 3)
+    txL, txR = L.tx.reshape(K, J + 1, IL + 1, 3), R.tx.reshape(K, J + 1, IR + 1, 3)
+    gL, gR = L.grd.reshape(K + 1, J + 1, IL + 1, ngrd), R.grd.reshape(K + 1, J + 1, IR + 1, ngrd)
+    fromL = (exL[:K, :J, IL - 1].copy(), txL[:K, :J, IL - 1].copy(), gL[1:K, 1:J, IL - 1].copy())
+    fromR = (exR[:K, :J, 1].copy(), txR[:K, :J, 1].copy(), gR[1:K, 1:J, 1].copy())
+    exR[:K, :J, 0], txR[:K, :J, 0] = fromL[0], fromL[1]
+    exL[:K, :J, IL], txL[:K, :J, IL] = fromR[0], fromR[1]
+    qs = [0, 4] if ngrd > 4 else [0]
+    gR[1:K, 1:J, 0, qs] = fromL[2][..., qs]
+    gL[1:K, 1:J, IL, qs] = fromR[2][..., qs]
+
+
 def split_i(case, cuts):
     """The same case cut into blocks along i at the given vertex indices
     (1-based), with interfaces between consecutive blocks.  Metrics are
-    recomputed per block (wall distance over all walls), the state is the
-    single block's, cut."""
+    computed per block (wall distance over all walls) and exchanged across
+    the cuts; the state is the single block's, cut."""
     (b0,) = case.blocks
     I, J, K = b0.imax, b0.jmax, b0.kmax
     bounds = [1] + list(cuts) + [I]
@@ -507,6 +535,8 @@ def split_i(case, cuts):
         if q < n - 1:  # imax face joins the next block's imin face
             nb.conn_desc.append(connection([Ib, 1, 1, Ib, J, K], [1, 1, 1, 1, J, K], [1, 2, 3], q + 1))
         finish_block(nb, case.ngrd, walls)
+        if q > 0:
+            exchange_i_metrics(blocks[-1], nb, case.ngrd)
         blocks.append(nb)
         sl = slice(lo - 1, hi + 5)          # ghosted i range of the sub-block: lo-2 .. hi+3 (Fortran)
         pv, dv, tv, qq = case.init[0]

@@ -1355,6 +1355,15 @@ struct cav3d_ctx {
     hipEvent_t ev[10] = {};
     double prof_ms[8] = {};
     std::vector<void *> allocs;
+    // grid from vertex coordinates (cav3d_set_grid / cav3d_build_metrics)
+    std::vector<std::vector<int>> condesc;   // per block: ncon*22 raw
+    double *xv = nullptr;     // vertices, SoA [3][NV], block b from v0[b]
+    long NV = 0;
+    std::vector<long> v0;
+    std::vector<char> has_x;
+    double *xc = nullptr;     // cell centroids grd(2:4), SoA [3][NW]
+    double *xbuf = nullptr;   // interface-exchange staging
+    long xbuf_n = 0;
 };
 
 namespace {
@@ -1686,8 +1695,11 @@ int cav3d_create(const cav3d_params *prm, int nblocks, const cav3d_block *blocks
     for (int d = 0; d < 3; d++) S.NF[d] = f0[d];
     // interfaces
     long hs = 0;
+    c->condesc.resize(nblocks);
     for (int b = 0; b < nblocks; b++) {
         const cav3d_block &cb = blocks[b];
+        if (cb.ncon < 0 || (cb.ncon > 0 && !cb.con)) { delete c; return fail("bad interface list", -1); }
+        c->condesc[b].assign(cb.con, cb.con + 22 * cb.ncon);
         for (int n = 0; n < cb.ncon; n++) {
             const int *d = cb.con + 22 * n;
             Con cn;
@@ -1729,7 +1741,7 @@ int cav3d_create(const cav3d_params *prm, int nblocks, const cav3d_block *blocks
     if (rc) { cav3d_destroy(c); return rc; }
     const int npv = c->npv;
     if ((rc = dalloc(c, &S.pv, npv * S.NG)) || (rc = dalloc(c, &S.dc, (long)ND * S.NG)) ||
-        (rc = dalloc(c, &S.vol, S.NW)) || (rc = dalloc(c, &S.cx, 3 * S.NCX)) || (rc = dalloc(c, &S.ex, 3 * S.NEX)) ||
+        (rc = dalloc(c, &S.vol, S.NW)) || (rc = dalloc(c, &c->xc, 3 * S.NW)) || (rc = dalloc(c, &S.cx, 3 * S.NCX)) || (rc = dalloc(c, &S.ex, 3 * S.NEX)) ||
         (rc = dalloc(c, &S.tx, 3 * S.NTX)) || (rc = dalloc(c, &S.res, npv * S.NC)) || (rc = dalloc(c, &S.dt, S.NC)) ||
         (rc = dalloc(c, &S.dqf, npv * S.NW)) || (rc = dalloc(c, &S.dqb, npv * S.NW)) || (rc = dalloc(c, &S.dcv, npv * S.NW)) ||
         (rc = dalloc(c, &S.scal, 32))) { cav3d_destroy(c); return rc; }
@@ -1807,6 +1819,7 @@ void cav3d_destroy(cav3d_ctx *c)
     if (c->st) hipStreamSynchronize(c->st);
     for (void *p : c->allocs) hipFree(p);
     if (c->stage) hipFree(c->stage);
+    if (c->xbuf) hipFree(c->xbuf);
     for (int q = 0; q < 10; q++) if (c->ev[q]) hipEventDestroy(c->ev[q]);
     if (c->st) hipStreamDestroy(c->st);
     delete c;
@@ -1829,6 +1842,7 @@ int cav3d_set_metrics(cav3d_ctx *c, int b, const double *cx, const double *ex, c
     }
     HIPCHK(hipMemcpyAsync(c->stage, grd, (long)c->ngrd * B.nw * sizeof(double), hipMemcpyHostToDevice, c->st));
     hipLaunchKernelGGL(k_aos_to_soa, dim3(nblk(B.nw, 256)), dim3(256), 0, c->st, (const double *)c->stage, 1, c->ngrd, 0, S.vol, S.NW, B.w0, B.nw);
+    hipLaunchKernelGGL(k_aos_to_soa, dim3(nblk(B.nw, 256)), dim3(256), 0, c->st, (const double *)c->stage, 3, c->ngrd, 1, c->xc, S.NW, B.w0, B.nw);
     if (S.ydn)
         hipLaunchKernelGGL(k_aos_to_soa, dim3(nblk(B.nw, 256)), dim3(256), 0, c->st, (const double *)c->stage, 1, c->ngrd, 4, S.ydn, S.NW, B.w0, B.nw);
     HIPCHK(hipStreamSynchronize(c->st));
@@ -2079,3 +2093,5 @@ int cav3d_synchronize(cav3d_ctx *c)
 }
 
 }  // extern "C"
+
+#include "metrics.hpp"

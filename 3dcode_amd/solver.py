@@ -27,7 +27,8 @@ BC = {
 }
 
 EXPORTS = [
-    "cav3d_create", "cav3d_set_metrics", "cav3d_put_state", "cav3d_get_state", "cav3d_timeinteg",
+    "cav3d_create", "cav3d_set_metrics", "cav3d_set_grid", "cav3d_build_metrics", "cav3d_get_metrics",
+    "cav3d_put_state", "cav3d_get_state", "cav3d_timeinteg",
     "cav3d_setbc", "cav3d_calrhs", "cav3d_timestep", "cav3d_residual", "cav3d_unsteady_shift",
     "cav3d_eos", "cav3d_eos_host", "cav3d_profile", "cav3d_synchronize", "cav3d_last_error",
     "cav3d_destroy",
@@ -66,6 +67,9 @@ def lib(build_if_missing: bool = True):
     L.cav3d_last_error.restype = C.c_char_p
     L.cav3d_create.argtypes = [C.POINTER(Params), C.c_int, C.POINTER(BlockDesc), C.POINTER(C.c_void_p)]
     L.cav3d_set_metrics.argtypes = [C.c_void_p, C.c_int, _D, _D, _D, _D]
+    L.cav3d_set_grid.argtypes = [C.c_void_p, C.c_int, _D]
+    L.cav3d_build_metrics.argtypes = [C.c_void_p]
+    L.cav3d_get_metrics.argtypes = [C.c_void_p, C.c_int, _D, _D, _D, _D]
     L.cav3d_put_state.argtypes = [C.c_void_p, C.c_int, _D, _D, _D, _D]
     L.cav3d_get_state.argtypes = [C.c_void_p, C.c_int, _D, _D, _D, _D]
     L.cav3d_timeinteg.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, _D]
@@ -123,7 +127,12 @@ def params_struct(case) -> Params:
 class Solver:
     """One GPU context holding every block of a case."""
 
-    def __init__(self, case, precision: int = 0):
+    def __init__(self, case, precision: int = 0, metrics: str = "host"):
+        """metrics "host": the case's metric arrays are copied in
+        (cav3d_set_metrics); "device": they are computed on the GPU from the
+        case's vertex coordinates (cav3d_set_grid + cav3d_build_metrics)."""
+        if metrics not in ("host", "device"):
+            raise ValueError("metrics must be 'host' or 'device'")
         L = lib()
         self.case = case
         self.npv, self.ntv = case.npv, case.ntv
@@ -141,8 +150,25 @@ class Solver:
         h = C.c_void_p()
         _check(L.cav3d_create(C.byref(P), len(case.blocks), descs, C.byref(h)))
         self.h = h
-        for ib, b in enumerate(case.blocks):
-            _check(L.cav3d_set_metrics(self.h, ib, _p(b.cx), _p(b.ex), _p(b.tx), _p(b.grd)))
+        if metrics == "host":
+            for ib, b in enumerate(case.blocks):
+                _check(L.cav3d_set_metrics(self.h, ib, _p(b.cx), _p(b.ex), _p(b.tx), _p(b.grd)))
+        else:
+            for ib, b in enumerate(case.blocks):
+                x = np.ascontiguousarray(np.asarray(b.X, dtype=np.float64).transpose(2, 1, 0, 3)).ravel()
+                _check(L.cav3d_set_grid(self.h, ib, _p(x)))
+            _check(L.cav3d_build_metrics(self.h))
+
+    def get_metrics(self, ib):
+        """cx, ex, tx, grd of block ib as the context holds them (Fortran layouts, flat)."""
+        b = self.case.blocks[ib]
+        I, J, K = b.imax, b.jmax, b.kmax
+        cx = np.zeros(3 * I * (J + 1) * (K + 1))
+        ex = np.zeros(3 * (I + 1) * J * (K + 1))
+        tx = np.zeros(3 * (I + 1) * (J + 1) * K)
+        grd = np.zeros(self.case.ngrd * (I + 1) * (J + 1) * (K + 1))
+        _check(lib().cav3d_get_metrics(self.h, ib, _p(cx), _p(ex), _p(tx), _p(grd)))
+        return cx, ex, tx, grd
 
     def close(self):
         if getattr(self, "h", None):

@@ -103,6 +103,26 @@ def pmc_summary(args, dfetch, dwrite, dcfetch, dcwrite):
     print("\n".join(lines))
 
 
+def grid_metrics_ms(pk, case, n):
+    """Best wall time of building the case's metrics on the device from its
+    vertex coordinates (context creation excluded)."""
+    best = None
+    for _ in range(n):
+        s = pk.Solver(case)
+        s.synchronize()
+        x = [np.ascontiguousarray(np.asarray(b.X, dtype=np.float64).transpose(2, 1, 0, 3)).ravel() for b in case.blocks]
+        L = pk.solver.lib()
+        t = time.perf_counter()
+        for ib, xb in enumerate(x):
+            pk.solver._check(L.cav3d_set_grid(s.h, ib, pk.solver._p(xb)))
+        pk.solver._check(L.cav3d_build_metrics(s.h))
+        s.synchronize()
+        ms = (time.perf_counter() - t) * 1e3
+        best = ms if best is None else min(best, ms)
+        s.close()
+    return round(best, 3)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -113,6 +133,8 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="run only a plain device-to-device copy of MIB MiB (5 times): the calibration "
                          "kernel for the FETCH_SIZE / WRITE_SIZE counters")
+    ap.add_argument("--grid-metrics", type=int, default=0, metavar="N",
+                    help="run only the device grid-metric build (face vectors, volumes, wall distance) N times")
     ap.add_argument("--pmc-summary", nargs=4, metavar=("FETCH", "WRITE", "COPY_FETCH", "COPY_WRITE"),
                     help="write the HBM-traffic summary from four rocprofv3 --pmc output folders "
                          "(bench FETCH_SIZE, bench WRITE_SIZE, copy probe FETCH_SIZE, copy probe WRITE_SIZE)")
@@ -130,6 +152,14 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_mib": args.copy_probe, "bytes_each_way": n * 8}), flush=True)
+        return
+    if args.grid_metrics:
+        import importlib
+        pk = importlib.import_module("3dcode_amd")
+        ni, nj, nk = (int(x) for x in args.size.split("x"))
+        case = pk.cases.config2(ni, nj, nk)
+        ms = grid_metrics_ms(pk, case, args.grid_metrics)
+        print(json.dumps({"grid_metrics_ms": ms, "cells": case.ncell, "repeats": args.grid_metrics}), flush=True)
         return
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the library keeps every block of a case on a single device "
@@ -196,6 +226,8 @@ def main():
                 "phase_ms": {k: round(v, 4) for k, v in zip(
                     ["timestep", "boundary", "rhs", "lusgs", "update_eddy", "total", "sweep_fwd", "sweep_bwd"], phase)}}
 
+    grid_ms = grid_metrics_ms(pk, case, 3)
+
     cpu = None
     if args.cpu_baseline:
         from oracle import oracle as orc
@@ -218,6 +250,7 @@ def main():
         "config": {"workload": f"cavitating bump channel {args.size} cells, k-omega SST, AUSM+-up, MLP, Merkle "
                                f"(species clip), 1 block", "cells": ncell},
         "roofline": roofline, "cpu_baseline": cpu,
+        "grid_metrics_ms": grid_ms,
     }
     print(json.dumps(out), flush=True)
 

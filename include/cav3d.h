@@ -24,6 +24,9 @@
  *   cav3d_put_state /    t_variable getpv/getdv/gettv/getqq + setters
  *   cav3d_get_state      src/variable_module.f90:16-21, 69-77
  *   cav3d_set_metrics    t_grid getcx/getex/gettx/getgrd src/grid_module.f90:37-39
+ *   cav3d_set_grid /     t_grid%construct: calnormal, calvolume, calydns
+ *   cav3d_build_metrics  src/grid_module.f90:75-332, 350-1145
+ *   cav3d_get_metrics    t_grid getcx/getex/gettx/getgrd
  *   cav3d_create         t_update%construct + t_bc%construct (BC faces, 12
  *                        edges, 8 corners, interfaces)
  *                        src/update_module.f90:77-178, src/bc_module.f90:71-1188
@@ -108,6 +111,19 @@ int cav3d_create(const cav3d_params *params, int nblocks, const cav3d_block *blo
  * ngrd = 4 (laminar/Euler) or 5 (turbulent). */
 int cav3d_set_metrics(cav3d_ctx *ctx, int b, const double *cx, const double *ex,
                       const double *tx, const double *grd);
+
+/* Grid metrics computed on the device from the vertex coordinates, in place
+ * of cav3d_set_metrics.  x(3,imax,jmax,kmax): the CGNS vertices of block b
+ * (t_grid%x, src/grid_module.f90:118-130).  cav3d_build_metrics, once every
+ * block has its coordinates, runs t_grid's calnormal, calvolume and (for
+ * iturb = 0) calydns over all blocks (src/grid_module.f90:328-330, 350-1145):
+ * face vectors, six-pyramid volumes, centroids, the wall distance to the
+ * nearest BCWall face centroid of any block, the ghost fills and the
+ * interface exchanges.  cav3d_get_metrics returns what the context holds, in
+ * the cav3d_set_metrics layouts; any output may be NULL. */
+int cav3d_set_grid(cav3d_ctx *ctx, int b, const double *x);
+int cav3d_build_metrics(cav3d_ctx *ctx);
+int cav3d_get_metrics(cav3d_ctx *ctx, int b, double *cx, double *ex, double *tx, double *grd);
 
 /* pv(npv,-1:imax+3,-1:jmax+3,-1:kmax+3), dv(18,...), tv(ntv,...),
  * qq(npv,2,2:imax,2:jmax,2:kmax).  dv/tv may be NULL: then dv and tv(1:2)

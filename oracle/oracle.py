@@ -48,6 +48,11 @@ _D = C.POINTER(C.c_double)
 _I = C.POINTER(C.c_int)
 
 
+class GBlock(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("imax", "jmax", "kmax", "nbc", "ncon")] + \
+        [("x", _D), ("bc", _I), ("con", _I), ("cx", _D), ("ex", _D), ("tx", _D), ("grd", _D)]
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -84,6 +89,7 @@ def lib():
         L.orc_residual.argtypes = [C.c_void_p, _D, _I, _D]
         L.orc_unsteady_shift.argtypes = [C.c_void_p]
         L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_grid_metrics.argtypes = [C.c_int, C.POINTER(GBlock), C.c_int]
         L.orc_alt_snd2_count_ptr = C.c_int.in_dll(L, "orc_alt_snd2_count")
         _lib = L
     return _lib
@@ -140,6 +146,28 @@ def deteos_prop(p, t, y1, y2):
     tv = np.zeros(3)
     lib().orc_deteos_prop(p, t, y1, y2, _p(dv), _p(tv))
     return dv, tv[:2]
+
+
+def grid_metrics(blocks, ngrd):
+    """cx, ex, tx, grd of every block (flat, Fortran layouts) from the vertex
+    arrays X (imax, jmax, kmax, 3), bc_desc and conn_desc of `blocks`."""
+    keep, out = [], []
+    arr = (GBlock * len(blocks))()
+    for q, b in enumerate(blocks):
+        I, J, K = b.imax, b.jmax, b.kmax
+        x = np.ascontiguousarray(np.asarray(b.X, dtype=np.float64).transpose(2, 1, 0, 3)).ravel()
+        bc = np.ascontiguousarray(np.asarray(b.bc_desc, dtype=np.int32).ravel()) if len(b.bc_desc) else np.zeros(7, np.int32)
+        con = np.ascontiguousarray(np.asarray(b.conn_desc, dtype=np.int32).ravel()) if len(b.conn_desc) else np.zeros(22, np.int32)
+        o = (np.zeros(3 * I * (J + 1) * (K + 1)), np.zeros(3 * (I + 1) * J * (K + 1)),
+             np.zeros(3 * (I + 1) * (J + 1) * K), np.zeros(ngrd * (I + 1) * (J + 1) * (K + 1)))
+        keep += [x, bc, con]
+        out.append(o)
+        arr[q] = GBlock(I, J, K, len(b.bc_desc), len(b.conn_desc), _p(x), _p(bc, C.c_int), _p(con, C.c_int),
+                        *(_p(a) for a in o))
+    rc = lib().orc_grid_metrics(len(blocks), arr, ngrd)
+    if rc != 0:
+        raise RuntimeError(f"orc_grid_metrics failed ({rc})")
+    return out
 
 
 def set_threads(n: int):

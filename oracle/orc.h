@@ -76,6 +76,20 @@ int orc_residual(orc_ctx *c, double *l2, int *ijkmax, double *resmax);
 int orc_unsteady_shift(orc_ctx *c);
 void orc_set_threads(int n);
 
+/* One block for orc_grid_metrics: vertex coordinates in, metrics out
+ * (Fortran layouts of t_grid: x(3,2:imax+1,2:jmax+1,2:kmax+1) holding the
+ * CGNS vertices, cx(3,1:imax,1:jmax+1,1:kmax+1), ex(3,1:imax+1,1:jmax,1:kmax+1),
+ * tx(3,1:imax+1,1:jmax+1,1:kmax), grd(ngrd,1:imax+1,1:jmax+1,1:kmax+1)). */
+typedef struct {
+    int imax, jmax, kmax;
+    int nbc, ncon;
+    const double *x;
+    const int *bc;     /* nbc x 7 {kind, istart[3], iend[3]} */
+    const int *con;    /* ncon x 22, as orc_set_conns takes them */
+    double *cx, *ex, *tx, *grd;
+} orc_gblock;
+int orc_grid_metrics(int nb, orc_gblock *blk, int ngrd);
+
 #ifdef __cplusplus
 }
 #endif

@@ -14,10 +14,11 @@ def test_connection_descriptor_matches_reference_rules(pkg):
     c = pkg.cases.config1(10, 6, 5)
     s = pkg.cases.split_i(c, [6])
     (d0,), (d1,) = s.blocks[0].conn_desc, s.blocks[1].conn_desc
-    # block 0 sends its last three cells i = 4..6 to block 1's ghosts -1..1
-    assert d0[:13] == [1, 4, 1, 1, 6, 7, 6, -1, 1, 1, 1, 7, 6]
+    # block 0 sends its last three cells i = 4..6 (cells j = 2..7, k = 2..6)
+    # to block 1's ghosts -1..1
+    assert d0[:13] == [1, 4, 2, 2, 6, 7, 6, -1, 2, 2, 1, 7, 6]
     # block 1 sends i = 2..4 to block 0's ghosts 7..9
-    assert d1[:13] == [0, 2, 1, 1, 4, 7, 6, 7, 1, 1, 9, 7, 6]
+    assert d1[:13] == [0, 2, 2, 2, 4, 7, 6, 7, 2, 2, 9, 7, 6]
     assert d0[13:] == d1[13:] == [1, 0, 0, 0, 1, 0, 0, 0, 1]
 
 
@@ -59,5 +60,30 @@ def test_inviscid_residual_does_not_see_the_cut(pkg, orc):
     parts = [o2.calrhs(ib)[0].reshape(b.kmax - 1, b.jmax - 1, b.imax - 1, 7) for ib, b in enumerate(s.blocks)]
     r2 = np.concatenate(parts, axis=2)
     assert r2.shape == r1.shape
+    inner = (slice(1, -1), slice(1, -1), slice(None))
+    assert np.array_equal(r2[inner], r1[inner])
+
+
+@pytest.mark.parametrize("which", ["config1", "config2"])
+def test_viscous_residual_does_not_see_the_cut(pkg, orc, which):
+    """With the interface exchange of the metrics (ghost volumes, wall
+    distances and tangential face vectors taken from the neighbour block), the
+    laminar and the SST + Merkle residuals of the cut channel equal the uncut
+    block's bit for bit away from the j/k boundary layers, where the
+    neighbour's edge ghosts come from the edge fill instead."""
+    c = getattr(pkg.cases, which)(14, 7, 5)
+    c.params["cav_clip"] = 1
+    s = pkg.cases.split_i(c, [6])
+    b, npv = c.blocks[0], c.npv
+    o1 = orc.Oracle(c)
+    o1.put_state(0, *c.init[0])
+    o1.setbc()
+    r1 = o1.calrhs(0)[0].reshape(b.kmax - 1, b.jmax - 1, b.imax - 1, npv)
+    o2 = orc.Oracle(s)
+    for ib, st in enumerate(s.init):
+        o2.put_state(ib, *st)
+    o2.setbc()
+    r2 = np.concatenate([o2.calrhs(ib)[0].reshape(q.kmax - 1, q.jmax - 1, q.imax - 1, npv)
+                         for ib, q in enumerate(s.blocks)], axis=2)
     inner = (slice(1, -1), slice(1, -1), slice(None))
     assert np.array_equal(r2[inner], r1[inner])
