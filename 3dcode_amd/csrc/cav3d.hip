@@ -1404,7 +1404,7 @@ int build_bcs(cav3d_ctx *c)
             Seg s;
             memset(&s, 0, sizeof(s));
             s.b = b;
-            int kind = dsc[7 * n];
+            int kind = dsc[7 * n] & CAV3D_BC_KIND_MASK;
             for (int d = 0; d < 3; d++) { s.is[d] = dsc[7 * n + 1 + d]; s.ie[d] = dsc[7 * n + 4 + d]; }
             if (s.is[0] == s.ie[0]) {
                 set3(s.dir, -1, 1, 1);
@@ -2095,3 +2095,4 @@ int cav3d_synchronize(cav3d_ctx *c)
 }  // extern "C"
 
 #include "metrics.hpp"
+#include "post.hpp"

@@ -423,7 +423,7 @@ std::vector<WSeg> wall_segments(const cav3d_ctx *c, int b, long &start, bool &ok
     const int dim[3] = {B.I, B.J, B.K};
     const std::vector<int> &dsc = c->bcdesc[b];
     for (size_t n = 0; n + 7 <= dsc.size(); n += 7) {
-        if (dsc[n] != CAV3D_BC_WALL) continue;
+        if ((dsc[n] & CAV3D_BC_KIND_MASK) != CAV3D_BC_WALL) continue;
         const int *s = &dsc[n + 1], *e = &dsc[n + 4];
         for (int ax = 0; ax < 3; ax++) {
             if (s[ax] != e[ax]) continue;
@@ -492,7 +492,7 @@ int cav3d_build_metrics(cav3d_ctx *c)
         hipLaunchKernelGGL(k_face_vectors<2>, dim3(nblk((long)(I - 1) * (J - 1) * K, 256)), dim3(256), 0, c->st, X, tx);
         const std::vector<int> &dsc = c->bcdesc[b];
         for (size_t n = 0; n + 7 <= dsc.size(); n += 7) {
-            if (dsc[n] != CAV3D_BC_DEGENERATE) continue;
+            if ((dsc[n] & CAV3D_BC_KIND_MASK) != CAV3D_BC_DEGENERATE) continue;
             const int *s = &dsc[n + 1], *e = &dsc[n + 4];
             if (s[0] == e[0]) plane(c, cx, 0, s[0] == 1 ? 1 : I, 0, 0, 0, 2, 1, J + 1, 1, K + 1);
             if (s[1] == e[1]) plane(c, ex, 1, s[1] == 1 ? 1 : J, 0, 0, 0, 2, 1, I + 1, 1, K + 1);

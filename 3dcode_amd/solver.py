@@ -26,8 +26,11 @@ BC = {
     "inflow_supersonic": 6, "outflow_supersonic": 7, "degenerate": 8, "inflow": 9, "outflow": 10,
 }
 
+SURFACE = 0x100   # CAV3D_BC_SURFACE: OR'ed into a kind, marks the force-integration surface
+
 EXPORTS = [
     "cav3d_create", "cav3d_set_metrics", "cav3d_set_grid", "cav3d_build_metrics", "cav3d_get_metrics",
+    "cav3d_integrals", "cav3d_export", "cav3d_import",
     "cav3d_put_state", "cav3d_get_state", "cav3d_timeinteg",
     "cav3d_setbc", "cav3d_calrhs", "cav3d_timestep", "cav3d_residual", "cav3d_unsteady_shift",
     "cav3d_eos", "cav3d_eos_host", "cav3d_profile", "cav3d_synchronize", "cav3d_last_error",
@@ -70,6 +73,9 @@ def lib(build_if_missing: bool = True):
     L.cav3d_set_grid.argtypes = [C.c_void_p, C.c_int, _D]
     L.cav3d_build_metrics.argtypes = [C.c_void_p]
     L.cav3d_get_metrics.argtypes = [C.c_void_p, C.c_int, _D, _D, _D, _D]
+    L.cav3d_integrals.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_double, _D]
+    L.cav3d_export.argtypes = [C.c_void_p, C.c_char_p, C.c_int, C.c_int, C.c_double]
+    L.cav3d_import.argtypes = [C.c_void_p, C.c_char_p, _I, _I, _D]
     L.cav3d_put_state.argtypes = [C.c_void_p, C.c_int, _D, _D, _D, _D]
     L.cav3d_get_state.argtypes = [C.c_void_p, C.c_int, _D, _D, _D, _D]
     L.cav3d_timeinteg.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, _D]
@@ -245,6 +251,28 @@ class Solver:
         tv = np.empty((n, 2))
         _check(lib().cav3d_eos(self.h, n, _p(p), _p(t), _p(y1), _p(y2), _p(dv), _p(tv)))
         return dv, tv
+
+    def set_grid(self):
+        """Hand the case's vertex coordinates to the context (surface forces
+        need them; cav3d_build_metrics is not run)."""
+        for ib, b in enumerate(self.case.blocks):
+            x = np.ascontiguousarray(np.asarray(b.X, dtype=np.float64).transpose(2, 1, 0, 3)).ravel()
+            _check(lib().cav3d_set_grid(self.h, ib, _p(x)))
+
+    def integrals(self, rhoref, uref, area, aoa=0.0):
+        """vapour volume, gas volume, pressure force (3), viscous force (3), cl, cd."""
+        out = np.zeros(10)
+        _check(lib().cav3d_integrals(self.h, rhoref, uref, area, aoa, _p(out)))
+        return out
+
+    def export(self, path, nt_phy, nt, time):
+        _check(lib().cav3d_export(self.h, os.fsencode(path), nt_phy, nt, time))
+
+    def import_(self, path):
+        """Read a restart file; returns (nps, nts, timeprev) as the reader advances them."""
+        nps, nts, tp = C.c_int(0), C.c_int(0), C.c_double(0.0)
+        _check(lib().cav3d_import(self.h, os.fsencode(path), C.byref(nps), C.byref(nts), C.byref(tp)))
+        return nps.value, nts.value, tp.value
 
     def profile(self, enable=True):
         out = np.zeros(8)

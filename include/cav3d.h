@@ -24,6 +24,10 @@
  *   cav3d_put_state /    t_variable getpv/getdv/gettv/getqq + setters
  *   cav3d_get_state      src/variable_module.f90:16-21, 69-77
  *   cav3d_set_metrics    t_grid getcx/getex/gettx/getgrd src/grid_module.f90:37-39
+ *   cav3d_integrals      t_post vof fields + clcd_writing
+ *                        src/postvariable_module.f90:282-295, 507-633
+ *   cav3d_export /       t_variable%export_variable src/variable_module.f90:108-182,
+ *   cav3d_import         t_ini_restart%restart src/initial_module.f90:245-389
  *   cav3d_set_grid /     t_grid%construct: calnormal, calvolume, calydns
  *   cav3d_build_metrics  src/grid_module.f90:75-332, 350-1145
  *   cav3d_get_metrics    t_grid getcx/getex/gettx/getgrd
@@ -55,6 +59,10 @@ enum {
     CAV3D_BC_INFLOW = 9,            /* 'BCInflow'  (no-op, as in the reference) */
     CAV3D_BC_OUTFLOW = 10           /* 'BCOutflow' (no-op, as in the reference) */
 };
+/* A boundary range of the reference's 'Surface' family (the force-integration
+ * surface of src/postvariable_module.f90:541) carries this flag OR'ed into its
+ * kind; the kind itself is kind & CAV3D_BC_KIND_MASK. */
+enum { CAV3D_BC_KIND_MASK = 0xff, CAV3D_BC_SURFACE = 0x100 };
 
 enum { CAV3D_FP64 = 0, CAV3D_MIXED = 1 };
 
@@ -124,6 +132,26 @@ int cav3d_set_metrics(cav3d_ctx *ctx, int b, const double *cx, const double *ex,
 int cav3d_set_grid(cav3d_ctx *ctx, int b, const double *x);
 int cav3d_build_metrics(cav3d_ctx *ctx);
 int cav3d_get_metrics(cav3d_ctx *ctx, int b, double *cx, double *ex, double *tx, double *grd);
+
+/* Integrals of the current state.  out[10]: vapour volume and gas volume
+ * (sum over interior cells of rho*y1/rho_v and rho*y2/rho_g times the cell
+ * volume, the vof fields of src/postvariable_module.f90:282-295), the
+ * pressure force fx_p, fy_p, fz_p and the viscous force fx_v, fy_v, fz_v over
+ * the CAV3D_BC_SURFACE ranges scaled by 2/(rhoref*uref^2)/area, and cl, cd at
+ * angle of attack aoa [rad] (clcd_writing, src/postvariable_module.f90:507-633).
+ * Surface forces need the vertex coordinates (cav3d_set_grid). */
+int cav3d_integrals(cav3d_ctx *ctx, double rhoref, double uref, double area, double aoa, double *out);
+
+/* Restart file of every block, the reference's out_<iter>.dat layout
+ * (export_variable, src/variable_module.f90:108-182): per block, int32 nt_phy,
+ * int32 nt, float64 time, pv(npv,-1:imax+3,..), emut(-1:imax+3,..) when
+ * turbulent, qq(npv,2,2:imax,..) when dual-time, rcav(2,2:imax,..) when
+ * cavitating, native byte order, no record markers.  cav3d_import reads it as
+ * the restart reader does (src/initial_module.f90:245-389): interior pv,
+ * emut, qq and rcav; dv and tv(1:2) from the EOS; ghosts are left to the next
+ * boundary fill; nps / nts / timeprev returned as the reader advances them. */
+int cav3d_export(cav3d_ctx *ctx, const char *path, int nt_phy, int nt, double time);
+int cav3d_import(cav3d_ctx *ctx, const char *path, int *nps, int *nts, double *timeprev);
 
 /* pv(npv,-1:imax+3,-1:jmax+3,-1:kmax+3), dv(18,...), tv(ntv,...),
  * qq(npv,2,2:imax,2:jmax,2:kmax).  dv/tv may be NULL: then dv and tv(1:2)

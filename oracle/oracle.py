@@ -90,6 +90,7 @@ def lib():
         L.orc_unsteady_shift.argtypes = [C.c_void_p]
         L.orc_set_threads.argtypes = [C.c_int]
         L.orc_grid_metrics.argtypes = [C.c_int, C.POINTER(GBlock), C.c_int]
+        L.orc_integrals.argtypes = [C.c_void_p, C.POINTER(_D), C.c_double, C.c_double, C.c_double, C.c_double, _D]
         L.orc_alt_snd2_count_ptr = C.c_int.in_dll(L, "orc_alt_snd2_count")
         _lib = L
     return _lib
@@ -271,6 +272,15 @@ class Oracle:
         rm = C.c_double(0.0)
         lib().orc_residual(self.h, _p(l2), _p(ijk, C.c_int), C.byref(rm))
         return l2, ijk, rm.value
+
+    def integrals(self, rhoref, uref, area, aoa=0.0):
+        """vapour volume, gas volume, pressure force (3), viscous force (3), cl, cd."""
+        xs = [np.ascontiguousarray(np.asarray(b.X, dtype=np.float64).transpose(2, 1, 0, 3)).ravel()
+              for b in self.case.blocks]
+        ptrs = (_D * len(xs))(*[_p(x) for x in xs])
+        out = np.zeros(10)
+        lib().orc_integrals(self.h, ptrs, rhoref, uref, area, aoa, _p(out))
+        return out
 
     def unsteady_shift(self):
         lib().orc_unsteady_shift(self.h)

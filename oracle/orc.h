@@ -39,6 +39,8 @@ enum {
     ORC_BC_OUTFLOW_SUPERSONIC = 7, ORC_BC_DEGENERATE = 8, ORC_BC_INFLOW = 9,
     ORC_BC_OUTFLOW = 10
 };
+/* flag OR'ed into a kind: the range belongs to the 'Surface' family */
+enum { ORC_BC_SURFACE = 0x100 };
 
 /* Run configuration (the reference's config_module fields that the path reads). */
 typedef struct {
@@ -75,6 +77,8 @@ int orc_setqres(orc_ctx *c);
 int orc_residual(orc_ctx *c, double *l2, int *ijkmax, double *resmax);
 int orc_unsteady_shift(orc_ctx *c);
 void orc_set_threads(int n);
+int orc_integrals(orc_ctx *c, const double *const *x, double rhoref, double uref, double area, double aoa,
+                  double *out);
 
 /* One block for orc_grid_metrics: vertex coordinates in, metrics out
  * (Fortran layouts of t_grid: x(3,2:imax+1,2:jmax+1,2:kmax+1) holding the

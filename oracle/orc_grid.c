@@ -147,7 +147,7 @@ static void face_vectors(orc_gblock *b)
     /* degenerate faces carry no area (src/grid_module.f90:674-708) */
     for (int n = 0; n < b->nbc; n++) {
         const int *d = b->bc + 7 * n;
-        if (d[0] != ORC_BC_DEGENERATE) continue;
+        if ((d[0] & 0xff) != ORC_BC_DEGENERATE) continue;
         if (d[1] == d[4]) {
             int i0 = d[1] == 1 ? 1 : I;
             for (int k = 1; k <= K + 1; k++) for (int j = 1; j <= J + 1; j++)
@@ -346,7 +346,7 @@ static long wall_points(const orc_gblock *blk, int nb, double *out)
         const orc_gblock *b = &blk[bb];
         for (int n = 0; n < b->nbc; n++) {
             const int *d = b->bc + 7 * n;
-            if (d[0] != ORC_BC_WALL) continue;
+            if ((d[0] & 0xff) != ORC_BC_WALL) continue;
             const int *s = d + 1, *e = d + 4;
             for (int ax = 0; ax < 3; ax++) {
                 if (s[ax] != e[ax]) continue;

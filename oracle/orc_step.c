@@ -31,8 +31,9 @@ enum { FN_NONE = 0, FN_WALLINV, FN_WALLVIS, FN_WALLVISKW, FN_INSUB, FN_OUTSUB, F
 enum { F_NONE = -1, F_IMIN = 0, F_IMAX, F_JMIN, F_JMAX, F_KMIN, F_KMAX };
 
 typedef struct {
-    int kind, fn, face;
+    int kind, fn, face, surface;
     int istart[3], iend[3], origin[3], dir[3], nb[6][3];
+    int ostart[3], oend[3];   /* the range as grid_module gives it */
 } bcinfo;
 
 typedef struct {
@@ -1031,8 +1032,10 @@ int orc_set_bcs(orc_ctx *c, int bb, int nbc, const int *desc)
     b->nbc = nbc;
     for (int n = 0; n < nbc; n++) {
         bcinfo *bi = &b->bc[n];
-        bi->kind = desc[7 * n];
+        bi->kind = desc[7 * n] & 0xff;
+        bi->surface = (desc[7 * n] & ORC_BC_SURFACE) != 0;
         for (int d = 0; d < 3; d++) { bi->istart[d] = desc[7 * n + 1 + d]; bi->iend[d] = desc[7 * n + 4 + d]; }
+        for (int d = 0; d < 3; d++) { bi->ostart[d] = bi->istart[d]; bi->oend[d] = bi->iend[d]; }
         if (build_face(c, bi)) return -1;
     }
     return 0;
@@ -1120,5 +1123,113 @@ int orc_get_bcinfo(orc_ctx *c, int bb, int *out)
         }
         for (int t = 0; t < 6; t++) for (int d = 0; d < 3; d++) o[14 + 3 * t + d] = bi->nb[t][d];
     }
+    return 0;
+}
+
+/* Integrals of the current state (restated for the checker):
+ *   out[0] vapour volume: sum over interior cells of dv(1)*pv(6)/dv(4)*grd(1)
+ *   out[1] gas volume:    sum of dv(1)*pv(7)/dv(5)*grd(1)
+ *          (the vof_y1 / vof_y2 fields, src/postvariable_module.f90:282-295)
+ *   out[2..7] pressure and viscous force over the ranges flagged as the
+ *          'Surface' family, scaled by 2/(rhoref*uref^2)/area; out[8..9] cl, cd
+ *          (clcd_writing, src/postvariable_module.f90:507-633)
+ * x[b]: vertex coordinates x(3,2:imax+1,2:jmax+1,2:kmax+1) of block b. */
+int orc_integrals(orc_ctx *c, const double *const *x, double rhoref, double uref, double area, double aoa,
+                  double *out)
+{
+    const int npv = c->c.npv;
+    double vv = 0.0, vg = 0.0;
+    double fp[3] = {0.0, 0.0, 0.0}, fv[3] = {0.0, 0.0, 0.0};
+    for (int l = 0; l < c->nb; l++) {
+        const blk *b = &c->b[l];
+        for (int k = 2; k <= b->kmax; k++)
+            for (int j = 2; j <= b->jmax; j++)
+                for (int i = 2; i <= b->imax; i++) {
+                    const double *pv = PV(c, b, i, j, k), *dv = DV(b, i, j, k);
+                    const double vol = GRD(c, b, i, j, k)[0];
+                    vv += dv[0] * pv[5] / dv[3] * vol;
+                    vg += dv[0] * pv[6] / dv[4] * vol;
+                }
+    }
+    for (int l = 0; l < c->nb; l++) {
+        const blk *b = &c->b[l];
+        const int I = b->imax, J = b->jmax;
+#define XX(cc, i, j, k) (x[l][(cc) + 3 * ((long)((i) - 2) + (long)I * ((long)((j) - 2) + (long)J * ((k) - 2)))])
+        for (int m = 0; m < b->nbc; m++) {
+            const bcinfo *bi = &b->bc[m];
+            if (!bi->surface) continue;
+            const int *s = bi->ostart, *e = bi->oend;
+            for (int k = s[2]; k <= e[2]; k++)
+                for (int j = s[1]; j <= e[1]; j++)
+                    for (int i = s[0]; i <= e[0]; i++) {
+                        double cx[3], xf[3];
+                        const double *grd, *pv, *tv;
+                        int ci, cj, ck;
+                        if (s[0] == e[0]) {
+                            if (s[0] == 1) {
+                                memcpy(cx, CX(b, i, j, k), sizeof(cx));
+                                for (int q = 0; q < 3; q++)
+                                    xf[q] = 0.25 * (XX(q, i + 1, j, k) + XX(q, i + 1, j + 1, k) + XX(q, i + 1, j, k + 1) + XX(q, i + 1, j + 1, k + 1));
+                                ci = i + 1; cj = j; ck = k;
+                            } else {
+                                for (int q = 0; q < 3; q++) cx[q] = -CX(b, i - 1, j, k)[q];
+                                for (int q = 0; q < 3; q++)
+                                    xf[q] = 0.25 * (XX(q, i, j, k) + XX(q, i, j + 1, k) + XX(q, i, j, k + 1) + XX(q, i, j + 1, k + 1));
+                                ci = i - 1; cj = j; ck = k;
+                            }
+                        } else if (s[1] == e[1]) {
+                            if (s[1] == 1) {
+                                memcpy(cx, EX(b, i, j, k), sizeof(cx));
+                                for (int q = 0; q < 3; q++)
+                                    xf[q] = 0.25 * (XX(q, i, j + 1, k) + XX(q, i + 1, j + 1, k) + XX(q, i, j + 1, k + 1) + XX(q, i + 1, j + 1, k + 1));
+                                ci = i; cj = j + 1; ck = k;
+                            } else {
+                                for (int q = 0; q < 3; q++) cx[q] = -EX(b, i, j - 1, k)[q];
+                                for (int q = 0; q < 3; q++)
+                                    xf[q] = 0.25 * (XX(q, i, j, k) + XX(q, i + 1, j, k) + XX(q, i, j, k + 1) + XX(q, i + 1, j, k + 1));
+                                ci = i; cj = j - 1; ck = k;
+                            }
+                        } else {
+                            if (s[2] == 1) {
+                                memcpy(cx, TX(b, i, j, k), sizeof(cx));
+                                for (int q = 0; q < 3; q++)
+                                    xf[q] = 0.25 * (XX(q, i, j, k + 1) + XX(q, i + 1, j, k + 1) + XX(q, i, j + 1, k + 1) + XX(q, i + 1, j + 1, k + 1));
+                                ci = i; cj = j; ck = k + 1;
+                            } else {
+                                for (int q = 0; q < 3; q++) cx[q] = -TX(b, i, j, k - 1)[q];
+                                for (int q = 0; q < 3; q++)
+                                    xf[q] = 0.25 * (XX(q, i, j, k) + XX(q, i + 1, j, k) + XX(q, i, j + 1, k) + XX(q, i + 1, j + 1, k));
+                                ci = i; cj = j; ck = k - 1;
+                            }
+                        }
+                        grd = GRD(c, b, ci, cj, ck);
+                        pv = PV(c, b, ci, cj, ck);
+                        tv = c->c.ntv ? TV(c, b, ci, cj, ck) : NULL;
+                        double dfp[3], dfv[3];
+                        for (int q = 0; q < 3; q++) dfp[q] = -pv[0] * cx[q];
+                        double dl = cx[0] * cx[0] + cx[1] * cx[1] + cx[2] * cx[2];
+                        double vel = cx[0] * pv[1] + cx[1] * pv[2] + cx[2] * pv[3];
+                        double vis;
+                        if (c->c.iturb == -1 || c->c.iturb == 0) vis = tv[0] + tv[2];
+                        else if (c->c.iturb == -2) vis = tv[0];
+                        else vis = 0.0;
+                        for (int q = 0; q < 3; q++) {
+                            double ut = pv[1 + q] - cx[q] * vel / dl;
+                            dfv[q] = vis * ut / sqrt((grd[1] - xf[0]) * (grd[1] - xf[0]) + (grd[2] - xf[1]) * (grd[2] - xf[1]) +
+                                                     (grd[3] - xf[2]) * (grd[3] - xf[2]));
+                        }
+                        for (int q = 0; q < 3; q++) dfv[q] = dfv[q] * sqrt(dl);
+                        for (int q = 0; q < 3; q++) { fp[q] += dfp[q]; fv[q] += dfv[q]; }
+                    }
+        }
+#undef XX
+    }
+    (void)npv;
+    const double ref = 2.0 / rhoref / (uref * uref) / area;
+    out[0] = vv;
+    out[1] = vg;
+    for (int q = 0; q < 3; q++) { out[2 + q] = fp[q] * ref; out[5 + q] = fv[q] * ref; }
+    out[8] = (out[3] + out[6]) * cos(aoa) - (out[2] + out[5]) * sin(aoa);
+    out[9] = (out[3] + out[6]) * sin(aoa) + (out[2] + out[5]) * cos(aoa);
     return 0;
 }
