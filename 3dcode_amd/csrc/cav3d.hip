@@ -653,7 +653,13 @@ __global__ __launch_bounds__(256) void k_combine(DS S)
     double vol = S.vol[widx(B, i, j, k)];
     if (P.ncav) {
         double rc, rv, ic[4];
-        merkle(P, pc, d, vol, rc, rv, ic);
+        if (P.ncav == 1) {
+            merkle(P, pc, d, vol, rc, rv, ic);
+        } else {
+            double dd[ND], full[18];
+            mixture_eos<double>(pc[0] + P.pref, pc[4], pc[5], pc[6], dd, false, full);
+            cav_model(P, pc, full, vol, rc, rv, ic);
+        }
         r[5] = r[5] + rv - rc;
         for (int q = 0; q < 4; q++) S.icav[q * S.NC + t] = ic[q];
         S.rcav[t] = rc;
@@ -1634,7 +1640,8 @@ int cav3d_create(const cav3d_params *prm, int nblocks, const cav3d_block *blocks
         return fail("iturb must be 0 (k-omega SST), -2 (laminar) or -3 (Euler)", -1);
     if (prm->nscheme != 1 && prm->nscheme != 4) return fail("nscheme must be 1 (Roe) or 4 (AUSM+-up)", -1);
     if (prm->nmuscl < 0 || prm->nmuscl > 2 || prm->nlim < 0 || prm->nlim > 10) return fail("bad nmuscl/nlim", -1);
-    if (prm->ncav != 0 && prm->ncav != 1) return fail("ncav must be 0 or 1 (Merkle)", -1);
+    if (prm->ncav < 0 || prm->ncav > 5)
+        return fail("ncav must be 0 (none), 1 (Merkle), 2 (Kunz), 3 (Singhal), 4 (Schnerr-Sauer) or 5 (Lee)", -1);
     if (prm->cav_clip != 0 && prm->cav_clip != 1) return fail("cav_clip must be 0 (as written) or 1 (species)", -1);
     if (prm->nsteady < 0 || prm->nsteady > 2) return fail("nsteady must be 0, 1 or 2", -1);
     if (!(prm->dt_phy > 0.0)) return fail("dt_phy must be > 0 (it divides the BDF term of getx)", -1);

@@ -133,6 +133,28 @@ HD double sat_p(double t)
     return 1.0e6 * (x2 * x2);
 }
 
+// IF97 region-4 saturation temperature (src/eos_module.f90:2340-2354)
+HD double sat_t(double p)
+{
+    double beta = pow(p / 1.0e6, 0.25);
+    double e = beta * beta - 0.17073846940092e2 * beta + 0.14915108613530e2;
+    double f = 0.11670521452767e4 * (beta * beta) + 0.12020824702470e5 * beta - 0.48232657361591e4;
+    double g = -0.72421316703206e6 * (beta * beta) - 0.32325550322333e7 * beta + 0.40511340542057e6;
+    double d = 2.0 * g / (-f - sqrt(f * f - 4.0 * e * g));
+    double s = 0.65017534844798e3 + d;
+    return 0.5 * (0.65017534844798e3 + d - sqrt(s * s - 4.0 * (-0.23855557567849e0 + 0.65017534844798e3 * d)));
+}
+
+// surface tension of water against its vapour (src/eos_module.f90:2430-2438);
+// constant integer powers as gfortran expands them: x**3 = (x*x)*x,
+// x**4 = (x*x)*(x*x), x**5 = ((x*x)*x)*(x*x)
+HD double h2o_sigma(double t)
+{
+    double t2 = t * t, t3 = t2 * t, t4 = t2 * t2, t5 = t3 * t2;
+    return 0.0881130429998226 + 0.0000616475914044782 * t - 4.4860167748009e-7 * t2 + 2.53682468718936e-10 * t3
+           - 2.62005668908913e-13 * t4 + 2.97878955533757e-16 * t5;
+}
+
 // real**integer as gfortran evaluates it with a run-time exponent (libgcc
 // __powidf2: binary exponentiation, reciprocal for negative powers); the
 // exponents here are compile-time table entries, so the loop unrolls into
@@ -1103,6 +1125,147 @@ HD void merkle(const Par &P, const double *pc, const double *d, double vol, doub
     }
     double s = (icav[2] < 0.0 || (icav[2] == 0.0 && signbit(icav[2]))) ? -1.0 : 1.0;
     for (int q = 0; q < 4; q++) icav[q] *= s;
+}
+
+// The other mass-transfer models (src/cav_module.f90:173-357), on the full
+// dv(1:18) of the cell (dv[2..4] phase densities, dv[14..17] the vapour and
+// liquid density derivatives).  Rate clip and coupling sign as in merkle().
+HD void cav_clip_rates(const Par &P, const double *pc, const double *dv, double &r_c, double &r_v)
+{
+    if (P.cav_clip == 0) {
+        r_c = mn(r_c, dv[0] * pc[4]);
+        r_v = mn(r_v, dv[0] * (1.0 - pc[4] - pc[5]));
+    } else {
+        r_c = mn(r_c, dv[0] * pc[5]);
+        r_v = mn(r_v, dv[0] * (1.0 - pc[5] - pc[6]));
+    }
+}
+HD void cav_flip(double *icav)
+{
+    double s = (icav[2] < 0.0 || (icav[2] == 0.0 && signbit(icav[2]))) ? -1.0 : 1.0;
+    for (int q = 0; q < 4; q++) icav[q] *= s;
+}
+
+// Kunz (ncav 2), src/cav_module.f90:173-211
+HD void kunz(const Par &P, const double *pc, const double *dv, double vol, double &rc, double &rv, double *icav)
+{
+    double pww = sat_p(pc[4]);
+    double rho1 = 1.0 / dv[0];
+    double av = dv[0] * pc[5] / dv[3], ag = dv[0] * pc[6] / dv[4], al = 1.0 - av - ag;
+    double r_c = P.c_c * dv[3] * av * ((al + ag) * (al + ag)) * P.t_ref;
+    double pv_ = mx(pww - pc[0] - P.pref, 0.0);
+    double r_v = P.c_v * dv[0] * (1.0 - pc[5] - pc[6]) * pv_ * P.dp_ref * P.t_ref;
+    cav_clip_rates(P, pc, dv, r_c, r_v);
+    rc = r_c * vol;
+    rv = r_v * vol;
+    double av1 = 1.0 - 4.0 * av + 3.0 * (av * av);
+    icav[0] = -(P.c_c * P.t_ref * (dv[6] * pc[5] * av1 + 2.0 * (av * av) * dv[14] * (al + ag)));
+    icav[1] = -(P.c_c * P.t_ref * (dv[7] * pc[5] * av1 + 2.0 * (av * av) * dv[15] * (al + ag)));
+    icav[2] = -(P.c_c * P.t_ref * (dv[8] * pc[5] + dv[0]) * av1);
+    icav[3] = -(P.c_c * P.t_ref * dv[9] * pc[5] * av1);
+    if (r_v != 0.0) {
+        icav[0] = r_v * dv[6] * rho1 - P.c_v * dv[0] * (1.0 - pc[5] - pc[6]) * P.dp_ref * P.t_ref;
+        icav[1] = r_v * dv[7] * rho1;
+        icav[2] = r_v * dv[8] * rho1 - P.c_v * dv[0] * pv_ * P.dp_ref * P.t_ref;
+        icav[3] = r_v * dv[9] * rho1 - P.c_v * dv[0] * pv_ * P.dp_ref * P.t_ref;
+    }
+}
+
+// Singhal (ncav 3), src/cav_module.f90:213-260
+HD void singhal(const Par &P, const double *pc, const double *dv, double vol, double &rc, double &rv, double *icav)
+{
+    double pww = sat_p(pc[4]);
+    double sigma = h2o_sigma(pc[4]);
+    double rho1 = 1.0 / dv[2], rho2 = 1.0 / dv[3];
+    double kc = P.c_c * P.uref / sigma * dv[2] * dv[3], kv = P.c_v * P.uref / sigma * dv[2] * dv[3];
+    double pc_ = mx(pc[0] + P.pref - pww, 0.0), pv_ = mx(pww - pc[0] - P.pref, 0.0);
+    double r_c = kc * sqrt(2.0 / 3.0 * pc_ * rho1) * pc[5];
+    double r_v = kv * sqrt(2.0 / 3.0 * pv_ * rho1) * (1.0 - pc[5] - pc[6]);
+    cav_clip_rates(P, pc, dv, r_c, r_v);
+    rc = r_c * vol;
+    rv = r_v * vol;
+    if (r_c != 0.0) {
+        icav[0] = -(r_c * (dv[14] * rho2 + 0.5 * dv[16] * rho1))
+                  - 0.5 * P.c_c * P.uref / sigma * dv[2] * dv[3] * sqrt(2.0 / 3.0 * rho1) * pc[5] * pow(pc[0] + P.pref - pww, -0.5);
+        icav[1] = -(r_c * (dv[15] * rho2 + 0.5 * dv[17] * rho1));
+        icav[2] = -(kc * sqrt(2.0 / 3.0 * pc_ * rho1));
+        icav[3] = 0.0;
+    }
+    if (r_v != 0.0) {
+        icav[0] = r_v * (dv[14] * rho2 + 0.5 * dv[16] * rho1)
+                  - 0.5 * P.c_v * P.uref / sigma * dv[2] * dv[3] * sqrt(2.0 / 3.0 * rho1) * (1.0 - pc[5] - pc[6])
+                        * pow(pww - pc[0] - P.pref, -0.5);
+        icav[1] = r_v * (dv[15] * rho2 + 0.5 * dv[17] * rho1);
+        icav[2] = -(kv * sqrt(2.0 / 3.0 * pv_ * rho1));
+        icav[3] = -(kv * sqrt(2.0 / 3.0 * pv_ * rho1));
+    }
+}
+
+// Schnerr-Sauer (ncav 4), src/cav_module.f90:262-316; bubble number density 1e13
+HD void schnerr_sauer(const Par &P, const double *pc, const double *dv, double vol, double &rc, double &rv, double *icav)
+{
+    const double pi = 4.0 * atan(1.0), eta = 1.0e13;
+    double pww = sat_p(pc[4]);
+    double rho1 = 1.0 / dv[0], rho3 = 1.0 / dv[2], rho4 = 1.0 / dv[3];
+    double y1 = pc[5], y2 = pc[6], yl = 1.0 - pc[5] - pc[6];
+    double kb = pow(4.0 * pi * eta / 3.0, 1.0 / 3.0), k23 = sqrt(2.0 / 3.0);
+    double r_c = P.c_c * 3.0 * kb * k23 * dv[0] * pow(dv[3], 1.0 / 3.0) * pow(dv[2], -5.0 / 6.0)
+                 * pow(y1, 2.0 / 3.0) * pow(yl, 4.0 / 3.0) * sqrt(mx(pc[0] + P.pref - pww, 0.0));
+    double r_v = P.c_v * 3.0 * kb * k23 * dv[0] * pow(dv[3], 1.0 / 3.0) * pow(dv[2], -5.0 / 6.0)
+                 * pow(y1 + y2, 2.0 / 3.0) * pow(yl, 4.0 / 3.0) * sqrt(mx(pww - pc[0] - P.pref, 0.0));
+    cav_clip_rates(P, pc, dv, r_c, r_v);
+    rc = r_c * vol;
+    rv = r_v * vol;
+    if (r_c != 0.0) {
+        icav[0] = -(r_c * (dv[6] * rho1 + dv[14] * rho4 / 3.0 - 5.0 * dv[16] * rho3 / 6.0 + 0.5 / (pc[0] + P.pref - pww)));
+        icav[1] = -(r_c * (dv[7] * rho1 + dv[15] * rho4 / 3.0 - 5.0 * dv[17] * rho3 / 6.0));
+        icav[2] = -(r_c * (dv[8] * rho1 + 2.0 / (3.0 * y1) - 4.0 / (3.0 * yl)));
+        icav[3] = -(r_c * (dv[9] * rho1 - 4.0 / (3.0 * yl)));
+    }
+    if (r_v != 0.0) {
+        icav[0] = r_v * (dv[6] * rho1 + dv[14] * rho4 / 3.0 - 5.0 * dv[16] * rho3 / 6.0 - 0.5 / (pww - pc[0] - P.pref));
+        icav[1] = r_v * (dv[7] * rho1 + dv[15] * rho4 / 3.0 - 5.0 * dv[17] * rho3 / 6.0);
+        icav[2] = r_v * (dv[8] * rho1 + 2.0 / (3.0 * (y1 + y2)) - 4.0 / (3.0 * yl));
+        icav[3] = r_v * (dv[9] * rho1 + 2.0 / (3.0 * (y1 + y2)) - 4.0 / (3.0 * yl));
+    }
+}
+
+// Lee (ncav 5), src/cav_module.f90:318-356
+HD void lee(const Par &P, const double *pc, const double *dv, double vol, double &rc, double &rv, double *icav)
+{
+    double tww = sat_t(pc[0] + P.pref);
+    double rho1 = 1.0 / dv[0];
+    double tc = mx(tww - pc[4], 0.0), tv_ = mx(pc[4] - tww, 0.0);
+    double r_c = P.c_c * dv[0] * pc[5] * tc / tww;
+    double r_v = P.c_v * dv[0] * (1.0 - pc[5] - pc[6]) * tv_ / tww;
+    cav_clip_rates(P, pc, dv, r_c, r_v);
+    rc = r_c * vol;
+    rv = r_v * vol;
+    if (r_c != 0.0) {
+        icav[0] = -(r_c * dv[6] * rho1);
+        icav[1] = -(r_c * dv[7] * rho1) + P.c_c * dv[0] * pc[5] / tww;
+        icav[2] = -(r_c * dv[8] * rho1) - P.c_c * dv[0] * tc / tww;
+        icav[3] = -(r_c * dv[9] * rho1);
+    }
+    if (r_v != 0.0) {
+        icav[0] = r_v * dv[6] * rho1;
+        icav[1] = r_v * dv[7] * rho1 + P.c_v * dv[0] * (1.0 - pc[5] - pc[6]) / tww;
+        icav[2] = r_v * dv[8] * rho1 - P.c_v * dv[0] * tv_ / tww;
+        icav[3] = r_v * dv[9] * rho1 - P.c_v * dv[0] * tv_ / tww;
+    }
+}
+
+// the model of ncav 2..5 (src/rhs_module.f90:98-110), full dv(1:18) of the cell
+HD void cav_model(const Par &P, const double *pc, const double *dv, double vol, double &rc, double &rv, double *icav)
+{
+    rc = 0.0; rv = 0.0;
+    icav[0] = icav[1] = icav[2] = icav[3] = 0.0;
+    if (pc[4] >= P.t_crit) return;
+    if (P.ncav == 2) kunz(P, pc, dv, vol, rc, rv, icav);
+    else if (P.ncav == 3) singhal(P, pc, dv, vol, rc, rv, icav);
+    else if (P.ncav == 4) schnerr_sauer(P, pc, dv, vol, rc, rv, icav);
+    else lee(P, pc, dv, vol, rc, rv, icav);
+    cav_flip(icav);
 }
 
 // k-omega SST source (ides = 0): src[2], itt[4], omega_cut

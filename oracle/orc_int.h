@@ -17,6 +17,9 @@ void orc_vsflux(const orc_cfg *c, const double *nx, const double *const e[4], co
                 const double *dvl, const double *dvr, const double *tvl, const double *tvr, double *fx);
 void orc_merkle(const orc_cfg *c, const double (*x)[ORC_NPV], const double *dv, const double *grd,
                 double *r_c, double *r_v, double *icav);
+void orc_cavsource(const orc_cfg *c, const double (*x)[ORC_NPV], const double *dv, const double *grd,
+                double *r_c, double *r_v, double *icav);
+double orc_h2o_sigma(double t);
 void orc_kwsst_source(const double *const f[6], const double *grd, const double (*x)[ORC_NPV],
                       const double *dv, const double *tv, double *source, double *itt, double *omega_cut);
 double orc_kwsst_eddy(const double *const f[6], const double *grd, const double (*x)[ORC_NPV],

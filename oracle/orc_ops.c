@@ -411,6 +411,182 @@ void orc_vsflux(const orc_cfg *c, const double *nx, const double *const e[4], co
 #undef ED
 
 /* ------------------------------------------------------------------ */
+/* The other mass-transfer models, src/cav_module.f90:173-357.  Common:
+ * nothing at or above the critical temperature; the rate clip as the
+ * Merkle model (as written on the temperature, or on the mass fractions);
+ * the coupling vector flipped by the sign of its y1 entry.  Constant integer
+ * powers are the products gfortran expands them into (x**2 = x*x,
+ * x**3 = (x*x)*x, x**4 = (x*x)*(x*x), x**5 = ((x*x)*x)*(x*x)); real powers
+ * are run-time pow. */
+static void cav_clip(const orc_cfg *c, const double (*x)[ORC_NPV], const double *dv, double *r_c, double *r_v)
+{
+    if (c->cav_clip == 0) {
+        *r_c = fmin_f(*r_c, dv[0] * X(2, 5));
+        *r_v = fmin_f(*r_v, dv[0] * (1.0 - X(2, 5) - X(2, 6)));
+    } else {
+        *r_c = fmin_f(*r_c, dv[0] * X(2, 6));
+        *r_v = fmin_f(*r_v, dv[0] * (1.0 - X(2, 6) - X(2, 7)));
+    }
+}
+
+static void cav_sign(double *icav)
+{
+    double lam = copysign(1.0, icav[2]);
+    double s = 1.0 * (1.0 - 0.5 * (1.0 - lam)) - 0.5 * (1.0 - lam);
+    for (int q = 0; q < 4; q++) icav[q] = icav[q] * s;
+}
+
+/* src/eos_module.f90:2430-2438 — surface tension of water against its vapour */
+double orc_h2o_sigma(double t)
+{
+    double t2 = t * t, t3 = t2 * t, t4 = t2 * t2, t5 = t3 * t2;
+    return 0.0881130429998226 + 0.0000616475914044782 * t - 4.4860167748009e-7 * t2 + 2.53682468718936e-10 * t3
+           - 2.62005668908913e-13 * t4 + 2.97878955533757e-16 * t5;
+}
+
+/* Kunz, src/cav_module.f90:173-211 */
+void orc_kunz(const orc_cfg *c, const double (*x)[ORC_NPV], const double *dv, const double *grd,
+              double *r_c_out, double *r_v_out, double *icav)
+{
+    *r_c_out = 0.0; *r_v_out = 0.0;
+    icav[0] = icav[1] = icav[2] = icav[3] = 0.0;
+    if (X(2, 5) >= c->t_crit) return;
+    double pww = orc_h2o_pww(X(2, 5));
+    double rho1 = 1.0 / dv[0];
+    double av = dv[0] * X(2, 6) / dv[3];
+    double ag = dv[0] * X(2, 7) / dv[4];
+    double al = 1.0 - av - ag;
+    double r_c = c->c_c * dv[3] * av * ((al + ag) * (al + ag)) * c->t_ref;
+    double r_v = c->c_v * dv[0] * (1.0 - X(2, 6) - X(2, 7)) * fmax_f(pww - X(2, 1) - c->pref, 0.0) * c->dp_ref * c->t_ref;
+    cav_clip(c, x, dv, &r_c, &r_v);
+    *r_c_out = r_c * grd[0];
+    *r_v_out = r_v * grd[0];
+    double av1 = 1.0 - 4.0 * av + 3.0 * (av * av);
+    icav[0] = -(c->c_c * c->t_ref * (dv[6] * X(2, 6) * av1 + 2.0 * (av * av) * dv[14] * (al + ag)));
+    icav[1] = -(c->c_c * c->t_ref * (dv[7] * X(2, 6) * av1 + 2.0 * (av * av) * dv[15] * (al + ag)));
+    icav[2] = -(c->c_c * c->t_ref * (dv[8] * X(2, 6) + dv[0]) * av1);
+    icav[3] = -(c->c_c * c->t_ref * dv[9] * X(2, 6) * av1);
+    if (r_v != 0.0) {
+        icav[0] = r_v * dv[6] * rho1 - c->c_v * dv[0] * (1.0 - X(2, 6) - X(2, 7)) * c->dp_ref * c->t_ref;
+        icav[1] = r_v * dv[7] * rho1;
+        icav[2] = r_v * dv[8] * rho1 - c->c_v * dv[0] * fmax_f(pww - X(2, 1) - c->pref, 0.0) * c->dp_ref * c->t_ref;
+        icav[3] = r_v * dv[9] * rho1 - c->c_v * dv[0] * fmax_f(pww - X(2, 1) - c->pref, 0.0) * c->dp_ref * c->t_ref;
+    }
+    cav_sign(icav);
+}
+
+/* Singhal, src/cav_module.f90:213-260 */
+void orc_singhal(const orc_cfg *c, const double (*x)[ORC_NPV], const double *dv, const double *grd,
+                 double *r_c_out, double *r_v_out, double *icav)
+{
+    *r_c_out = 0.0; *r_v_out = 0.0;
+    icav[0] = icav[1] = icav[2] = icav[3] = 0.0;
+    if (X(2, 5) >= c->t_crit) return;
+    double pww = orc_h2o_pww(X(2, 5));
+    double sigma = orc_h2o_sigma(X(2, 5));
+    double rho1 = 1.0 / dv[2], rho2 = 1.0 / dv[3];
+    double kc = c->c_c * c->uref / sigma * dv[2] * dv[3], kv = c->c_v * c->uref / sigma * dv[2] * dv[3];
+    double r_c = kc * sqrt(2.0 / 3.0 * fmax_f(X(2, 1) + c->pref - pww, 0.0) * rho1) * X(2, 6);
+    double r_v = kv * sqrt(2.0 / 3.0 * fmax_f(pww - X(2, 1) - c->pref, 0.0) * rho1) * (1.0 - X(2, 6) - X(2, 7));
+    cav_clip(c, x, dv, &r_c, &r_v);
+    *r_c_out = r_c * grd[0];
+    *r_v_out = r_v * grd[0];
+    if (r_c != 0.0) {
+        icav[0] = -(r_c * (dv[14] * rho2 + 0.5 * dv[16] * rho1))
+                  - 0.5 * c->c_c * c->uref / sigma * dv[2] * dv[3] * sqrt(2.0 / 3.0 * rho1) * X(2, 6)
+                        * pow(X(2, 1) + c->pref - pww, -0.5);
+        icav[1] = -(r_c * (dv[15] * rho2 + 0.5 * dv[17] * rho1));
+        icav[2] = -(kc * sqrt(2.0 / 3.0 * fmax_f(X(2, 1) + c->pref - pww, 0.0) * rho1));
+        icav[3] = 0.0;
+    }
+    if (r_v != 0.0) {
+        icav[0] = r_v * (dv[14] * rho2 + 0.5 * dv[16] * rho1)
+                  - 0.5 * c->c_v * c->uref / sigma * dv[2] * dv[3] * sqrt(2.0 / 3.0 * rho1) * (1.0 - X(2, 6) - X(2, 7))
+                        * pow(pww - X(2, 1) - c->pref, -0.5);
+        icav[1] = r_v * (dv[15] * rho2 + 0.5 * dv[17] * rho1);
+        icav[2] = -(kv * sqrt(2.0 / 3.0 * fmax_f(pww - X(2, 1) - c->pref, 0.0) * rho1));
+        icav[3] = -(kv * sqrt(2.0 / 3.0 * fmax_f(pww - X(2, 1) - c->pref, 0.0) * rho1));
+    }
+    cav_sign(icav);
+}
+
+/* Schnerr-Sauer, src/cav_module.f90:262-316 (bubble number density 1e13) */
+void orc_schnerr_sauer(const orc_cfg *c, const double (*x)[ORC_NPV], const double *dv, const double *grd,
+                       double *r_c_out, double *r_v_out, double *icav)
+{
+    *r_c_out = 0.0; *r_v_out = 0.0;
+    icav[0] = icav[1] = icav[2] = icav[3] = 0.0;
+    if (X(2, 5) >= c->t_crit) return;
+    const double pi = 4.0 * atan(1.0), eta = 1.0e13;
+    double pww = orc_h2o_pww(X(2, 5));
+    double rho1 = 1.0 / dv[0], rho3 = 1.0 / dv[2], rho4 = 1.0 / dv[3];
+    double y1 = X(2, 6), y2 = X(2, 7), yl = 1.0 - X(2, 6) - X(2, 7);
+    double kb = pow(4.0 * pi * eta / 3.0, 1.0 / 3.0), k23 = sqrt(2.0 / 3.0);
+    double r_c = c->c_c * 3.0 * kb * k23 * dv[0] * pow(dv[3], 1.0 / 3.0) * pow(dv[2], -5.0 / 6.0)
+                 * pow(y1, 2.0 / 3.0) * pow(yl, 4.0 / 3.0) * sqrt(fmax_f(X(2, 1) + c->pref - pww, 0.0));
+    double r_v = c->c_v * 3.0 * kb * k23 * dv[0] * pow(dv[3], 1.0 / 3.0) * pow(dv[2], -5.0 / 6.0)
+                 * pow(y1 + y2, 2.0 / 3.0) * pow(yl, 4.0 / 3.0) * sqrt(fmax_f(pww - X(2, 1) - c->pref, 0.0));
+    cav_clip(c, x, dv, &r_c, &r_v);
+    *r_c_out = r_c * grd[0];
+    *r_v_out = r_v * grd[0];
+    if (r_c != 0.0) {
+        icav[0] = -(r_c * (dv[6] * rho1 + dv[14] * rho4 / 3.0 - 5.0 * dv[16] * rho3 / 6.0 + 0.5 / (X(2, 1) + c->pref - pww)));
+        icav[1] = -(r_c * (dv[7] * rho1 + dv[15] * rho4 / 3.0 - 5.0 * dv[17] * rho3 / 6.0));
+        icav[2] = -(r_c * (dv[8] * rho1 + 2.0 / (3.0 * y1) - 4.0 / (3.0 * yl)));
+        icav[3] = -(r_c * (dv[9] * rho1 - 4.0 / (3.0 * yl)));
+    }
+    if (r_v != 0.0) {
+        icav[0] = r_v * (dv[6] * rho1 + dv[14] * rho4 / 3.0 - 5.0 * dv[16] * rho3 / 6.0 - 0.5 / (pww - X(2, 1) - c->pref));
+        icav[1] = r_v * (dv[7] * rho1 + dv[15] * rho4 / 3.0 - 5.0 * dv[17] * rho3 / 6.0);
+        icav[2] = r_v * (dv[8] * rho1 + 2.0 / (3.0 * (y1 + y2)) - 4.0 / (3.0 * yl));
+        icav[3] = r_v * (dv[9] * rho1 + 2.0 / (3.0 * (y1 + y2)) - 4.0 / (3.0 * yl));
+    }
+    cav_sign(icav);
+}
+
+/* Lee, src/cav_module.f90:318-356 */
+void orc_lee(const orc_cfg *c, const double (*x)[ORC_NPV], const double *dv, const double *grd,
+             double *r_c_out, double *r_v_out, double *icav)
+{
+    *r_c_out = 0.0; *r_v_out = 0.0;
+    icav[0] = icav[1] = icav[2] = icav[3] = 0.0;
+    if (X(2, 5) >= c->t_crit) return;
+    double tww = orc_h2o_tww(X(2, 1) + c->pref);
+    double rho1 = 1.0 / dv[0];
+    double r_c = c->c_c * dv[0] * X(2, 6) * fmax_f(tww - X(2, 5), 0.0) / tww;
+    double r_v = c->c_v * dv[0] * (1.0 - X(2, 6) - X(2, 7)) * fmax_f(X(2, 5) - tww, 0.0) / tww;
+    cav_clip(c, x, dv, &r_c, &r_v);
+    *r_c_out = r_c * grd[0];
+    *r_v_out = r_v * grd[0];
+    if (r_c != 0.0) {
+        icav[0] = -(r_c * dv[6] * rho1);
+        icav[1] = -(r_c * dv[7] * rho1) + c->c_c * dv[0] * X(2, 6) / tww;
+        icav[2] = -(r_c * dv[8] * rho1) - c->c_c * dv[0] * fmax_f(tww - X(2, 5), 0.0) / tww;
+        icav[3] = -(r_c * dv[9] * rho1);
+    }
+    if (r_v != 0.0) {
+        icav[0] = r_v * dv[6] * rho1;
+        icav[1] = r_v * dv[7] * rho1 + c->c_v * dv[0] * (1.0 - X(2, 6) - X(2, 7)) / tww;
+        icav[2] = r_v * dv[8] * rho1 - c->c_v * dv[0] * fmax_f(X(2, 5) - tww, 0.0) / tww;
+        icav[3] = r_v * dv[9] * rho1 - c->c_v * dv[0] * fmax_f(X(2, 5) - tww, 0.0) / tww;
+    }
+    cav_sign(icav);
+}
+
+/* the model of config ncav (src/rhs_module.f90:98-110) */
+void orc_cavsource(const orc_cfg *c, const double (*x)[ORC_NPV], const double *dv, const double *grd,
+                   double *r_c, double *r_v, double *icav)
+{
+    switch (c->ncav) {
+    case 2: orc_kunz(c, x, dv, grd, r_c, r_v, icav); break;
+    case 3: orc_singhal(c, x, dv, grd, r_c, r_v, icav); break;
+    case 4: orc_schnerr_sauer(c, x, dv, grd, r_c, r_v, icav); break;
+    case 5: orc_lee(c, x, dv, grd, r_c, r_v, icav); break;
+    default: orc_merkle(c, x, dv, grd, r_c, r_v, icav); break;
+    }
+}
+
+/* ------------------------------------------------------------------ */
 /* Merkle mass transfer, src/cav_module.f90:136-171                    */
 void orc_merkle(const orc_cfg *c, const double (*x)[ORC_NPV], const double *dv, const double *grd,
                 double *r_c_out, double *r_v_out, double *icav)

@@ -700,7 +700,7 @@ static void calrhs_block(orc_ctx *c, blk *b)
             double *r = b->res + npv * q;
             if (g->ncav) {
                 double rc, rv, ic[4];
-                orc_merkle(g, (const double (*)[ORC_NPV])x, dv, grd, &rc, &rv, ic);
+                orc_cavsource(g, (const double (*)[ORC_NPV])x, dv, grd, &rc, &rv, ic);
                 r[5] = r[5] + rv - rc;
                 memcpy(b->icav + 4 * q, ic, sizeof(ic));
                 b->rcav[2 * q] = rc;

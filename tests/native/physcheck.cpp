@@ -143,6 +143,18 @@ void pc_merkle(const int *iv, const double *dv, const double *pc, const double *
     for (int q = 0; q < 4; q++) out[2 + q] = ic[q];
 }
 
+void pc_cav_model(const int *iv, const double *dv, const double *pc, const double *full, double vol, double *out)
+{
+    Par P = mkpar(iv, dv);
+    double rc, rv, ic[4];
+    cav_model(P, pc, full, vol, rc, rv, ic);
+    out[0] = rc; out[1] = rv;
+    for (int q = 0; q < 4; q++) out[2 + q] = ic[q];
+}
+
+double pc_sat_t(double p) { return sat_t(p); }
+double pc_h2o_sigma(double t) { return h2o_sigma(t); }
+
 void pc_sst_source(const double *x7, const double *f6, double vol, double ydns, const double *d, double mut,
                    double *out)
 {

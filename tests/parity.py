@@ -18,7 +18,18 @@ def small_cases(pk):
         "sst_cav_bdf1": lambda: _clip(bc(12, 8, 6, True, True, 4, 2, 8, cfl=5.0, nsteady=1, dt_phy=2e-4, prec=2,
                                          seed=8), 1),
         "sst_cav_3block": lambda: _clip(pk.cases.split_i(pk.cases.config2(14, 8, 6), [5, 10]), 1),
+        "sst_kunz": lambda: _model(pk.cases.config2(12, 8, 6), 2),
+        "sst_singhal": lambda: _model(pk.cases.config2(12, 8, 6), 3),
+        "sst_schnerr_sauer": lambda: _model(pk.cases.config2(12, 8, 6), 4),
+        "sst_lee": lambda: _model(pk.cases.config2(12, 8, 6), 5),
     }
+
+
+def _model(case, ncav):
+    """the config-2 state with another mass-transfer model (species clip)"""
+    case.params["ncav"] = ncav
+    case.params["cav_clip"] = 1
+    return case
 
 
 def _clip(case, mode):

@@ -28,6 +28,11 @@ def build(outdir):
     L.pc_jac_rows.argtypes = [_I, _D, _D, _D, C.c_double, C.c_double, _D, C.c_int, C.c_double, _D, _D]
     L.pc_getx_sel.argtypes = [_I, _D, _D, _D, C.c_double, C.c_double, _D, C.c_double, _D, _D, _D, _D]
     L.pc_merkle.argtypes = [_I, _D, _D, _D, C.c_double, _D]
+    L.pc_cav_model.argtypes = [_I, _D, _D, _D, C.c_double, _D]
+    L.pc_sat_t.argtypes = [C.c_double]
+    L.pc_sat_t.restype = C.c_double
+    L.pc_h2o_sigma.argtypes = [C.c_double]
+    L.pc_h2o_sigma.restype = C.c_double
     L.pc_sst_source.argtypes = [_D, _D, C.c_double, C.c_double, _D, C.c_double, _D]
     L.pc_sst_eddy.argtypes = [_D, _D, C.c_double, C.c_double, _D]
     L.pc_liquid.argtypes = [C.c_double, C.c_double, _D]

@@ -42,6 +42,9 @@ def OL(orc):
     L.orc_getx.argtypes = [C.c_void_p, PP, D, D, D, D, C.c_double, D, D, D, D]
     L.orc_caljac.argtypes = [C.c_void_p, D, D, D, D, D, C.c_int, D]
     L.orc_merkle.argtypes = [C.c_void_p, D, D, D, D, D, D]
+    L.orc_cavsource.argtypes = [C.c_void_p, D, D, D, D, D, D]
+    L.orc_h2o_sigma.argtypes = [C.c_double]
+    L.orc_h2o_sigma.restype = C.c_double
     L.orc_kwsst_source.argtypes = [PP, D, D, D, D, D, D, D]
     L.orc_kwsst_eddy.argtypes = [PP, D, D, D, D]
     L.orc_kwsst_eddy.restype = C.c_double
@@ -270,3 +273,97 @@ def test_sweep_row_forms_laminar(L, pkg):
             L.pc_jac_rows(ivp, dp(dvp), dp(pv), dp(d13), 0.0, vol, dp(n3), sg, sgn, dp(x), dp(yr))
             L.pc_jac_coef(ivp, dp(dvp), dp(pv), dp(d13), 0.0, vol, dp(n3), sg, sgn, dp(x), dp(yc))
             assert np.array_equal(yr[:7], yc[:7])
+
+
+def _cav_state(rng, pww, pref, branch, dp=(50.0, 3e3), y1=(1e-4, 0.05)):
+    """A cell state on the condensing (p > p_sat) or vaporising side."""
+    d = rng.uniform(*dp) * (1 if branch == "cond" else -1)
+    return np.array([pww + d - pref, rng.uniform(-10, 10), rng.uniform(-2, 2), rng.uniform(-2, 2),
+                     0.0, rng.uniform(*y1), rng.choice([0.0, 1e-7, rng.uniform(0, 1e-6)]),
+                     rng.uniform(1e-4, 0.1), rng.uniform(10, 1e4)])
+
+
+@pytest.mark.parametrize("ncav", [2, 3, 4, 5])
+@pytest.mark.parametrize("clip", [0, 1])
+def test_cavitation_models_bit_exact(L, OL, orc, pkg, ncav, clip):
+    """Kunz, Singhal, Schnerr-Sauer and Lee: the device source compiled for the
+    host against the oracle, on both sides of saturation."""
+    case = pkg.cases.config2(8, 6, 6)
+    case.params.update(ncav=ncav, cav_clip=clip, c_c=3.0, c_v=5.0)
+    cfg = case.cfg_dict()
+    iv, dvp = H.par_arrays(cfg)
+    cf = cfg_struct(orc, cfg)
+    ivp = H.p(iv, C.c_int)
+    rng = np.random.default_rng(100 + ncav + 10 * clip)
+    for n in range(300):
+        t = rng.uniform(280.0, 330.0)
+        pv = _cav_state(rng, orc.h2o_pww(t), cfg["pref"], "cond" if n % 2 else "vap")
+        pv[4] = t
+        dvv, _ = orc.deteos_prop(pv[0] + cfg["pref"], t, pv[5], pv[6])
+        vol = rng.uniform(1e-9, 1e-6)
+        x7 = np.ascontiguousarray(np.tile(pv, (7, 1)))
+        grd = np.array([vol, 0.0, 0.0, 0.0, 1e-3])
+        mo = np.zeros(6)
+        L.pc_cav_model(ivp, dp(dvp), H.p(pv), dp(dvv), vol, dp(mo))
+        rc, rv, ic = C.c_double(), C.c_double(), np.zeros(4)
+        OL.orc_cavsource(C.byref(cf), dp(x7), dp(dvv), dp(grd), C.byref(rc), C.byref(rv), dp(ic))
+        # the rate clip as written (on T) makes r_v negative everywhere, so the
+        # vaporising linearisation runs on the condensing side too; where that
+        # takes a negative number to the power -0.5 both sides give NaN
+        assert np.array_equal(mo, [rc.value, rv.value, *ic], equal_nan=True), (n, mo, rc.value, rv.value, ic)
+    for t in (273.15, 300.0, 373.15, 500.0):
+        assert L.pc_h2o_sigma(t) == OL.orc_h2o_sigma(t)
+    for p in (2e3, 1.01325e5, 1e6, 1e7):
+        assert L.pc_sat_t(p) == orc.h2o_tww(p)
+
+
+# (model, branch) -> the state components whose coupling entry is the exact
+# derivative of r_v - r_c (the others drop the saturation-line or
+# surface-tension slope, src/cav_module.f90:157-168, 196-207, 240-257, 296-313, 338-354)
+_EXACT = {(1, "cond"): (0, 2, 3), (1, "vap"): (0, 2, 3),
+          (2, "cond"): (0, 1, 2, 3),
+          (3, "cond"): (0, 2, 3), (3, "vap"): (0, 2, 3),
+          (4, "cond"): (0, 2, 3), (4, "vap"): (0, 2, 3),
+          (5, "cond"): (1, 2, 3), (5, "vap"): (1, 2, 3)}
+
+
+@pytest.mark.parametrize("ncav,branch", sorted(_EXACT))
+def test_cavitation_linearisation_is_the_rate_derivative(OL, orc, pkg, ncav, branch):
+    """icav = d(r_v - r_c)/d(p, T, y1, y2) per unit volume, flipped by the
+    sign of its y1 entry: central differences with the EOS re-evaluated,
+    for every entry the model linearises exactly (Kunz with y2 = 0 and only
+    where condensation alone acts)."""
+    case = pkg.cases.config2(8, 6, 6)
+    k = {1: 0.05, 2: 1e-3, 3: 0.05, 4: 1e-6, 5: 0.05}[ncav]     # rates well inside the clip
+    case.params.update(ncav=ncav, cav_clip=1, c_c=k, c_v=k)
+    cfg = case.cfg_dict()
+    cf = cfg_struct(orc, cfg)
+    rng = np.random.default_rng(7 * ncav + (branch == "vap"))
+
+    def rates(pv):
+        dvv, _ = orc.deteos_prop(pv[0] + cfg["pref"], pv[4], pv[5], pv[6])
+        x7 = np.ascontiguousarray(np.tile(pv, (7, 1)))
+        rc, rv, ic = C.c_double(), C.c_double(), np.zeros(4)
+        OL.orc_cavsource(C.byref(cf), dp(x7), dp(dvv), dp(np.array([1.0, 0, 0, 0, 1e-3])), C.byref(rc),
+                         C.byref(rv), dp(ic))
+        return rv.value - rc.value, ic, dvv
+
+    for _ in range(20):
+        t = rng.uniform(285.0, 320.0)
+        pv = _cav_state(rng, orc.h2o_pww(t), cfg["pref"], branch, dp=(20.0, 300.0), y1=(1e-6, 1e-5))
+        pv[4] = t
+        if ncav == 2:
+            pv[6] = 0.0
+        f0, ic, dvv = rates(pv)
+        assert f0 != 0.0
+        assert (-f0 if branch == "cond" else f0) < dvv[0] * (pv[5] if branch == "cond" else 1 - pv[5] - pv[6])
+        deriv = np.zeros(4)
+        for q, idx in enumerate((0, 4, 5, 6)):
+            h = 1e-6 * max(abs(pv[idx]), (1e3, 1.0, 1e-3, 1e-3)[q])
+            a, b = pv.copy(), pv.copy()
+            a[idx] += h
+            b[idx] -= h
+            deriv[q] = (rates(a)[0] - rates(b)[0]) / (2 * h)
+        s = 1.0 if deriv[2] >= 0 else -1.0
+        for q in _EXACT[(ncav, branch)]:
+            assert ic[q] == pytest.approx(s * deriv[q], rel=2e-5, abs=1e-9 * np.abs(deriv).max()), (q, ic, deriv)
