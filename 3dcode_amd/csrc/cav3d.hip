@@ -511,7 +511,9 @@ __device__ __forceinline__ long stencil_off(const BlkD &B, int ll)
 
 // Inviscid flux of every face of direction D (src/rhs_module.f90:282-337 and
 // the j/k analogues): reconstruction, clamps, face EOS, Roe / AUSM+-up.
-template <int D>
+// SENSOR: the Roe-M / AUSMPW+ instantiation (18 stencil pressures, face
+// sound speeds); the other schemes run the lighter one.
+template <int D, bool SENSOR>
 __global__ __launch_bounds__(128) void k_face(DS S)
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -532,10 +534,19 @@ __global__ __launch_bounds__(128) void k_face(DS S)
     const long gq = g9 + sN;
     double n3[3];
     face_vec(S, B, D, i, j, k, n3);
-    double rh9[2] = {S.dc[D_RHO * S.NG + g9], S.dc[D_H * S.NG + g9]};
-    double rh10[2] = {S.dc[D_RHO * S.NG + gq], S.dc[D_H * S.NG + gq]};
+    double rh9[3] = {S.dc[D_RHO * S.NG + g9], S.dc[D_H * S.NG + g9], 0.0};
+    double rh10[3] = {S.dc[D_RHO * S.NG + gq], S.dc[D_H * S.NG + gq], 0.0};
     double fx[NPV];
-    inviscid_face<double>(P, npv, n3, c9, c10, xl, xr, rh9, rh10, fx);
+    if constexpr (SENSOR) {
+        // the pressure sensor of Roe-M / AUSMPW+: x(1:18,1) of the face stencil
+        double sd[18];
+        for (int ll = 1; ll <= 18; ll++) sd[ll - 1] = f0[stencil_off<D>(B, ll)];
+        rh9[2] = S.dc[D_C2 * S.NG + g9];
+        rh10[2] = S.dc[D_C2 * S.NG + gq];
+        inviscid_face<double, true>(P, npv, n3, c9, c10, xl, xr, rh9, rh10, fx, sd);
+    } else {
+        inviscid_face<double, false>(P, npv, n3, c9, c10, xl, xr, rh9, rh10, fx);
+    }
     double *out = S.ff[D];
     const long NF = S.NF[D];
     for (int n = 0; n < NPV; n++) if (n < npv) out[n * NF + t] = fx[n];
@@ -624,6 +635,9 @@ __device__ __forceinline__ long fidx(const BlkD &B, int D, int i, int j, int k)
 }
 
 // flux differences in the reference's order, then the cell sources
+// MODELS: the instantiation with the Kunz / Singhal / Schnerr-Sauer / Lee
+// sources (a full EOS per cell); Merkle and no cavitation run the lighter one.
+template <bool MODELS>
 __global__ __launch_bounds__(256) void k_combine(DS S)
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -653,12 +667,12 @@ __global__ __launch_bounds__(256) void k_combine(DS S)
     double vol = S.vol[widx(B, i, j, k)];
     if (P.ncav) {
         double rc, rv, ic[4];
-        if (P.ncav == 1) {
-            merkle(P, pc, d, vol, rc, rv, ic);
-        } else {
+        if constexpr (MODELS) {
             double dd[ND], full[18];
             mixture_eos<double>(pc[0] + P.pref, pc[4], pc[5], pc[6], dd, false, full);
             cav_model(P, pc, full, vol, rc, rv, ic);
+        } else {
+            merkle(P, pc, d, vol, rc, rv, ic);
         }
         r[5] = r[5] + rv - rc;
         for (int q = 0; q < 4; q++) S.icav[q * S.NC + t] = ic[q];
@@ -1555,15 +1569,22 @@ int launch_setbc(cav3d_ctx *c)
 int launch_rhs(cav3d_ctx *c)
 {
     DS &S = c->S;
-    hipLaunchKernelGGL(k_face<0>, dim3(nblk(S.NF[0], 128)), dim3(128), 0, c->st, S);
-    hipLaunchKernelGGL(k_face<1>, dim3(nblk(S.NF[1], 128)), dim3(128), 0, c->st, S);
-    hipLaunchKernelGGL(k_face<2>, dim3(nblk(S.NF[2], 128)), dim3(128), 0, c->st, S);
+    if (S.P.nscheme == 2 || S.P.nscheme == 3) {
+        hipLaunchKernelGGL((k_face<0, true>), dim3(nblk(S.NF[0], 128)), dim3(128), 0, c->st, S);
+        hipLaunchKernelGGL((k_face<1, true>), dim3(nblk(S.NF[1], 128)), dim3(128), 0, c->st, S);
+        hipLaunchKernelGGL((k_face<2, true>), dim3(nblk(S.NF[2], 128)), dim3(128), 0, c->st, S);
+    } else {
+        hipLaunchKernelGGL((k_face<0, false>), dim3(nblk(S.NF[0], 128)), dim3(128), 0, c->st, S);
+        hipLaunchKernelGGL((k_face<1, false>), dim3(nblk(S.NF[1], 128)), dim3(128), 0, c->st, S);
+        hipLaunchKernelGGL((k_face<2, false>), dim3(nblk(S.NF[2], 128)), dim3(128), 0, c->st, S);
+    }
     if (c->prm.iturb >= -2) {
         hipLaunchKernelGGL(k_vface<0>, dim3(nblk(S.NF[0], 128)), dim3(128), 0, c->st, S);
         hipLaunchKernelGGL(k_vface<1>, dim3(nblk(S.NF[1], 128)), dim3(128), 0, c->st, S);
         hipLaunchKernelGGL(k_vface<2>, dim3(nblk(S.NF[2], 128)), dim3(128), 0, c->st, S);
     }
-    hipLaunchKernelGGL(k_combine, dim3(nblk(S.NC, 128)), dim3(128), 0, c->st, S);
+    if (S.P.ncav >= 2) hipLaunchKernelGGL(k_combine<true>, dim3(nblk(S.NC, 128)), dim3(128), 0, c->st, S);
+    else hipLaunchKernelGGL(k_combine<false>, dim3(nblk(S.NC, 128)), dim3(128), 0, c->st, S);
     return 0;
 }
 
@@ -1638,7 +1659,8 @@ int cav3d_create(const cav3d_params *prm, int nblocks, const cav3d_block *blocks
     if (nblocks < 1 || nblocks > MAXB) return fail("nblocks must be 1.." + std::to_string(MAXB), -1);
     if (prm->iturb != 0 && prm->iturb != -2 && prm->iturb != -3)
         return fail("iturb must be 0 (k-omega SST), -2 (laminar) or -3 (Euler)", -1);
-    if (prm->nscheme != 1 && prm->nscheme != 4) return fail("nscheme must be 1 (Roe) or 4 (AUSM+-up)", -1);
+    if (prm->nscheme < 1 || prm->nscheme > 4)
+        return fail("nscheme must be 1 (Roe), 2 (Roe-M), 3 (AUSMPW+) or 4 (AUSM+-up)", -1);
     if (prm->nmuscl < 0 || prm->nmuscl > 2 || prm->nlim < 0 || prm->nlim > 10) return fail("bad nmuscl/nlim", -1);
     if (prm->ncav < 0 || prm->ncav > 5)
         return fail("ncav must be 0 (none), 1 (Merkle), 2 (Kunz), 3 (Singhal), 4 (Schnerr-Sauer) or 5 (Lee)", -1);

@@ -396,6 +396,14 @@ template <class R> HD R sndp2(const Par &P, R snd2, R u2)
     return mn(snd2, mx(mx(u2, ur2), R(P.str) * R(P.str)));
 }
 
+// the dissipation's speed of sound: steady preconditioning whenever any is on
+// (getsndp2_du, src/flux_module.f90:107-116)
+template <class R> HD R sndp2_du(const Par &P, R snd2, R u2)
+{
+    if (P.prec == 0) return snd2;
+    return mn(snd2, mx(u2, R(P.uref) * R(P.uref)));
+}
+
 // ------------------------------------------------------------ reconstruction
 // x[ll][n]: 38-point stencil (ll = Fortran index - 1)
 template <class R> struct LimState { R r[2], r1[2], r2[2], alp[2]; };
@@ -497,7 +505,7 @@ template <class R> HD void muscl_clamps(int npv, R pref, const R *c9, const R *c
 // ------------------------------------------------------------ inviscid flux
 // pl/pr primitive face states (pl[0] gauge p), rl/rr densities, hl/hr static
 // enthalpies, c2m the mixture c^2 at the Roe-averaged state.
-template <class R> struct FaceState { R p[NPV]; R rho, h; };
+template <class R> struct FaceState { R p[NPV]; R rho, h, c2; };
 
 template <class R> HD void roe_flux(const Par &P, int npv, const R *n3, const FaceState<R> &L,
                                     const FaceState<R> &Rt, R c2avg, R *fx)
@@ -595,6 +603,164 @@ template <class R> HD void ausm_flux(const Par &P, int npv, const R *n3, const F
     for (int k = 5; k < NPV; k++) if (k < npv) fx[k] = (ml * pl[k] + mr * pr[k]) * dl;
 }
 
+// Shock-sensor pressures sd[18] (gauge, the face's 18-point stencil), lifted
+// by p_ref and a tenth of the smaller rho*c^2; the smallest of the ratios the
+// Roe-M and AUSMPW+ fluxes compare (src/flux_module.f90:375-386, 499-508)
+template <class R> HD void press_ratios(const Par &P, const FaceState<R> &L, const FaceState<R> &Rt, const R *sd,
+                                        R &m910, R &mall)
+{
+    R lift = R(0.1) * mn(L.rho * L.c2, Rt.rho * Rt.c2);
+    R s[18];
+#pragma unroll
+    for (int k = 0; k < 18; k++) s[k] = sd[k] + R(P.pref) + lift;
+    // 1-based pairs of the reference: (9,10) (11,9) (9,7) (9,3) (9,15) (10,12) (10,8) (4,10) (10,16)
+    const int a[9] = {8, 10, 8, 8, 8, 9, 9, 3, 9}, b[9] = {9, 8, 6, 2, 14, 11, 7, 9, 15};
+    m910 = mn(s[8] / s[9], s[9] / s[8]);
+    mall = m910;
+#pragma unroll
+    for (int q = 1; q < 9; q++) mall = mn(mall, mn(s[a[q]] / s[b[q]], s[b[q]] / s[a[q]]));
+}
+
+// Roe-M (nscheme 2), src/flux_module.f90:278-418
+template <class R> HD void roem_flux(const Par &P, int npv, const R *n3, const FaceState<R> &L, const FaceState<R> &Rt,
+                                     R c2avg, const R *sd, R *fx)
+{
+    R dl = sqrt(n3[0] * n3[0] + n3[1] * n3[1] + n3[2] * n3[2]);
+    if (dl == R(0)) { for (int k = 0; k < NPV; k++) if (k < npv) fx[k] = 0; return; }
+    R nx = n3[0] / dl, ny = n3[1] / dl, nz = n3[2] / dl;
+    const R *pl = L.p, *pr = Rt.p;
+    R uull = nx * pl[1] + ny * pl[2] + nz * pl[3];
+    R uurr = nx * pr[1] + ny * pr[2] + nz * pr[3];
+    R sl = sqrt(L.rho), sr = sqrt(Rt.rho);
+    R ravd = R(1) / (sl + sr);
+    R ra[NPV];
+#pragma unroll
+    for (int k = 1; k < NPV; k++) if (k != 4 && k < npv) ra[k] = (sl * pl[k] + sr * pr[k]) * ravd;
+    R hl = L.h + R(0.5) * (pl[1] * pl[1] + pl[2] * pl[2] + pl[3] * pl[3]);
+    R hr = Rt.h + R(0.5) * (pr[1] * pr[1] + pr[2] * pr[2] + pr[3] * pr[3]);
+    R raht = (sl * hl + sr * hr) * ravd;
+    R u2 = ra[1] * ra[1] + ra[2] * ra[2] + ra[3] * ra[3];
+    R rr = sqrt(L.rho * Rt.rho);
+    R uuu = nx * ra[1] + ny * ra[2] + nz * ra[3];
+    R am2du = sndp2_du(P, c2avg, u2) / c2avg;
+    R am2 = sndp2(P, c2avg, u2) / c2avg;
+    R fmid = sqrt(am2) * (R(2) - sqrt(am2));
+    R fmdu = sqrt(am2du) * (R(2) - sqrt(am2du));
+    R amid = sqrt(c2avg);
+    R b1 = mx(mx(R(0), uuu + amid), uurr + amid);
+    R b2 = mn(mn(R(0), uuu - amid), uull - amid);
+    R b1du = mx(mx(R(0), uuu + fmdu * amid), uurr + fmdu * amid);
+    R b2du = mn(mn(R(0), uuu - fmdu * amid), uull - fmdu * amid);
+    R sgn = signbit(uuu) ? R(-1) : R(1);
+    R mt = sgn * mn(fabs(uuu) / amid, R(1));
+    R mtdu = sgn * mn(fabs(uuu) / (fmdu * amid), R(1));
+    R m910, mall;
+    press_ratios<R>(P, L, Rt, sd, m910, mall);
+    R ff = R(1) - mall, gg = R(1) - m910;
+    ff = uuu != R(0) ? pow(fabs(uuu) / amid, ff) : R(1);
+    gg = uuu != R(0) ? pow(fabs(uuu) / amid, gg) : R(1);
+    R add = R(1) + fabs(mt), aaa = R(1) + fabs(mtdu);
+    R dq[NPV], fl[NPV], fr[NPV], bdq[NPV];
+    dq[0] = Rt.rho - L.rho;
+    fl[0] = L.rho * uull;
+    fr[0] = Rt.rho * uurr;
+#pragma unroll
+    for (int k = 1; k < NPV; k++) if (k < npv) {
+        R ql = k == 4 ? hl : pl[k], qr = k == 4 ? hr : pr[k];
+        dq[k] = Rt.rho * qr - L.rho * ql;
+        fl[k] = L.rho * uull * ql;
+        fr[k] = Rt.rho * uurr * qr;
+    }
+    fl[1] = fl[1] + nx * (pl[0] + R(P.pref)); fr[1] = fr[1] + nx * (pr[0] + R(P.pref));
+    fl[2] = fl[2] + ny * (pl[0] + R(P.pref)); fr[2] = fr[2] + ny * (pr[0] + R(P.pref));
+    fl[3] = fl[3] + nz * (pl[0] + R(P.pref)); fr[3] = fr[3] + nz * (pr[0] + R(P.pref));
+    bdq[0] = dq[0] - ff * (pr[0] - pl[0]) / c2avg / fmid;
+#pragma unroll
+    for (int k = 1; k < NPV; k++) if (k < npv)
+        bdq[k] = k == 4 ? bdq[0] * raht + rr * (hr - hl) : bdq[0] * ra[k] + rr * (pr[k] - pl[k]);
+#pragma unroll
+    for (int k = 0; k < NPV; k++) if (k < npv)
+        fx[k] = (b1 * fl[k] - b2 * fr[k] + b1 * b2 * (dq[k] - gg * bdq[k] / add)) / (b1 - b2) * dl;
+    R du = gg * b1du * b2du * rr * (uurr - uull);
+    fx[1] = fx[1] + du * nx / aaa / (b1du - b2du) * dl;
+    fx[2] = fx[2] + du * ny / aaa / (b1du - b2du) * dl;
+    fx[3] = fx[3] + du * nz / aaa / (b1du - b2du) * dl;
+}
+
+// AUSMPW+ (nscheme 3), src/flux_module.f90:420-546
+template <class R> HD void ausmpwp_flux(const Par &P, int npv, const R *n3, const FaceState<R> &L,
+                                        const FaceState<R> &Rt, R c2avg, const R *sd, R *fx)
+{
+    const R beta = R(0.125), ku = R(1);
+    R dl = sqrt(n3[0] * n3[0] + n3[1] * n3[1] + n3[2] * n3[2]);
+    if (dl == R(0)) { for (int k = 0; k < NPV; k++) if (k < npv) fx[k] = 0; return; }
+    R nx = n3[0] / dl, ny = n3[1] / dl, nz = n3[2] / dl;
+    const R *pl = L.p, *pr = Rt.p;
+    R uull = nx * pl[1] + ny * pl[2] + nz * pl[3];
+    R uurr = nx * pr[1] + ny * pr[2] + nz * pr[3];
+    R sl = sqrt(L.rho), sr = sqrt(Rt.rho);
+    R ravd = R(1) / (sl + sr);
+    R ua = (sl * pl[1] + sr * pr[1]) * ravd, va = (sl * pl[2] + sr * pr[2]) * ravd, wa = (sl * pl[3] + sr * pr[3]) * ravd;
+    R rr = sqrt(L.rho * Rt.rho);
+    R amid = sqrt(c2avg);
+    R zmr = uurr / amid, zml = uull / amid;
+    R am2 = ua * ua + va * va + wa * wa;
+    R am2r1 = sndp2_du(P, c2avg, am2) / c2avg;
+    R am2r = sndp2(P, c2avg, am2) / c2avg;
+    R fmid = sqrt(am2r) * (R(2) - sqrt(am2r));
+    R fmid1 = sqrt(am2r1) * (R(2) - sqrt(am2r1));
+    R alpha = R(0.1875) * (R(-4) + R(5) * (fmid1 * fmid1));
+    alpha = mx(mn(alpha, R(0.1875)), R(-0.75));
+    R zmmr, pmr, zmpl, ppl;
+    if (fabs(zmr) >= R(1)) {
+        zmmr = R(0.5) * (zmr - fabs(zmr));
+        pmr = R(0.5) * (R(1) - zmr / fabs(zmr));
+    } else {
+        R t1 = (zmr - R(1)) * (zmr - R(1)), t2 = (zmr * zmr - R(1)) * (zmr * zmr - R(1));
+        zmmr = -(R(0.25) * t1) - beta * t2;
+        pmr = R(0.25) * t1 * (R(2) + zmr) - alpha * zmr * t2;
+    }
+    if (fabs(zml) >= R(1)) {
+        zmpl = R(0.5) * (zml + fabs(zml));
+        ppl = R(0.5) * (R(1) + zml / fabs(zml));
+    } else {
+        R t1 = (zml + R(1)) * (zml + R(1)), t2 = (zml * zml - R(1)) * (zml * zml - R(1));
+        zmpl = R(0.25) * t1 + beta * t2;
+        ppl = R(0.25) * t1 * (R(2) - zml) + alpha * zml * t2;
+    }
+    R zmid = zmpl + zmmr;
+    R pmid = ppl * (pl[0] + R(P.pref)) + pmr * (pr[0] + R(P.pref)) - R(2) * ku * ppl * pmr * rr * fmid1 * amid * (uurr - uull);
+    R m910, mall;
+    press_ratios<R>(P, L, Rt, sd, m910, mall);
+    R ww1 = R(1) - m910 * m910 * m910;
+    R ww2 = R(1) - mall * mall;
+    R ww = mx(ww1, ww2);
+    R pmt = rr * c2avg;
+    R pwl = ((pl[0] + R(P.pref) + rr * c2avg) / pmt - R(1)) * (R(1) - ww) / fmid;
+    R pwr = ((pr[0] + R(P.pref) + rr * c2avg) / pmt - R(1)) * (R(1) - ww) / fmid;
+    R zmpl1, zmmr1;
+    if (zmid >= R(0)) {
+        pwl = pwl * rr / L.rho;
+        pwr = pwr * rr / L.rho;
+        zmpl1 = zmpl + zmmr * ((R(1) - ww) * (R(1) + pwr) - pwl);
+        zmmr1 = zmmr * ww * (R(1) + pwr);
+    } else {
+        pwl = pwl * rr / Rt.rho;
+        pwr = pwr * rr / Rt.rho;
+        zmpl1 = zmpl * ww * (R(1) + pwl);
+        zmmr1 = zmmr + zmpl * ((R(1) - ww) * (R(1) + pwl) - pwr);
+    }
+    R hl = L.h + R(0.5) * (pl[1] * pl[1] + pl[2] * pl[2] + pl[3] * pl[3]);
+    R hr = Rt.h + R(0.5) * (pr[1] * pr[1] + pr[2] * pr[2] + pr[3] * pr[3]);
+    fx[0] = (zmpl1 * amid * L.rho + zmmr1 * amid * Rt.rho) * dl;
+    fx[1] = (zmpl1 * amid * L.rho * pl[1] + zmmr1 * amid * Rt.rho * pr[1] + pmid * nx) * dl;
+    fx[2] = (zmpl1 * amid * L.rho * pl[2] + zmmr1 * amid * Rt.rho * pr[2] + pmid * ny) * dl;
+    fx[3] = (zmpl1 * amid * L.rho * pl[3] + zmmr1 * amid * Rt.rho * pr[3] + pmid * nz) * dl;
+    fx[4] = (zmpl1 * amid * L.rho * hl + zmmr1 * amid * Rt.rho * hr) * dl;
+#pragma unroll
+    for (int k = 5; k < NPV; k++) if (k < npv) fx[k] = (zmpl1 * amid * L.rho * pl[k] + zmmr1 * amid * Rt.rho * pr[k]) * dl;
+}
+
 // ------------------------------------------------------------ one face
 // Reconstruction of every variable at a face from its 38-point stencil,
 // xs(ll, n) = stencil value at Fortran index ll of variable n
@@ -625,25 +791,38 @@ HD void face_recon(const Par &P, int npv, X xs, R *c9, R *c10, R *xl, R *xr, R g
 // Inviscid flux of a face from the reconstructed states: clamps, face EOS,
 // Roe-averaged c^2 and the Roe / AUSM+-up flux (src/rhs_module.f90:314-337).
 // rh9/rh10 = {rho, h} of the two cells (first-order faces read the cells').
-template <class R>
+// SENSOR: Roe-M / AUSMPW+ (they read the 18 stencil pressures sd and the
+// face sound speeds); otherwise Roe / AUSM+-up.
+template <class R, bool SENSOR = false>
 HD void inviscid_face(const Par &P, int npv, const R *n3, const R *c9, const R *c10, R *xl, R *xr,
-                      const R *rh9, const R *rh10, R *fx)
+                      const R *rh9, const R *rh10, R *fx, const R *sd = nullptr)
 {
     FaceState<R> L, Rt;
+    constexpr bool sensor = SENSOR;
     if (P.nmuscl > 0) {
         muscl_clamps<R>(npv, R(P.pref), c9, c10, xl, xr);
         face_rho_h<R>(xl[0] + R(P.pref), xl[4], xl[5], xl[6], L.rho, L.h);
         face_rho_h<R>(xr[0] + R(P.pref), xr[4], xr[5], xr[6], Rt.rho, Rt.h);
+        if (sensor) {
+            L.c2 = face_c2<R>(xl[0] + R(P.pref), xl[4], xl[5], xl[6]);
+            Rt.c2 = face_c2<R>(xr[0] + R(P.pref), xr[4], xr[5], xr[6]);
+        }
     } else {
         L.rho = rh9[0]; L.h = rh9[1];
         Rt.rho = rh10[0]; Rt.h = rh10[1];
+        if (sensor) { L.c2 = rh9[2]; Rt.c2 = rh10[2]; }
     }
     for (int n = 0; n < NPV; n++) if (n < npv) { L.p[n] = xl[n]; Rt.p[n] = xr[n]; }
     R sl = sqrt(L.rho), sr = sqrt(Rt.rho), rd = R(1) / (sl + sr);
     R ya1 = (sl * xl[5] + sr * xr[5]) * rd, ya2 = (sl * xl[6] + sr * xr[6]) * rd;
     R c2a = face_c2<R>(R(0.5) * (xl[0] + xr[0]) + R(P.pref), R(0.5) * (xl[4] + xr[4]), ya1, ya2);
-    if (P.nscheme == 1) roe_flux<R>(P, npv, n3, L, Rt, c2a, fx);
-    else ausm_flux<R>(P, npv, n3, L, Rt, c2a, fx);
+    if constexpr (SENSOR) {
+        if (P.nscheme == 2) roem_flux<R>(P, npv, n3, L, Rt, c2a, sd, fx);
+        else ausmpwp_flux<R>(P, npv, n3, L, Rt, c2a, sd, fx);
+    } else {
+        if (P.nscheme == 1) roe_flux<R>(P, npv, n3, L, Rt, c2a, fx);
+        else ausm_flux<R>(P, npv, n3, L, Rt, c2a, fx);
+    }
 }
 
 // ------------------------------------------------------------ Jacobian / getx

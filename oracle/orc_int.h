@@ -10,6 +10,10 @@ void orc_tvd(const orc_cfg *c, const double (*x)[ORC_NPV], double *xl, double *x
 void orc_mlp(const orc_cfg *c, const double (*x)[ORC_NPV], double *xl, double *xr, int *flags);
 void orc_roe(const orc_cfg *c, const double *n3, const double *pvl, const double *pvr,
              const double *dvl, const double *dvr, double *fx);
+void orc_roem(const orc_cfg *c, const double *n3, const double *pvl, const double *pvr,
+              const double *dvl, const double *dvr, const double *sd, double *fx);
+void orc_ausmpwp(const orc_cfg *c, const double *n3, const double *pvl, const double *pvr,
+                 const double *dvl, const double *dvr, const double *sd, double *fx);
 void orc_ausmpup(const orc_cfg *c, const double *n3, const double *pvl, const double *pvr,
                  const double *dvl, const double *dvr, double *fx, int *flag);
 void orc_vsflux(const orc_cfg *c, const double *nx, const double *const e[4], const double *const t[4],

@@ -251,6 +251,168 @@ void orc_roe(const orc_cfg *c, const double *n3, const double *pvl, const double
         fx[k] = 0.5 * (dvl[0] * uull * pvl[k] + dvr[0] * uurr * pvr[k] - df[k]) * dl;
 }
 
+/* getsndp2_du: no preconditioning, or the steady form for prec 1 and 2
+   (src/flux_module.f90:107-116) */
+static double sndp2_du(const orc_cfg *c, double snd2, double uuu2)
+{
+    if (c->prec == 0) return snd2;
+    return fmin_f(snd2, fmax_f(uuu2, c->uref * c->uref));
+}
+
+/* the lifted stencil pressures and their two ratio minima of Roe-M and
+   AUSMPW+ (src/flux_module.f90:375-386, 499-508); sd = x(1:18,1) */
+static void sensor(const orc_cfg *c, const double *dvl, const double *dvr, const double *sd, double *m910, double *mall)
+{
+    double s[19];
+    for (int k = 1; k <= 18; k++) s[k] = sd[k - 1] + c->pref + 0.1 * fmin_f(dvl[0] * dvl[5], dvr[0] * dvr[5]);
+    double r[18] = {s[9] / s[10], s[10] / s[9], s[11] / s[9], s[9] / s[11], s[9] / s[7], s[7] / s[9],
+                    s[9] / s[3], s[3] / s[9], s[9] / s[15], s[15] / s[9], s[10] / s[12], s[12] / s[10],
+                    s[10] / s[8], s[8] / s[10], s[4] / s[10], s[10] / s[4], s[10] / s[16], s[16] / s[10]};
+    *m910 = fmin_f(r[0], r[1]);
+    double m = r[0];
+    for (int k = 1; k < 18; k++) m = fmin_f(m, r[k]);
+    *mall = m;
+}
+
+/* Roe-M, src/flux_module.f90:278-418 */
+void orc_roem(const orc_cfg *c, const double *n3, const double *pvl, const double *pvr,
+              const double *dvl, const double *dvr, const double *sd, double *fx)
+{
+    int npv = c->npv;
+    double dl = sqrt(n3[0] * n3[0] + n3[1] * n3[1] + n3[2] * n3[2]);
+    if (dl == 0.0) { for (int k = 0; k < npv; k++) fx[k] = 0.0; return; }
+    double nx = n3[0] / dl, ny = n3[1] / dl, nz = n3[2] / dl;
+    double uull = nx * pvl[1] + ny * pvl[2] + nz * pvl[3];
+    double uurr = nx * pvr[1] + ny * pvr[2] + nz * pvr[3];
+    double ravg[ORC_NPV], rdv[18];
+    double ravg_d = 1.0 / (sqrt(dvl[0]) + sqrt(dvr[0]));
+    ravg[0] = 0.5 * (pvl[0] + pvr[0]);
+    ravg[4] = 0.5 * (pvl[4] + pvr[4]);
+    for (int k = 1; k <= 3; k++) ravg[k] = (sqrt(dvl[0]) * pvl[k] + sqrt(dvr[0]) * pvr[k]) * ravg_d;
+    for (int k = 5; k < npv; k++) ravg[k] = (sqrt(dvl[0]) * pvl[k] + sqrt(dvr[0]) * pvr[k]) * ravg_d;
+    double hl = enthalpy(pvl, dvl), hr = enthalpy(pvr, dvr);
+    double ravg_ht = (sqrt(dvl[0]) * hl + sqrt(dvr[0]) * hr) * ravg_d;
+    orc_deteos_simple(ravg[0] + c->pref, ravg[4], ravg[5], ravg[6], rdv);
+    double u2 = ravg[1] * ravg[1] + ravg[2] * ravg[2] + ravg[3] * ravg[3];
+    rdv[0] = sqrt(dvl[0] * dvr[0]);
+    double uuu = nx * ravg[1] + ny * ravg[2] + nz * ravg[3];
+    double am2rmid_du = sndp2_du(c, rdv[5], u2) / rdv[5];
+    double am2rmid = orc_sndp2(c, rdv[5], u2) / rdv[5];
+    double fmid = sqrt(am2rmid) * (2.0 - sqrt(am2rmid));
+    double fmid_du = sqrt(am2rmid_du) * (2.0 - sqrt(am2rmid_du));
+    double amid = sqrt(rdv[5]);
+    double b1 = fmax_f(fmax_f(0.0, uuu + amid), uurr + amid);
+    double b2 = fmin_f(fmin_f(0.0, uuu - amid), uull - amid);
+    double b1_du = fmax_f(fmax_f(0.0, uuu + fmid_du * amid), uurr + fmid_du * amid);
+    double b2_du = fmin_f(fmin_f(0.0, uuu - fmid_du * amid), uull - fmid_du * amid);
+    double m_tilde = copysign(1.0, uuu) * fmin_f(fabs(uuu) / amid, 1.0);
+    double m_tilde_du = copysign(1.0, uuu) * fmin_f(fabs(uuu) / (fmid_du * amid), 1.0);
+    double dqp[ORC_NPV], dq[ORC_NPV], fl[ORC_NPV], fr[ORC_NPV], bdq[ORC_NPV];
+    for (int k = 0; k < npv; k++) dqp[k] = pvr[k] - pvl[k];
+    dq[0] = dvr[0] - dvl[0];
+    for (int k = 1; k <= 3; k++) dq[k] = dvr[0] * pvr[k] - dvl[0] * pvl[k];
+    dq[4] = dvr[0] * hr - dvl[0] * hl;
+    for (int k = 5; k < npv; k++) dq[k] = dvr[0] * pvr[k] - dvl[0] * pvl[k];
+    const double nn[3] = {nx, ny, nz};
+    fl[0] = dvl[0] * uull;
+    fr[0] = dvr[0] * uurr;
+    for (int k = 1; k <= 3; k++) {
+        fl[k] = dvl[0] * uull * pvl[k] + nn[k - 1] * (pvl[0] + c->pref);
+        fr[k] = dvr[0] * uurr * pvr[k] + nn[k - 1] * (pvr[0] + c->pref);
+    }
+    fl[4] = dvl[0] * uull * hl;
+    fr[4] = dvr[0] * uurr * hr;
+    for (int k = 5; k < npv; k++) { fl[k] = dvl[0] * uull * pvl[k]; fr[k] = dvr[0] * uurr * pvr[k]; }
+    double m910, mall;
+    sensor(c, dvl, dvr, sd, &m910, &mall);
+    double ff = 1.0 - mall, gg = 1.0 - m910;
+    ff = (uuu != 0.0) ? pow(fabs(uuu) / amid, ff) : 1.0;
+    gg = (uuu != 0.0) ? pow(fabs(uuu) / amid, gg) : 1.0;
+    double add = 1.0 + fabs(m_tilde), aaa = 1.0 + fabs(m_tilde_du);
+    bdq[0] = dq[0] - ff * dqp[0] / rdv[5] / fmid;
+    for (int k = 1; k <= 3; k++) bdq[k] = bdq[0] * ravg[k] + rdv[0] * dqp[k];
+    bdq[4] = bdq[0] * ravg_ht + rdv[0] * (hr - hl);
+    for (int k = 5; k < npv; k++) bdq[k] = bdq[0] * ravg[k] + rdv[0] * dqp[k];
+    for (int k = 0; k < npv; k++)
+        fx[k] = (b1 * fl[k] - b2 * fr[k] + b1 * b2 * (dq[k] - gg * bdq[k] / add)) / (b1 - b2) * dl;
+    for (int k = 1; k <= 3; k++)
+        fx[k] = fx[k] + gg * b1_du * b2_du * rdv[0] * (uurr - uull) * nn[k - 1] / aaa / (b1_du - b2_du) * dl;
+}
+
+/* AUSMPW+, src/flux_module.f90:420-546 */
+void orc_ausmpwp(const orc_cfg *c, const double *n3, const double *pvl, const double *pvr,
+                 const double *dvl, const double *dvr, const double *sd, double *fx)
+{
+    const double beta = 0.125, ku = 1.0;
+    int npv = c->npv;
+    double dl = sqrt(n3[0] * n3[0] + n3[1] * n3[1] + n3[2] * n3[2]);
+    if (dl == 0.0) { for (int k = 0; k < npv; k++) fx[k] = 0.0; return; }
+    double nx = n3[0] / dl, ny = n3[1] / dl, nz = n3[2] / dl;
+    double uull = nx * pvl[1] + ny * pvl[2] + nz * pvl[3];
+    double uurr = nx * pvr[1] + ny * pvr[2] + nz * pvr[3];
+    double ravg[ORC_NPV], rdv[18];
+    double ravg_d = 1.0 / (sqrt(dvl[0]) + sqrt(dvr[0]));
+    ravg[0] = 0.5 * (pvl[0] + pvr[0]);
+    ravg[4] = 0.5 * (pvl[4] + pvr[4]);
+    for (int k = 1; k <= 3; k++) ravg[k] = (sqrt(dvl[0]) * pvl[k] + sqrt(dvr[0]) * pvr[k]) * ravg_d;
+    for (int k = 5; k < npv; k++) ravg[k] = (sqrt(dvl[0]) * pvl[k] + sqrt(dvr[0]) * pvr[k]) * ravg_d;
+    orc_deteos_simple(ravg[0] + c->pref, ravg[4], ravg[5], ravg[6], rdv);
+    rdv[0] = sqrt(dvl[0] * dvr[0]);
+    double amid = sqrt(rdv[5]);
+    double zmr = uurr / amid, zml = uull / amid;
+    double am2mid = ravg[1] * ravg[1] + ravg[2] * ravg[2] + ravg[3] * ravg[3];
+    double am2rmid1 = sndp2_du(c, rdv[5], am2mid) / rdv[5];
+    double am2rmid = orc_sndp2(c, rdv[5], am2mid) / rdv[5];
+    double fmid = sqrt(am2rmid) * (2.0 - sqrt(am2rmid));
+    double fmid1 = sqrt(am2rmid1) * (2.0 - sqrt(am2rmid1));
+    double alpha = 3.0 / 16.0 * (-4.0 + 5.0 * (fmid1 * fmid1));
+    alpha = fmax_f(fmin_f(alpha, 3.0 / 16.0), -3.0 / 4.0);
+    double zmmr, pmr, zmpl, ppl;
+    if (fabs(zmr) >= 1.0) {
+        zmmr = 0.5 * (zmr - fabs(zmr));
+        pmr = 0.5 * (1.0 - zmr / fabs(zmr));
+    } else {
+        zmmr = -(0.25 * ((zmr - 1.0) * (zmr - 1.0))) - beta * ((zmr * zmr - 1.0) * (zmr * zmr - 1.0));
+        pmr = 0.25 * ((zmr - 1.0) * (zmr - 1.0)) * (2.0 + zmr) - alpha * zmr * ((zmr * zmr - 1.0) * (zmr * zmr - 1.0));
+    }
+    if (fabs(zml) >= 1.0) {
+        zmpl = 0.5 * (zml + fabs(zml));
+        ppl = 0.5 * (1.0 + zml / fabs(zml));
+    } else {
+        zmpl = 0.25 * ((zml + 1.0) * (zml + 1.0)) + beta * ((zml * zml - 1.0) * (zml * zml - 1.0));
+        ppl = 0.25 * ((zml + 1.0) * (zml + 1.0)) * (2.0 - zml) + alpha * zml * ((zml * zml - 1.0) * (zml * zml - 1.0));
+    }
+    double zmid = zmpl + zmmr;
+    double pmid = ppl * (pvl[0] + c->pref) + pmr * (pvr[0] + c->pref) - 2.0 * ku * ppl * pmr * rdv[0] * fmid1 * amid * (uurr - uull);
+    double m910, mall;
+    sensor(c, dvl, dvr, sd, &m910, &mall);
+    double ww1 = 1.0 - (m910 * m910) * m910;
+    double ww2 = 1.0 - mall * mall;
+    double ww = fmax_f(ww1, ww2);
+    double pmt = rdv[0] * rdv[5];
+    double pwl = ((pvl[0] + c->pref + rdv[0] * rdv[5]) / pmt - 1.0) * (1.0 - ww) / fmid;
+    double pwr = ((pvr[0] + c->pref + rdv[0] * rdv[5]) / pmt - 1.0) * (1.0 - ww) / fmid;
+    double zmpl1, zmmr1;
+    if (zmid >= 0.0) {
+        pwl = pwl * rdv[0] / dvl[0];
+        pwr = pwr * rdv[0] / dvl[0];
+        zmpl1 = zmpl + zmmr * ((1.0 - ww) * (1.0 + pwr) - pwl);
+        zmmr1 = zmmr * ww * (1.0 + pwr);
+    } else {
+        pwl = pwl * rdv[0] / dvr[0];
+        pwr = pwr * rdv[0] / dvr[0];
+        zmpl1 = zmpl * ww * (1.0 + pwl);
+        zmmr1 = zmmr + zmpl * ((1.0 - ww) * (1.0 + pwl) - pwr);
+    }
+    double hl = enthalpy(pvl, dvl), hr = enthalpy(pvr, dvr);
+    fx[0] = (zmpl1 * amid * dvl[0] + zmmr1 * amid * dvr[0]) * dl;
+    fx[1] = (zmpl1 * amid * dvl[0] * pvl[1] + zmmr1 * amid * dvr[0] * pvr[1] + pmid * nx) * dl;
+    fx[2] = (zmpl1 * amid * dvl[0] * pvl[2] + zmmr1 * amid * dvr[0] * pvr[2] + pmid * ny) * dl;
+    fx[3] = (zmpl1 * amid * dvl[0] * pvl[3] + zmmr1 * amid * dvr[0] * pvr[3] + pmid * nz) * dl;
+    fx[4] = (zmpl1 * amid * dvl[0] * hl + zmmr1 * amid * dvr[0] * hr) * dl;
+    for (int k = 5; k < npv; k++) fx[k] = (zmpl1 * amid * dvl[0] * pvl[k] + zmmr1 * amid * dvr[0] * pvr[k]) * dl;
+}
+
 /* src/flux_module.f90:548-641; *flag (if non-NULL) = 1 when zmid > 0 */
 void orc_ausmpup(const orc_cfg *c, const double *n3, const double *pvl, const double *pvr,
                  const double *dvl, const double *dvr, double *fx, int *flag)

@@ -541,7 +541,11 @@ static void face_flux(const orc_ctx *c, const double (*x)[ORC_NPV], const double
         memcpy(dvr, dvc_r, sizeof(dvr));
     }
     (void)grdl; (void)grdr;
+    double sd[18];
+    for (int q = 0; q < 18; q++) sd[q] = x[q][0];      /* x(1:18,1), src/rhs_module.f90:336 */
     if (g->nscheme == 1) orc_roe(g, n3, pvl, pvr, dvl, dvr, fx);
+    else if (g->nscheme == 2) orc_roem(g, n3, pvl, pvr, dvl, dvr, sd, fx);
+    else if (g->nscheme == 3) orc_ausmpwp(g, n3, pvl, pvr, dvl, dvr, sd, fx);
     else orc_ausmpup(g, n3, pvl, pvr, dvl, dvr, fx, NULL);
 }
 

@@ -52,9 +52,11 @@ void pc_face(const int *iv, const double *dv, const double *x, const double *n3,
              const double *rh10, double *fx, double *xl, double *xr, double *g3)
 {
     Par P = mkpar(iv, dv);
-    double c9[NPV], c10[NPV], g[6][3] = {};
+    double c9[NPV], c10[NPV], g[6][3] = {}, sd[18];
+    for (int ll = 1; ll <= 18; ll++) sd[ll - 1] = x[(ll - 1) * 9];
     face_recon<double>(P, P.npv, [&](int ll, int n) { return x[(ll - 1) * 9 + n]; }, c9, c10, xl, xr, g);
-    inviscid_face<double>(P, P.npv, n3, c9, c10, xl, xr, rh9, rh10, fx);
+    if (P.nscheme == 2 || P.nscheme == 3) inviscid_face<double, true>(P, P.npv, n3, c9, c10, xl, xr, rh9, rh10, fx, sd);
+    else inviscid_face<double, false>(P, P.npv, n3, c9, c10, xl, xr, rh9, rh10, fx);
     for (int q = 0; q < 18; q++) g3[q] = g[q / 3][q % 3];
 }
 

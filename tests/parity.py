@@ -18,6 +18,9 @@ def small_cases(pk):
         "sst_cav_bdf1": lambda: _clip(bc(12, 8, 6, True, True, 4, 2, 8, cfl=5.0, nsteady=1, dt_phy=2e-4, prec=2,
                                          seed=8), 1),
         "sst_cav_3block": lambda: _clip(pk.cases.split_i(pk.cases.config2(14, 8, 6), [5, 10]), 1),
+        "sst_cav_roem_mlp": lambda: _clip(bc(12, 8, 6, True, True, 2, 2, 8, cfl=3.0, seed=11), 1),
+        "lam_ausmpwp_tvd": lambda: bc(12, 8, 6, False, False, 3, 1, 2, cfl=5.0, seed=12),
+        "sst_cav_ausmpwp_first": lambda: _clip(bc(12, 8, 6, True, True, 3, 0, 0, cfl=3.0, seed=13), 1),
         "sst_kunz": lambda: _model(pk.cases.config2(12, 8, 6), 2),
         "sst_singhal": lambda: _model(pk.cases.config2(12, 8, 6), 3),
         "sst_schnerr_sauer": lambda: _model(pk.cases.config2(12, 8, 6), 4),

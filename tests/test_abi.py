@@ -56,10 +56,14 @@ def test_create_without_device_fails_cleanly(pkg):
 
 def test_create_rejects_unsupported_options(pkg):
     case = pkg.cases.config1(8, 6, 6)
-    case.params["nscheme"] = 2
+    case.params["nscheme"] = 5
     with pytest.raises(pkg.Cav3dError, match="nscheme"):
         pkg.Solver(case)
     case.params["nscheme"] = 1
+    case.params["ncav"] = 6
+    with pytest.raises(pkg.Cav3dError, match="ncav"):
+        pkg.Solver(case)
+    case.params["ncav"] = 0
     case.params["dt_phy"] = 0.0
     with pytest.raises(pkg.Cav3dError, match="dt_phy"):
         pkg.Solver(case)

@@ -27,6 +27,8 @@ def OL(orc):
     L.orc_mlp.argtypes = [C.c_void_p, D, D, D, C.c_void_p]
     L.orc_roe.argtypes = [C.c_void_p, D, D, D, D, D, D]
     L.orc_ausmpup.argtypes = [C.c_void_p, D, D, D, D, D, D, C.c_void_p]
+    L.orc_roem.argtypes = [C.c_void_p, D, D, D, D, D, D, D]
+    L.orc_ausmpwp.argtypes = [C.c_void_p, D, D, D, D, D, D, D]
     L.orc_localdt.argtypes = [C.c_void_p, PP, D, D, D, D]
     L.orc_localdt.restype = C.c_double
     L.orc_getx.argtypes = [C.c_void_p, PP, D, D, D, D, C.c_double, D, D, D, D]
@@ -61,13 +63,22 @@ def phys_flux(orc, pref, q, n3, npv):
     return f
 
 
-def flux(OL, cf, orc, cfg, nscheme, ql, qr, n3):
+def flux(OL, cf, orc, cfg, nscheme, ql, qr, n3, sd=None):
+    """sd: the 18 stencil pressures of Roe-M / AUSMPW+ (default: the left
+    and right pressures on the ii = 0 / 1 points)"""
     dl = orc.deteos_simple(ql[0] + cfg["pref"], ql[4], ql[5], ql[6])
     dr = orc.deteos_simple(qr[0] + cfg["pref"], qr[4], qr[5], qr[6])
     f = np.zeros(9)
     ql, qr, n3 = (np.ascontiguousarray(a, dtype=np.float64) for a in (ql, qr, n3))
+    if sd is None:
+        sd = np.array([ql[0], qr[0]] * 9)
+    sd = np.ascontiguousarray(sd, dtype=np.float64)
     if nscheme == 1:
         OL.orc_roe(C.byref(cf), dp(n3), dp(ql), dp(qr), dp(dl), dp(dr), dp(f))
+    elif nscheme == 2:
+        OL.orc_roem(C.byref(cf), dp(n3), dp(ql), dp(qr), dp(dl), dp(dr), dp(sd), dp(f))
+    elif nscheme == 3:
+        OL.orc_ausmpwp(C.byref(cf), dp(n3), dp(ql), dp(qr), dp(dl), dp(dr), dp(sd), dp(f))
     else:
         OL.orc_ausmpup(C.byref(cf), dp(n3), dp(ql), dp(qr), dp(dl), dp(dr), dp(f), None)
     return f
@@ -83,7 +94,7 @@ def rand_state(rng, npv, vapour=False):
     return q
 
 
-@pytest.mark.parametrize("nscheme", [1, 4])
+@pytest.mark.parametrize("nscheme", [1, 2, 3, 4])
 @pytest.mark.parametrize("prec", [0, 1])
 def test_flux_consistency(orc, pkg, OL, nscheme, prec):
     """F(q, q, n) is the physical flux (the dissipation vanishes)."""
@@ -97,28 +108,33 @@ def test_flux_consistency(orc, pkg, OL, nscheme, prec):
         scale = np.abs(ref) + abs(q[0] + cfg["pref"]) * np.linalg.norm(n3) * 1e-3
         # AUSM+-up builds the mass flux from split-Mach polynomials whose terms are
         # O(1) while M ~ 1e-2: rounding is amplified by 1/M
-        tol = 1e-12 if nscheme == 1 else 1e-10
+        tol = 1e-12 if nscheme in (1, 2) else 1e-10
         assert np.all(np.abs(f - ref) <= tol * scale + 1e-300), (f, ref)
 
 
-@pytest.mark.parametrize("nscheme", [1, 4])
+@pytest.mark.parametrize("nscheme", [1, 2, 3, 4])
 def test_flux_antisymmetry(orc, pkg, OL, nscheme):
-    """F(L, R, n) = -F(R, L, -n)."""
+    """F(L, R, n) = -F(R, L, -n), the sensor stencil mirrored with the states
+    (its ii = 0 and ii = 1 points exchanged)."""
     case, cfg, cf = cfg_of(orc, pkg)
     rng = np.random.default_rng(20 + nscheme)
     for _ in range(100):
         ql, qr = rand_state(rng, 7), rand_state(rng, 7)
         qr[5] = ql[5]
         n3 = rng.standard_normal(3) * 1e-3
-        f1 = flux(OL, cf, orc, cfg, nscheme, ql, qr, n3)[:7]
-        f2 = flux(OL, cf, orc, cfg, nscheme, qr, ql, -n3)[:7]
+        sd = np.array([ql[0], qr[0]] * 9) + rng.uniform(-500, 500, 18)
+        sm = sd.reshape(9, 2)[:, ::-1].ravel()
+        f1 = flux(OL, cf, orc, cfg, nscheme, ql, qr, n3, sd)[:7]
+        f2 = flux(OL, cf, orc, cfg, nscheme, qr, ql, -n3, sm)[:7]
         scale = np.abs(f1) + (abs(ql[0]) + abs(qr[0]) + cfg["pref"]) * np.linalg.norm(n3)
         assert np.all(np.abs(f1 + f2) <= 1e-12 * scale), (f1, f2)
 
 
-def test_ausm_supersonic_upwind(orc, pkg, OL):
-    """Both sides supersonic toward +n: AUSM+-up returns the left physical flux."""
-    case, cfg, cf = cfg_of(orc, pkg, nscheme=4)
+@pytest.mark.parametrize("nscheme", [2, 3, 4])
+def test_supersonic_upwind(orc, pkg, OL, nscheme):
+    """Both sides supersonic toward +n: Roe-M, AUSMPW+ and AUSM+-up return
+    the left physical flux."""
+    case, cfg, cf = cfg_of(orc, pkg, nscheme=nscheme)
     rng = np.random.default_rng(3)
     for _ in range(50):
         # pure vapour at low pressure, u well above its sound speed (~430 m/s)
@@ -127,7 +143,7 @@ def test_ausm_supersonic_upwind(orc, pkg, OL):
         qr[1] += rng.uniform(0, 100)
         qr[0] += rng.uniform(-100, 100)
         n3 = np.array([1e-3, 0, 0])
-        f = flux(OL, cf, orc, cfg, 4, ql, qr, n3)[:7]
+        f = flux(OL, cf, orc, cfg, nscheme, ql, qr, n3)[:7]
         ref = phys_flux(orc, cfg["pref"], ql, n3, 7)
         assert np.allclose(f, ref, rtol=1e-12, atol=1e-14 * np.max(np.abs(ref)))
 

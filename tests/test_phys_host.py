@@ -36,6 +36,8 @@ def OL(orc):
     L.orc_mlp.argtypes = [C.c_void_p, D, D, D, C.c_void_p]
     L.orc_roe.argtypes = [C.c_void_p, D, D, D, D, D, D]
     L.orc_ausmpup.argtypes = [C.c_void_p, D, D, D, D, D, D, C.c_void_p]
+    L.orc_roem.argtypes = [C.c_void_p, D, D, D, D, D, D, D]
+    L.orc_ausmpwp.argtypes = [C.c_void_p, D, D, D, D, D, D, D]
     L.orc_vsflux.argtypes = [C.c_void_p, D, PP, PP, D, D, D, D, D, D, D, D]
     L.orc_localdt.argtypes = [C.c_void_p, PP, D, D, D, D]
     L.orc_localdt.restype = C.c_double
@@ -95,13 +97,15 @@ def _stencil(rng, P4, npv, amp):
     return x
 
 
-SCHEMES = [(1, 0, 0), (1, 1, 1), (1, 2, 8), (4, 1, 2), (4, 2, 8), (4, 1, 5), (1, 2, 3)]
+SCHEMES = [(1, 0, 0), (1, 1, 1), (1, 2, 8), (4, 1, 2), (4, 2, 8), (4, 1, 5), (1, 2, 3),
+           (2, 0, 0), (2, 1, 1), (2, 2, 8), (3, 0, 0), (3, 1, 2), (3, 2, 8)]
 
 
 @pytest.mark.parametrize("cfgname", ["config1", "config2"])
 @pytest.mark.parametrize("nscheme,nmuscl,nlim", SCHEMES)
 def test_inviscid_face_bit_exact(L, OL, orc, pkg, cfgname, nscheme, nmuscl, nlim):
-    """Reconstruction + clamps + face EOS + Roe/AUSM+-up (src/rhs_module.f90:314-337)."""
+    """Reconstruction + clamps + face EOS + Roe / Roe-M / AUSMPW+ / AUSM+-up
+    (src/rhs_module.f90:314-337)."""
     case = getattr(pkg.cases, cfgname)(10, 8, 8)
     case.params.update(nscheme=nscheme, nmuscl=nmuscl, nlim=nlim)
     cfg = case.cfg_dict()
@@ -116,7 +120,8 @@ def test_inviscid_face_bit_exact(L, OL, orc, pkg, cfgname, nscheme, nmuscl, nlim
         d9 = orc.deteos_simple(x[8, 0] + cfg["pref"], x[8, 4], x[8, 5], x[8, 6])
         d10 = orc.deteos_simple(x[9, 0] + cfg["pref"], x[9, 4], x[9, 5], x[9, 6])
         fx, xl, xr, g3 = np.zeros(9), np.zeros(9), np.zeros(9), np.zeros(18)
-        L.pc_face(H.p(iv, C.c_int), dp(dvp), H.p(x), dp(n3), H.p(d9[:2]), H.p(d10[:2]), dp(fx), dp(xl), dp(xr), dp(g3))
+        L.pc_face(H.p(iv, C.c_int), dp(dvp), H.p(x), dp(n3), H.p(d9[[0, 1, 5]]), H.p(d10[[0, 1, 5]]), dp(fx),
+                  dp(xl), dp(xr), dp(g3))
         ol, orr = np.zeros(9), np.zeros(9)
         if nmuscl == 1:
             OL.orc_tvd(C.byref(cf), dp(x), dp(ol), dp(orr))
@@ -130,8 +135,13 @@ def test_inviscid_face_bit_exact(L, OL, orc, pkg, cfgname, nscheme, nmuscl, nlim
         else:
             dl, dr = d9, d10
         fo = np.zeros(9)
+        sd = np.ascontiguousarray(x[:18, 0])
         if nscheme == 1:
             OL.orc_roe(C.byref(cf), dp(n3), dp(ol), dp(orr), dp(dl), dp(dr), dp(fo))
+        elif nscheme == 2:
+            OL.orc_roem(C.byref(cf), dp(n3), dp(ol), dp(orr), dp(dl), dp(dr), dp(sd), dp(fo))
+        elif nscheme == 3:
+            OL.orc_ausmpwp(C.byref(cf), dp(n3), dp(ol), dp(orr), dp(dl), dp(dr), dp(sd), dp(fo))
         else:
             OL.orc_ausmpup(C.byref(cf), dp(n3), dp(ol), dp(orr), dp(dl), dp(dr), dp(fo), None)
         n = case.npv
