@@ -65,7 +65,7 @@ class Case:
                 "uref", "str", "dt_phy", "c_v", "c_c", "dp_ref", "t_ref", "t_crit", "bc_pv", "bc_pressure",
                 "emutref", "cav_clip"]
         d = {k: p[k] for k in keys}
-        d.update(npv=self.npv, ntv=self.ntv, ndv=18, ngrd=self.ngrd, ides=0, timemethod=3, nprint=1)
+        d.update(npv=self.npv, ntv=self.ntv, ndv=18, ngrd=self.ngrd, ides=p.get("ides", 0), timemethod=3, nprint=1)
         return d
 
     @property

@@ -636,7 +636,8 @@ __device__ __forceinline__ long fidx(const BlkD &B, int D, int i, int j, int k)
 
 // flux differences in the reference's order, then the cell sources
 // MODELS: the instantiation with the Kunz / Singhal / Schnerr-Sauer / Lee
-// sources (a full EOS per cell); Merkle and no cavitation run the lighter one.
+// sources (a full EOS per cell) and the DES length scales; Merkle, no
+// cavitation and RANS SST run the lighter one.
 template <bool MODELS>
 __global__ __launch_bounds__(256) void k_combine(DS S)
 {
@@ -668,9 +669,13 @@ __global__ __launch_bounds__(256) void k_combine(DS S)
     if (P.ncav) {
         double rc, rv, ic[4];
         if constexpr (MODELS) {
-            double dd[ND], full[18];
-            mixture_eos<double>(pc[0] + P.pref, pc[4], pc[5], pc[6], dd, false, full);
-            cav_model(P, pc, full, vol, rc, rv, ic);
+            if (P.ncav >= 2) {
+                double dd[ND], full[18];
+                mixture_eos<double>(pc[0] + P.pref, pc[4], pc[5], pc[6], dd, false, full);
+                cav_model(P, pc, full, vol, rc, rv, ic);
+            } else {
+                merkle(P, pc, d, vol, rc, rv, ic);
+            }
         } else {
             merkle(P, pc, d, vol, rc, rv, ic);
         }
@@ -687,7 +692,7 @@ __global__ __launch_bounds__(256) void k_combine(DS S)
             for (int n = 0; n < NPV; n++) if (n < npv) x7[q][n] = S.pv[n * S.NG + gn];
         }
         cell_faces(S, B, i, j, k, f6);
-        sst_source(x7, f6, vol, S.ydn[widx(B, i, j, k)], d, S.mut[g], src, it, oc);
+        sst_source<MODELS>(P, x7, f6, vol, S.ydn[widx(B, i, j, k)], d, S.mut[g], src, it, oc);
         S.ocut[t] = oc;
         r[7] = r[7] + src[0];
         r[8] = r[8] + src[1];
@@ -1583,7 +1588,7 @@ int launch_rhs(cav3d_ctx *c)
         hipLaunchKernelGGL(k_vface<1>, dim3(nblk(S.NF[1], 128)), dim3(128), 0, c->st, S);
         hipLaunchKernelGGL(k_vface<2>, dim3(nblk(S.NF[2], 128)), dim3(128), 0, c->st, S);
     }
-    if (S.P.ncav >= 2) hipLaunchKernelGGL(k_combine<true>, dim3(nblk(S.NC, 128)), dim3(128), 0, c->st, S);
+    if (S.P.ncav >= 2 || S.P.ides > 0) hipLaunchKernelGGL(k_combine<true>, dim3(nblk(S.NC, 128)), dim3(128), 0, c->st, S);
     else hipLaunchKernelGGL(k_combine<false>, dim3(nblk(S.NC, 128)), dim3(128), 0, c->st, S);
     return 0;
 }
@@ -1662,6 +1667,7 @@ int cav3d_create(const cav3d_params *prm, int nblocks, const cav3d_block *blocks
     if (prm->nscheme < 1 || prm->nscheme > 4)
         return fail("nscheme must be 1 (Roe), 2 (Roe-M), 3 (AUSMPW+) or 4 (AUSM+-up)", -1);
     if (prm->nmuscl < 0 || prm->nmuscl > 2 || prm->nlim < 0 || prm->nlim > 10) return fail("bad nmuscl/nlim", -1);
+    if (prm->ides < 0 || prm->ides > 4) return fail("ides must be 0 (RANS), 1 (DES), 2 (DDES), 3 (IDDES) or 4 (S-IDDES)", -1);
     if (prm->ncav < 0 || prm->ncav > 5)
         return fail("ncav must be 0 (none), 1 (Merkle), 2 (Kunz), 3 (Singhal), 4 (Schnerr-Sauer) or 5 (Lee)", -1);
     if (prm->cav_clip != 0 && prm->cav_clip != 1) return fail("cav_clip must be 0 (as written) or 1 (species)", -1);
@@ -1681,6 +1687,7 @@ int cav3d_create(const cav3d_params *prm, int nblocks, const cav3d_block *blocks
     P.npv = c->npv; P.ntv = c->ntv; P.iturb = prm->iturb; P.nscheme = prm->nscheme; P.nmuscl = prm->nmuscl;
     P.nlim = prm->nlim; P.local = prm->local; P.prec = prm->prec; P.ncav = prm->ncav; P.nsteady = prm->nsteady;
     P.cav_clip = prm->cav_clip;
+    P.ides = prm->iturb == 0 ? prm->ides : 0;
     P.cfl = prm->cfl; P.pref = prm->pref; P.uref = prm->uref; P.str = prm->str; P.dt_phy = prm->dt_phy;
     P.c_v = prm->c_v; P.c_c = prm->c_c; P.dp_ref = prm->dp_ref; P.t_ref = prm->t_ref; P.t_crit = prm->t_crit;
     for (int q = 0; q < 9; q++) P.bc_pv[q] = prm->bc_pv[q];

@@ -33,7 +33,7 @@ constexpr int NPV = 9;
 enum { D_RHO = 0, D_H, D_C2, D_RP, D_RT, D_RY1, D_RY2, D_HP, D_HT, D_HY1, D_HY2, D_MU, D_KAP, ND };
 
 struct Par {
-    int npv, ntv, iturb, nscheme, nmuscl, nlim, local, prec, ncav, nsteady, cav_clip;
+    int npv, ntv, iturb, nscheme, nmuscl, nlim, local, prec, ncav, nsteady, cav_clip, ides;
     double cfl, pref, uref, str, dt_phy, c_v, c_c, dp_ref, t_ref, t_crit;
     double bc_pv[9], bc_pressure, emutref, bdf_b;
 };
@@ -1447,8 +1447,54 @@ HD void cav_model(const Par &P, const double *pc, const double *dv, double vol, 
     cav_flip(icav);
 }
 
-// k-omega SST source (ides = 0): src[2], itt[4], omega_cut
-HD void sst_source(const double x7[7][NPV], const double f6[6][3], double vol, double ydns, const double *d,
+// Turbulent length scale of the SST destruction term (calturblength,
+// src/turbsource_module.f90:406-576): RANS sqrt(k)/(9 omega)*100, or its DES
+// (1), DDES (2), IDDES (3) or S-IDDES (4) blend with the LES scale C_DES h_max,
+// h from the cell volume over its mean face areas.  Constant integer powers
+// as gfortran expands them (x**3 = (x*x)*x, x**10 = x5*x5 with x5 = x3*x2).
+HD double des_length(int ides, double k, double om, const double f6[6][3], double vol, double ydns, double mu,
+                     double mut, double bigf, double s2, double w2)
+{
+    double tl_rans = sqrt(k) / 9.0 / om * 100.0;
+    if (ides == 0) return tl_rans;
+    double gs[3];
+    for (int q = 0; q < 3; q++) {
+        const double *a = f6[2 * q], *b = f6[2 * q + 1];
+        gs[q] = 0.5 * (sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]) + sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]));
+        gs[q] = vol / gs[q];
+    }
+    double hmax = mx(mx(gs[0], gs[1]), gs[2]);
+    if (ides == 1) return mn(tl_rans, 0.61 * hmax);
+    double cdes = 0.78 * bigf + 0.61 * (1.0 - bigf);
+    double den = 0.41 * 0.41 * (ydns * ydns) * sqrt(0.5 * (s2 + w2));
+    if (ides == 2) {
+        double tl_les = cdes * hmax;
+        double rd = (mu + mut) / den, x = 20.0 * rd;
+        double fd = 1.0 - tanh(x * x * x);
+        return tl_rans - fd * mx(0.0, tl_rans - tl_les);
+    }
+    double tl_les = cdes * mn(0.15 * mx(ydns, hmax), hmax);
+    double alpha = 0.25 - ydns / hmax;
+    double fb = mn(2.0 * exp(-9.0 * (alpha * alpha)), 1.0);
+    double rdt = mut / den, xt = 20.0 * rdt;
+    double fdt = 1.0 - tanh(xt * xt * xt);
+    double fd = mx(1.0 - fdt, fb);
+    if (ides == 4) return fd * tl_rans + (1.0 - fd) * tl_les;
+    double rdl = mu / den;
+    double xl = 25.0 * rdl, xl2 = xl * xl, xl3 = xl2 * xl, xl5 = xl3 * xl2;
+    double fl = tanh(xl5 * xl5);
+    double x3 = 3.4969 * rdt;
+    double ft = tanh(x3 * x3 * x3);
+    double fe2 = 1.0 - mx(ft, fl);
+    double fe1 = alpha >= 0.0 ? 2.0 * exp(-11.09 * (alpha * alpha)) : 2.0 * exp(-9.0 * (alpha * alpha));
+    double fe = fe2 * mx(fe1 - 1.0, 0.0);
+    return fd * (1.0 + fe) * tl_rans + (1.0 - fd) * tl_les;
+}
+
+// k-omega SST source: src[2], itt[4], omega_cut.  DES: the length-scale
+// variants are compiled in (the RANS-only instantiation keeps its registers).
+template <bool DES = false>
+HD void sst_source(const Par &P, const double x7[7][NPV], const double f6[6][3], double vol, double ydns, const double *d,
                    double mut, double *src, double *itt, double &ocut)
 {
     const double c23 = 2.0 / 3.0, c59 = 5.0 / 9.0;
@@ -1477,7 +1523,14 @@ HD void sst_source(const double x7[7][NPV], const double f6[6][3], double vol, d
     double ta = bf * c59 + (1.0 - bf) * 0.44;
     double tb = bf * 0.075 + (1.0 - bf) * 0.0828;
     double ki = 1.0 / k, oi = 1.0 / om, ei = 1.0 / mut;
-    double dest = rho * pow(k, 1.5) / (sqrt(k) / 9.0 / om * 100.0);
+    double tl;
+    if constexpr (DES) {
+        double w2 = (gu[1] - gv[0]) * (gu[1] - gv[0]) + (gu[2] - gw[0]) * (gu[2] - gw[0]) + (gv[2] - gw[1]) * (gv[2] - gw[1]);
+        tl = des_length(P.ides, k, om, f6, vol, ydns, d[D_MU], mut, bf, s2, w2);
+    } else {
+        tl = sqrt(k) / 9.0 / om * 100.0;
+    }
+    double dest = rho * pow(k, 1.5) / tl;
     src[0] = (prod - dest) * vol;
     src[1] = (ta * rho * ei * prod - tb * rho * (om * om) + 1.712 * (1.0 - bf) * rho * oi * pkw) * vol;
     double dpdk = prod * ki;

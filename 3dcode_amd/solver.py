@@ -41,7 +41,7 @@ EXPORTS = [
 class Params(C.Structure):
     _fields_ = [(n, C.c_int) for n in ("iturb", "nscheme", "nmuscl", "nlim", "local", "prec", "ncav",
                                        "nsteady", "precision", "cav_clip")] + \
-        [("reserved", C.c_int * 2)] + \
+        [("ides", C.c_int), ("reserved", C.c_int)] + \
         [(n, C.c_double) for n in ("cfl", "pref", "uref", "str", "dt_phy", "c_v", "c_c", "dp_ref",
                                    "t_ref", "t_crit")] + \
         [("bc_pv", C.c_double * 9), ("bc_pressure", C.c_double), ("emutref", C.c_double)]

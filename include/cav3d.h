@@ -77,7 +77,7 @@ enum { CAV3D_CLIP_AS_WRITTEN = 0, CAV3D_CLIP_SPECIES = 1 };
 /* Run parameters: the config_module fields the path reads
  * (src/config_module.f90:7-28, 90-199), integer codes unchanged. */
 typedef struct {
-    int iturb;      /* -3 Euler, -2 laminar, 0 k-omega SST (ides = 0) */
+    int iturb;      /* -3 Euler, -2 laminar, 0 k-omega SST */
     int nscheme;    /* 1 preconditioned Roe, 4 AUSM+-up */
     int nmuscl;     /* 0 first order, 1 MUSCL-TVD, 2 MLP */
     int nlim;       /* limiter 0..10 */
@@ -87,7 +87,8 @@ typedef struct {
     int nsteady;    /* 0 steady, 1 BDF1, 2 BDF2 dual time */
     int precision;  /* CAV3D_FP64 or CAV3D_MIXED */
     int cav_clip;   /* CAV3D_CLIP_AS_WRITTEN or CAV3D_CLIP_SPECIES */
-    int reserved[2];
+    int ides;       /* SST length scale: 0 RANS, 1 DES, 2 DDES, 3 IDDES, 4 S-IDDES */
+    int reserved;
     double cfl, pref, uref, str, dt_phy, c_v, c_c;
     double dp_ref;  /* 2/(rho_ref*uref^2)  (src/cav_module.f90:81) */
     double t_ref;   /* uref/l_chord        (src/cav_module.f90:82) */

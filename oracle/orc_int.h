@@ -26,6 +26,8 @@ void orc_cavsource(const orc_cfg *c, const double (*x)[ORC_NPV], const double *d
 double orc_h2o_sigma(double t);
 void orc_kwsst_source(const double *const f[6], const double *grd, const double (*x)[ORC_NPV],
                       const double *dv, const double *tv, double *source, double *itt, double *omega_cut);
+void orc_kwsst_source_des(int ides, const double *const f[6], const double *grd, const double (*x)[ORC_NPV],
+                      const double *dv, const double *tv, double *source, double *itt, double *omega_cut);
 double orc_kwsst_eddy(const double *const f[6], const double *grd, const double (*x)[ORC_NPV],
                       const double *dv, const double *tv);
 double orc_eigenvis(const orc_cfg *c, const double *dv, const double *tv);

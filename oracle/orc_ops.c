@@ -801,9 +801,68 @@ static void gg(const double (*x)[ORC_NPV], int n, const double *const f[6], doub
         g[m] = (a2 * f[1][m] + a4 * f[3][m] + a6 * f[5][m] - a1 * f[0][m] - a3 * f[2][m] - a5 * f[4][m]) * vol;
 }
 
+/* calturblength, src/turbsource_module.f90:406-576: no_des, sst_des,
+   sst_ddes, sst_iddes, sst_s_iddes (ides 0..4).  f: the cell's face
+   vectors cx1, cx2, ex1, ex2, tx1, tx2. */
+static double turb_length(int ides, const double *const f[6], const double *grd, const double (*x)[ORC_NPV],
+                          const double *tv, double bigf, double strain_sq, double vorticity_sq)
+{
+    double tl_rans = sqrt(X(2, 8)) / 9.0 / X(2, 9) * 100.0;
+    if (ides == 0) return tl_rans;
+    double gsp[3];
+    for (int q = 0; q < 3; q++) {
+        const double *a = f[2 * q], *b = f[2 * q + 1];
+        gsp[q] = 0.5 * (sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]) + sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]));
+    }
+    for (int q = 0; q < 3; q++) gsp[q] = grd[0] / gsp[q];
+    double hmax = fmax_f(fmax_f(gsp[0], gsp[1]), gsp[2]);
+    if (ides == 1) {
+        double tl_les = 0.61 * hmax;
+        return fmin_f(tl_rans, tl_les);
+    }
+    double cdes = 0.78 * bigf + 0.61 * (1.0 - bigf);
+    if (ides == 2) {
+        double tl_les = cdes * hmax;
+        double rd_num = tv[0] + tv[2];
+        double rd_den = 0.41 * 0.41 * (grd[4] * grd[4]) * sqrt(0.5 * (strain_sq + vorticity_sq));
+        double rd = rd_num / rd_den;
+        double y = 20.0 * rd;
+        double fd = 1.0 - tanh((y * y) * y);
+        return tl_rans - fd * fmax_f(0.0, tl_rans - tl_les);
+    }
+    double tl_les = cdes * fmin_f(0.15 * fmax_f(grd[4], hmax), hmax);
+    double alpha = 0.25 - grd[4] / hmax;
+    double fb = fmin_f(2.0 * exp(-9.0 * (alpha * alpha)), 1.0);
+    double rdt_den = 0.41 * 0.41 * (grd[4] * grd[4]) * sqrt(0.5 * (strain_sq + vorticity_sq));
+    double rdt = tv[2] / rdt_den;
+    double y = 20.0 * rdt;
+    double fdt = 1.0 - tanh((y * y) * y);
+    double fd = fmax_f(1.0 - fdt, fb);
+    if (ides == 4) return fd * tl_rans + (1.0 - fd) * tl_les;
+    double rdl = tv[0] / rdt_den;
+    double z = 25.0 * rdl, z2 = z * z, z3 = z2 * z, z5 = z3 * z2;
+    double fl = tanh(z5 * z5);
+    double w = 3.4969 * rdt;
+    double ft = tanh((w * w) * w);
+    double fe2 = 1.0 - fmax_f(ft, fl);
+    double fe1 = (alpha >= 0.0) ? 2.0 * exp(-11.09 * (alpha * alpha)) : 2.0 * exp(-9.0 * (alpha * alpha));
+    double fe = fe2 * fmax_f(fe1 - 1.0, 0.0);
+    return fd * (1.0 + fe) * tl_rans + (1.0 - fd) * tl_les;
+}
+
+void orc_kwsst_source_des(int ides, const double *const f[6], const double *grd, const double (*x)[ORC_NPV],
+                          const double *dv, const double *tv, double *source, double *itt, double *omega_cut);
+
 /* src/turbsource_module.f90:269-414 (ides = 0) */
 void orc_kwsst_source(const double *const f[6], const double *grd, const double (*x)[ORC_NPV],
                       const double *dv, const double *tv, double *source, double *itt, double *omega_cut)
+{
+    orc_kwsst_source_des(0, f, grd, x, dv, tv, source, itt, omega_cut);
+}
+
+/* src/turbsource_module.f90:269-383 with the length scale of ides */
+void orc_kwsst_source_des(int ides, const double *const f[6], const double *grd, const double (*x)[ORC_NPV],
+                          const double *dv, const double *tv, double *source, double *itt, double *omega_cut)
 {
     const double c23 = 2.0 / 3.0, c59 = 5.0 / 9.0;
     double vol = 1.0 / grd[0];
@@ -836,7 +895,8 @@ void orc_kwsst_source(const double *const f[6], const double *grd, const double 
     double talpha = bigf * c59 + (1.0 - bigf) * 0.44;
     double tbeta = bigf * 0.075 + (1.0 - bigf) * 0.0828;
     double ki = 1.0 / k, oi = 1.0 / om, ei = 1.0 / tv[2];
-    double tl = sqrt(k) / 9.0 / om * 100.0;            /* no_des, :406-414 */
+    double vorticity_sq = (dudy - dvdx) * (dudy - dvdx) + (dudz - dwdx) * (dudz - dwdx) + (dvdz - dwdy) * (dvdz - dwdy);
+    double tl = turb_length(ides, f, grd, x, tv, bigf, strain_sq, vorticity_sq);
     double dest = dv[0] * pow(k, 1.5) / tl;
     source[0] = (prod - dest) * grd[0];
     source[1] = (talpha * dv[0] * ei * prod - tbeta * dv[0] * (om * om) + 1.712 * (1.0 - bigf) * dv[0] * oi * pkw) * grd[0];

@@ -712,7 +712,7 @@ static void calrhs_block(orc_ctx *c, blk *b)
             }
             if (g->iturb == 0) {
                 double src[2], it[4], oc;
-                orc_kwsst_source(f6, grd, (const double (*)[ORC_NPV])x, dv, tv, src, it, &oc);
+                orc_kwsst_source_des(g->ides, f6, grd, (const double (*)[ORC_NPV])x, dv, tv, src, it, &oc);
                 b->ocut[q] = oc;
                 r[7] = r[7] + src[0];
                 r[8] = r[8] + src[1];

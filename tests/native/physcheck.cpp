@@ -9,7 +9,7 @@ static Par mkpar(const int *iv, const double *dv)
 {
     Par P{};
     P.npv = iv[0]; P.ntv = iv[1]; P.iturb = iv[2]; P.nscheme = iv[3]; P.nmuscl = iv[4]; P.nlim = iv[5];
-    P.local = iv[6]; P.prec = iv[7]; P.ncav = iv[8]; P.nsteady = iv[9]; P.cav_clip = iv[10];
+    P.local = iv[6]; P.prec = iv[7]; P.ncav = iv[8]; P.nsteady = iv[9]; P.cav_clip = iv[10]; P.ides = iv[11];
     P.cfl = dv[0]; P.pref = dv[1]; P.uref = dv[2]; P.str = dv[3]; P.dt_phy = dv[4]; P.c_v = dv[5];
     P.c_c = dv[6]; P.dp_ref = dv[7]; P.t_ref = dv[8]; P.t_crit = dv[9];
     for (int q = 0; q < 9; q++) P.bc_pv[q] = dv[10 + q];
@@ -164,7 +164,22 @@ void pc_sst_source(const double *x7, const double *f6, double vol, double ydns, 
     for (int q = 0; q < 63; q++) x[q / 9][q % 9] = x7[q];
     for (int q = 0; q < 18; q++) f[q / 3][q % 3] = f6[q];
     double src[2], itt[4], oc;
-    sst_source(x, f, vol, ydns, d, mut, src, itt, oc);
+    Par P{};
+    sst_source<false>(P, x, f, vol, ydns, d, mut, src, itt, oc);
+    out[0] = src[0]; out[1] = src[1];
+    for (int q = 0; q < 4; q++) out[2 + q] = itt[q];
+    out[6] = oc;
+}
+
+void pc_sst_source_des(const int *iv, const double *dv, const double *x7, const double *f6, double vol, double ydns, const double *d, double mut,
+                   double *out)
+{
+    double x[7][NPV], f[6][3];
+    for (int q = 0; q < 63; q++) x[q / 9][q % 9] = x7[q];
+    for (int q = 0; q < 18; q++) f[q / 3][q % 3] = f6[q];
+    double src[2], itt[4], oc;
+    Par P = mkpar(iv, dv);
+    sst_source<true>(P, x, f, vol, ydns, d, mut, src, itt, oc);
     out[0] = src[0]; out[1] = src[1];
     for (int q = 0; q < 4; q++) out[2 + q] = itt[q];
     out[6] = oc;

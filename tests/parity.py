@@ -21,11 +21,35 @@ def small_cases(pk):
         "sst_cav_roem_mlp": lambda: _clip(bc(12, 8, 6, True, True, 2, 2, 8, cfl=3.0, seed=11), 1),
         "lam_ausmpwp_tvd": lambda: bc(12, 8, 6, False, False, 3, 1, 2, cfl=5.0, seed=12),
         "sst_cav_ausmpwp_first": lambda: _clip(bc(12, 8, 6, True, True, 3, 0, 0, cfl=3.0, seed=13), 1),
+        "sst_des": lambda: _des(pk.cases.config2(12, 8, 6), 1),
+        "sst_ddes": lambda: _des(pk.cases.config2(12, 8, 6), 2),
+        "sst_iddes": lambda: _des(pk.cases.config2(12, 8, 6), 3),
+        "sst_siddes": lambda: _des(pk.cases.config2(12, 8, 6), 4),
         "sst_kunz": lambda: _model(pk.cases.config2(12, 8, 6), 2),
         "sst_singhal": lambda: _model(pk.cases.config2(12, 8, 6), 3),
         "sst_schnerr_sauer": lambda: _model(pk.cases.config2(12, 8, 6), 4),
         "sst_lee": lambda: _model(pk.cases.config2(12, 8, 6), 5),
     }
+
+
+def _des(case, ides):
+    """the config-2 state with a DES length scale (species clip), made to
+    reach the LES branches: omega lowered 100-fold (the RANS length
+    sqrt(k)/(0.09 omega) exceeds the cell size), a shear u += 2 m/s per cell
+    across the channel and mu_t = 1e-6 mu (the shielding functions open)"""
+    case.params["ides"] = ides
+    case.params["cav_clip"] = 1
+    for q, (pv, dv, tv, qq) in enumerate(case.init):
+        b = case.blocks[q]
+        pv = pv.copy().reshape(b.kmax + 5, b.jmax + 5, b.imax + 5, case.npv)
+        pv[..., 8] *= 0.01
+        pv[..., 1] += 2.0 * np.arange(b.jmax + 5)[None, :, None]
+        if tv is not None:
+            tv = tv.copy().reshape(-1, case.ntv)
+            tv[:, 2] = 1e-6 * tv[:, 0]
+            tv = tv.ravel()
+        case.init[q] = (pv.ravel(), dv, tv, qq)
+    return case
 
 
 def _model(case, ncav):

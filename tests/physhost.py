@@ -29,6 +29,7 @@ def build(outdir):
     L.pc_getx_sel.argtypes = [_I, _D, _D, _D, C.c_double, C.c_double, _D, C.c_double, _D, _D, _D, _D]
     L.pc_merkle.argtypes = [_I, _D, _D, _D, C.c_double, _D]
     L.pc_cav_model.argtypes = [_I, _D, _D, _D, C.c_double, _D]
+    L.pc_sst_source_des.argtypes = [_I, _D, _D, _D, C.c_double, C.c_double, _D, C.c_double, _D]
     L.pc_sat_t.argtypes = [C.c_double]
     L.pc_sat_t.restype = C.c_double
     L.pc_h2o_sigma.argtypes = [C.c_double]
@@ -51,7 +52,8 @@ def p(a, t=C.c_double):
 def par_arrays(cfg):
     """int / double parameter vectors of pc_* from an oracle-style cfg dict."""
     iv = np.array([cfg["npv"], cfg["ntv"], cfg["iturb"], cfg["nscheme"], cfg["nmuscl"], cfg["nlim"], cfg["local"],
-                   cfg["prec"], cfg["ncav"], cfg["nsteady"], cfg.get("cav_clip", 0)], dtype=np.int32)
+                   cfg["prec"], cfg["ncav"], cfg["nsteady"], cfg.get("cav_clip", 0), cfg.get("ides", 0)],
+                  dtype=np.int32)
     bdf = 1.0 if cfg["nsteady"] == 1 else (0.75 if cfg["nsteady"] == 2 else 0.0)
     dv = np.array([cfg["cfl"], cfg["pref"], cfg["uref"], cfg["str"], cfg["dt_phy"], cfg["c_v"], cfg["c_c"],
                    cfg["dp_ref"], cfg["t_ref"], cfg["t_crit"], *list(cfg["bc_pv"]), cfg["bc_pressure"],

@@ -5,6 +5,7 @@ Every routine is required to agree bit for bit: the kernels follow the
 reference's operation order, so the only device-side differences left are
 the device math library's exp/pow/tanh roundings."""
 import ctypes as C
+import math
 import importlib
 
 import numpy as np
@@ -48,6 +49,7 @@ def OL(orc):
     L.orc_h2o_sigma.argtypes = [C.c_double]
     L.orc_h2o_sigma.restype = C.c_double
     L.orc_kwsst_source.argtypes = [PP, D, D, D, D, D, D, D]
+    L.orc_kwsst_source_des.argtypes = [C.c_int, PP, D, D, D, D, D, D, D]
     L.orc_kwsst_eddy.argtypes = [PP, D, D, D, D]
     L.orc_kwsst_eddy.restype = C.c_double
     return L
@@ -377,3 +379,74 @@ def test_cavitation_linearisation_is_the_rate_derivative(OL, orc, pkg, ncav, bra
         s = 1.0 if deriv[2] >= 0 else -1.0
         for q in _EXACT[(ncav, branch)]:
             assert ic[q] == pytest.approx(s * deriv[q], rel=2e-5, abs=1e-9 * np.abs(deriv).max()), (q, ic, deriv)
+
+
+@pytest.mark.parametrize("ides", [1, 2, 3, 4])
+def test_des_sources_bit_exact(L, OL, orc, pkg, ides):
+    """SST source with the DES / DDES / IDDES / S-IDDES length scale
+    (src/turbsource_module.f90:416-576): device code compiled for the host
+    against the oracle, near and far from the wall."""
+    case = pkg.cases.config2(8, 6, 6)
+    case.params.update(ides=ides)
+    cfg = case.cfg_dict()
+    iv, dvp = H.par_arrays(cfg)
+    ivp = H.p(iv, C.c_int)
+    rng = np.random.default_rng(300 + ides)
+    for n in range(300):
+        pv = np.array([rng.uniform(-3e4, 3e4), rng.uniform(-15, 15), rng.uniform(-3, 3), rng.uniform(-3, 3),
+                       rng.uniform(290, 310), 0.0, 0.0, rng.uniform(1e-4, 0.1), rng.uniform(10, 1e4)])
+        dvv, tv2 = orc.deteos_prop(pv[0] + cfg["pref"], pv[4], pv[5], pv[6])
+        mut = rng.uniform(1e-6, 1e-2)
+        tv = np.array([tv2[0], tv2[1], mut])
+        d13 = H.compact(dvv, tv2)
+        h = rng.uniform(1e-4, 1e-2)
+        vol = h ** 3 * rng.uniform(0.5, 2.0)
+        grd = np.array([vol, 0.1, 0.05, 0.02, h * rng.choice([0.05, 0.5, 5.0, 50.0])])
+        f6 = [np.eye(3)[q // 2] * h * h * rng.uniform(0.7, 1.3) + rng.standard_normal(3) * 1e-2 * h * h for q in range(6)]
+        f6a = np.concatenate(f6)
+        x7 = np.ascontiguousarray(np.tile(pv, (7, 1)) * (1 + 1e-2 * rng.standard_normal((7, 9))))
+        so = np.zeros(7)
+        L.pc_sst_source_des(ivp, dp(dvp), dp(x7), dp(f6a), vol, grd[4], dp(d13), mut, dp(so))
+        src, itt, oc = np.zeros(2), np.zeros(4), C.c_double()
+        OL.orc_kwsst_source_des(ides, ptrs(f6), dp(grd), dp(x7), dp(dvv), dp(tv), dp(src), dp(itt), C.byref(oc))
+        assert np.array_equal(so, [*src, *itt, oc.value]), (n, so, src, itt)
+
+
+def test_des_length_closed_forms(OL, orc, pkg):
+    """A cube of side h: DES destroys k with rho k^1.5 / min(l_RANS, 0.61 h);
+    with l_LES above l_RANS, DES and DDES fall back to RANS; far from the
+    wall (d > 0.53 h) IDDES and S-IDDES coincide."""
+    cfg = pkg.cases.config2(8, 6, 6).cfg_dict()
+    rng = np.random.default_rng(5)
+
+    def source(ides, pv, h, d, mut):
+        dvv, tv2 = orc.deteos_prop(pv[0] + cfg["pref"], pv[4], pv[5], pv[6])
+        tv = np.array([tv2[0], tv2[1], mut])
+        grd = np.array([h ** 3, 0, 0, 0, d])
+        f6 = [np.eye(3)[q // 2] * h * h for q in range(6)]
+        x7 = np.ascontiguousarray(np.tile(pv, (7, 1)) * (1 + 1e-3 * noise))
+        out = []
+        for i in (0, ides):
+            src, itt, oc = np.zeros(2), np.zeros(4), C.c_double()
+            OL.orc_kwsst_source_des(i, ptrs(f6), dp(grd), dp(x7), dp(dvv), dp(tv), dp(src), dp(itt), C.byref(oc))
+            out.append(src[0])
+        return out, dvv[0]
+
+    for _ in range(20):
+        noise = rng.standard_normal((7, 9))
+        noise[1] = 0.0                      # the cell itself keeps pv (its k, omega set l_RANS)
+        pv = np.array([0.0, rng.uniform(-5, 5), 0.5, 0.2, 300.0, 0.0, 0.0, rng.uniform(1e-3, 0.1), rng.uniform(10, 1e3)])
+        k, om = pv[7], pv[8]
+        l_rans = math.sqrt(k) / 9.0 / om * 100.0
+        h = rng.uniform(0.2, 5.0) * l_rans / 0.61          # l_LES on either side of l_RANS
+        (s0, s1), rho = source(1, pv, h, 0.1 * h, 1e-4)
+        vol = h ** 3
+        want = s0 + rho * k ** 1.5 * (1.0 / l_rans - 1.0 / min(l_rans, 0.61 * h)) * vol
+        assert s1 == pytest.approx(want, rel=1e-10, abs=1e-12 * abs(s0))
+        hbig = 50.0 * l_rans
+        for ides in (1, 2):
+            (a, b), _ = source(ides, pv, hbig, 5.0 * hbig, 1e-4)
+            assert b == pytest.approx(a, rel=1e-12), ides
+        (_, c3), _ = source(3, pv, h, 0.6 * h, 1e-4)
+        (_, c4), _ = source(4, pv, h, 0.6 * h, 1e-4)
+        assert c3 == c4
