@@ -512,8 +512,12 @@ __device__ __forceinline__ long stencil_off(const BlkD &B, int ll)
 // Inviscid flux of every face of direction D (src/rhs_module.f90:282-337 and
 // the j/k analogues): reconstruction, clamps, face EOS, Roe / AUSM+-up.
 // SENSOR: the Roe-M / AUSMPW+ instantiation (18 stencil pressures, face
-// sound speeds); the other schemes run the lighter one.
-template <int D, bool SENSOR>
+// sound speeds); the other schemes run the lighter one.  R: the arithmetic
+// of reconstruction, face EOS and flux -- double, or float for the mixed
+// precision mode (the state is read in FP64 and rounded once; gauge
+// pressure keeps the low-Mach pressure differences; the metastable-vapour
+// switch compares against the FP64 saturation pressure).
+template <int D, bool SENSOR, class R>
 __global__ __launch_bounds__(128) void k_face(DS S)
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -524,37 +528,38 @@ __global__ __launch_bounds__(128) void k_face(DS S)
     const int npv = S.npv;
     const long g9 = gidx(B, i, j, k);
     const long sN = D == 0 ? 1 : (D == 1 ? (long)B.nx : (long)B.nx * B.ny);
-    double c9[NPV], c10[NPV], xl[NPV], xr[NPV];
-    double g3[6][3];
+    R c9[NPV], c10[NPV], xl[NPV], xr[NPV];
+    R g3[6][3];
     const double *f0 = S.pv + g9;
     Par Pi = P;
     Pi.iturb = -3;          // no viscous differences here
-    face_recon<double>(Pi, npv, [&](int ll, int n) -> double { return f0[(long)n * S.NG + stencil_off<D>(B, ll)]; },
-                       c9, c10, xl, xr, g3);
+    face_recon<R>(Pi, npv, [&](int ll, int n) -> R { return R(f0[(long)n * S.NG + stencil_off<D>(B, ll)]); },
+                  c9, c10, xl, xr, g3);
     const long gq = g9 + sN;
-    double n3[3];
-    face_vec(S, B, D, i, j, k, n3);
-    double rh9[3] = {S.dc[D_RHO * S.NG + g9], S.dc[D_H * S.NG + g9], 0.0};
-    double rh10[3] = {S.dc[D_RHO * S.NG + gq], S.dc[D_H * S.NG + gq], 0.0};
-    double fx[NPV];
+    double n3d[3];
+    face_vec(S, B, D, i, j, k, n3d);
+    const R n3[3] = {R(n3d[0]), R(n3d[1]), R(n3d[2])};
+    R rh9[3] = {R(S.dc[D_RHO * S.NG + g9]), R(S.dc[D_H * S.NG + g9]), R(0)};
+    R rh10[3] = {R(S.dc[D_RHO * S.NG + gq]), R(S.dc[D_H * S.NG + gq]), R(0)};
+    R fx[NPV];
     if constexpr (SENSOR) {
         // the pressure sensor of Roe-M / AUSMPW+: x(1:18,1) of the face stencil
-        double sd[18];
-        for (int ll = 1; ll <= 18; ll++) sd[ll - 1] = f0[stencil_off<D>(B, ll)];
-        rh9[2] = S.dc[D_C2 * S.NG + g9];
-        rh10[2] = S.dc[D_C2 * S.NG + gq];
-        inviscid_face<double, true>(P, npv, n3, c9, c10, xl, xr, rh9, rh10, fx, sd);
+        R sd[18];
+        for (int ll = 1; ll <= 18; ll++) sd[ll - 1] = R(f0[stencil_off<D>(B, ll)]);
+        rh9[2] = R(S.dc[D_C2 * S.NG + g9]);
+        rh10[2] = R(S.dc[D_C2 * S.NG + gq]);
+        inviscid_face<R, true>(P, npv, n3, c9, c10, xl, xr, rh9, rh10, fx, sd);
     } else {
-        inviscid_face<double, false>(P, npv, n3, c9, c10, xl, xr, rh9, rh10, fx);
+        inviscid_face<R, false>(P, npv, n3, c9, c10, xl, xr, rh9, rh10, fx);
     }
     double *out = S.ff[D];
     const long NF = S.NF[D];
-    for (int n = 0; n < NPV; n++) if (n < npv) out[n * NF + t] = fx[n];
+    for (int n = 0; n < NPV; n++) if (n < npv) out[n * NF + t] = double(fx[n]);
 }
 
 // Viscous flux of every face of direction D (src/vsflux_module.f90:182-398),
-// with the metric averaging of src/rhs_module.f90:340-357.
-template <int D>
+// with the metric averaging of src/rhs_module.f90:340-357.  R as k_face.
+template <int D, class R>
 __global__ __launch_bounds__(128) void k_vface(DS S)
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -565,66 +570,70 @@ __global__ __launch_bounds__(128) void k_vface(DS S)
     const int npv = S.npv;
     const long g9 = gidx(B, i, j, k);
     const long sN = D == 0 ? 1 : (D == 1 ? (long)B.nx : (long)B.nx * B.ny);
-    double g3[6][3] = {};
-    double c9[NPV], c10[NPV];
+    R g3[6][3] = {};
+    R c9[NPV], c10[NPV];
 #pragma unroll
     for (int n = 0; n < NPV; n++) {
-        c9[n] = c10[n] = 0.0;
+        c9[n] = c10[n] = R(0);
         if (n >= npv) continue;
         const double *f = S.pv + (long)n * S.NG + g9;
-        auto xs = [&](int ll) -> double { return f[stencil_off<D>(B, ll)]; };
-        c9[n] = f[0];
-        c10[n] = f[sN];
+        auto xs = [&](int ll) -> R { return R(f[stencil_off<D>(B, ll)]); };
+        c9[n] = R(f[0]);
+        c10[n] = R(f[sN]);
         if ((n >= 1 && n <= 4) || n >= 7) {
             const int q = (n <= 4) ? n - 1 : n - 3;     // u, v, w, T, k, omega
             g3[q][0] = xs(10) - xs(9);
-            g3[q][1] = 0.0625 * (-xs(1) - xs(2) + xs(5) + xs(6) + 2.0 * (-xs(7) - xs(8) + xs(11) + xs(12))
-                                 - xs(13) - xs(14) + xs(17) + xs(18));
-            g3[q][2] = 0.0625 * (-xs(1) - xs(2) + xs(13) + xs(14) + 2.0 * (-xs(3) - xs(4) + xs(15) + xs(16))
-                                 - xs(5) - xs(6) + xs(17) + xs(18));
+            g3[q][1] = R(0.0625) * (-xs(1) - xs(2) + xs(5) + xs(6) + R(2) * (-xs(7) - xs(8) + xs(11) + xs(12))
+                                    - xs(13) - xs(14) + xs(17) + xs(18));
+            g3[q][2] = R(0.0625) * (-xs(1) - xs(2) + xs(13) + xs(14) + R(2) * (-xs(3) - xs(4) + xs(15) + xs(16))
+                                    - xs(5) - xs(6) + xs(17) + xs(18));
         }
     }
     int ip = i, jp = j, kp = k;
     if (D == 0) ip++; else if (D == 1) jp++; else kp++;
     const long gq = g9 + sN;
-    double n3[3];
-    face_vec(S, B, D, i, j, k, n3);
-    double e4[4][3], t4[4][3];
+    double n3d[3];
+    face_vec(S, B, D, i, j, k, n3d);
+    double e4d[4][3], t4d[4][3];
     const int E = (D + 1) % 3, T = (D + 2) % 3;
     int p0[3] = {i, j, k}, p1[3] = {ip, jp, kp};
     int a0[3], a1[3];
     // e-family: (P,e-1), (P+1,e-1), (P,e), (P+1,e)
     for (int q = 0; q < 3; q++) { a0[q] = p0[q]; a1[q] = p1[q]; }
     a0[E]--; a1[E]--;
-    face_vec(S, B, E, a0[0], a0[1], a0[2], e4[0]);
-    face_vec(S, B, E, a1[0], a1[1], a1[2], e4[1]);
-    face_vec(S, B, E, p0[0], p0[1], p0[2], e4[2]);
-    face_vec(S, B, E, p1[0], p1[1], p1[2], e4[3]);
+    face_vec(S, B, E, a0[0], a0[1], a0[2], e4d[0]);
+    face_vec(S, B, E, a1[0], a1[1], a1[2], e4d[1]);
+    face_vec(S, B, E, p0[0], p0[1], p0[2], e4d[2]);
+    face_vec(S, B, E, p1[0], p1[1], p1[2], e4d[3]);
     // t-family: i faces (P,t-1),(P+1,t-1),(P,t),(P+1,t); j and k faces (P,t-1),(P,t),(P+1,t-1),(P+1,t)
     for (int q = 0; q < 3; q++) { a0[q] = p0[q]; a1[q] = p1[q]; }
     a0[T]--; a1[T]--;
-    face_vec(S, B, T, a0[0], a0[1], a0[2], t4[0]);
+    face_vec(S, B, T, a0[0], a0[1], a0[2], t4d[0]);
     if (D == 0) {
-        face_vec(S, B, T, a1[0], a1[1], a1[2], t4[1]);
-        face_vec(S, B, T, p0[0], p0[1], p0[2], t4[2]);
+        face_vec(S, B, T, a1[0], a1[1], a1[2], t4d[1]);
+        face_vec(S, B, T, p0[0], p0[1], p0[2], t4d[2]);
     } else {
-        face_vec(S, B, T, p0[0], p0[1], p0[2], t4[1]);
-        face_vec(S, B, T, a1[0], a1[1], a1[2], t4[2]);
+        face_vec(S, B, T, p0[0], p0[1], p0[2], t4d[1]);
+        face_vec(S, B, T, a1[0], a1[1], a1[2], t4d[2]);
     }
-    face_vec(S, B, T, p1[0], p1[1], p1[2], t4[3]);
-    double gl[2], gr[2], dl[ND], dr[ND];
+    face_vec(S, B, T, p1[0], p1[1], p1[2], t4d[3]);
+    R n3[3], e4[4][3], t4[4][3];
+    for (int q = 0; q < 3; q++) n3[q] = R(n3d[q]);
+    for (int a = 0; a < 4; a++)
+        for (int q = 0; q < 3; q++) { e4[a][q] = R(e4d[a][q]); t4[a][q] = R(t4d[a][q]); }
+    R gl[2], gr[2], dl[ND], dr[ND];
     long wl = widx(B, i, j, k), wr = widx(B, ip, jp, kp);
-    gl[0] = S.vol[wl]; gr[0] = S.vol[wr];
-    gl[1] = S.ydn ? S.ydn[wl] : 1.0; gr[1] = S.ydn ? S.ydn[wr] : 1.0;
-    for (int m = 0; m < ND; m++) { dl[m] = S.dc[m * S.NG + g9]; dr[m] = S.dc[m * S.NG + gq]; }
-    double ml = S.mut ? S.mut[g9] : 0.0, mr = S.mut ? S.mut[gq] : 0.0;
-    double cav[5] = {0.5 * (c9[1] + c10[1]), 0.5 * (c9[2] + c10[2]), 0.5 * (c9[3] + c10[3]), 0.0, 0.0};
-    if (npv > 7) { cav[3] = 0.5 * (c9[7] + c10[7]); cav[4] = 0.5 * (c9[8] + c10[8]); }
-    double fv[NPV];
-    visc_flux<double>(P, n3, e4, t4, gl, gr, g3, cav, dl, dr, ml, mr, fv);
+    gl[0] = R(S.vol[wl]); gr[0] = R(S.vol[wr]);
+    gl[1] = S.ydn ? R(S.ydn[wl]) : R(1); gr[1] = S.ydn ? R(S.ydn[wr]) : R(1);
+    for (int m = 0; m < ND; m++) { dl[m] = R(S.dc[m * S.NG + g9]); dr[m] = R(S.dc[m * S.NG + gq]); }
+    R ml = S.mut ? R(S.mut[g9]) : R(0), mr = S.mut ? R(S.mut[gq]) : R(0);
+    R cav[5] = {R(0.5) * (c9[1] + c10[1]), R(0.5) * (c9[2] + c10[2]), R(0.5) * (c9[3] + c10[3]), R(0), R(0)};
+    if (npv > 7) { cav[3] = R(0.5) * (c9[7] + c10[7]); cav[4] = R(0.5) * (c9[8] + c10[8]); }
+    R fv[NPV];
+    visc_flux<R>(P, n3, e4, t4, gl, gr, g3, cav, dl, dr, ml, mr, fv);
     double *out = S.ff[D];
     const long NF = S.NF[D];
-    for (int n = 0; n < NPV; n++) if (n < npv) out[(npv + n) * NF + t] = fv[n];
+    for (int n = 0; n < NPV; n++) if (n < npv) out[(npv + n) * NF + t] = double(fv[n]);
 }
 
 __device__ __forceinline__ long fidx(const BlkD &B, int D, int i, int j, int k)
@@ -1574,19 +1583,31 @@ int launch_setbc(cav3d_ctx *c)
 int launch_rhs(cav3d_ctx *c)
 {
     DS &S = c->S;
-    if (S.P.nscheme == 2 || S.P.nscheme == 3) {
-        hipLaunchKernelGGL((k_face<0, true>), dim3(nblk(S.NF[0], 128)), dim3(128), 0, c->st, S);
-        hipLaunchKernelGGL((k_face<1, true>), dim3(nblk(S.NF[1], 128)), dim3(128), 0, c->st, S);
-        hipLaunchKernelGGL((k_face<2, true>), dim3(nblk(S.NF[2], 128)), dim3(128), 0, c->st, S);
-    } else {
-        hipLaunchKernelGGL((k_face<0, false>), dim3(nblk(S.NF[0], 128)), dim3(128), 0, c->st, S);
-        hipLaunchKernelGGL((k_face<1, false>), dim3(nblk(S.NF[1], 128)), dim3(128), 0, c->st, S);
-        hipLaunchKernelGGL((k_face<2, false>), dim3(nblk(S.NF[2], 128)), dim3(128), 0, c->st, S);
-    }
+    const bool sensor = S.P.nscheme == 2 || S.P.nscheme == 3;
+    // AUSMPW+ stays FP64 in the mixed mode: its pressure weight
+    // (p + p_ref + rho c^2)/(rho c^2) - 1 is ~1e-5 in water and cancels in FP32
+    const bool mixed = c->prm.precision == CAV3D_MIXED && S.P.nscheme != 3;
+#define CAV3D_FACES(SEN, R)                                                                             \
+    do {                                                                                                \
+        hipLaunchKernelGGL((k_face<0, SEN, R>), dim3(nblk(S.NF[0], 128)), dim3(128), 0, c->st, S);      \
+        hipLaunchKernelGGL((k_face<1, SEN, R>), dim3(nblk(S.NF[1], 128)), dim3(128), 0, c->st, S);      \
+        hipLaunchKernelGGL((k_face<2, SEN, R>), dim3(nblk(S.NF[2], 128)), dim3(128), 0, c->st, S);      \
+    } while (0)
+    if (sensor && mixed) CAV3D_FACES(true, float);
+    else if (sensor) CAV3D_FACES(true, double);
+    else if (mixed) CAV3D_FACES(false, float);
+    else CAV3D_FACES(false, double);
+#undef CAV3D_FACES
     if (c->prm.iturb >= -2) {
-        hipLaunchKernelGGL(k_vface<0>, dim3(nblk(S.NF[0], 128)), dim3(128), 0, c->st, S);
-        hipLaunchKernelGGL(k_vface<1>, dim3(nblk(S.NF[1], 128)), dim3(128), 0, c->st, S);
-        hipLaunchKernelGGL(k_vface<2>, dim3(nblk(S.NF[2], 128)), dim3(128), 0, c->st, S);
+        if (c->prm.precision == CAV3D_MIXED) {
+            hipLaunchKernelGGL((k_vface<0, float>), dim3(nblk(S.NF[0], 128)), dim3(128), 0, c->st, S);
+            hipLaunchKernelGGL((k_vface<1, float>), dim3(nblk(S.NF[1], 128)), dim3(128), 0, c->st, S);
+            hipLaunchKernelGGL((k_vface<2, float>), dim3(nblk(S.NF[2], 128)), dim3(128), 0, c->st, S);
+        } else {
+            hipLaunchKernelGGL((k_vface<0, double>), dim3(nblk(S.NF[0], 128)), dim3(128), 0, c->st, S);
+            hipLaunchKernelGGL((k_vface<1, double>), dim3(nblk(S.NF[1], 128)), dim3(128), 0, c->st, S);
+            hipLaunchKernelGGL((k_vface<2, double>), dim3(nblk(S.NF[2], 128)), dim3(128), 0, c->st, S);
+        }
     }
     if (S.P.ncav >= 2 || S.P.ides > 0) hipLaunchKernelGGL(k_combine<true>, dim3(nblk(S.NC, 128)), dim3(128), 0, c->st, S);
     else hipLaunchKernelGGL(k_combine<false>, dim3(nblk(S.NC, 128)), dim3(128), 0, c->st, S);
@@ -1673,7 +1694,8 @@ int cav3d_create(const cav3d_params *prm, int nblocks, const cav3d_block *blocks
     if (prm->cav_clip != 0 && prm->cav_clip != 1) return fail("cav_clip must be 0 (as written) or 1 (species)", -1);
     if (prm->nsteady < 0 || prm->nsteady > 2) return fail("nsteady must be 0, 1 or 2", -1);
     if (!(prm->dt_phy > 0.0)) return fail("dt_phy must be > 0 (it divides the BDF term of getx)", -1);
-    if (prm->precision != CAV3D_FP64) return fail("only the FP64 path is built in this library", -1);
+    if (prm->precision != CAV3D_FP64 && prm->precision != CAV3D_MIXED)
+        return fail("precision must be CAV3D_FP64 (0) or CAV3D_MIXED (1)", -1);
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail("no HIP device", -4);
     cav3d_ctx *c = new cav3d_ctx();

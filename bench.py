@@ -123,6 +123,54 @@ def grid_metrics_ms(pk, case, n):
     return round(best, 3)
 
 
+FIELDS = ["p", "u", "v", "w", "T", "y1", "y2", "k", "omega"]
+
+
+def interior_fields(pk, s, case):
+    b = case.blocks[0]
+    pv = s.get_state(0)[0].reshape(b.kmax + 5, b.jmax + 5, b.imax + 5, case.npv)
+    return pv[3:b.kmax + 2, 3:b.jmax + 2, 3:b.imax + 2]
+
+
+def mixed_precision_run(pk, case, args, ms64):
+    """The same warm-up and timed iterations with CAV3D_MIXED (reconstruction,
+    face EOS, inviscid and viscous fluxes in FP32; sources, D^-1, sweeps,
+    update and norms in FP64), and the per-field difference of the final
+    state from the FP64 run's, relative to each field's largest magnitude."""
+    out = {}
+    states = {}
+    for prec in (0, 1):
+        s = pk.Solver(case, precision=prec)
+        for ib, st in enumerate(case.init):
+            s.put_state(ib, *st)
+        n = args.warmup + args.steps
+        s.synchronize()
+        t1 = t2 = None
+        for it in range(n):
+            if it == args.warmup:
+                s.synchronize()
+                t1 = time.perf_counter()
+            s.timeinteg(1, it + 1, 0.0, want_time=False)
+        s.synchronize()
+        t2 = time.perf_counter()
+        states[prec] = (interior_fields(pk, s, case), s.residual()[0])
+        if prec == 1:
+            out["ms_per_step"] = round((t2 - t1) * 1e3 / max(args.steps, 1), 4)
+        s.close()
+    (a, l64), (b, l32) = states[0], states[1]
+    out["value"] = round(case.ncell / (out["ms_per_step"] * 1e-3) / 1e6, 4)
+    out["unit"] = "Mcell-iter/s"
+    out["speedup_vs_fp64"] = round(ms64 / out["ms_per_step"], 4)
+    out["iterations_compared"] = args.warmup + args.steps
+    err = {}
+    for n in range(case.npv):
+        scale = float(np.max(np.abs(a[..., n])))
+        err[FIELDS[n]] = float(np.max(np.abs(a[..., n] - b[..., n])) / scale) if scale > 0 else 0.0
+    out["field_rel_diff_vs_fp64"] = {k: float(f"{v:.3e}") for k, v in err.items()}
+    out["residual_l2_rel_diff"] = float(f"{float(np.max(np.abs(l32 - l64) / np.maximum(np.abs(l64), 1e-300))):.3e}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -227,6 +275,7 @@ def main():
                     ["timestep", "boundary", "rhs", "lusgs", "update_eddy", "total", "sweep_fwd", "sweep_bwd"], phase)}}
 
     grid_ms = grid_metrics_ms(pk, case, 3)
+    mixed = mixed_precision_run(pk, case, args, ms)
 
     cpu = None
     if args.cpu_baseline:
@@ -251,6 +300,7 @@ def main():
                                f"(species clip), 1 block", "cells": ncell},
         "roofline": roofline, "cpu_baseline": cpu,
         "grid_metrics_ms": grid_ms,
+        "mixed": mixed,
     }
     print(json.dumps(out), flush=True)
 

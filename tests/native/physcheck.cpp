@@ -60,6 +60,21 @@ void pc_face(const int *iv, const double *dv, const double *x, const double *n3,
     for (int q = 0; q < 18; q++) g3[q] = g[q / 3][q % 3];
 }
 
+// the same face in FP32 (the mixed-precision kernels' arithmetic)
+void pc_face32(const int *iv, const double *dv, const double *x, const double *n3, const double *rh9,
+               const double *rh10, double *fx)
+{
+    Par P = mkpar(iv, dv);
+    float c9[NPV], c10[NPV], xl[NPV], xr[NPV], g[6][3] = {}, sd[18], f[NPV];
+    for (int ll = 1; ll <= 18; ll++) sd[ll - 1] = float(x[(ll - 1) * 9]);
+    face_recon<float>(P, P.npv, [&](int ll, int n) { return float(x[(ll - 1) * 9 + n]); }, c9, c10, xl, xr, g);
+    const float n3f[3] = {float(n3[0]), float(n3[1]), float(n3[2])};
+    const float r9[3] = {float(rh9[0]), float(rh9[1]), float(rh9[2])}, r10[3] = {float(rh10[0]), float(rh10[1]), float(rh10[2])};
+    if (P.nscheme == 2 || P.nscheme == 3) inviscid_face<float, true>(P, P.npv, n3f, c9, c10, xl, xr, r9, r10, f, sd);
+    else inviscid_face<float, false>(P, P.npv, n3f, c9, c10, xl, xr, r9, r10, f);
+    for (int n = 0; n < P.npv; n++) fx[n] = double(f[n]);
+}
+
 void pc_visc(const int *iv, const double *dv, const double *x, const double *n3, const double *e4,
              const double *t4, const double *gl, const double *gr, const double *dl, const double *dr,
              double mutl, double mutr, double *fv)

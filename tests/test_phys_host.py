@@ -450,3 +450,39 @@ def test_des_length_closed_forms(OL, orc, pkg):
         (_, c3), _ = source(3, pv, h, 0.6 * h, 1e-4)
         (_, c4), _ = source(4, pv, h, 0.6 * h, 1e-4)
         assert c3 == c4
+
+
+@pytest.mark.parametrize("nscheme,nmuscl,nlim", [(1, 1, 1), (1, 0, 0), (4, 2, 8), (2, 2, 8)])
+def test_fp32_face_within_single_precision(L, OL, orc, pkg, nscheme, nmuscl, nlim):
+    """The mixed-precision face (reconstruction, face EOS, flux in FP32 on
+    gauge pressure) against the FP64 face: each flux component within 2e-5
+    of that component's largest magnitude over the sample.  (AUSMPW+ keeps
+    FP64 in the mixed mode: its pressure weight cancels to ~4e-4 in FP32.)"""
+    case = pkg.cases.config2(10, 8, 8)
+    case.params.update(nscheme=nscheme, nmuscl=nmuscl, nlim=nlim)
+    cfg = case.cfg_dict()
+    iv, dvp = H.par_arrays(cfg)
+    cf = cfg_struct(orc, cfg)
+    b = case.blocks[0]
+    P4 = case.init[0][0].reshape(b.kmax + 5, b.jmax + 5, b.imax + 5, case.npv)
+    rng = np.random.default_rng(900 + nscheme)
+    got, want = [], []
+    for trial in range(150):
+        x = _stencil(rng, P4, case.npv, 1e-2 if trial % 2 else 1e-6)
+        n3 = rng.standard_normal(3) * 1e-3
+        d9 = orc.deteos_simple(x[8, 0] + cfg["pref"], x[8, 4], x[8, 5], x[8, 6])
+        d10 = orc.deteos_simple(x[9, 0] + cfg["pref"], x[9, 4], x[9, 5], x[9, 6])
+        f32 = np.zeros(9)
+        L.pc_face32(H.p(iv, C.c_int), dp(dvp), H.p(x), dp(n3), H.p(d9[[0, 1, 5]]), H.p(d10[[0, 1, 5]]), dp(f32))
+        f64, xl, xr, g3 = np.zeros(9), np.zeros(9), np.zeros(9), np.zeros(18)
+        L.pc_face(H.p(iv, C.c_int), dp(dvp), H.p(x), dp(n3), H.p(d9[[0, 1, 5]]), H.p(d10[[0, 1, 5]]), dp(f64),
+                  dp(xl), dp(xr), dp(g3))
+        got.append(f32[:case.npv])
+        want.append(f64[:case.npv])
+    got, want = np.array(got), np.array(want)
+    scale = np.abs(want).max(axis=0)
+    zero = scale == 0.0                      # a species absent everywhere
+    assert np.all(got[:, zero] == 0.0)
+    err = np.abs(got[:, ~zero] - want[:, ~zero]).max(axis=0) / scale[~zero]
+    assert np.all(err < 2e-5), err
+    assert np.any(got != want)
