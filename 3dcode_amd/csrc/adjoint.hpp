@@ -1,0 +1,548 @@
+// Discrete adjoint of the residual: cav3d_rhs_vjp.
+//
+// The operator is the residual cav3d_calrhs evaluates (src/rhs_module.f90:
+// 260-643: inviscid and viscous face-flux differences, the Merkle and SST
+// sources, the BDF term), taken as a function of every block's ghosted
+// primitive state pv, with the cell properties re-derived from pv by the EOS
+// (rho, h, c^2, the EOS derivatives, mu, kappa: deteos_prop) and with the
+// eddy viscosity, the grid metrics, the BDF history and the ghost layers
+// held fixed (a ghost cell is an input like an interior cell).  It returns
+//   pv_bar = (d res / d pv)^T res_bar
+// and the two scalar derivatives of res_bar . res with respect to p_ref and
+// u_ref.  The reference has no adjoint; this is the VJP of its residual.
+//
+// Method: reverse accumulation over faces and cells, local derivatives in
+// forward mode.  Each face recomputes its reconstruction and flux (nothing is
+// kept from a forward pass) and takes the gradient of
+//   phi_f = g_f . F_f,   g_f = res_bar(right cell) - res_bar(left cell)
+// through dual-number passes (Dn<W>: W inputs seeded per pass) of the same
+// templated physics the residual kernels run, split where the input count is
+// smallest:
+//   flux core       the two face states, rho and h (c^2) of each side, the
+//                   Roe-averaged c^2, [the 18 sensor pressures], u_ref
+//   averaged c^2    the ten quantities it is built from
+//   face EOS        (p, T, y1, y2) of each side
+//   clamps          each clamped value comes from the reconstruction, the
+//                   cell or a constant (recorded in the forward recompute)
+//   reconstruction  each variable's stencil (38 points under MLP, 6 under TVD)
+// The per-face bars of every stencil point go to a buffer and each cell
+// gathers its own in a fixed order: the result is deterministic, with no
+// atomics.  The cell sources are handled per cell the same way, and each
+// cell's property bars return to (p, T, y1, y2) through one 4-lane dual pass
+// of the EOS.  Arithmetic is FP64 throughout.
+//
+// d/d(p_ref) is the sum of pv_bar over the pressure component: the residual
+// reads gauge pressure only as p + p_ref or in differences (the pressure
+// floor included), so shifting p_ref shifts it as shifting every gauge
+// pressure does.  d/d(u_ref) is carried on one extra lane of the flux-core
+// passes (the low-Mach preconditioning is the only reader).
+//
+// Included once, from cav3d.hip, after the context definition.
+
+#include "adjoint_point.hpp"
+
+namespace {
+
+struct AD {
+    const double *rb;       // res_bar [npv][NC]
+    double *fb[3];          // bars of a face's stencil points [NSLOT][npv][NF]
+    double *db[3];          // cell-property bars of a face's two cells [2][ND][NF]
+    double *ub[3];          // d(phi_f)/d(u_ref) [NF]
+    double *cb;             // cell-source bars on the 7-point stencil [7][npv][NC]
+    double *cdb;            // cell-source property bars [ND][NC]
+    double *pb;             // pv_bar [npv][NG]
+    unsigned long long mask;   // stencil points a face writes (bit ll-1)
+};
+
+__device__ __forceinline__ bool interior_cell(const BlkD &B, int i, int j, int k)
+{
+    return i >= 2 && i <= B.I && j >= 2 && j <= B.J && k >= 2 && k <= B.K;
+}
+
+__device__ __forceinline__ double rbar(const DS &S, const AD &A, const BlkD &B, int i, int j, int k, int n)
+{
+    return interior_cell(B, i, j, k) ? A.rb[(long)n * S.NC + cidx(B, i, j, k)] : 0.0;
+}
+
+// (normal, first transverse, second transverse) offset of stencil point ll
+// from the face's left cell; the same table as stencil_off
+__device__ __forceinline__ void stencil_ijk(int ll, int &dn, int &de, int &dt)
+{
+    int q = ll - 1;
+    if (q < 18) { dn = q & 1; de = (q >> 1) % 3 - 1; dt = q / 6 - 1; }
+    else if (q < 27) { q -= 18; dn = -1; de = q % 3 - 1; dt = q / 3 - 1; }
+    else if (q < 36) { q -= 27; dn = 2; de = q % 3 - 1; dt = q / 3 - 1; }
+    else { dn = q == 36 ? -2 : 3; de = 0; dt = 0; }
+}
+
+// does the face of direction D with left cell (i,j,k) exist
+__device__ __forceinline__ bool face_exists(const BlkD &B, int D, int i, int j, int k)
+{
+    const int lo[3] = {D == 0 ? 1 : 2, D == 1 ? 1 : 2, D == 2 ? 1 : 2};
+    return i >= lo[0] && i <= B.I && j >= lo[1] && j <= B.J && k >= lo[2] && k <= B.K;
+}
+
+template <int D, bool SENSOR>
+__global__ __launch_bounds__(128) void k_face_vjp(DS S, AD A)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= S.NF[D]) return;
+    int i, j, k;
+    const BlkD &B = face_of<D>(S, t, i, j, k);
+    const Par &P = S.P;
+    const int npv = S.npv;
+    const long NF = S.NF[D];
+    const long g9 = gidx(B, i, j, k);
+    const long sN = D == 0 ? 1 : (D == 1 ? (long)B.nx : (long)B.nx * B.ny);
+    const long gq = g9 + sN;
+    const int ip = i + (D == 0), jp = j + (D == 1), kp = k + (D == 2);
+    double g[NPV];
+    bool any = false;
+    for (int n = 0; n < NPV; n++) {
+        g[n] = n < npv ? rbar(S, A, B, ip, jp, kp, n) - rbar(S, A, B, i, j, k, n) : 0.0;
+        any = any || g[n] != 0.0;
+    }
+    double *fb = A.fb[D], *db = A.db[D];
+    if (!any) {
+        for (int ll = 1; ll <= NSLOT; ll++)
+            if ((A.mask >> (ll - 1)) & 1ull)
+                for (int n = 0; n < npv; n++) fb[((long)(ll - 1) * npv + n) * NF + t] = 0.0;
+        for (int m = 0; m < 2 * ND; m++) db[m * NF + t] = 0.0;
+        A.ub[D][t] = 0.0;
+        return;
+    }
+    const double *f0 = S.pv + g9;
+    long off[NSLOT];
+    for (int ll = 1; ll <= NSLOT; ll++) off[ll - 1] = stencil_off<D>(B, ll);
+    double n3[3];
+    face_vec(S, B, D, i, j, k, n3);
+    const double cellp[6] = {S.dc[D_RHO * S.NG + g9], S.dc[D_H * S.NG + g9], S.dc[D_C2 * S.NG + g9],
+                             S.dc[D_RHO * S.NG + gq], S.dc[D_H * S.NG + gq], S.dc[D_C2 * S.NG + gq]};
+    double ub;
+    face_adj<SENSOR>(P, npv, f0, S.NG, off, n3, cellp, g, A.mask, fb + t, (long)npv * NF, NF, db + t, NF, ub);
+    A.ub[D][t] = ub;
+}
+
+// Viscous part of a face, added to the inviscid bars (stencil points 1..18,
+// the two cells' mu, kappa and, under SST, h_T and rho).
+template <int D>
+__global__ __launch_bounds__(128) void k_vface_vjp(DS S, AD A)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= S.NF[D]) return;
+    int i, j, k;
+    const BlkD &B = face_of<D>(S, t, i, j, k);
+    const Par &P = S.P;
+    const int npv = S.npv;
+    const long NF = S.NF[D];
+    const long g9 = gidx(B, i, j, k);
+    const long sN = D == 0 ? 1 : (D == 1 ? (long)B.nx : (long)B.nx * B.ny);
+    const long gq = g9 + sN;
+    int ip = i, jp = j, kp = k;
+    if (D == 0) ip++; else if (D == 1) jp++; else kp++;
+    double g[NPV];
+    bool any = false;
+    for (int n = 0; n < NPV; n++) {
+        g[n] = n < npv ? rbar(S, A, B, i, j, k, n) - rbar(S, A, B, ip, jp, kp, n) : 0.0;
+        any = any || g[n] != 0.0;
+    }
+    if (!any) return;
+    const bool turb = P.iturb == 0;
+    const int nv = turb ? 6 : 4;
+    const int vars[6] = {1, 2, 3, 4, 7, 8};
+    auto xs = [&](int ll, int n) -> double { return S.pv[(long)n * S.NG + g9 + stencil_off<D>(B, ll)]; };
+    // inputs: the normal / transverse differences, the face averages, the cell properties
+    double g3[6][3], cav[5];
+    for (int q = 0; q < nv; q++) {
+        const int n = vars[q];
+        g3[q][0] = xs(10, n) - xs(9, n);
+        g3[q][1] = 0.0625 * (-xs(1, n) - xs(2, n) + xs(5, n) + xs(6, n) + 2.0 * (-xs(7, n) - xs(8, n) + xs(11, n) + xs(12, n))
+                             - xs(13, n) - xs(14, n) + xs(17, n) + xs(18, n));
+        g3[q][2] = 0.0625 * (-xs(1, n) - xs(2, n) + xs(13, n) + xs(14, n) + 2.0 * (-xs(3, n) - xs(4, n) + xs(15, n) + xs(16, n))
+                             - xs(5, n) - xs(6, n) + xs(17, n) + xs(18, n));
+    }
+    const int ncv = turb ? 5 : 3;
+    const int cvar[5] = {1, 2, 3, 7, 8};
+    for (int q = 0; q < ncv; q++) cav[q] = 0.5 * (xs(9, cvar[q]) + xs(10, cvar[q]));
+    double dl[ND], dr[ND];
+    for (int m = 0; m < ND; m++) { dl[m] = S.dc[m * S.NG + g9]; dr[m] = S.dc[m * S.NG + gq]; }
+    const double ml = S.mut ? S.mut[g9] : 0.0, mr = S.mut ? S.mut[gq] : 0.0;
+    // metrics (as k_vface)
+    double n3[3], e4[4][3], t4[4][3];
+    face_vec(S, B, D, i, j, k, n3);
+    const int E = (D + 1) % 3, T = (D + 2) % 3;
+    int p0[3] = {i, j, k}, p1[3] = {ip, jp, kp};
+    int a0[3], a1[3];
+    for (int q = 0; q < 3; q++) { a0[q] = p0[q]; a1[q] = p1[q]; }
+    a0[E]--; a1[E]--;
+    face_vec(S, B, E, a0[0], a0[1], a0[2], e4[0]);
+    face_vec(S, B, E, a1[0], a1[1], a1[2], e4[1]);
+    face_vec(S, B, E, p0[0], p0[1], p0[2], e4[2]);
+    face_vec(S, B, E, p1[0], p1[1], p1[2], e4[3]);
+    for (int q = 0; q < 3; q++) { a0[q] = p0[q]; a1[q] = p1[q]; }
+    a0[T]--; a1[T]--;
+    face_vec(S, B, T, a0[0], a0[1], a0[2], t4[0]);
+    if (D == 0) {
+        face_vec(S, B, T, a1[0], a1[1], a1[2], t4[1]);
+        face_vec(S, B, T, p0[0], p0[1], p0[2], t4[2]);
+    } else {
+        face_vec(S, B, T, p0[0], p0[1], p0[2], t4[1]);
+        face_vec(S, B, T, a1[0], a1[1], a1[2], t4[2]);
+    }
+    face_vec(S, B, T, p1[0], p1[1], p1[2], t4[3]);
+    const long wl = widx(B, i, j, k), wr = widx(B, ip, jp, kp);
+    const double gl[2] = {S.vol[wl], S.ydn ? S.ydn[wl] : 1.0}, gr[2] = {S.vol[wr], S.ydn ? S.ydn[wr] : 1.0};
+    // z = {g3[nv][3], cav[ncv], muL, kapL, muR, kapR, [hTL, hTR, rhoL, rhoR]}
+    constexpr int NZ = 18 + 5 + 8;
+    const int og = 3 * nv, oc = og + ncv;
+    const int nz = oc + (turb ? 8 : 4);
+    double zx[NZ], zb[NZ];
+    for (int q = 0; q < nv; q++) for (int c = 0; c < 3; c++) zx[3 * q + c] = g3[q][c];
+    for (int q = 0; q < ncv; q++) zx[og + q] = cav[q];
+    zx[oc] = dl[D_MU]; zx[oc + 1] = dl[D_KAP]; zx[oc + 2] = dr[D_MU]; zx[oc + 3] = dr[D_KAP];
+    if (turb) { zx[oc + 4] = dl[D_HT]; zx[oc + 5] = dr[D_HT]; zx[oc + 6] = dl[D_RHO]; zx[oc + 7] = dr[D_RHO]; }
+    visc_adj(P, npv, nv, ncv, nz, zx, n3, e4, t4, gl, gr, dl, dr, ml, mr, g, zb);
+    // the differences and averages are linear in the stencil
+    double *fb = A.fb[D];
+    constexpr double C1[18] = {-1, -1, 0, 0, 1, 1, -2, -2, 0, 0, 2, 2, -1, -1, 0, 0, 1, 1};
+    constexpr double C2[18] = {-1, -1, -2, -2, -1, -1, 0, 0, 0, 0, 0, 0, 1, 1, 2, 2, 1, 1};
+    for (int q = 0; q < nv; q++) {
+        const int n = vars[q];
+        for (int ll = 1; ll <= 18; ll++) {
+            double v = 0.0625 * (C1[ll - 1] * zb[3 * q + 1] + C2[ll - 1] * zb[3 * q + 2]);
+            if (ll == 9) v -= zb[3 * q];
+            if (ll == 10) v += zb[3 * q];
+            if (v != 0.0) fb[((long)(ll - 1) * npv + n) * NF + t] += v;
+        }
+    }
+    for (int q = 0; q < ncv; q++) {
+        const int n = cvar[q];
+        fb[((long)8 * npv + n) * NF + t] += 0.5 * zb[og + q];
+        fb[((long)9 * npv + n) * NF + t] += 0.5 * zb[og + q];
+    }
+    double *db = A.db[D];
+    db[D_MU * NF + t] += zb[oc];
+    db[D_KAP * NF + t] += zb[oc + 1];
+    db[(ND + D_MU) * NF + t] += zb[oc + 2];
+    db[(ND + D_KAP) * NF + t] += zb[oc + 3];
+    if (turb) {
+        db[D_HT * NF + t] += zb[oc + 4];
+        db[(ND + D_HT) * NF + t] += zb[oc + 5];
+        db[D_RHO * NF + t] += zb[oc + 6];
+        db[(ND + D_RHO) * NF + t] += zb[oc + 7];
+    }
+}
+
+// the 7-point stencil of the cell sources (x7 of k_combine)
+__device__ __forceinline__ void src_off(int q, int &di, int &dj, int &dk)
+{
+    di = q == 0 ? -1 : (q == 2 ? 1 : 0);
+    dj = q == 3 ? -1 : (q == 4 ? 1 : 0);
+    dk = q == 5 ? -1 : (q == 6 ? 1 : 0);
+}
+
+// Cell sources of one interior cell: Merkle, SST (RANS or DES length), BDF.
+template <bool DES>
+__global__ __launch_bounds__(128) void k_cell_vjp(DS S, AD A)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= S.NC) return;
+    const BlkD &B = S.b[find_blk(S, t, 0)];
+    const Par &P = S.P;
+    const int npv = S.npv;
+    int i, j, k;
+    cell_of(B, t - B.c0, i, j, k);
+    const long g = gidx(B, i, j, k);
+    double rb[NPV];
+    for (int n = 0; n < NPV; n++) rb[n] = n < npv ? A.rb[(long)n * S.NC + t] : 0.0;
+    const bool sst = P.iturb == 0, cav_ = P.ncav == 1, bdf = P.nsteady > 0;
+    // z = {pc[npv], rho, h, mu, neighbours' (u, v, w, k, omega) x 6}
+    constexpr int NZ = NPV + 3 + 30;
+    const int od = npv, ox = npv + 3;
+    const int nz = (cav_ || bdf || sst) ? (sst ? ox + 30 : ox) : 0;
+    double zx[NZ], zb[NZ];
+    for (int q = 0; q < NZ; q++) zx[q] = zb[q] = 0.0;
+    const int nbv[5] = {1, 2, 3, 7, 8};
+    double d[ND];
+    for (int m = 0; m < ND; m++) d[m] = S.dc[m * S.NG + g];
+    double x7[7][NPV];
+    for (int q = 0; q < 7; q++) {
+        int di, dj, dk;
+        src_off(q, di, dj, dk);
+        const long gn = gidx(B, i + di, j + dj, k + dk);
+        for (int n = 0; n < NPV; n++) x7[q][n] = n < npv ? S.pv[(long)n * S.NG + gn] : 0.0;
+    }
+    if (nz) {
+        for (int n = 0; n < npv; n++) zx[n] = x7[1][n];
+        zx[od] = d[D_RHO]; zx[od + 1] = d[D_H]; zx[od + 2] = d[D_MU];
+        if (sst) {
+            int c = 0;
+            for (int q = 0; q < 7; q++) {
+                if (q == 1) continue;
+                for (int v = 0; v < 5; v++) zx[ox + 5 * c + v] = x7[q][nbv[v]];
+                c++;
+            }
+        }
+        const double vol = S.vol[widx(B, i, j, k)];
+        double f6[6][3];
+        cell_faces(S, B, i, j, k, f6);
+        const double ydn = sst ? S.ydn[widx(B, i, j, k)] : 1.0;
+        const double mut = S.mut ? S.mut[g] : 0.0;
+        double q1[NPV], q2[NPV];
+        for (int n = 0; n < NPV; n++) {
+            q1[n] = (bdf && n < npv) ? S.qq[(long)n * S.NC + t] : 0.0;
+            q2[n] = (bdf && n < npv) ? S.qq[(long)(npv + n) * S.NC + t] : 0.0;
+        }
+        constexpr int W = 2;
+        using T = Dn<W>;
+        dn_grad<W>(nz, [&](int q) -> double { return zx[q]; }, [&](auto in, int) -> T {
+            T pc[NPV], dt[ND], x7t[7][NPV];
+            for (int n = 0; n < NPV; n++) pc[n] = n < npv ? in(n) : T(0.0);
+            for (int m = 0; m < ND; m++) dt[m] = T(d[m]);
+            dt[D_RHO] = in(od); dt[D_H] = in(od + 1); dt[D_MU] = in(od + 2);
+            T phi(0.0);
+            if (cav_) {
+                T rc, rv, ic[4];
+                merkle<T>(P, pc, dt, vol, rc, rv, ic);
+                phi = phi + rb[5] * (rv - rc);
+            }
+            if (sst) {
+                int c = 0;
+                for (int q = 0; q < 7; q++) {
+                    for (int n = 0; n < NPV; n++) x7t[q][n] = T(x7[q][n]);
+                    if (q == 1) { for (int n = 0; n < npv; n++) x7t[1][n] = pc[n]; continue; }
+                    for (int v = 0; v < 5; v++) x7t[q][nbv[v]] = in(ox + 5 * c + v);
+                    c++;
+                }
+                T src[2], it[4], oc;
+                sst_source<DES, T>(P, x7t, f6, vol, ydn, dt, T(mut), src, it, oc);
+                phi = phi + rb[7] * src[0] + rb[8] * src[1];
+            }
+            if (bdf) {
+                T fx[NPV];
+                fx[0] = dt[D_RHO];
+                fx[1] = dt[D_RHO] * pc[1];
+                fx[2] = dt[D_RHO] * pc[2];
+                fx[3] = dt[D_RHO] * pc[3];
+                fx[4] = dt[D_RHO] * (dt[D_H] + 0.5 * (pc[1] * pc[1] + pc[2] * pc[2] + pc[3] * pc[3])) - (pc[0] + P.pref);
+                for (int n = 5; n < NPV; n++) fx[n] = n < npv ? dt[D_RHO] * pc[n] : T(0.0);
+                for (int n = 0; n < npv; n++) {
+                    T v = (double)P.nsteady * (P.bdf_b * fx[n] - q1[n]) + 0.5 * ((double)P.nsteady - 1.0) * q2[n];
+                    phi = phi - rb[n] * (v * vol / P.dt_phy);
+                }
+            }
+            return phi;
+        }, zb);
+    }
+    // outputs: centre (q = 1) and neighbour bars, the cell's property bars
+    for (int q = 0; q < 7; q++)
+        for (int n = 0; n < npv; n++) A.cb[((long)q * npv + n) * S.NC + t] = 0.0;
+    for (int n = 0; n < npv; n++) A.cb[((long)1 * npv + n) * S.NC + t] = zb[n];
+    if (sst) {
+        int c = 0;
+        for (int q = 0; q < 7; q++) {
+            if (q == 1) continue;
+            for (int v = 0; v < 5; v++) A.cb[((long)q * npv + nbv[v]) * S.NC + t] = zb[ox + 5 * c + v];
+            c++;
+        }
+    }
+    for (int m = 0; m < ND; m++) A.cdb[(long)m * S.NC + t] = 0.0;
+    A.cdb[(long)D_RHO * S.NC + t] = zb[od];
+    A.cdb[(long)D_H * S.NC + t] = zb[od + 1];
+    A.cdb[(long)D_MU * S.NC + t] = zb[od + 2];
+}
+
+// pv_bar of every ghosted cell: its bars from the faces whose stencils hold
+// it and from the sources of its neighbours, in a fixed order, then its
+// property bars through the EOS.
+__global__ __launch_bounds__(128) void k_adj_gather(DS S, AD A)
+{
+    long gl = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gl >= S.NG) return;
+    const BlkD &B = S.b[find_blk(S, gl, 1)];
+    const Par &P = S.P;
+    const int npv = S.npv;
+    const long r = gl - B.g0;
+    const int i = (int)(r % B.nx) - 1, j = (int)((r / B.nx) % B.ny) - 1, k = (int)(r / ((long)B.nx * B.ny)) - 1;
+    double acc[NPV], dcb[ND];
+    for (int n = 0; n < NPV; n++) acc[n] = 0.0;
+    for (int m = 0; m < ND; m++) dcb[m] = 0.0;
+    const int c3[3] = {i, j, k};
+    for (int D = 0; D < 3; D++) {
+        const int E = (D + 1) % 3, T = (D + 2) % 3;
+        const long NF = S.NF[D];
+        for (int ll = 1; ll <= NSLOT; ll++) {
+            if (!((A.mask >> (ll - 1)) & 1ull)) continue;
+            int dn, de, dt;
+            stencil_ijk(ll, dn, de, dt);
+            int f[3];
+            f[D] = c3[D] - dn; f[E] = c3[E] - de; f[T] = c3[T] - dt;
+            if (!face_exists(B, D, f[0], f[1], f[2])) continue;
+            const long fi = fidx(B, D, f[0], f[1], f[2]);
+            for (int n = 0; n < npv; n++) acc[n] += A.fb[D][((long)(ll - 1) * npv + n) * NF + fi];
+        }
+        if (face_exists(B, D, i, j, k)) {
+            const long fi = fidx(B, D, i, j, k);
+            for (int m = 0; m < ND; m++) dcb[m] += A.db[D][(long)m * NF + fi];
+        }
+        const int im = i - (D == 0), jm = j - (D == 1), km = k - (D == 2);
+        if (face_exists(B, D, im, jm, km)) {
+            const long fi = fidx(B, D, im, jm, km);
+            for (int m = 0; m < ND; m++) dcb[m] += A.db[D][(long)(ND + m) * NF + fi];
+        }
+    }
+    for (int q = 0; q < 7; q++) {
+        int di, dj, dk;
+        src_off(q, di, dj, dk);
+        const int si = i - di, sj = j - dj, sk = k - dk;
+        if (!interior_cell(B, si, sj, sk)) continue;
+        const long ct = cidx(B, si, sj, sk);
+        for (int n = 0; n < npv; n++) acc[n] += A.cb[((long)q * npv + n) * S.NC + ct];
+    }
+    if (interior_cell(B, i, j, k)) {
+        const long ct = cidx(B, i, j, k);
+        for (int m = 0; m < ND; m++) dcb[m] += A.cdb[(long)m * S.NC + ct];
+    }
+    bool any = false;
+    for (int m = 0; m < ND; m++) any = any || dcb[m] != 0.0;
+    if (any) {
+        double x4[4] = {S.pv[gl] + P.pref, S.pv[4 * S.NG + gl], S.pv[5 * S.NG + gl], S.pv[6 * S.NG + gl]}, gx[4];
+        eos_adj(P.iturb != -3, x4, dcb, gx);
+        acc[0] += gx[0]; acc[4] += gx[1]; acc[5] += gx[2]; acc[6] += gx[3];
+    }
+    for (int n = 0; n < npv; n++) A.pb[(long)n * S.NG + gl] = acc[n];
+}
+
+// deterministic sum of n doubles (one workgroup: strided partial sums, then a tree)
+__global__ __launch_bounds__(256) void k_sum_fixed(const double *a, long n, double *out)
+{
+    __shared__ double sh[256];
+    double s = 0.0;
+    for (long q = threadIdx.x; q < n; q += 256) s += a[q];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = sh[0];
+}
+
+// stencil points a face's bars cover under the configuration
+unsigned long long adj_mask(const Par &P)
+{
+    unsigned long long m = 0;
+    auto set = [&](int ll) { m |= 1ull << (ll - 1); };
+    set(9); set(10);
+    if (P.nmuscl == 1) { set(23); set(32); set(37); set(38); }
+    if (P.nmuscl == 2) for (int ll = 1; ll <= NSLOT; ll++) set(ll);
+    if (P.nscheme == 2 || P.nscheme == 3 || P.iturb >= -2) for (int ll = 1; ll <= 18; ll++) set(ll);
+    return m;
+}
+
+int adj_alloc(cav3d_ctx *c)
+{
+    if (c->adj_ready) return 0;
+    DS &S = c->S;
+    const int npv = c->npv;
+    int rc;
+    if ((rc = dalloc(c, &c->a_rb, (long)npv * S.NC)) || (rc = dalloc(c, &c->a_cb, 7L * npv * S.NC)) ||
+        (rc = dalloc(c, &c->a_cdb, (long)ND * S.NC)) || (rc = dalloc(c, &c->a_pb, (long)npv * S.NG)) ||
+        (rc = dalloc(c, &c->a_sum, 8)))
+        return rc;
+    for (int d = 0; d < 3; d++)
+        if ((rc = dalloc(c, &c->a_fb[d], (long)NSLOT * npv * S.NF[d])) || (rc = dalloc(c, &c->a_db[d], 2L * ND * S.NF[d])) ||
+            (rc = dalloc(c, &c->a_ub[d], S.NF[d])))
+            return rc;
+    HIPCHK(hipStreamSynchronize(c->st));
+    c->adj_ready = true;
+    return 0;
+}
+
+int launch_vjp(cav3d_ctx *c)
+{
+    DS &S = c->S;
+    AD A;
+    A.rb = c->a_rb;
+    for (int d = 0; d < 3; d++) { A.fb[d] = c->a_fb[d]; A.db[d] = c->a_db[d]; A.ub[d] = c->a_ub[d]; }
+    A.cb = c->a_cb; A.cdb = c->a_cdb; A.pb = c->a_pb;
+    A.mask = adj_mask(S.P);
+    const bool sensor = S.P.nscheme == 2 || S.P.nscheme == 3;
+    if (sensor) {
+        hipLaunchKernelGGL((k_face_vjp<0, true>), dim3(nblk(S.NF[0], 128)), dim3(128), 0, c->st, S, A);
+        hipLaunchKernelGGL((k_face_vjp<1, true>), dim3(nblk(S.NF[1], 128)), dim3(128), 0, c->st, S, A);
+        hipLaunchKernelGGL((k_face_vjp<2, true>), dim3(nblk(S.NF[2], 128)), dim3(128), 0, c->st, S, A);
+    } else {
+        hipLaunchKernelGGL((k_face_vjp<0, false>), dim3(nblk(S.NF[0], 128)), dim3(128), 0, c->st, S, A);
+        hipLaunchKernelGGL((k_face_vjp<1, false>), dim3(nblk(S.NF[1], 128)), dim3(128), 0, c->st, S, A);
+        hipLaunchKernelGGL((k_face_vjp<2, false>), dim3(nblk(S.NF[2], 128)), dim3(128), 0, c->st, S, A);
+    }
+    if (S.P.iturb >= -2) {
+        hipLaunchKernelGGL((k_vface_vjp<0>), dim3(nblk(S.NF[0], 128)), dim3(128), 0, c->st, S, A);
+        hipLaunchKernelGGL((k_vface_vjp<1>), dim3(nblk(S.NF[1], 128)), dim3(128), 0, c->st, S, A);
+        hipLaunchKernelGGL((k_vface_vjp<2>), dim3(nblk(S.NF[2], 128)), dim3(128), 0, c->st, S, A);
+    }
+    if (S.P.ides > 0) hipLaunchKernelGGL(k_cell_vjp<true>, dim3(nblk(S.NC, 128)), dim3(128), 0, c->st, S, A);
+    else hipLaunchKernelGGL(k_cell_vjp<false>, dim3(nblk(S.NC, 128)), dim3(128), 0, c->st, S, A);
+    hipLaunchKernelGGL(k_adj_gather, dim3(nblk(S.NG, 128)), dim3(128), 0, c->st, S, A);
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cav3d_rhs_vjp(cav3d_ctx *c, const double *res_bar, double *pv_bar, double *scal_bar)
+{
+    if (!c || !res_bar || !pv_bar) return fail("bad argument to cav3d_rhs_vjp", -1);
+    if (c->prm.ncav >= 2)
+        return fail("cav3d_rhs_vjp: the adjoint differentiates the Merkle source only (ncav 0 or 1)", -2);
+    int rc;
+    if ((rc = adj_alloc(c))) return rc;
+    DS &S = c->S;
+    const int npv = c->npv;
+    long maxn = 0;
+    for (int b = 0; b < c->nb; b++) maxn = std::max(maxn, (long)npv * c->blk[b].ng);
+    if ((rc = ensure_stage(c, maxn))) return rc;
+    long off = 0;
+    for (int b = 0; b < c->nb; b++) {
+        const BlkD &B = c->blk[b];
+        HIPCHK(hipMemcpyAsync(c->stage, res_bar + off, (long)npv * B.nc * sizeof(double), hipMemcpyHostToDevice, c->st));
+        hipLaunchKernelGGL(k_cell_from_aos, dim3(nblk(B.nc, 256)), dim3(256), 0, c->st, (const double *)c->stage, npv, S.NC,
+                           B.c0, c->a_rb, B.nc);
+        HIPCHK(hipStreamSynchronize(c->st));
+        off += (long)npv * B.nc;
+    }
+    if (c->profile) hipEventRecord(c->ev[10], c->st);
+    launch_vjp(c);
+    if (c->profile) {
+        hipEventRecord(c->ev[11], c->st);
+        hipEventSynchronize(c->ev[11]);
+        float ms;
+        hipEventElapsedTime(&ms, c->ev[10], c->ev[11]);
+        c->prof_ms[8] = ms;
+    }
+    if ((rc = check_launch("cav3d_rhs_vjp"))) return rc;
+    off = 0;
+    for (int b = 0; b < c->nb; b++) {
+        const BlkD &B = c->blk[b];
+        hipLaunchKernelGGL(k_soa_to_aos, dim3(nblk(B.ng, 256)), dim3(256), 0, c->st, (const double *)c->a_pb, npv, S.NG, B.g0,
+                           c->stage, B.ng);
+        HIPCHK(hipMemcpyAsync(pv_bar + off, c->stage, (long)npv * B.ng * sizeof(double), hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+        off += (long)npv * B.ng;
+    }
+    if (scal_bar) {
+        hipLaunchKernelGGL(k_sum_fixed, dim3(1), dim3(256), 0, c->st, (const double *)c->a_pb, S.NG, c->a_sum);
+        for (int d = 0; d < 3; d++)
+            hipLaunchKernelGGL(k_sum_fixed, dim3(1), dim3(256), 0, c->st, (const double *)c->a_ub[d], S.NF[d], c->a_sum + 1 + d);
+        double h[4];
+        HIPCHK(hipMemcpyAsync(h, c->a_sum, sizeof(h), hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+        scal_bar[0] = h[0];
+        scal_bar[1] = (h[1] + h[2]) + h[3];
+    }
+    return check_launch("cav3d_rhs_vjp");
+}
+
+}  // extern "C"

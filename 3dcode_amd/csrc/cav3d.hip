@@ -1386,8 +1386,8 @@ struct cav3d_ctx {
     int ntiles = 0;
     int *sweep_err = nullptr; // set when a sweep wait gave up
     bool profile = false;
-    hipEvent_t ev[10] = {};
-    double prof_ms[8] = {};
+    hipEvent_t ev[14] = {};
+    double prof_ms[10] = {};  // [8] last cav3d_rhs_vjp, [9] last cav3d_calrhs (device ms)
     std::vector<void *> allocs;
     // grid from vertex coordinates (cav3d_set_grid / cav3d_build_metrics)
     std::vector<std::vector<int>> condesc;   // per block: ncon*22 raw
@@ -1398,6 +1398,10 @@ struct cav3d_ctx {
     double *xc = nullptr;     // cell centroids grd(2:4), SoA [3][NW]
     double *xbuf = nullptr;   // interface-exchange staging
     long xbuf_n = 0;
+    // adjoint work arrays (allocated by the first cav3d_rhs_vjp)
+    bool adj_ready = false;
+    double *a_rb = nullptr, *a_fb[3] = {}, *a_db[3] = {}, *a_ub[3] = {}, *a_cb = nullptr, *a_cdb = nullptr,
+           *a_pb = nullptr, *a_sum = nullptr;
 };
 
 namespace {
@@ -1865,7 +1869,7 @@ int cav3d_create(const cav3d_params *prm, int nblocks, const cav3d_block *blocks
         hipMemcpyAsync(S.con, c->cons.data(), c->cons.size() * sizeof(Con), hipMemcpyHostToDevice, c->st) != hipSuccess) {
         cav3d_destroy(c); return fail("interface upload failed", -3);
     }
-    for (int q = 0; q < 10; q++) hipEventCreate(&c->ev[q]);
+    for (int q = 0; q < 14; q++) hipEventCreate(&c->ev[q]);
     if (hipStreamSynchronize(c->st) != hipSuccess) { cav3d_destroy(c); return fail("device set-up failed", -3); }
     *out = c;
     return 0;
@@ -1878,7 +1882,7 @@ void cav3d_destroy(cav3d_ctx *c)
     for (void *p : c->allocs) hipFree(p);
     if (c->stage) hipFree(c->stage);
     if (c->xbuf) hipFree(c->xbuf);
-    for (int q = 0; q < 10; q++) if (c->ev[q]) hipEventDestroy(c->ev[q]);
+    for (int q = 0; q < 14; q++) if (c->ev[q]) hipEventDestroy(c->ev[q]);
     if (c->st) hipStreamDestroy(c->st);
     delete c;
 }
@@ -2015,7 +2019,15 @@ int cav3d_calrhs(cav3d_ctx *c, double time, int b, double *res, double *icav, do
     const BlkD &B = c->blk[b];
     int rc;
     if ((rc = ensure_stage(c, (long)c->npv * B.nc))) return rc;
+    if (c->profile) hipEventRecord(c->ev[12], c->st);
     launch_rhs(c);
+    if (c->profile) {
+        hipEventRecord(c->ev[13], c->st);
+        hipEventSynchronize(c->ev[13]);
+        float ms;
+        hipEventElapsedTime(&ms, c->ev[12], c->ev[13]);
+        c->prof_ms[9] = ms;
+    }
     struct { double *h; const double *d; int m; } outs[4] = {{res, S.res, c->npv}, {icav, S.icav, 4}, {itt, S.itt, 4}, {omega_cut, S.ocut, 1}};
     for (auto &o : outs) {
         if (!o.h) continue;
@@ -2139,7 +2151,7 @@ int cav3d_profile(cav3d_ctx *c, int enable, double *out)
 {
     if (!c) return fail("null context", -1);
     c->profile = enable != 0;
-    if (out) for (int q = 0; q < 8; q++) out[q] = c->prof_ms[q];
+    if (out) for (int q = 0; q < 10; q++) out[q] = c->prof_ms[q];
     return 0;
 }
 
@@ -2154,3 +2166,4 @@ int cav3d_synchronize(cav3d_ctx *c)
 
 #include "metrics.hpp"
 #include "post.hpp"
+#include "adjoint.hpp"

@@ -34,12 +34,149 @@ enum { D_RHO = 0, D_H, D_C2, D_RP, D_RT, D_RY1, D_RY2, D_HP, D_HT, D_HY1, D_HY2,
 
 struct Par {
     int npv, ntv, iturb, nscheme, nmuscl, nlim, local, prec, ncav, nsteady, cav_clip, ides;
+    int uref_seed;   // dual lane + 1 that carries d/d(uref) in the adjoint's passes (0: none)
     double cfl, pref, uref, str, dt_phy, c_v, c_c, dp_ref, t_ref, t_crit;
     double bc_pv[9], bc_pressure, emutref, bdf_b;
 };
 
 template <class R> HD R mx(R a, R b) { return (b > a) ? b : a; }
 template <class R> HD R mn(R a, R b) { return (b < a) ? b : a; }
+
+// ------------------------------------------------------------ dual numbers
+// (the double/float overloads stay visible beside the dual ones below)
+using ::sqrt;
+using ::fabs;
+using ::exp;
+using ::log;
+using ::tanh;
+using ::pow;
+using ::signbit;
+
+// Forward-mode dual number with W tangent lanes.  The adjoint (VJP) kernels
+// instantiate the templates below with it: each pass seeds up to W inputs of
+// a pointwise function and reads the W directional derivatives of every
+// output, so the derivative follows the same branches, clamps and
+// min/max selections as the value.
+template <int W> struct Dn {
+    double v, d[W];
+    HD Dn() {}
+    HD Dn(double x) : v(x)
+    {
+#pragma unroll
+        for (int q = 0; q < W; q++) d[q] = 0.0;
+    }
+    HD explicit operator double() const { return v; }
+    HD static Dn seed(double x, int lane)
+    {
+        Dn r(x);
+        if (lane >= 0 && lane < W) r.d[lane] = 1.0;
+        return r;
+    }
+};
+template <int W> HD Dn<W> dn_map(const Dn<W> &a, double v, double da)
+{
+    Dn<W> r;
+    r.v = v;
+#pragma unroll
+    for (int q = 0; q < W; q++) r.d[q] = da * a.d[q];
+    return r;
+}
+template <int W> HD Dn<W> operator-(const Dn<W> &a) { return dn_map(a, -a.v, -1.0); }
+template <int W> HD Dn<W> operator+(const Dn<W> &a, const Dn<W> &b)
+{
+    Dn<W> r;
+    r.v = a.v + b.v;
+#pragma unroll
+    for (int q = 0; q < W; q++) r.d[q] = a.d[q] + b.d[q];
+    return r;
+}
+template <int W> HD Dn<W> operator-(const Dn<W> &a, const Dn<W> &b)
+{
+    Dn<W> r;
+    r.v = a.v - b.v;
+#pragma unroll
+    for (int q = 0; q < W; q++) r.d[q] = a.d[q] - b.d[q];
+    return r;
+}
+template <int W> HD Dn<W> operator*(const Dn<W> &a, const Dn<W> &b)
+{
+    Dn<W> r;
+    r.v = a.v * b.v;
+#pragma unroll
+    for (int q = 0; q < W; q++) r.d[q] = a.d[q] * b.v + a.v * b.d[q];
+    return r;
+}
+template <int W> HD Dn<W> operator/(const Dn<W> &a, const Dn<W> &b)
+{
+    Dn<W> r;
+    r.v = a.v / b.v;
+    const double ib = 1.0 / b.v;
+#pragma unroll
+    for (int q = 0; q < W; q++) r.d[q] = (a.d[q] - r.v * b.d[q]) * ib;
+    return r;
+}
+template <int W> HD Dn<W> operator+(const Dn<W> &a, double b) { return a + Dn<W>(b); }
+template <int W> HD Dn<W> operator+(double a, const Dn<W> &b) { return Dn<W>(a) + b; }
+template <int W> HD Dn<W> operator-(const Dn<W> &a, double b) { return a - Dn<W>(b); }
+template <int W> HD Dn<W> operator-(double a, const Dn<W> &b) { return Dn<W>(a) - b; }
+template <int W> HD Dn<W> operator*(const Dn<W> &a, double b) { return dn_map(a, a.v * b, b); }
+template <int W> HD Dn<W> operator*(double a, const Dn<W> &b) { return dn_map(b, a * b.v, a); }
+template <int W> HD Dn<W> operator/(const Dn<W> &a, double b) { return a / Dn<W>(b); }
+template <int W> HD Dn<W> operator/(double a, const Dn<W> &b) { return Dn<W>(a) / b; }
+#define CAV_DN_CMP(OP)                                                                        \
+    template <int W> HD bool operator OP(const Dn<W> &a, const Dn<W> &b) { return a.v OP b.v; } \
+    template <int W> HD bool operator OP(const Dn<W> &a, double b) { return a.v OP b; }         \
+    template <int W> HD bool operator OP(double a, const Dn<W> &b) { return a OP b.v; }
+CAV_DN_CMP(<)
+CAV_DN_CMP(>)
+CAV_DN_CMP(<=)
+CAV_DN_CMP(>=)
+CAV_DN_CMP(==)
+CAV_DN_CMP(!=)
+#undef CAV_DN_CMP
+template <int W> HD Dn<W> sqrt(const Dn<W> &a) { double s = ::sqrt(a.v); return dn_map(a, s, 0.5 / s); }
+template <int W> HD Dn<W> fabs(const Dn<W> &a) { return dn_map(a, ::fabs(a.v), a.v < 0.0 ? -1.0 : 1.0); }
+template <int W> HD Dn<W> exp(const Dn<W> &a) { double e = ::exp(a.v); return dn_map(a, e, e); }
+template <int W> HD Dn<W> log(const Dn<W> &a) { return dn_map(a, ::log(a.v), 1.0 / a.v); }
+template <int W> HD Dn<W> tanh(const Dn<W> &a) { double t = ::tanh(a.v); return dn_map(a, t, 1.0 - t * t); }
+template <int W> HD Dn<W> pow(const Dn<W> &a, double b)
+{
+    double p = ::pow(a.v, b);
+    return dn_map(a, p, b == 0.0 ? 0.0 : b * ::pow(a.v, b - 1.0));
+}
+template <int W> HD Dn<W> pow(const Dn<W> &a, const Dn<W> &b)
+{
+    Dn<W> r;
+    r.v = ::pow(a.v, b.v);
+    const double la = a.v > 0.0 ? ::log(a.v) : 0.0, ea = b.v * ::pow(a.v, b.v - 1.0);
+#pragma unroll
+    for (int q = 0; q < W; q++) r.d[q] = ea * a.d[q] + r.v * la * b.d[q];
+    return r;
+}
+template <int W> HD bool signbit(const Dn<W> &a) { return ::signbit(a.v); }
+
+template <class R> struct IsDual { static constexpr bool v = false; };
+template <int W> struct IsDual<Dn<W>> { static constexpr bool v = true; };
+
+// the preconditioning reference speed as R; a dual carries d/d(uref) on the
+// lane Par::uref_seed names
+template <class R> struct RefSpeed {
+    HD static R of(const Par &P) { return R(P.uref); }
+};
+template <int W> struct RefSpeed<Dn<W>> {
+    HD static Dn<W> of(const Par &P) { return Dn<W>::seed(P.uref, P.uref_seed - 1); }
+};
+
+// the face EOS skips a phase whose mass fraction is zero (its terms are exact
+// zeros); a dual keeps the phase when the fraction's derivative is non-zero
+template <class R> HD bool has_mass(R y) { return y != R(0); }
+template <int W> HD bool has_mass(const Dn<W> &y)
+{
+    bool nz = y.v != 0.0;
+#pragma unroll
+    for (int q = 0; q < W; q++) nz = nz || y.d[q] != 0.0;
+    return nz;
+}
 
 
 // ------------------------------------------------------------ IF97 tables
@@ -120,16 +257,18 @@ constexpr double C_RT2B = (double)0.0074304703986495518f;
 
 template <class R> struct E2 { R rho, h, drdp, drdt, dhdp, dhdt; };
 
-// IF97 region-4 saturation pressure, always double (src/eos_module.f90:2266-2279)
-HD double sat_p(double t)
+// IF97 region-4 saturation pressure (src/eos_module.f90:2266-2279): double
+// on every evaluation path (the face EOS rounds its temperature to double for
+// it); a dual only in the adjoint's cavitation source
+template <class R> HD R sat_p(R t)
 {
-    double tt = mn(mx(t, 273.15), 647.096);
-    double th = tt - 0.23855557567849e0 / (tt - 0.65017534844798e3);
-    double a = th * th + 0.11670521452767e4 * th - 0.72421316703206e6;
-    double b = -0.17073846940092e2 * (th * th) + 0.12020824702470e5 * th - 0.32325550322333e7;
-    double c = 0.14915108613530e2 * (th * th) - 0.48232657361591e4 * th + 0.40511340542057e6;
-    double x = 2.0 * c / (-b + sqrt(b * b - 4.0 * a * c));
-    double x2 = x * x;
+    R tt = mn(mx(t, R(273.15)), R(647.096));
+    R th = tt - 0.23855557567849e0 / (tt - 0.65017534844798e3);
+    R a = th * th + 0.11670521452767e4 * th - 0.72421316703206e6;
+    R b = -0.17073846940092e2 * (th * th) + 0.12020824702470e5 * th - 0.32325550322333e7;
+    R c = 0.14915108613530e2 * (th * th) - 0.48232657361591e4 * th + 0.40511340542057e6;
+    R x = 2.0 * c / (-b + sqrt(b * b - 4.0 * a * c));
+    R x2 = x * x;
     return 1.0e6 * (x2 * x2);
 }
 
@@ -174,12 +313,13 @@ template <class R> HD R powi(R x, int m)
 // operation order so the FP64 path reproduces its rounding.
 template <class R> HD E2<R> liquid(R p, R t)
 {
+    constexpr int EOS_UNROLL = IsDual<R>::v ? 1 : 64;   // dual instantiations stay rolled
     R pi = p * R(0.00000006049607);
     R tau = R(1386.0) / t;
     R pi1 = R(1) / (R(7.1) - pi);
     R tau1 = R(1) / (tau - R(1.222));
     R gt = 0, gp = 0, gpp = 0, gtt = 0, gpt = 0;
-#pragma unroll
+#pragma unroll EOS_UNROLL
     for (int k = 0; k < 34; k++) {
         const int I = R1C[k].i, J = R1C[k].j;
         R g = R(R1C[k].n) * powi(R(7.1) - pi, I) * powi(tau - R(1.222), J);
@@ -210,6 +350,7 @@ template <class R> HD E2<R> liquid(R p, R t)
 // equation whenever p > p_sat(T), in the reference's operation order.
 template <class R> HD E2<R> vapour(R p, R t, double psat)
 {
+    constexpr int EOS_UNROLL = IsDual<R>::v ? 1 : 64;   // dual instantiations stay rolled
     R pi = p * R(1.0e-6);
     R tau = R(540.0) / t;
     R pi1 = R(1.0e6) / p;
@@ -218,7 +359,7 @@ template <class R> HD E2<R> vapour(R p, R t, double psat)
     R g0t = 0, g0tt = 0;
     R grt = 0, grp = 0, grpp = 0, grtt = 0, grpt = 0;
     const bool meta = (double)p > psat;
-#pragma unroll
+#pragma unroll EOS_UNROLL
     for (int k = 0; k < 9; k++) {
         const int J = R2J0[k];
         double n = R2N0[k];
@@ -243,10 +384,10 @@ template <class R> HD E2<R> vapour(R p, R t, double psat)
         grpt = grpt + gr5;
     };
     if (meta) {
-#pragma unroll
+#pragma unroll EOS_UNROLL
         for (int k = 0; k < 13; k++) term(R2MC[k]);
     } else {
-#pragma unroll
+#pragma unroll EOS_UNROLL
         for (int k = 0; k < 43; k++) term(R2C[k]);
     }
     R gp1 = R(1) / (pi1 + grp);
@@ -279,23 +420,24 @@ template <class R> HD E2<R> gas(R p, R t)
 // (src/eos_module.f90:1508-1537), in the reference's operation order.
 template <class R> HD void iaps85(R t, R rho, R &mu, R &kap)
 {
+    constexpr int EOS_UNROLL = IsDual<R>::v ? 1 : 64;   // dual instantiations stay rolled
     R tt = t * R(0.00154495032985);
     R rr = mx(R(1.0e-3), mn(R(1332.4), rho)) * R(0.00314699949333);
     R tt1 = R(647.27) / t;
     R mu0 = sqrt(tt) / (R(TPH[0]) + R(TPH[1]) * tt1 + R(TPH[2]) * (tt1 * tt1) + R(TPH[3]) * (tt1 * tt1 * tt1));
     R mu1 = 0;
-#pragma unroll
+#pragma unroll EOS_UNROLL
     for (int i = 0; i <= 5; i++)
-#pragma unroll
+#pragma unroll EOS_UNROLL
         for (int j = 0; j <= 6; j++)
             mu1 = mu1 + R(TPHH[i][j]) * powi(tt1 - R(1), i) * powi(rr - R(1), j);
     mu1 = exp(rr * mu1);
     mu = R(55.071e-6) * mu0 * mu1 * R(1);
     R lam0 = sqrt(tt) / (R(TPL[0]) + R(TPL[1]) * tt1 + R(TPL[2]) * (tt1 * tt1) + R(TPL[3]) * (tt1 * tt1 * tt1));
     R lam1 = 0;
-#pragma unroll
+#pragma unroll EOS_UNROLL
     for (int i = 0; i <= 4; i++)
-#pragma unroll
+#pragma unroll EOS_UNROLL
         for (int j = 0; j <= 5; j++)
             lam1 = lam1 + R(TPLL[i][j]) * powi(tt1 - R(1), i) * powi(rr - R(1), j);
     lam1 = exp(rr * lam1);
@@ -356,8 +498,8 @@ template <class R> HD void face_rho_h(R p, R t, R y1, R y2, R &rho, R &h)
     E2<R> l = liquid(p, t);
     R yl = R(1) - y1 - y2;
     R iv = 0, ig = 0, hv = 0, hg = 0;
-    if (y1 != R(0)) { E2<R> v = vapour(p, t, sat_p((double)t)); iv = R(1) / v.rho; hv = v.h; }
-    if (y2 != R(0)) { E2<R> g = gas(p, t); ig = R(1) / g.rho; hg = g.h; }
+    if (has_mass(y1)) { E2<R> v = vapour(p, t, sat_p((double)t)); iv = R(1) / v.rho; hv = v.h; }
+    if (has_mass(y2)) { E2<R> g = gas(p, t); ig = R(1) / g.rho; hg = g.h; }
     rho = R(1) / (yl * (R(1) / l.rho) + y1 * iv + y2 * ig);
     h = yl * l.h + y1 * hv + y2 * hg;
 }
@@ -366,7 +508,7 @@ template <class R> HD void face_rho_h(R p, R t, R y1, R y2, R &rho, R &h)
 template <class R> HD R face_c2(R p, R t, R y1, R y2)
 {
     E2<R> l = liquid(p, t), v{}, g{};
-    const bool hv = (y1 != R(0)), hg = (y2 != R(0));
+    const bool hv = has_mass(y1), hg = has_mass(y2);
     if (hv) v = vapour(p, t, sat_p((double)t));
     if (hg) g = gas(p, t);
     R il = R(1) / l.rho, iv = hv ? R(1) / v.rho : R(0), ig = hg ? R(1) / g.rho : R(0);
@@ -390,7 +532,8 @@ template <class R> HD R face_c2(R p, R t, R y1, R y2)
 // ------------------------------------------------------------ preconditioning
 template <class R> HD R sndp2(const Par &P, R snd2, R u2)
 {
-    R ur2 = R(P.uref) * R(P.uref);
+    const R ur = RefSpeed<R>::of(P);
+    R ur2 = ur * ur;
     if (P.prec == 0) return snd2;
     if (P.prec == 1) return mn(snd2, mx(u2, ur2));
     return mn(snd2, mx(mx(u2, ur2), R(P.str) * R(P.str)));
@@ -401,7 +544,8 @@ template <class R> HD R sndp2(const Par &P, R snd2, R u2)
 template <class R> HD R sndp2_du(const Par &P, R snd2, R u2)
 {
     if (P.prec == 0) return snd2;
-    return mn(snd2, mx(u2, R(P.uref) * R(P.uref)));
+    const R ur = RefSpeed<R>::of(P);
+    return mn(snd2, mx(u2, ur * ur));
 }
 
 // ------------------------------------------------------------ reconstruction
@@ -1261,26 +1405,28 @@ HD void visc_flux(const Par &P, const R *nx, const R e4[4][3], const R t4[4][3],
 
 // ------------------------------------------------------------ cell sources
 // x7[q][n]: cell (q=1), its i-, i+, j-, j+, k-, k+ neighbours (q = 0,2..6)
-HD void gg_grad(const double x7[7][NPV], int n, const double f6[6][3], double ivol, double *g)
+template <class R>
+HD void gg_grad(const R x7[7][NPV], int n, const double f6[6][3], double ivol, R *g)
 {
-    double a1 = 0.5 * (x7[0][n] + x7[1][n]), a2 = 0.5 * (x7[2][n] + x7[1][n]);
-    double a3 = 0.5 * (x7[3][n] + x7[1][n]), a4 = 0.5 * (x7[4][n] + x7[1][n]);
-    double a5 = 0.5 * (x7[5][n] + x7[1][n]), a6 = 0.5 * (x7[6][n] + x7[1][n]);
+    R a1 = 0.5 * (x7[0][n] + x7[1][n]), a2 = 0.5 * (x7[2][n] + x7[1][n]);
+    R a3 = 0.5 * (x7[3][n] + x7[1][n]), a4 = 0.5 * (x7[4][n] + x7[1][n]);
+    R a5 = 0.5 * (x7[5][n] + x7[1][n]), a6 = 0.5 * (x7[6][n] + x7[1][n]);
     for (int m = 0; m < 3; m++)
         g[m] = (a2 * f6[1][m] + a4 * f6[3][m] + a6 * f6[5][m] - a1 * f6[0][m] - a3 * f6[2][m] - a5 * f6[4][m]) * ivol;
 }
 
 // Merkle: returns r_v - r_c (volume-integrated) and the coupling icav[4]
-HD void merkle(const Par &P, const double *pc, const double *d, double vol, double &rc, double &rv, double *icav)
+template <class R>
+HD void merkle(const Par &P, const R *pc, const R *d, double vol, R &rc, R &rv, R *icav)
 {
     rc = 0.0; rv = 0.0;
     icav[0] = icav[1] = icav[2] = icav[3] = 0.0;
     if (pc[4] >= P.t_crit) return;
-    double pww = sat_p(pc[4]);
-    double irho = 1.0 / d[D_RHO];
-    double pc_ = mx(pc[0] + P.pref - pww, 0.0), pv_ = mx(pww - pc[0] - P.pref, 0.0);
-    double r_c = P.c_c * d[D_RHO] * pc[5] * pc_ * P.dp_ref * P.t_ref;
-    double r_v = P.c_v * d[D_RHO] * (1.0 - pc[5] - pc[6]) * pv_ * P.dp_ref * P.t_ref;
+    R pww = sat_p(pc[4]);
+    R irho = 1.0 / d[D_RHO];
+    R pc_ = mx(pc[0] + P.pref - pww, R(0.0)), pv_ = mx(pww - pc[0] - P.pref, R(0.0));
+    R r_c = P.c_c * d[D_RHO] * pc[5] * pc_ * P.dp_ref * P.t_ref;
+    R r_v = P.c_v * d[D_RHO] * (1.0 - pc[5] - pc[6]) * pv_ * P.dp_ref * P.t_ref;
     if (P.cav_clip == 0) {   // as written in src/cav_module.f90:151-152 (pc[4] is T)
         r_c = mn(r_c, d[D_RHO] * pc[4]);
         r_v = mn(r_v, d[D_RHO] * (1.0 - pc[4] - pc[5]));
@@ -1303,7 +1449,7 @@ HD void merkle(const Par &P, const double *pc, const double *d, double vol, doub
         icav[3] = r_v * d[D_RY2] * irho - P.c_v * d[D_RHO] * pv_ * P.dp_ref * P.t_ref;
     }
     double s = (icav[2] < 0.0 || (icav[2] == 0.0 && signbit(icav[2]))) ? -1.0 : 1.0;
-    for (int q = 0; q < 4; q++) icav[q] *= s;
+    for (int q = 0; q < 4; q++) icav[q] = icav[q] * s;
 }
 
 // The other mass-transfer models (src/cav_module.f90:173-357), on the full
@@ -1452,10 +1598,10 @@ HD void cav_model(const Par &P, const double *pc, const double *dv, double vol, 
 // (1), DDES (2), IDDES (3) or S-IDDES (4) blend with the LES scale C_DES h_max,
 // h from the cell volume over its mean face areas.  Constant integer powers
 // as gfortran expands them (x**3 = (x*x)*x, x**10 = x5*x5 with x5 = x3*x2).
-HD double des_length(int ides, double k, double om, const double f6[6][3], double vol, double ydns, double mu,
-                     double mut, double bigf, double s2, double w2)
+template <class R>
+HD R des_length(int ides, R k, R om, const double f6[6][3], double vol, double ydns, R mu, R mut, R bigf, R s2, R w2)
 {
-    double tl_rans = sqrt(k) / 9.0 / om * 100.0;
+    R tl_rans = sqrt(k) / 9.0 / om * 100.0;
     if (ides == 0) return tl_rans;
     double gs[3];
     for (int q = 0; q < 3; q++) {
@@ -1464,86 +1610,86 @@ HD double des_length(int ides, double k, double om, const double f6[6][3], doubl
         gs[q] = vol / gs[q];
     }
     double hmax = mx(mx(gs[0], gs[1]), gs[2]);
-    if (ides == 1) return mn(tl_rans, 0.61 * hmax);
-    double cdes = 0.78 * bigf + 0.61 * (1.0 - bigf);
-    double den = 0.41 * 0.41 * (ydns * ydns) * sqrt(0.5 * (s2 + w2));
+    if (ides == 1) return mn(tl_rans, R(0.61 * hmax));
+    R cdes = 0.78 * bigf + 0.61 * (1.0 - bigf);
+    R den = 0.41 * 0.41 * (ydns * ydns) * sqrt(0.5 * (s2 + w2));
     if (ides == 2) {
-        double tl_les = cdes * hmax;
-        double rd = (mu + mut) / den, x = 20.0 * rd;
-        double fd = 1.0 - tanh(x * x * x);
-        return tl_rans - fd * mx(0.0, tl_rans - tl_les);
+        R tl_les = cdes * hmax;
+        R rd = (mu + mut) / den, x = 20.0 * rd;
+        R fd = 1.0 - tanh(x * x * x);
+        return tl_rans - fd * mx(R(0.0), tl_rans - tl_les);
     }
-    double tl_les = cdes * mn(0.15 * mx(ydns, hmax), hmax);
+    R tl_les = cdes * mn(0.15 * mx(ydns, hmax), hmax);
     double alpha = 0.25 - ydns / hmax;
     double fb = mn(2.0 * exp(-9.0 * (alpha * alpha)), 1.0);
-    double rdt = mut / den, xt = 20.0 * rdt;
-    double fdt = 1.0 - tanh(xt * xt * xt);
-    double fd = mx(1.0 - fdt, fb);
+    R rdt = mut / den, xt = 20.0 * rdt;
+    R fdt = 1.0 - tanh(xt * xt * xt);
+    R fd = mx(1.0 - fdt, R(fb));
     if (ides == 4) return fd * tl_rans + (1.0 - fd) * tl_les;
-    double rdl = mu / den;
-    double xl = 25.0 * rdl, xl2 = xl * xl, xl3 = xl2 * xl, xl5 = xl3 * xl2;
-    double fl = tanh(xl5 * xl5);
-    double x3 = 3.4969 * rdt;
-    double ft = tanh(x3 * x3 * x3);
-    double fe2 = 1.0 - mx(ft, fl);
+    R rdl = mu / den;
+    R xl = 25.0 * rdl, xl2 = xl * xl, xl3 = xl2 * xl, xl5 = xl3 * xl2;
+    R fl = tanh(xl5 * xl5);
+    R x3 = 3.4969 * rdt;
+    R ft = tanh(x3 * x3 * x3);
+    R fe2 = 1.0 - mx(ft, fl);
     double fe1 = alpha >= 0.0 ? 2.0 * exp(-11.09 * (alpha * alpha)) : 2.0 * exp(-9.0 * (alpha * alpha));
-    double fe = fe2 * mx(fe1 - 1.0, 0.0);
+    R fe = fe2 * mx(fe1 - 1.0, 0.0);
     return fd * (1.0 + fe) * tl_rans + (1.0 - fd) * tl_les;
 }
 
 // k-omega SST source: src[2], itt[4], omega_cut.  DES: the length-scale
 // variants are compiled in (the RANS-only instantiation keeps its registers).
-template <bool DES = false>
-HD void sst_source(const Par &P, const double x7[7][NPV], const double f6[6][3], double vol, double ydns, const double *d,
-                   double mut, double *src, double *itt, double &ocut)
+template <bool DES = false, class R = double>
+HD void sst_source(const Par &P, const R x7[7][NPV], const double f6[6][3], double vol, double ydns, const R *d,
+                   R mut, R *src, R *itt, R &ocut)
 {
     const double c23 = 2.0 / 3.0, c59 = 5.0 / 9.0;
     double iv = 1.0 / vol;
-    double gu[3], gv[3], gw[3], gk[3], go[3];
+    R gu[3], gv[3], gw[3], gk[3], go[3];
     gg_grad(x7, 7, f6, iv, gk);
     gg_grad(x7, 8, f6, iv, go);
     gg_grad(x7, 1, f6, iv, gu);
     gg_grad(x7, 2, f6, iv, gv);
     gg_grad(x7, 3, f6, iv, gw);
-    double div = gu[0] + gv[1] + gw[2];
-    double s2 = 2.0 * (gu[0] * gu[0] + gv[1] * gv[1] + gw[2] * gw[2]) + (gu[1] + gv[0]) * (gu[1] + gv[0])
-              + (gu[2] + gw[0]) * (gu[2] + gw[0]) + (gv[2] + gw[1]) * (gv[2] + gw[1]);
-    double k = x7[1][7], om = x7[1][8], rho = d[D_RHO];
-    double prod = mut * (s2 - c23 * (div * div)) - c23 * rho * k * div;
+    R div = gu[0] + gv[1] + gw[2];
+    R s2 = 2.0 * (gu[0] * gu[0] + gv[1] * gv[1] + gw[2] * gw[2]) + (gu[1] + gv[0]) * (gu[1] + gv[0])
+         + (gu[2] + gw[0]) * (gu[2] + gw[0]) + (gv[2] + gw[1]) * (gv[2] + gw[1]);
+    R k = x7[1][7], om = x7[1][8], rho = d[D_RHO];
+    R prod = mut * (s2 - c23 * (div * div)) - c23 * rho * k * div;
     prod = mn(prod, 0.9 * rho * k * om);
     ocut = sqrt(s2 - c23 * (div * div));
-    double pkw = gk[0] * go[0] + gk[1] * go[1] + gk[2] * go[2];
-    double cdkw = mx(1.712 * rho / om * pkw, 1.0e-10);
-    double t1 = sqrt(k) / (0.09 * om * ydns);
-    double t2 = 500.0 * d[D_MU] / rho / (om * (ydns * ydns));
-    double t3 = (3.424 * rho * k) / (cdkw * (ydns * ydns));
-    double a1 = mn(mx(t1, t2), t3);
-    double a2 = a1 * a1;
-    double bf = tanh(a2 * a2);
-    double ta = bf * c59 + (1.0 - bf) * 0.44;
-    double tb = bf * 0.075 + (1.0 - bf) * 0.0828;
-    double ki = 1.0 / k, oi = 1.0 / om, ei = 1.0 / mut;
-    double tl;
+    R pkw = gk[0] * go[0] + gk[1] * go[1] + gk[2] * go[2];
+    R cdkw = mx(1.712 * rho / om * pkw, R(1.0e-10));
+    R t1 = sqrt(k) / (0.09 * om * ydns);
+    R t2 = 500.0 * d[D_MU] / rho / (om * (ydns * ydns));
+    R t3 = (3.424 * rho * k) / (cdkw * (ydns * ydns));
+    R a1 = mn(mx(t1, t2), t3);
+    R a2 = a1 * a1;
+    R bf = tanh(a2 * a2);
+    R ta = bf * c59 + (1.0 - bf) * 0.44;
+    R tb = bf * 0.075 + (1.0 - bf) * 0.0828;
+    R ki = 1.0 / k, oi = 1.0 / om, ei = 1.0 / mut;
+    R tl;
     if constexpr (DES) {
-        double w2 = (gu[1] - gv[0]) * (gu[1] - gv[0]) + (gu[2] - gw[0]) * (gu[2] - gw[0]) + (gv[2] - gw[1]) * (gv[2] - gw[1]);
-        tl = des_length(P.ides, k, om, f6, vol, ydns, d[D_MU], mut, bf, s2, w2);
+        R w2 = (gu[1] - gv[0]) * (gu[1] - gv[0]) + (gu[2] - gw[0]) * (gu[2] - gw[0]) + (gv[2] - gw[1]) * (gv[2] - gw[1]);
+        tl = des_length<R>(P.ides, k, om, f6, vol, ydns, d[D_MU], mut, bf, s2, w2);
     } else {
         tl = sqrt(k) / 9.0 / om * 100.0;
     }
-    double dest = rho * pow(k, 1.5) / tl;
+    R dest = rho * pow(k, 1.5) / tl;
     src[0] = (prod - dest) * vol;
     src[1] = (ta * rho * ei * prod - tb * rho * (om * om) + 1.712 * (1.0 - bf) * rho * oi * pkw) * vol;
-    double dpdk = prod * ki;
-    double dpdo = -((prod + c23 * rho * k * div) * oi);
+    R dpdk = prod * ki;
+    R dpdo = -((prod + c23 * rho * k * div) * oi);
     itt[0] = dpdk - dest * ki;
     itt[1] = dpdo - dest * oi;
     itt[2] = 0.0;
     itt[3] = ta * rho * ei * dpdo + ta * rho * ei * prod * oi - 2.0 * tb * rho * om - 1.712 * (1.0 - bf) * rho * (oi * oi) * pkw;
-    double t = itt[0] + itt[3], det = itt[0] * itt[3] - itt[1] * itt[2];
-    double sq = sqrt(mx(0.25 * (t * t) - det, 0.0));
-    double l1 = 0.5 * t + sq, l2 = 0.5 * t - sq;
+    R t = itt[0] + itt[3], det = itt[0] * itt[3] - itt[1] * itt[2];
+    R sq = sqrt(mx(0.25 * (t * t) - det, R(0.0)));
+    R l1 = 0.5 * t + sq, l2 = 0.5 * t - sq;
     double s1 = signbit(l1) ? -1.0 : 1.0, s2s = signbit(l2) ? -1.0 : 1.0;
-    itt[0] *= s1; itt[1] *= s1; itt[2] *= s2s; itt[3] *= s2s;
+    itt[0] = itt[0] * s1; itt[1] = itt[1] * s1; itt[2] = itt[2] * s2s; itt[3] = itt[3] * s2s;
 }
 
 // SST eddy viscosity mu_t = 0.31 rho k / max(0.31 omega, S F2)

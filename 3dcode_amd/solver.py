@@ -30,7 +30,7 @@ SURFACE = 0x100   # CAV3D_BC_SURFACE: OR'ed into a kind, marks the force-integra
 
 EXPORTS = [
     "cav3d_create", "cav3d_set_metrics", "cav3d_set_grid", "cav3d_build_metrics", "cav3d_get_metrics",
-    "cav3d_integrals", "cav3d_export", "cav3d_import",
+    "cav3d_integrals", "cav3d_export", "cav3d_import", "cav3d_rhs_vjp",
     "cav3d_put_state", "cav3d_get_state", "cav3d_timeinteg",
     "cav3d_setbc", "cav3d_calrhs", "cav3d_timestep", "cav3d_residual", "cav3d_unsteady_shift",
     "cav3d_eos", "cav3d_eos_host", "cav3d_profile", "cav3d_synchronize", "cav3d_last_error",
@@ -76,6 +76,7 @@ def lib(build_if_missing: bool = True):
     L.cav3d_integrals.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_double, _D]
     L.cav3d_export.argtypes = [C.c_void_p, C.c_char_p, C.c_int, C.c_int, C.c_double]
     L.cav3d_import.argtypes = [C.c_void_p, C.c_char_p, _I, _I, _D]
+    L.cav3d_rhs_vjp.argtypes = [C.c_void_p, _D, _D, _D]
     L.cav3d_put_state.argtypes = [C.c_void_p, C.c_int, _D, _D, _D, _D]
     L.cav3d_get_state.argtypes = [C.c_void_p, C.c_int, _D, _D, _D, _D]
     L.cav3d_timeinteg.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, _D]
@@ -274,8 +275,59 @@ class Solver:
         _check(lib().cav3d_import(self.h, os.fsencode(path), C.byref(nps), C.byref(nts), C.byref(tp)))
         return nps.value, nts.value, tp.value
 
+    # ---- adjoint
+    def rhs_vjp(self, res_bars):
+        """(d res/d pv)^T res_bar over every block (cav3d_rhs_vjp).
+
+        res_bars: one array per block in calrhs's res layout.  Returns the
+        per-block pv_bar arrays (pv layout, ghosted) and [d/d p_ref, d/d u_ref]
+        of res_bar . res."""
+        blocks = self.case.blocks
+        rb = np.ascontiguousarray(np.concatenate([np.asarray(r, dtype=np.float64).ravel() for r in res_bars]))
+        nc = [self.npv * (b.imax - 1) * (b.jmax - 1) * (b.kmax - 1) for b in blocks]
+        if rb.size != sum(nc):
+            raise ValueError(f"res_bar: expected {sum(nc)} values, got {rb.size}")
+        ng = [self.npv * (b.imax + 5) * (b.jmax + 5) * (b.kmax + 5) for b in blocks]
+        pb = np.zeros(sum(ng))
+        sc = np.zeros(2)
+        _check(lib().cav3d_rhs_vjp(self.h, _p(rb), _p(pb), _p(sc)))
+        return np.split(pb, np.cumsum(ng)[:-1]), sc
+
+    def frozen(self):
+        """The parts of the current state the adjoint's map holds fixed: the
+        eddy viscosity (ghosted, per block) and the BDF history (per block)."""
+        muts, qqs = [], []
+        for ib in range(len(self.case.blocks)):
+            _, _, tv, qq = self.get_state(ib)
+            muts.append(tv.reshape(-1, self.ntv)[:, 2].copy() if self.ntv == 3 else None)
+            qqs.append(qq.copy() if self.case.params["nsteady"] >= 1 else None)
+        return {"mut": muts, "qq": qqs}
+
+    def residual_of(self, pvs, muts=None, qqs=None):
+        """The map the adjoint differentiates: every block's residual for the
+        ghosted states pvs, with the cell properties re-derived by the
+        device EOS and the eddy viscosity muts (per block, ghosted) and BDF
+        history qqs held as given.  Loads the states into the context."""
+        pref = self.case.params["pref"]
+        out = []
+        for ib, pv in enumerate(pvs):
+            a = np.asarray(pv, dtype=np.float64).reshape(-1, self.npv)
+            dv, t2 = self.eos(a[:, 0] + pref, a[:, 4], a[:, 5], a[:, 6])
+            tv = None
+            if self.ntv:
+                tv = np.zeros((a.shape[0], self.ntv))
+                tv[:, :2] = t2
+                if self.ntv == 3:
+                    tv[:, 2] = muts[ib]
+            qq = None if qqs is None else qqs[ib]
+            self.put_state(ib, np.ascontiguousarray(a.ravel()), np.ascontiguousarray(dv.ravel()),
+                           None if tv is None else np.ascontiguousarray(tv.ravel()), qq)
+        for ib in range(len(pvs)):
+            out.append(self.calrhs(ib)[0])
+        return out
+
     def profile(self, enable=True):
-        out = np.zeros(8)
+        out = np.zeros(10)
         _check(lib().cav3d_profile(self.h, 1 if enable else 0, _p(out)))
         return out
 

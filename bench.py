@@ -171,6 +171,26 @@ def mixed_precision_run(pk, case, args, ms64):
     return out
 
 
+def adjoint_cost(pk, s, case, reps=3):
+    """Device time of one discrete-adjoint product (cav3d_rhs_vjp) against one
+    evaluation of the residual it differentiates (cav3d_calrhs), on the
+    current state, from HIP events on the library's stream (best of reps)."""
+    rng = np.random.default_rng(0)
+    rb = [rng.standard_normal(case.npv * (b.imax - 1) * (b.jmax - 1) * (b.kmax - 1)) for b in case.blocks]
+    s.profile(True)
+    s.rhs_vjp(rb)                 # the first call allocates the work arrays
+    vjp, rhs = [], []
+    for _ in range(reps):
+        s.rhs_vjp(rb)
+        vjp.append(s.profile(True)[8])
+        s.calrhs(0)
+        rhs.append(s.profile(True)[9])
+    s.profile(False)
+    v, r = min(vjp), min(rhs)
+    return {"vjp_ms": round(v, 4), "rhs_ms": round(r, 4), "adjoint_over_primal": round(v / r, 3),
+            "vjp_mcell_per_s": round(case.ncell / (v * 1e-3) / 1e6, 3), "arithmetic": "f64"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -245,9 +265,10 @@ def main():
     acc = np.zeros(8)
     for it in range(nprof):
         s.timeinteg(1, 1, 0.0, want_time=False)
-        acc += s.profile(True)
+        acc += s.profile(True)[:8]
     s.profile(False)
     phase = acc / nprof
+    adjoint = adjoint_cost(pk, s, case)
     s.close()
 
     # dominant kernel: the forward LU-SGS sweep, one launch over all cells.  Its
@@ -301,6 +322,7 @@ def main():
         "roofline": roofline, "cpu_baseline": cpu,
         "grid_metrics_ms": grid_ms,
         "mixed": mixed,
+        "adjoint": adjoint,
     }
     print(json.dumps(out), flush=True)
 

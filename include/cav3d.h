@@ -154,6 +154,24 @@ int cav3d_integrals(cav3d_ctx *ctx, double rhoref, double uref, double area, dou
 int cav3d_export(cav3d_ctx *ctx, const char *path, int nt_phy, int nt, double time);
 int cav3d_import(cav3d_ctx *ctx, const char *path, int *nps, int *nts, double *timeprev);
 
+/* Discrete adjoint (vector-Jacobian product) of the residual cav3d_calrhs
+ * returns.  No counterpart in the reference, which has no adjoint.
+ * The differentiated map is res(pv): every block's ghosted pv, with the cell
+ * properties (rho, h, c^2, EOS derivatives, mu, kappa) re-derived from pv by
+ * the EOS, and with the eddy viscosity, the grid metrics, the BDF history qq
+ * and the ghost layers held fixed (ghost cells are inputs like the interior).
+ *   res_bar   every block's res_bar in cav3d_calrhs's res layout
+ *             (n, 2:imax, 2:jmax, 2:kmax), blocks concatenated in order
+ *   pv_bar    out: (d res/d pv)^T res_bar in the pv layout
+ *             (n, -1:imax+3, -1:jmax+3, -1:kmax+3), blocks concatenated
+ *   scal_bar  out (may be NULL): [0] d(res_bar . res)/d(p_ref) at fixed
+ *             gauge pv, [1] d(res_bar . res)/d(u_ref) (low-Mach
+ *             preconditioning speed)
+ * FP64 throughout, also in a CAV3D_MIXED context (it is the FP64 residual's
+ * VJP).  Differentiates the Merkle source only: ncav >= 2 returns an error.
+ * Deterministic (fixed-order gathers, no atomics). */
+int cav3d_rhs_vjp(cav3d_ctx *ctx, const double *res_bar, double *pv_bar, double *scal_bar);
+
 /* pv(npv,-1:imax+3,-1:jmax+3,-1:kmax+3), dv(18,...), tv(ntv,...),
  * qq(npv,2,2:imax,2:jmax,2:kmax).  dv/tv may be NULL: then dv and tv(1:2)
  * are evaluated from pv on the device and tv(3) is set to emutref. qq may be
@@ -193,7 +211,8 @@ int cav3d_eos_host(int n, const double *p, const double *t, const double *y1, co
 /* Device time, in milliseconds, of the last cav3d_timeinteg's kernels by phase
  * (HIP events on the context's stream, recorded while enabled):
  * out[0] timestep, [1] boundary, [2] rhs, [3] lusgs, [4] update+eddy, [5] total,
- * [6] forward sweep kernel, [7] backward sweep kernel.  out holds 8 doubles. */
+ * [6] forward sweep kernel, [7] backward sweep kernel; [8] the kernels of the
+ * last cav3d_rhs_vjp, [9] those of the last cav3d_calrhs.  out holds 10 doubles. */
 int cav3d_profile(cav3d_ctx *ctx, int enable, double *out);
 
 int cav3d_synchronize(cav3d_ctx *ctx);

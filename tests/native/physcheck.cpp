@@ -2,6 +2,7 @@
 // with C entry points, so CPU tests can compare each routine with the
 // oracle on identical inputs.  Test infrastructure only.
 #include "../../3dcode_amd/csrc/phys.hpp"
+#include "../../3dcode_amd/csrc/adjoint_point.hpp"
 
 using namespace cav;
 
@@ -208,4 +209,24 @@ double pc_sst_eddy(const double *x7, const double *f6, double vol, double ydns, 
     return sst_eddy(x, f, vol, ydns, d);
 }
 
+
+// gradient of g . F of one face (the adjoint's inviscid part) on the 38x9
+// stencil x[(ll-1)*9 + n]; rh9/rh10 = {rho, h, c^2} of the two cells
+void pc_face_adj(const int *iv, const double *dv, const double *x, const double *n3, const double *rh9,
+                 const double *rh10, const double *g, double *sbo, double *dcb, double *ub)
+{
+    Par P = mkpar(iv, dv);
+    long off[NSLOT];
+    for (int ll = 1; ll <= NSLOT; ll++) off[ll - 1] = (long)(ll - 1) * 9;
+    const double cellp[6] = {rh9[0], rh9[1], rh9[2], rh10[0], rh10[1], rh10[2]};
+    const unsigned long long all = (1ull << NSLOT) - 1;
+    if (P.nscheme == 2 || P.nscheme == 3) face_adj<true>(P, P.npv, x, 1, off, n3, cellp, g, all, sbo, 1, NSLOT, dcb, 1, *ub);
+    else face_adj<false>(P, P.npv, x, 1, off, n3, cellp, g, all, sbo, 1, NSLOT, dcb, 1, *ub);
+}
+
+// gradient of phi = sum_m w[m] d[m] of the cell properties with respect to (p, T, y1, y2)
+void pc_eos_adj(int prop, const double *x4, const double *w, double *gx)
+{
+    eos_adj(prop != 0, x4, w, gx);
+}
 }

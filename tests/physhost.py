@@ -41,6 +41,8 @@ def build(outdir):
     L.pc_mixture.argtypes = [C.c_double] * 4 + [_D, _D]
     L.pc_face.argtypes = [_I, _D, _D, _D, _D, _D, _D, _D, _D, _D]
     L.pc_face32.argtypes = [_I, _D, _D, _D, _D, _D, _D]
+    L.pc_face_adj.argtypes = [_I, _D, _D, _D, _D, _D, _D, _D, _D, _D]
+    L.pc_eos_adj.argtypes = [C.c_int, _D, _D, _D]
     L.pc_visc.argtypes = [_I, _D, _D, _D, _D, _D, _D, _D, _D, _D, C.c_double, C.c_double, _D]
     return L
 

@@ -1,0 +1,155 @@
+"""The adjoint's pointwise pieces (3dcode_amd/csrc/adjoint_point.hpp,
+compiled for the host) against central differences of the same physics on
+the CPU: the gradient of g . F of one face over its 38-point stencil, the
+two cells' properties and u_ref, for every flux / reconstruction / limiter
+family of the face tests; and the EOS property-bar pass.  The primal face
+flux these differentiate is pinned bit for bit to the oracle
+(test_phys_host.py), so this pins the transposed flux, limiter, clamps and
+face EOS."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import physhost as H
+from test_phys_host import SCHEMES, _stencil, dp
+
+pytestmark = pytest.mark.filterwarnings("ignore")
+
+
+@pytest.fixture(scope="module")
+def L(tmp_path_factory):
+    return H.build(str(tmp_path_factory.mktemp("adjhost")))
+
+
+def _face(L, iv, dvp, x, n3, r9, r10):
+    fx, xl, xr, g3 = np.zeros(9), np.zeros(9), np.zeros(9), np.zeros(18)
+    L.pc_face(H.p(iv, C.c_int), dp(dvp), H.p(x), dp(n3), H.p(r9), H.p(r10), dp(fx), dp(xl), dp(xr), dp(g3))
+    return fx
+
+
+def _agree(ad, f, x0, h):
+    """Central differences at steps h and h/4 (f returns the flux vector and
+    the weights; differencing per component keeps the small components above
+    the rounding of the large ones).  A mismatch counts only where the two
+    agree with each other (no kink of a limiter, clamp or |.| in the step)."""
+    (fp, g), (fm, _) = f(x0 + h), f(x0 - h)
+    (fp4, _), (fm4, _) = f(x0 + h / 4), f(x0 - h / 4)
+    d1 = float(np.dot(g, fp - fm)) / (2 * h)
+    d2 = float(np.dot(g, fp4 - fm4)) / (h / 2)
+    # rounding of the h/4 difference: the face EOS (IF97 sums) and the flux
+    # cancel, so each flux carries up to ~1e5 ulps of noise
+    fl = 1e-11 * float(np.dot(np.abs(g), np.abs(fp))) / h
+    return d1, d2, fl
+
+
+@pytest.mark.parametrize("cfgname", ["config1", "config2"])
+@pytest.mark.parametrize("nscheme,nmuscl,nlim", SCHEMES)
+def test_face_gradient_matches_differences(L, orc, pkg, cfgname, nscheme, nmuscl, nlim):
+    case = getattr(pkg.cases, cfgname)(10, 8, 8)
+    case.params.update(nscheme=nscheme, nmuscl=nmuscl, nlim=nlim)
+    cfg = case.cfg_dict()
+    iv, dvp = H.par_arrays(cfg)
+    npv = case.npv
+    b = case.blocks[0]
+    P4 = case.init[0][0].reshape(b.kmax + 5, b.jmax + 5, b.imax + 5, npv)
+    rng = np.random.default_rng(1000 + nscheme * 100 + nmuscl * 10 + nlim)
+    bad = total = checked = 0
+    for trial in range(12):
+        x = _stencil(rng, P4, npv, 1e-2)
+        # velocities off exact zeros: a limiter ratio of zero differences is
+        # set by its 1e-30 regularisation, which no finite difference resolves
+        x[:, 1:4] += 0.05 * rng.standard_normal((38, 3))
+        n3 = rng.standard_normal(3) * 1e-3
+        d9 = orc.deteos_simple(x[8, 0] + cfg["pref"], x[8, 4], x[8, 5], x[8, 6])
+        d10 = orc.deteos_simple(x[9, 0] + cfg["pref"], x[9, 4], x[9, 5], x[9, 6])
+        r9, r10 = np.ascontiguousarray(d9[[0, 1, 5]]), np.ascontiguousarray(d10[[0, 1, 5]])
+        g = np.zeros(9)
+        g[:npv] = rng.standard_normal(npv)
+        sbo, dcb, ub = np.zeros(9 * 38), np.zeros(26), np.zeros(1)
+        L.pc_face_adj(H.p(iv, C.c_int), dp(dvp), H.p(x), dp(n3), H.p(r9), H.p(r10), dp(g), dp(sbo), dp(dcb), dp(ub))
+        sbo = sbo.reshape(9, 38)
+        phi = lambda xx, a=r9, b_=r10: (_face(L, iv, dvp, xx, n3, a, b_), g)
+        # steps on each variable's own scale over the stencil (absolute pressure,
+        # |u| of at least 1 m/s); tolerance relative to that variable's
+        # derivative scale, kinks inside a step skipped (the two differences
+        # disagree there)
+        for n in range(npv):
+            if n == 6:
+                continue                    # gas fraction: zero throughout
+            col = np.abs(x[:, n] + (cfg["pref"] if n == 0 else 0.0))
+            if n == 5 and col.max() == 0.0:
+                continue                    # single phase: y1 = 0 is a one-sided bound
+            # temperature moves the IF97 sums: a longer step keeps their noise down
+            h = (1e-4 if n == 4 else 1e-5) * max(col.max(), 1.0 if 1 <= n <= 3 else 0.0)
+            sn = np.abs(sbo[n]).max()
+            for ll in range(38):
+                if n == 5 and x[ll, n] < 2 * h:
+                    continue                # the step would cross y1 = 0
+                total += 1
+
+                def f(v, ll=ll, n=n):
+                    xx = x.copy()
+                    xx[ll, n] = v
+                    return phi(xx)
+                d1, d2, fl = _agree(sbo[n, ll], f, x[ll, n], h)
+                if abs(d1 - d2) > 1e-4 * abs(d2) + 1e-3 * fl:
+                    continue
+                checked += 1
+                if abs(d2 - sbo[n, ll]) > max(3.0 * abs(d1 - d2), 1e-6 * abs(d2)) + fl:
+                    bad += 1
+                    print("mismatch", cfgname, nscheme, nmuscl, nlim, trial, ll + 1, n, sbo[n, ll], d1, d2)
+        if nmuscl == 0:
+            for side, r in ((0, r9), (1, r10)):
+                for q, m in ((0, 0), (1, 1), (2, 2)):
+                    if m == 2 and nscheme not in (2, 3):
+                        continue
+                    hh = 1e-5 * abs(r[q])
+
+                    def f(v, side=side, q=q):
+                        a, b_ = r9.copy(), r10.copy()
+                        (a if side == 0 else b_)[q] = v
+                        return phi(x, a, b_)
+                    d1, d2, fl = _agree(None, f, r[q], hh)
+                    ad = dcb[13 * side + m]
+                    if abs(d1 - d2) <= 1e-4 * abs(d2) + 1e-3 * fl and abs(d2 - ad) > max(3.0 * abs(d1 - d2), 1e-6 * abs(d2)) + fl:
+                        bad += 1
+                        print("cell mismatch", cfgname, nscheme, trial, side, m, ad, d1, d2)
+        # u_ref
+        hu = 1e-5 * cfg["uref"]
+
+        def fu(v):
+            dd = dvp.copy()
+            dd[2] = v
+            fx = np.zeros(9); xl = np.zeros(9); xr = np.zeros(9); g3 = np.zeros(18)
+            L.pc_face(H.p(iv, C.c_int), dp(dd), H.p(x), dp(n3), H.p(r9), H.p(r10), dp(fx), dp(xl), dp(xr), dp(g3))
+            return fx, g
+        d1, d2, fl = _agree(ub[0], fu, cfg["uref"], hu)
+        assert abs(d2 - ub[0]) <= max(3.0 * abs(d1 - d2), 1e-6 * abs(d2)) + fl, (d1, d2, ub[0])
+    assert bad == 0
+    assert checked >= 0.8 * total, (checked, total)
+
+
+def test_eos_property_bars(L, pkg):
+    """eos_adj: sum_m w_m d_m(p, T, y1, y2) differentiated by its dual pass
+    against central differences of the library's host EOS."""
+    rng = np.random.default_rng(9)
+    cols = [0, 1, 5, 6, 7, 8, 9, 10, 11, 12, 13]   # dv entries kept per cell (rho .. dh/dy2)
+    for trial in range(40):
+        x4 = np.array([rng.uniform(5e3, 3e5), rng.uniform(280, 360), rng.choice([0.0, rng.uniform(1e-6, 1e-2)]), 0.0])
+        w = rng.standard_normal(13)
+        gx = np.zeros(4)
+        L.pc_eos_adj(1, dp(x4), dp(w), dp(gx))
+
+        def f(v, q):
+            y = x4.copy()
+            y[q] = v
+            dv, tv = pkg.eos_host(y[0], y[1], y[2], y[3])
+            d = np.concatenate([dv[0, cols], tv[0]])
+            return float(np.dot(w, d))
+        for q in range(3):
+            if q == 2 and x4[2] == 0.0:
+                continue
+            h = 1e-7 * abs(x4[q])
+            fd = (f(x4[q] + h, q) - f(x4[q] - h, q)) / (2 * h)
+            assert abs(fd - gx[q]) <= 1e-6 * max(abs(fd), abs(gx[q])), (trial, q, fd, gx[q])

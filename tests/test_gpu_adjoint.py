@@ -1,0 +1,302 @@
+"""The discrete adjoint cav3d_rhs_vjp against central finite differences of
+the residual it differentiates: the device residual (cav3d_calrhs with the
+cell properties re-derived by the EOS) and the oracle's residual
+(src/rhs_module.f90:260-643 restated), along seeded random directions over
+every block's ghosted state; the p_ref and u_ref derivatives; determinism;
+and the torch.autograd harness."""
+import copy
+import os
+import zlib
+
+import numpy as np
+import pytest
+
+from parity import small_cases
+
+pytestmark = pytest.mark.gpu
+
+# d(res_bar . res) along a direction: FD vs adjoint, relative to their size
+TOL = 1e-5
+
+# AUSMPW+ (nscheme 3) is piecewise at every face -- minima over nine stencil
+# pressure ratios, the larger of two pressure weights, the smaller rho c^2 of
+# the two sides -- so at the residual level no step of a random direction is
+# reliably free of kinks; its face gradient is checked against differences
+# face by face on the host (tests/test_adjoint_host.py, schemes 3-*-*).
+AUSMPWP = pytest.mark.skip(reason="AUSMPW+ is piecewise at every face; its face gradient is pinned on the host")
+
+CASES = ["lam_roe_tvd", "lam_roe_mlp", "lam_ausm_tvd_vanleer", "sst_cav_ausm_mlp", "sst_cav_roe_first",
+         "lam_roe_2block", "lam_roe_bdf2", "sst_cav_bdf1", "sst_cav_roem_mlp",
+         pytest.param("lam_ausmpwp_tvd", marks=AUSMPWP), "sst_ddes", "sst_cav_3block"]
+
+
+_OPEN = []
+
+
+@pytest.fixture(autouse=True)
+def _close_contexts():
+    """Every context a test opens is destroyed when it ends, passed or not
+    (a failed test's traceback would otherwise keep its solver alive)."""
+    yield
+    while _OPEN:
+        _OPEN.pop().close()
+
+
+def prepared(pk, name, steps=2, generic=True):
+    """A case after a few pseudo-time steps, the solver holding it, and its
+    frozen parts.  The ghosted state is then made generic: ghost layers that
+    copy the interior and the spanwise velocity's exact zeros give limiter
+    ratios of zero differences, which the 1e-30 regularisation sets and no
+    finite step resolves; every ghosted cell is an input of the map, so any
+    values are admissible."""
+    case = small_cases(pk)[name]()
+    s = pk.Solver(case)
+    _OPEN.append(s)
+    for ib, st in enumerate(case.init):
+        s.put_state(ib, *st)
+    for it in range(steps):
+        s.timeinteg(1, it + 1, 0.0)
+    s.setbc()       # ghost layers of the updated state (a step fills them at its start)
+    rng = np.random.default_rng(zlib.crc32(("state" + name).encode()))
+    pref = case.params["pref"]
+    pvs = []
+    for ib in range(len(case.blocks)):
+        a = s.get_state(ib)[0].reshape(-1, case.npv).copy()
+        live = a[:, 4] > 0.0                    # cells the boundary fill has set
+        m = live.sum()
+        a[live, 0] += 1e-4 * np.abs(a[live, 0] + pref) * rng.standard_normal(m)
+        a[live, 1:4] += 1e-3 * rng.standard_normal((m, 3))
+        a[live, 4] *= 1.0 + 1e-5 * rng.standard_normal(m)
+        a[live, 5] *= 1.0 + 1e-3 * rng.standard_normal(m)
+        if case.npv > 7:
+            a[live, 8] *= 1.0 + 1e-3 * rng.standard_normal(m)
+            # k away from walls only: a wall's ghost layer holds -k (zero face average)
+            mk = _k_movable(case, ib, a)
+            a[mk, 7] *= 1.0 + 1e-3 * rng.standard_normal(mk.sum())
+        pvs.append(a.ravel())
+    return case, s, pvs, s.frozen()
+
+
+def _k_movable(case, ib, a):
+    """Cells whose k may move: k > 0 there and in the six neighbours (a wall's
+    ghost layer holds -k, so the wall face's average k is zero and any step
+    of k beside it leaves the domain of its square root)."""
+    b = case.blocks[ib]
+    k = a[:, 7].reshape(b.kmax + 5, b.jmax + 5, b.imax + 5)
+    ok = k > 0.0
+    m = ok.copy()
+    for ax in range(3):
+        for sh in (1, -1):
+            m &= np.roll(ok, sh, axis=ax)
+    return m.ravel()
+
+
+def directions(case, pvs, rng, k):
+    """k directions over every ghosted entry, each component scaled to 1e-9
+    of its magnitude in that cell (gauge pressure by the absolute one); the
+    gas fraction, zero throughout, is not moved, nor k beside a wall.  Under
+    AUSMPW+ the pressure is not moved either: its sensor takes minima over
+    nine stencil pressure ratios at every face, so nearly every step of the
+    pressure crosses a kink somewhere (its pressure bars are checked face by
+    face on the host, tests/test_adjoint_host.py)."""
+    out = []
+    pref = case.params["pref"]
+    for _ in range(k):
+        d = []
+        for ib, pv in enumerate(pvs):
+            a = pv.reshape(-1, case.npv)
+            sc = 1e-9 * np.abs(a)
+            sc[:, 0] = 1e-9 * np.abs(a[:, 0] + pref)
+            sc[:, 1:4] = 1e-9 * np.maximum(np.abs(a[:, 1:4]), 1.0)
+            sc[:, 6] = 0.0
+            if case.params["nscheme"] == 3:
+                sc[:, 0] = 0.0
+            if case.npv > 7:
+                sc[~_k_movable(case, ib, a), 7] = 0.0
+            d.append((rng.standard_normal(a.shape) * sc).ravel())
+        out.append(d)
+    return out
+
+
+def seeds(case, res, rng):
+    """res_bar: standard normal per entry over the rms of its component."""
+    out = []
+    rms = np.sqrt(np.mean(np.concatenate([r.reshape(-1, case.npv) for r in res]) ** 2, axis=0))
+    rms = np.where(rms > 0.0, rms, 1.0)
+    for r in res:
+        out.append((rng.standard_normal(r.reshape(-1, case.npv).shape) / rms).ravel())
+    return out
+
+
+def phi(rb, res):
+    return sum(float(np.dot(a, b)) for a, b in zip(rb, res))
+
+
+def kink_aware(diff, adj):
+    """Compare an adjoint directional derivative with central differences
+    diff(c) at steps c = 1, 1/4, 1/16: the first pair of consecutive steps
+    that agree to 1e-4 (no limiter / clamp / |.| kink inside the longer one)
+    is used, the adjoint is compared with the nearer of the two, allowing
+    twice their difference (the differences' rounding, which shorter steps
+    magnify).  Returns the relative error, or None when no pair agrees."""
+    prev = diff(1.0)
+    for c in (0.25, 0.0625):
+        cur = diff(c)
+        if abs(prev - cur) <= 1e-4 * max(abs(prev), abs(cur)):
+            miss = min(abs(prev - adj), abs(cur - adj)) - 2.0 * abs(prev - cur)
+            return max(miss, 0.0) / max(abs(prev), abs(adj), 1e-300)
+        prev = cur
+    return None
+
+
+def check_directions(f, pvs, pb, rb, dirs, need):
+    """Every direction whose differences settle (kink_aware) must match the
+    adjoint to TOL, and at least `need` directions must settle."""
+    worst, counted = 0.0, 0
+
+    for d in dirs:
+        def diff(c, d=d):
+            # residuals differenced entry by entry before the weighting, so small
+            # entries are not lost in the rounding of large ones
+            rp = f([p + c * v for p, v in zip(pvs, d)])
+            rm = f([p - c * v for p, v in zip(pvs, d)])
+            return 0.5 * sum(float(np.dot(w, a - b)) for w, a, b in zip(rb, rp, rm)) / c
+        adj = sum(float(np.dot(a, v)) for a, v in zip(pb, d))
+        err = kink_aware(diff, adj)
+        if err is None:
+            print(f"  direction skipped: no two steps agree (adjoint {adj:.9e})", flush=True)
+            continue
+        counted += 1
+        worst = max(worst, err)
+        assert err < TOL, (adj, err)
+    assert counted >= need, (counted, len(dirs))
+    return worst
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_vjp_matches_central_differences(pkg, name):
+    print(f"case {name} adjoint vs FD", flush=True)
+    case, s, pvs, fr = prepared(pkg, name)
+    rng = np.random.default_rng(zlib.crc32(name.encode()))
+    res0 = s.residual_of(pvs, fr["mut"], fr["qq"])
+    rb = seeds(case, res0, rng)
+    pb, _ = s.rhs_vjp(rb)
+    dirs = directions(case, pvs, rng, 8)
+    worst = check_directions(lambda p: s.residual_of(p, fr["mut"], fr["qq"]), pvs, pb, rb, dirs, 5)
+    print(f"case {name}: worst relative difference {worst:.2e} over {len(dirs)} directions", flush=True)
+    s.close()
+
+
+def test_vjp_sixty_four_directions_config2(pkg):
+    """The config-2 physics (SST, Merkle, AUSM+-up, MLP) on 64 directions."""
+    print("case sst_cav_ausm_mlp 64 directions", flush=True)
+    case, s, pvs, fr = prepared(pkg, "sst_cav_ausm_mlp")
+    rng = np.random.default_rng(64)
+    res0 = s.residual_of(pvs, fr["mut"], fr["qq"])
+    rb = seeds(case, res0, rng)
+    pb, _ = s.rhs_vjp(rb)
+    check_directions(lambda p: s.residual_of(p, fr["mut"], fr["qq"]), pvs, pb, rb, directions(case, pvs, rng, 64), 48)
+    s.close()
+
+
+@pytest.mark.parametrize("name", ["sst_cav_ausm_mlp", "lam_roe_2block", "sst_cav_roem_mlp"])
+def test_vjp_matches_oracle_differences(pkg, orc, name):
+    """Central differences of the oracle's residual (its own flux, limiter,
+    source code) along the same kind of directions."""
+    print(f"case {name} adjoint vs oracle FD", flush=True)
+    case, s, pvs, fr = prepared(pkg, name)
+    rng = np.random.default_rng(7)
+    o = orc.Oracle(case)
+    pref = case.params["pref"]
+
+    def f_orc(ps):
+        out = []
+        for ib, pv in enumerate(ps):
+            a = pv.reshape(-1, case.npv)
+            dv, t2 = pkg.eos_host(a[:, 0] + pref, a[:, 4], a[:, 5], a[:, 6])
+            tv = None
+            if case.ntv:
+                tv = np.zeros((a.shape[0], case.ntv))
+                tv[:, :2] = t2
+                if case.ntv == 3:
+                    tv[:, 2] = fr["mut"][ib]
+                tv = tv.ravel()
+            o.put_state(ib, np.ascontiguousarray(pv), np.ascontiguousarray(dv.ravel()), tv, fr["qq"][ib])
+        return [o.calrhs(ib)[0] for ib in range(len(ps))]
+
+    rb = seeds(case, f_orc(pvs), rng)
+    s.residual_of(pvs, fr["mut"], fr["qq"])
+    pb, _ = s.rhs_vjp(rb)
+    check_directions(f_orc, pvs, pb, rb, directions(case, pvs, rng, 6), 4)
+    s.close()
+
+
+@pytest.mark.parametrize("name", ["sst_cav_ausm_mlp", "lam_roe_tvd"])
+def test_scalar_derivatives(pkg, name):
+    """d(res_bar . res)/d p_ref at fixed gauge state and d/d u_ref against
+    central differences over contexts built with the shifted scalar."""
+    print(f"case {name} scalar derivatives", flush=True)
+    case, s, pvs, fr = prepared(pkg, name)
+    rng = np.random.default_rng(11)
+    rb = seeds(case, s.residual_of(pvs, fr["mut"], fr["qq"]), rng)
+    _, sc = s.rhs_vjp(rb)
+    def at(key, v):
+        c2 = copy.copy(case)
+        c2.params = dict(case.params)
+        c2.params[key] = v
+        s2 = pkg.Solver(c2)
+        try:
+            return s2.residual_of(pvs, fr["mut"], fr["qq"])
+        finally:
+            s2.close()
+
+    def diff(key, v0, h):
+        rp, rm = at(key, v0 + h), at(key, v0 - h)
+        return sum(float(np.dot(w, a - b)) for w, a, b in zip(rb, rp, rm)) / (2.0 * h)
+
+    pb, _ = s.rhs_vjp(rb)
+    for q, key in enumerate(("pref", "uref")):
+        v0 = case.params[key]
+        h = 1e-7 * abs(v0)
+        d1, d2 = diff(key, v0, h), diff(key, v0, h / 4)
+        fd = d1
+        # the p_ref derivative is a sum over every cell's pressure bar: its
+        # cancellation sets the scale of what a difference resolves
+        floor = 1e-9 * sum(np.abs(x.reshape(-1, case.npv)[:, 0]).sum() for x in pb) if key == "pref" else 0.0
+        assert abs(d1 - d2) <= 1e-4 * abs(d1) + floor, (key, d1, d2)      # no kink inside the step
+        miss = max(abs(fd - sc[q]) - 2.0 * abs(d1 - d2), 0.0)
+        err = miss / max(abs(fd), abs(sc[q]), floor / TOL, 1e-300)
+        print(f"  d/d{key}: adjoint {sc[q]:.6e} FD {fd:.6e} rel {err:.1e}", flush=True)
+        assert err < TOL, (key, fd, sc[q])
+    s.close()
+
+
+def test_vjp_is_deterministic_and_linear(pkg):
+    case, s, pvs, fr = prepared(pkg, "sst_cav_ausm_mlp")
+    rng = np.random.default_rng(3)
+    res0 = s.residual_of(pvs, fr["mut"], fr["qq"])
+    r1, r2 = seeds(case, res0, rng), seeds(case, res0, rng)
+    a1, _ = s.rhs_vjp(r1)
+    a1b, _ = s.rhs_vjp(r1)
+    assert all(np.array_equal(x, y) for x, y in zip(a1, a1b))
+    a2, _ = s.rhs_vjp(r2)
+    a12, _ = s.rhs_vjp([x + 2.0 * y for x, y in zip(r1, r2)])
+    for x, y, z in zip(a1, a2, a12):
+        scale = np.abs(z).max()
+        assert np.abs(x + 2.0 * y - z).max() <= 1e-12 * scale
+    s.close()
+
+
+def test_autograd_harness():
+    """torch reverse mode through ResidualFunction gives the library's VJP,
+    and its gradient of a quadratic objective matches central differences.
+    Runs in a child process that imports torch before the library (torch
+    carries its own HIP runtime; loading it after the library would put two
+    runtimes in this process)."""
+    import subprocess
+    import sys
+    here = os.path.dirname(os.path.abspath(__file__))
+    r = subprocess.run([sys.executable, "-u", os.path.join(here, "adjoint_torch_check.py")], capture_output=True,
+                       text=True, timeout=240)
+    print(r.stdout[-2000:], r.stderr[-2000:], flush=True)
+    assert r.returncode == 0
