@@ -148,13 +148,23 @@ __host__ __device__ __noinline__ void face_adj(const Par &P, int npv, const doub
     for (int n = 0; n < NPV; n++) { qlb[n] = n < npv ? zb[n] : 0.0; qrb[n] = n < npv ? zb[npv + n] : 0.0; }
     double rhoLb = zb[o], hLb = zb[o + 1], rhoRb = zb[o + 2], hRb = zb[o + 3];
     double c2Lb = SENSOR ? zb[o + 5] : 0.0, c2Rb = SENSOR ? zb[o + 6] : 0.0;
-    {   // averaged c^2
+    {   // averaged c^2 = c^2(pa, ta, ya1, ya2): one 4-lane pass of the EOS, then
+        // the closed forms of pa, ta and the Roe-averaged fractions
         const double c2ab = zb[o + 4];
-        double gz[10];
-        dn_grad<5>(10, zav, [&](auto in, int) -> Dn<5> { return c2ab * c2_average<Dn<5>>(P, in); }, gz);
-        qlb[0] += gz[0]; qrb[0] += gz[1]; qlb[4] += gz[2]; qrb[4] += gz[3];
-        qlb[5] += gz[4]; qrb[5] += gz[5]; qlb[6] += gz[6]; qrb[6] += gz[7];
-        rhoLb += gz[8]; rhoRb += gz[9];
+        const double sl = sqrt(L.rho), sr = sqrt(Rt.rho), S = sl + sr, rd = 1.0 / S;
+        const double ya1 = (sl * ql[5] + sr * qr[5]) * rd, ya2 = (sl * ql[6] + sr * qr[6]) * rd;
+        const double x4[4] = {0.5 * (ql[0] + qr[0]) + P.pref, 0.5 * (ql[4] + qr[4]), ya1, ya2};
+        double g4[4];
+        dn_grad<4>(4, [&](int k) -> double { return x4[k]; }, [&](auto in, int) -> Dn<4> {
+            return c2ab * face_c2<Dn<4>>(in(0), in(1), in(2), in(3));
+        }, g4);
+        qlb[0] += 0.5 * g4[0]; qrb[0] += 0.5 * g4[0];
+        qlb[4] += 0.5 * g4[1]; qrb[4] += 0.5 * g4[1];
+        qlb[5] += g4[2] * sl * rd; qrb[5] += g4[2] * sr * rd;
+        qlb[6] += g4[3] * sl * rd; qrb[6] += g4[3] * sr * rd;
+        // d ya / d sl = (x_L - ya) / S, d sl / d rho_L = 0.5 / sl
+        rhoLb += (g4[2] * (ql[5] - ya1) + g4[3] * (ql[6] - ya2)) * rd * (0.5 / sl);
+        rhoRb += (g4[2] * (qr[5] - ya1) + g4[3] * (qr[6] - ya2)) * rd * (0.5 / sr);
     }
     double dcb[2 * ND];
     for (int m = 0; m < 2 * ND; m++) dcb[m] = 0.0;
@@ -163,17 +173,22 @@ __host__ __device__ __noinline__ void face_adj(const Par &P, int npv, const doub
             const double *q = side ? qr : ql;
             double *qb = side ? qrb : qlb;
             const double rb_ = side ? rhoRb : rhoLb, hb_ = side ? hRb : hLb, cb_ = side ? c2Rb : c2Lb;
-            const double x4[4] = {q[0] + P.pref, q[4], q[5], q[6]};
-            double gx[4];
-            dn_grad<4>(4, [&](int k) -> double { return x4[k]; }, [&](auto in, int) -> Dn<4> {
-                Dn<4> rho, h;
-                const Dn<4> p = in(0), t = in(1), y1 = in(2), y2 = in(3);
-                face_rho_h<Dn<4>>(p, t, y1, y2, rho, h);
-                Dn<4> phi = rb_ * rho + hb_ * h;
-                if (SENSOR) phi = phi + cb_ * face_c2<Dn<4>>(p, t, y1, y2);
-                return phi;
-            }, gx);
-            qb[0] += gx[0]; qb[4] += gx[1]; qb[5] += gx[2]; qb[6] += gx[3];
+            // rho and h of the face state: the mixture EOS's own derivatives
+            // (d rho/dp, dT, dy1, dy2; dh/dp, ...) are their exact gradients
+            double d[ND];
+            mixture_eos<double>(q[0] + P.pref, q[4], q[5], q[6], d, false);
+            qb[0] += rb_ * d[D_RP] + hb_ * d[D_HP];
+            qb[4] += rb_ * d[D_RT] + hb_ * d[D_HT];
+            qb[5] += rb_ * d[D_RY1] + hb_ * d[D_HY1];
+            qb[6] += rb_ * d[D_RY2] + hb_ * d[D_HY2];
+            if (SENSOR && cb_ != 0.0) {     // c^2 of the face state (sensor schemes)
+                const double x4[4] = {q[0] + P.pref, q[4], q[5], q[6]};
+                double gx[4];
+                dn_grad<4>(4, [&](int k) -> double { return x4[k]; }, [&](auto in, int) -> Dn<4> {
+                    return cb_ * face_c2<Dn<4>>(in(0), in(1), in(2), in(3));
+                }, gx);
+                qb[0] += gx[0]; qb[4] += gx[1]; qb[5] += gx[2]; qb[6] += gx[3];
+            }
         }
     } else {
         dcb[D_RHO] = rhoLb; dcb[D_H] = hLb; dcb[ND + D_RHO] = rhoRb; dcb[ND + D_H] = hRb;
@@ -197,12 +212,41 @@ __host__ __device__ __noinline__ void face_adj(const Par &P, int npv, const doub
             sb[9] += xrb[n];
         } else if (xlb[n] != 0.0 || xrb[n] != 0.0) {
             const double wl = xlb[n], wr = xrb[n];
-            if (P.nmuscl == 2) {       // every stencil point; sb is still zero here
-                dn_grad<2>(NSLOT, [&](int q) -> double { return xs(q + 1, n); }, [&](auto in, int) -> Dn<2> {
+            if (P.nmuscl == 2) {
+                // the points the MLP value can depend on: the fourteen it reads
+                // directly and the extremum each min / max group selects (the
+                // first one, as mn / mx keep it); the others enter as constants
+                int sup[NSLOT], ns = 0;
+                unsigned long long seen = 0;
+                auto add = [&](int ll) { if (!((seen >> ll) & 1ull)) { seen |= 1ull << ll; sup[ns++] = ll; } };
+                constexpr int DIRECT[14] = {9, 10, 23, 32, 37, 38, 3, 4, 7, 8, 11, 12, 15, 16};
+                for (int q = 0; q < 14; q++) add(DIRECT[q]);
+                auto ext = [&](int lo, int hi, int step) {
+                    int amin = lo, amax = lo;
+                    double vmin = xs(lo, n), vmax = vmin;
+                    for (int q = lo + step; q <= hi; q += step) {
+                        const double v = xs(q, n);
+                        if (v < vmin) { vmin = v; amin = q; }
+                        if (v > vmax) { vmax = v; amax = q; }
+                    }
+                    add(amin);
+                    add(amax);
+                };
+                ext(1, 18, 1); ext(19, 27, 1); ext(28, 36, 1); ext(1, 17, 2); ext(2, 18, 2);
+                // two support points per pass, each read compared against them
+                for (int q = 0; q < ns; q += 2) {
+                    const int s0 = sup[q], s1 = q + 1 < ns ? sup[q + 1] : -1;
                     Dn<2> a, b;
-                    muscl_var<Dn<2>>(P.nlim, true, [&](int ll) -> Dn<2> { return in(ll - 1); }, a, b);
-                    return wl * a + wr * b;
-                }, sb);
+                    muscl_var<Dn<2>>(P.nlim, true, [&](int ll) -> Dn<2> {
+                        Dn<2> r(xs(ll, n));
+                        if (ll == s0) r.d[0] = 1.0;
+                        if (ll == s1) r.d[1] = 1.0;
+                        return r;
+                    }, a, b);
+                    const Dn<2> phi = wl * a + wr * b;
+                    sb[s0 - 1] += phi.d[0];
+                    if (s1 > 0) sb[s1 - 1] += phi.d[1];
+                }
             } else {
                 double gx[6];
                 dn_grad<6>(6, [&](int q) -> double { return xs(TVD6[q], n); }, [&](auto in, int) -> Dn<6> {

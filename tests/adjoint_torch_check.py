@@ -20,8 +20,8 @@ import test_gpu_adjoint as T  # noqa: E402
 def main():
     pk = importlib.import_module("3dcode_amd")
     ag = importlib.import_module("3dcode_amd.autograd")
-    # a generic state without pseudo-time steps (the LU-SGS sweeps'
-    # cooperative launch is refused under the HIP runtime torch brings)
+    # a generic state without pseudo-time steps: the parent test process holds
+    # the device's cooperative-launch queue (the LU-SGS sweeps)
     case, s, pvs, fr = T.prepared(pk, "sst_cav_roe_first", steps=0)
     try:
         pv = torch.from_numpy(np.concatenate(pvs)).requires_grad_(True)
