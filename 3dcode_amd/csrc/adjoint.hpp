@@ -413,19 +413,23 @@ __global__ __launch_bounds__(128) void k_adj_gather(DS S, AD A)
     for (int n = 0; n < npv; n++) A.pb[(long)n * S.NG + gl] = acc[n];
 }
 
-// deterministic sum of n doubles (one workgroup: strided partial sums, then a tree)
-__global__ __launch_bounds__(256) void k_sum_fixed(const double *a, long n, double *out)
+// deterministic sum of n doubles in two stages: workgroup b sums its strided
+// share (q = b*256 + t + k*256*gridDim) by a fixed tree into part[b]; then
+// one workgroup sums the parts the same way.  The order never depends on
+// timing, so the result is reproducible bit for bit.
+constexpr int SUM_WG = 512;
+__global__ __launch_bounds__(256) void k_sum_fixed(const double *a, long n, double *part)
 {
     __shared__ double sh[256];
     double s = 0.0;
-    for (long q = threadIdx.x; q < n; q += 256) s += a[q];
+    for (long q = (long)blockIdx.x * 256 + threadIdx.x; q < n; q += 256L * gridDim.x) s += a[q];
     sh[threadIdx.x] = s;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) out[0] = sh[0];
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
 
 // stencil points a face's bars cover under the configuration
@@ -448,7 +452,7 @@ int adj_alloc(cav3d_ctx *c)
     int rc;
     if ((rc = dalloc(c, &c->a_rb, (long)npv * S.NC)) || (rc = dalloc(c, &c->a_cb, 7L * npv * S.NC)) ||
         (rc = dalloc(c, &c->a_cdb, (long)ND * S.NC)) || (rc = dalloc(c, &c->a_pb, (long)npv * S.NG)) ||
-        (rc = dalloc(c, &c->a_sum, 8)))
+        (rc = dalloc(c, &c->a_sum, 8 + 4L * SUM_WG)))
         return rc;
     for (int d = 0; d < 3; d++)
         if ((rc = dalloc(c, &c->a_fb[d], (long)NSLOT * npv * S.NF[d])) || (rc = dalloc(c, &c->a_db[d], 2L * ND * S.NF[d])) ||
@@ -533,9 +537,14 @@ int cav3d_rhs_vjp(cav3d_ctx *c, const double *res_bar, double *pv_bar, double *s
         off += (long)npv * B.ng;
     }
     if (scal_bar) {
-        hipLaunchKernelGGL(k_sum_fixed, dim3(1), dim3(256), 0, c->st, (const double *)c->a_pb, S.NG, c->a_sum);
-        for (int d = 0; d < 3; d++)
-            hipLaunchKernelGGL(k_sum_fixed, dim3(1), dim3(256), 0, c->st, (const double *)c->a_ub[d], S.NF[d], c->a_sum + 1 + d);
+        // stage one into a_sum[8 + SUM_WG * q], stage two into a_sum[q]
+        const double *src[4] = {c->a_pb, c->a_ub[0], c->a_ub[1], c->a_ub[2]};
+        const long len[4] = {S.NG, S.NF[0], S.NF[1], S.NF[2]};
+        for (int q = 0; q < 4; q++) {
+            double *part = c->a_sum + 8 + (long)SUM_WG * q;
+            hipLaunchKernelGGL(k_sum_fixed, dim3(SUM_WG), dim3(256), 0, c->st, src[q], len[q], part);
+            hipLaunchKernelGGL(k_sum_fixed, dim3(1), dim3(256), 0, c->st, (const double *)part, (long)SUM_WG, c->a_sum + q);
+        }
         double h[4];
         HIPCHK(hipMemcpyAsync(h, c->a_sum, sizeof(h), hipMemcpyDeviceToHost, c->st));
         HIPCHK(hipStreamSynchronize(c->st));
