@@ -233,19 +233,20 @@ __host__ __device__ __noinline__ void face_adj(const Par &P, int npv, const doub
                     add(amax);
                 };
                 ext(1, 18, 1); ext(19, 27, 1); ext(28, 36, 1); ext(1, 17, 2); ext(2, 18, 2);
-                // two support points per pass, each read compared against them
-                for (int q = 0; q < ns; q += 2) {
-                    const int s0 = sup[q], s1 = q + 1 < ns ? sup[q + 1] : -1;
-                    Dn<2> a, b;
-                    muscl_var<Dn<2>>(P.nlim, true, [&](int ll) -> Dn<2> {
-                        Dn<2> r(xs(ll, n));
-                        if (ll == s0) r.d[0] = 1.0;
-                        if (ll == s1) r.d[1] = 1.0;
+                // four support points per pass, each read compared against them
+                for (int q = 0; q < ns; q += 4) {
+                    int sl4[4];
+                    for (int l = 0; l < 4; l++) sl4[l] = q + l < ns ? sup[q + l] : -1;
+                    Dn<4> a, b;
+                    muscl_var<Dn<4>>(P.nlim, true, [&](int ll) -> Dn<4> {
+                        Dn<4> r(xs(ll, n));
+                        for (int l = 0; l < 4; l++)
+                            if (ll == sl4[l]) r.d[l] = 1.0;
                         return r;
                     }, a, b);
-                    const Dn<2> phi = wl * a + wr * b;
-                    sb[s0 - 1] += phi.d[0];
-                    if (s1 > 0) sb[s1 - 1] += phi.d[1];
+                    const Dn<4> phi = wl * a + wr * b;
+                    for (int l = 0; l < 4; l++)
+                        if (sl4[l] > 0) sb[sl4[l] - 1] += phi.d[l];
                 }
             } else {
                 double gx[6];
