@@ -256,13 +256,13 @@ __global__ __launch_bounds__(128) void k_cell_vjp(DS S, AD A)
     double rb[NPV];
     for (int n = 0; n < NPV; n++) rb[n] = n < npv ? A.rb[(long)n * S.NC + t] : 0.0;
     const bool sst = P.iturb == 0, cav_ = P.ncav == 1, bdf = P.nsteady > 0;
-    // z = {pc[npv], rho, h, mu, neighbours' (u, v, w, k, omega) x 6}
-    constexpr int NZ = NPV + 3 + 30;
-    const int od = npv, ox = npv + 3;
-    const int nz = (cav_ || bdf || sst) ? (sst ? ox + 30 : ox) : 0;
+    // z = {pc[npv], rho, h, mu, the Green-Gauss gradients of u, v, w, k, omega (15)}
+    constexpr int NZ = NPV + 3 + 15;
+    const int od = npv, og = npv + 3;
+    const int nz = (cav_ || bdf || sst) ? (sst ? og + 15 : og) : 0;
     double zx[NZ], zb[NZ];
     for (int q = 0; q < NZ; q++) zx[q] = zb[q] = 0.0;
-    const int nbv[5] = {1, 2, 3, 7, 8};
+    const int gvar[5] = {1, 2, 3, 7, 8};
     double d[ND];
     for (int m = 0; m < ND; m++) d[m] = S.dc[m * S.NG + g];
     double x7[7][NPV];
@@ -272,20 +272,15 @@ __global__ __launch_bounds__(128) void k_cell_vjp(DS S, AD A)
         const long gn = gidx(B, i + di, j + dj, k + dk);
         for (int n = 0; n < NPV; n++) x7[q][n] = n < npv ? S.pv[(long)n * S.NG + gn] : 0.0;
     }
+    const double vol = S.vol[widx(B, i, j, k)];
+    double f6[6][3];
+    cell_faces(S, B, i, j, k, f6);
+    const double iv = 1.0 / vol;
     if (nz) {
         for (int n = 0; n < npv; n++) zx[n] = x7[1][n];
         zx[od] = d[D_RHO]; zx[od + 1] = d[D_H]; zx[od + 2] = d[D_MU];
-        if (sst) {
-            int c = 0;
-            for (int q = 0; q < 7; q++) {
-                if (q == 1) continue;
-                for (int v = 0; v < 5; v++) zx[ox + 5 * c + v] = x7[q][nbv[v]];
-                c++;
-            }
-        }
-        const double vol = S.vol[widx(B, i, j, k)];
-        double f6[6][3];
-        cell_faces(S, B, i, j, k, f6);
+        if (sst)
+            for (int v = 0; v < 5; v++) gg_grad<double>(x7, gvar[v], f6, iv, zx + og + 3 * v);
         const double ydn = sst ? S.ydn[widx(B, i, j, k)] : 1.0;
         const double mut = S.mut ? S.mut[g] : 0.0;
         double q1[NPV], q2[NPV];
@@ -296,7 +291,7 @@ __global__ __launch_bounds__(128) void k_cell_vjp(DS S, AD A)
         constexpr int W = 2;
         using T = Dn<W>;
         dn_grad<W>(nz, [&](int q) -> double { return zx[q]; }, [&](auto in, int) -> T {
-            T pc[NPV], dt[ND], x7t[7][NPV];
+            T pc[NPV], dt[ND];
             for (int n = 0; n < NPV; n++) pc[n] = n < npv ? in(n) : T(0.0);
             for (int m = 0; m < ND; m++) dt[m] = T(d[m]);
             dt[D_RHO] = in(od); dt[D_H] = in(od + 1); dt[D_MU] = in(od + 2);
@@ -307,15 +302,12 @@ __global__ __launch_bounds__(128) void k_cell_vjp(DS S, AD A)
                 phi = phi + rb[5] * (rv - rc);
             }
             if (sst) {
-                int c = 0;
-                for (int q = 0; q < 7; q++) {
-                    for (int n = 0; n < NPV; n++) x7t[q][n] = T(x7[q][n]);
-                    if (q == 1) { for (int n = 0; n < npv; n++) x7t[1][n] = pc[n]; continue; }
-                    for (int v = 0; v < 5; v++) x7t[q][nbv[v]] = in(ox + 5 * c + v);
-                    c++;
-                }
+                T gr[5][3];
+                for (int v = 0; v < 5; v++)
+                    for (int c = 0; c < 3; c++) gr[v][c] = in(og + 3 * v + c);
                 T src[2], it[4], oc;
-                sst_source<DES, T>(P, x7t, f6, vol, ydn, dt, T(mut), src, it, oc);
+                sst_source_g<DES, T>(P, gr[0], gr[1], gr[2], gr[3], gr[4], pc[7], pc[8], f6, vol, ydn, dt, T(mut),
+                                     src, it, oc);
                 phi = phi + rb[7] * src[0] + rb[8] * src[1];
             }
             if (bdf) {
@@ -334,18 +326,29 @@ __global__ __launch_bounds__(128) void k_cell_vjp(DS S, AD A)
             return phi;
         }, zb);
     }
-    // outputs: centre (q = 1) and neighbour bars, the cell's property bars
-    for (int q = 0; q < 7; q++)
-        for (int n = 0; n < npv; n++) A.cb[((long)q * npv + n) * S.NC + t] = 0.0;
-    for (int n = 0; n < npv; n++) A.cb[((long)1 * npv + n) * S.NC + t] = zb[n];
-    if (sst) {
-        int c = 0;
-        for (int q = 0; q < 7; q++) {
-            if (q == 1) continue;
-            for (int v = 0; v < 5; v++) A.cb[((long)q * npv + nbv[v]) * S.NC + t] = zb[ox + 5 * c + v];
-            c++;
-        }
+    // outputs: centre (q = 1) and neighbour bars, the cell's property bars;
+    // a gradient component g[c] = iv * sum_q w_q[c] x_q with the Green-Gauss
+    // weights w_q = -f6[0]/2, (face sum)/2, +f6[1]/2, -f6[2]/2, +f6[3]/2,
+    // -f6[4]/2, +f6[5]/2 for q = 0..6
+    double wq[7][3];
+    for (int c = 0; c < 3; c++) {
+        wq[0][c] = -0.5 * f6[0][c] * iv;
+        wq[2][c] = 0.5 * f6[1][c] * iv;
+        wq[3][c] = -0.5 * f6[2][c] * iv;
+        wq[4][c] = 0.5 * f6[3][c] * iv;
+        wq[5][c] = -0.5 * f6[4][c] * iv;
+        wq[6][c] = 0.5 * f6[5][c] * iv;
+        wq[1][c] = 0.5 * (f6[1][c] + f6[3][c] + f6[5][c] - f6[0][c] - f6[2][c] - f6[4][c]) * iv;
     }
+    for (int q = 0; q < 7; q++)
+        for (int n = 0; n < npv; n++) {
+            double v = q == 1 ? zb[n] : 0.0;
+            if (sst)
+                for (int gv = 0; gv < 5; gv++)
+                    if (gvar[gv] == n)
+                        v += wq[q][0] * zb[og + 3 * gv] + wq[q][1] * zb[og + 3 * gv + 1] + wq[q][2] * zb[og + 3 * gv + 2];
+            A.cb[((long)q * npv + n) * S.NC + t] = v;
+        }
     for (int m = 0; m < ND; m++) A.cdb[(long)m * S.NC + t] = 0.0;
     A.cdb[(long)D_RHO * S.NC + t] = zb[od];
     A.cdb[(long)D_H * S.NC + t] = zb[od + 1];

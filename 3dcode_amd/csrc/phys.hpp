@@ -1637,24 +1637,18 @@ HD R des_length(int ides, R k, R om, const double f6[6][3], double vol, double y
     return fd * (1.0 + fe) * tl_rans + (1.0 - fd) * tl_les;
 }
 
-// k-omega SST source: src[2], itt[4], omega_cut.  DES: the length-scale
-// variants are compiled in (the RANS-only instantiation keeps its registers).
+// The SST source from the cell's Green-Gauss gradients of u, v, w, k, omega
+// and its own k, omega (the adjoint differentiates this form: the neighbours
+// enter only through the gradients, which are linear in them).
 template <bool DES = false, class R = double>
-HD void sst_source(const Par &P, const R x7[7][NPV], const double f6[6][3], double vol, double ydns, const R *d,
-                   R mut, R *src, R *itt, R &ocut)
+HD void sst_source_g(const Par &P, const R *gu, const R *gv, const R *gw, const R *gk, const R *go, R k, R om,
+                     const double f6[6][3], double vol, double ydns, const R *d, R mut, R *src, R *itt, R &ocut)
 {
     const double c23 = 2.0 / 3.0, c59 = 5.0 / 9.0;
-    double iv = 1.0 / vol;
-    R gu[3], gv[3], gw[3], gk[3], go[3];
-    gg_grad(x7, 7, f6, iv, gk);
-    gg_grad(x7, 8, f6, iv, go);
-    gg_grad(x7, 1, f6, iv, gu);
-    gg_grad(x7, 2, f6, iv, gv);
-    gg_grad(x7, 3, f6, iv, gw);
     R div = gu[0] + gv[1] + gw[2];
     R s2 = 2.0 * (gu[0] * gu[0] + gv[1] * gv[1] + gw[2] * gw[2]) + (gu[1] + gv[0]) * (gu[1] + gv[0])
          + (gu[2] + gw[0]) * (gu[2] + gw[0]) + (gv[2] + gw[1]) * (gv[2] + gw[1]);
-    R k = x7[1][7], om = x7[1][8], rho = d[D_RHO];
+    R rho = d[D_RHO];
     R prod = mut * (s2 - c23 * (div * div)) - c23 * rho * k * div;
     prod = mn(prod, 0.9 * rho * k * om);
     ocut = sqrt(s2 - c23 * (div * div));
@@ -1690,6 +1684,22 @@ HD void sst_source(const Par &P, const R x7[7][NPV], const double f6[6][3], doub
     R l1 = 0.5 * t + sq, l2 = 0.5 * t - sq;
     double s1 = signbit(l1) ? -1.0 : 1.0, s2s = signbit(l2) ? -1.0 : 1.0;
     itt[0] = itt[0] * s1; itt[1] = itt[1] * s1; itt[2] = itt[2] * s2s; itt[3] = itt[3] * s2s;
+}
+
+// k-omega SST source: src[2], itt[4], omega_cut.  DES: the length-scale
+// variants are compiled in (the RANS-only instantiation keeps its registers).
+template <bool DES = false, class R = double>
+HD void sst_source(const Par &P, const R x7[7][NPV], const double f6[6][3], double vol, double ydns, const R *d,
+                   R mut, R *src, R *itt, R &ocut)
+{
+    double iv = 1.0 / vol;
+    R gu[3], gv[3], gw[3], gk[3], go[3];
+    gg_grad(x7, 7, f6, iv, gk);
+    gg_grad(x7, 8, f6, iv, go);
+    gg_grad(x7, 1, f6, iv, gu);
+    gg_grad(x7, 2, f6, iv, gv);
+    gg_grad(x7, 3, f6, iv, gw);
+    sst_source_g<DES, R>(P, gu, gv, gw, gk, go, x7[1][7], x7[1][8], f6, vol, ydns, d, mut, src, itt, ocut);
 }
 
 // SST eddy viscosity mu_t = 0.31 rho k / max(0.31 omega, S F2)
