@@ -243,7 +243,7 @@ __device__ __forceinline__ void src_off(int q, int &di, int &dj, int &dk)
 
 // Cell sources of one interior cell: Merkle, SST (RANS or DES length), BDF.
 template <bool DES>
-__global__ __launch_bounds__(128) void k_cell_vjp(DS S, AD A)
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_cell_vjp(DS S, AD A)
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= S.NC) return;
@@ -358,7 +358,7 @@ __global__ __launch_bounds__(128) void k_cell_vjp(DS S, AD A)
 // pv_bar of every ghosted cell: its bars from the faces whose stencils hold
 // it and from the sources of its neighbours, in a fixed order, then its
 // property bars through the EOS.
-__global__ __launch_bounds__(128) void k_adj_gather(DS S, AD A)
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_adj_gather(DS S, AD A)
 {
     long gl = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (gl >= S.NG) return;
