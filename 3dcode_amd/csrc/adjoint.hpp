@@ -1,7 +1,7 @@
 // Discrete adjoint of the residual: cav3d_rhs_vjp.
 //
 // The operator is the residual cav3d_calrhs evaluates (src/rhs_module.f90:
-// 260-643: inviscid and viscous face-flux differences, the Merkle and SST
+// 260-643: inviscid and viscous face-flux differences, the mass-transfer and SST
 // sources, the BDF term), taken as a function of every block's ghosted
 // primitive state pv, with the cell properties re-derived from pv by the EOS
 // (rho, h, c^2, the EOS derivatives, mu, kappa: deteos_prop) and with the
@@ -241,8 +241,10 @@ __device__ __forceinline__ void src_off(int q, int &di, int &dj, int &dk)
     dk = q == 5 ? -1 : (q == 6 ? 1 : 0);
 }
 
-// Cell sources of one interior cell: Merkle, SST (RANS or DES length), BDF.
-template <bool DES>
+// Cell sources of one interior cell: the mass transfer, SST, BDF.  MODELS:
+// the instantiation with the Kunz / Singhal / Schnerr-Sauer / Lee sources
+// (a full EOS of the cell) and the DES length scales, as k_combine<true>.
+template <bool MODELS>
 __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_cell_vjp(DS S, AD A)
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -255,7 +257,7 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     const long g = gidx(B, i, j, k);
     double rb[NPV];
     for (int n = 0; n < NPV; n++) rb[n] = n < npv ? A.rb[(long)n * S.NC + t] : 0.0;
-    const bool sst = P.iturb == 0, cav_ = P.ncav == 1, bdf = P.nsteady > 0;
+    const bool sst = P.iturb == 0, cav_ = P.ncav >= 1, bdf = P.nsteady > 0;
     // z = {pc[npv], rho, h, mu, the Green-Gauss gradients of u, v, w, k, omega (15)}
     constexpr int NZ = NPV + 3 + 15;
     const int od = npv, og = npv + 3;
@@ -298,7 +300,17 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
             T phi(0.0);
             if (cav_) {
                 T rc, rv, ic[4];
-                merkle<T>(P, pc, dt, vol, rc, rv, ic);
+                if constexpr (MODELS) {
+                    if (P.ncav >= 2) {      // on the cell's full EOS, as k_combine<true>
+                        T dd[ND], full[18];
+                        mixture_eos<T>(pc[0] + P.pref, pc[4], pc[5], pc[6], dd, false, full);
+                        cav_model<T>(P, pc, full, vol, rc, rv, ic);
+                    } else {
+                        merkle<T>(P, pc, dt, vol, rc, rv, ic);
+                    }
+                } else {
+                    merkle<T>(P, pc, dt, vol, rc, rv, ic);
+                }
                 phi = phi + rb[5] * (rv - rc);
             }
             if (sst) {
@@ -306,7 +318,7 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
                 for (int v = 0; v < 5; v++)
                     for (int c = 0; c < 3; c++) gr[v][c] = in(og + 3 * v + c);
                 T src[2], it[4], oc;
-                sst_source_g<DES, T>(P, gr[0], gr[1], gr[2], gr[3], gr[4], pc[7], pc[8], f6, vol, ydn, dt, T(mut),
+                sst_source_g<MODELS, T>(P, gr[0], gr[1], gr[2], gr[3], gr[4], pc[7], pc[8], f6, vol, ydn, dt, T(mut),
                                      src, it, oc);
                 phi = phi + rb[7] * src[0] + rb[8] * src[1];
             }
@@ -489,7 +501,7 @@ int launch_vjp(cav3d_ctx *c)
         hipLaunchKernelGGL((k_vface_vjp<1>), dim3(nblk(S.NF[1], 128)), dim3(128), 0, c->st, S, A);
         hipLaunchKernelGGL((k_vface_vjp<2>), dim3(nblk(S.NF[2], 128)), dim3(128), 0, c->st, S, A);
     }
-    if (S.P.ides > 0) hipLaunchKernelGGL(k_cell_vjp<true>, dim3(nblk(S.NC, 128)), dim3(128), 0, c->st, S, A);
+    if (S.P.ides > 0 || S.P.ncav >= 2) hipLaunchKernelGGL(k_cell_vjp<true>, dim3(nblk(S.NC, 128)), dim3(128), 0, c->st, S, A);
     else hipLaunchKernelGGL(k_cell_vjp<false>, dim3(nblk(S.NC, 128)), dim3(128), 0, c->st, S, A);
     hipLaunchKernelGGL(k_adj_gather, dim3(nblk(S.NG, 128)), dim3(128), 0, c->st, S, A);
     return 0;
@@ -502,8 +514,10 @@ extern "C" {
 int cav3d_rhs_vjp(cav3d_ctx *c, const double *res_bar, double *pv_bar, double *scal_bar)
 {
     if (!c || !res_bar || !pv_bar) return fail("bad argument to cav3d_rhs_vjp", -1);
-    if (c->prm.ncav >= 2)
-        return fail("cav3d_rhs_vjp: the adjoint differentiates the Merkle source only (ncav 0 or 1)", -2);
+    if (c->prm.ncav == 2 || c->prm.ncav == 3)
+        return fail("cav3d_rhs_vjp: the Kunz and Singhal sources (ncav 2, 3) are not differentiated "
+                    "(their products differ from central differences by up to 5e-4); "
+                    "ncav 0, 1, 4, 5 are", -2);
     int rc;
     if ((rc = adj_alloc(c))) return rc;
     DS &S = c->S;

@@ -134,7 +134,14 @@ CAV_DN_CMP(>=)
 CAV_DN_CMP(==)
 CAV_DN_CMP(!=)
 #undef CAV_DN_CMP
-template <int W> HD Dn<W> sqrt(const Dn<W> &a) { double s = ::sqrt(a.v); return dn_map(a, s, 0.5 / s); }
+// at an argument of exactly 0 the derivatives below are taken as 0 (the
+// rate and blending terms reach 0 only through a max(., 0) or a zero mass
+// fraction, where the one-sided derivative that matters is 0)
+template <int W> HD Dn<W> sqrt(const Dn<W> &a)
+{
+    double s = ::sqrt(a.v);
+    return dn_map(a, s, s > 0.0 ? 0.5 / s : 0.0);
+}
 template <int W> HD Dn<W> fabs(const Dn<W> &a) { return dn_map(a, ::fabs(a.v), a.v < 0.0 ? -1.0 : 1.0); }
 template <int W> HD Dn<W> exp(const Dn<W> &a) { double e = ::exp(a.v); return dn_map(a, e, e); }
 template <int W> HD Dn<W> log(const Dn<W> &a) { return dn_map(a, ::log(a.v), 1.0 / a.v); }
@@ -142,7 +149,7 @@ template <int W> HD Dn<W> tanh(const Dn<W> &a) { double t = ::tanh(a.v); return 
 template <int W> HD Dn<W> pow(const Dn<W> &a, double b)
 {
     double p = ::pow(a.v, b);
-    return dn_map(a, p, b == 0.0 ? 0.0 : b * ::pow(a.v, b - 1.0));
+    return dn_map(a, p, b == 0.0 ? 0.0 : (a.v == 0.0 ? (b == 1.0 ? 1.0 : 0.0) : b * ::pow(a.v, b - 1.0)));
 }
 template <int W> HD Dn<W> pow(const Dn<W> &a, const Dn<W> &b)
 {
@@ -273,23 +280,23 @@ template <class R> HD R sat_p(R t)
 }
 
 // IF97 region-4 saturation temperature (src/eos_module.f90:2340-2354)
-HD double sat_t(double p)
+template <class R> HD R sat_t(R p)
 {
-    double beta = pow(p / 1.0e6, 0.25);
-    double e = beta * beta - 0.17073846940092e2 * beta + 0.14915108613530e2;
-    double f = 0.11670521452767e4 * (beta * beta) + 0.12020824702470e5 * beta - 0.48232657361591e4;
-    double g = -0.72421316703206e6 * (beta * beta) - 0.32325550322333e7 * beta + 0.40511340542057e6;
-    double d = 2.0 * g / (-f - sqrt(f * f - 4.0 * e * g));
-    double s = 0.65017534844798e3 + d;
+    R beta = pow(p / 1.0e6, 0.25);
+    R e = beta * beta - 0.17073846940092e2 * beta + 0.14915108613530e2;
+    R f = 0.11670521452767e4 * (beta * beta) + 0.12020824702470e5 * beta - 0.48232657361591e4;
+    R g = -0.72421316703206e6 * (beta * beta) - 0.32325550322333e7 * beta + 0.40511340542057e6;
+    R d = 2.0 * g / (-f - sqrt(f * f - 4.0 * e * g));
+    R s = 0.65017534844798e3 + d;
     return 0.5 * (0.65017534844798e3 + d - sqrt(s * s - 4.0 * (-0.23855557567849e0 + 0.65017534844798e3 * d)));
 }
 
 // surface tension of water against its vapour (src/eos_module.f90:2430-2438);
 // constant integer powers as gfortran expands them: x**3 = (x*x)*x,
 // x**4 = (x*x)*(x*x), x**5 = ((x*x)*x)*(x*x)
-HD double h2o_sigma(double t)
+template <class R> HD R h2o_sigma(R t)
 {
-    double t2 = t * t, t3 = t2 * t, t4 = t2 * t2, t5 = t3 * t2;
+    R t2 = t * t, t3 = t2 * t, t4 = t2 * t2, t5 = t3 * t2;
     return 0.0881130429998226 + 0.0000616475914044782 * t - 4.4860167748009e-7 * t2 + 2.53682468718936e-10 * t3
            - 2.62005668908913e-13 * t4 + 2.97878955533757e-16 * t5;
 }
@@ -1455,7 +1462,7 @@ HD void merkle(const Par &P, const R *pc, const R *d, double vol, R &rc, R &rv, 
 // The other mass-transfer models (src/cav_module.f90:173-357), on the full
 // dv(1:18) of the cell (dv[2..4] phase densities, dv[14..17] the vapour and
 // liquid density derivatives).  Rate clip and coupling sign as in merkle().
-HD void cav_clip_rates(const Par &P, const double *pc, const double *dv, double &r_c, double &r_v)
+template <class R> HD void cav_clip_rates(const Par &P, const R *pc, const R *dv, R &r_c, R &r_v)
 {
     if (P.cav_clip == 0) {
         r_c = mn(r_c, dv[0] * pc[4]);
@@ -1465,25 +1472,26 @@ HD void cav_clip_rates(const Par &P, const double *pc, const double *dv, double 
         r_v = mn(r_v, dv[0] * (1.0 - pc[5] - pc[6]));
     }
 }
-HD void cav_flip(double *icav)
+template <class R> HD void cav_flip(R *icav)
 {
     double s = (icav[2] < 0.0 || (icav[2] == 0.0 && signbit(icav[2]))) ? -1.0 : 1.0;
-    for (int q = 0; q < 4; q++) icav[q] *= s;
+    for (int q = 0; q < 4; q++) icav[q] = icav[q] * s;
 }
 
 // Kunz (ncav 2), src/cav_module.f90:173-211
-HD void kunz(const Par &P, const double *pc, const double *dv, double vol, double &rc, double &rv, double *icav)
+template <class R>
+HD void kunz(const Par &P, const R *pc, const R *dv, double vol, R &rc, R &rv, R *icav)
 {
-    double pww = sat_p(pc[4]);
-    double rho1 = 1.0 / dv[0];
-    double av = dv[0] * pc[5] / dv[3], ag = dv[0] * pc[6] / dv[4], al = 1.0 - av - ag;
-    double r_c = P.c_c * dv[3] * av * ((al + ag) * (al + ag)) * P.t_ref;
-    double pv_ = mx(pww - pc[0] - P.pref, 0.0);
-    double r_v = P.c_v * dv[0] * (1.0 - pc[5] - pc[6]) * pv_ * P.dp_ref * P.t_ref;
+    R pww = sat_p(pc[4]);
+    R rho1 = 1.0 / dv[0];
+    R av = dv[0] * pc[5] / dv[3], ag = dv[0] * pc[6] / dv[4], al = 1.0 - av - ag;
+    R r_c = P.c_c * dv[3] * av * ((al + ag) * (al + ag)) * P.t_ref;
+    R pv_ = mx(pww - pc[0] - P.pref, R(0.0));
+    R r_v = P.c_v * dv[0] * (1.0 - pc[5] - pc[6]) * pv_ * P.dp_ref * P.t_ref;
     cav_clip_rates(P, pc, dv, r_c, r_v);
     rc = r_c * vol;
     rv = r_v * vol;
-    double av1 = 1.0 - 4.0 * av + 3.0 * (av * av);
+    R av1 = 1.0 - 4.0 * av + 3.0 * (av * av);
     icav[0] = -(P.c_c * P.t_ref * (dv[6] * pc[5] * av1 + 2.0 * (av * av) * dv[14] * (al + ag)));
     icav[1] = -(P.c_c * P.t_ref * (dv[7] * pc[5] * av1 + 2.0 * (av * av) * dv[15] * (al + ag)));
     icav[2] = -(P.c_c * P.t_ref * (dv[8] * pc[5] + dv[0]) * av1);
@@ -1497,15 +1505,16 @@ HD void kunz(const Par &P, const double *pc, const double *dv, double vol, doubl
 }
 
 // Singhal (ncav 3), src/cav_module.f90:213-260
-HD void singhal(const Par &P, const double *pc, const double *dv, double vol, double &rc, double &rv, double *icav)
+template <class R>
+HD void singhal(const Par &P, const R *pc, const R *dv, double vol, R &rc, R &rv, R *icav)
 {
-    double pww = sat_p(pc[4]);
-    double sigma = h2o_sigma(pc[4]);
-    double rho1 = 1.0 / dv[2], rho2 = 1.0 / dv[3];
-    double kc = P.c_c * P.uref / sigma * dv[2] * dv[3], kv = P.c_v * P.uref / sigma * dv[2] * dv[3];
-    double pc_ = mx(pc[0] + P.pref - pww, 0.0), pv_ = mx(pww - pc[0] - P.pref, 0.0);
-    double r_c = kc * sqrt(2.0 / 3.0 * pc_ * rho1) * pc[5];
-    double r_v = kv * sqrt(2.0 / 3.0 * pv_ * rho1) * (1.0 - pc[5] - pc[6]);
+    R pww = sat_p(pc[4]);
+    R sigma = h2o_sigma(pc[4]);
+    R rho1 = 1.0 / dv[2], rho2 = 1.0 / dv[3];
+    R kc = P.c_c * P.uref / sigma * dv[2] * dv[3], kv = P.c_v * P.uref / sigma * dv[2] * dv[3];
+    R pc_ = mx(pc[0] + P.pref - pww, R(0.0)), pv_ = mx(pww - pc[0] - P.pref, R(0.0));
+    R r_c = kc * sqrt(2.0 / 3.0 * pc_ * rho1) * pc[5];
+    R r_v = kv * sqrt(2.0 / 3.0 * pv_ * rho1) * (1.0 - pc[5] - pc[6]);
     cav_clip_rates(P, pc, dv, r_c, r_v);
     rc = r_c * vol;
     rv = r_v * vol;
@@ -1514,7 +1523,7 @@ HD void singhal(const Par &P, const double *pc, const double *dv, double vol, do
                   - 0.5 * P.c_c * P.uref / sigma * dv[2] * dv[3] * sqrt(2.0 / 3.0 * rho1) * pc[5] * pow(pc[0] + P.pref - pww, -0.5);
         icav[1] = -(r_c * (dv[15] * rho2 + 0.5 * dv[17] * rho1));
         icav[2] = -(kc * sqrt(2.0 / 3.0 * pc_ * rho1));
-        icav[3] = 0.0;
+        icav[3] = R(0.0);
     }
     if (r_v != 0.0) {
         icav[0] = r_v * (dv[14] * rho2 + 0.5 * dv[16] * rho1)
@@ -1527,17 +1536,18 @@ HD void singhal(const Par &P, const double *pc, const double *dv, double vol, do
 }
 
 // Schnerr-Sauer (ncav 4), src/cav_module.f90:262-316; bubble number density 1e13
-HD void schnerr_sauer(const Par &P, const double *pc, const double *dv, double vol, double &rc, double &rv, double *icav)
+template <class R>
+HD void schnerr_sauer(const Par &P, const R *pc, const R *dv, double vol, R &rc, R &rv, R *icav)
 {
     const double pi = 4.0 * atan(1.0), eta = 1.0e13;
-    double pww = sat_p(pc[4]);
-    double rho1 = 1.0 / dv[0], rho3 = 1.0 / dv[2], rho4 = 1.0 / dv[3];
-    double y1 = pc[5], y2 = pc[6], yl = 1.0 - pc[5] - pc[6];
+    R pww = sat_p(pc[4]);
+    R rho1 = 1.0 / dv[0], rho3 = 1.0 / dv[2], rho4 = 1.0 / dv[3];
+    R y1 = pc[5], y2 = pc[6], yl = 1.0 - pc[5] - pc[6];
     double kb = pow(4.0 * pi * eta / 3.0, 1.0 / 3.0), k23 = sqrt(2.0 / 3.0);
-    double r_c = P.c_c * 3.0 * kb * k23 * dv[0] * pow(dv[3], 1.0 / 3.0) * pow(dv[2], -5.0 / 6.0)
-                 * pow(y1, 2.0 / 3.0) * pow(yl, 4.0 / 3.0) * sqrt(mx(pc[0] + P.pref - pww, 0.0));
-    double r_v = P.c_v * 3.0 * kb * k23 * dv[0] * pow(dv[3], 1.0 / 3.0) * pow(dv[2], -5.0 / 6.0)
-                 * pow(y1 + y2, 2.0 / 3.0) * pow(yl, 4.0 / 3.0) * sqrt(mx(pww - pc[0] - P.pref, 0.0));
+    R r_c = P.c_c * 3.0 * kb * k23 * dv[0] * pow(dv[3], 1.0 / 3.0) * pow(dv[2], -5.0 / 6.0)
+                 * pow(y1, 2.0 / 3.0) * pow(yl, 4.0 / 3.0) * sqrt(mx(pc[0] + P.pref - pww, R(0.0)));
+    R r_v = P.c_v * 3.0 * kb * k23 * dv[0] * pow(dv[3], 1.0 / 3.0) * pow(dv[2], -5.0 / 6.0)
+                 * pow(y1 + y2, 2.0 / 3.0) * pow(yl, 4.0 / 3.0) * sqrt(mx(pww - pc[0] - P.pref, R(0.0)));
     cav_clip_rates(P, pc, dv, r_c, r_v);
     rc = r_c * vol;
     rv = r_v * vol;
@@ -1556,13 +1566,14 @@ HD void schnerr_sauer(const Par &P, const double *pc, const double *dv, double v
 }
 
 // Lee (ncav 5), src/cav_module.f90:318-356
-HD void lee(const Par &P, const double *pc, const double *dv, double vol, double &rc, double &rv, double *icav)
+template <class R>
+HD void lee(const Par &P, const R *pc, const R *dv, double vol, R &rc, R &rv, R *icav)
 {
-    double tww = sat_t(pc[0] + P.pref);
-    double rho1 = 1.0 / dv[0];
-    double tc = mx(tww - pc[4], 0.0), tv_ = mx(pc[4] - tww, 0.0);
-    double r_c = P.c_c * dv[0] * pc[5] * tc / tww;
-    double r_v = P.c_v * dv[0] * (1.0 - pc[5] - pc[6]) * tv_ / tww;
+    R tww = sat_t(pc[0] + P.pref);
+    R rho1 = 1.0 / dv[0];
+    R tc = mx(tww - pc[4], R(0.0)), tv_ = mx(pc[4] - tww, R(0.0));
+    R r_c = P.c_c * dv[0] * pc[5] * tc / tww;
+    R r_v = P.c_v * dv[0] * (1.0 - pc[5] - pc[6]) * tv_ / tww;
     cav_clip_rates(P, pc, dv, r_c, r_v);
     rc = r_c * vol;
     rv = r_v * vol;
@@ -1581,10 +1592,11 @@ HD void lee(const Par &P, const double *pc, const double *dv, double vol, double
 }
 
 // the model of ncav 2..5 (src/rhs_module.f90:98-110), full dv(1:18) of the cell
-HD void cav_model(const Par &P, const double *pc, const double *dv, double vol, double &rc, double &rv, double *icav)
+template <class R>
+HD void cav_model(const Par &P, const R *pc, const R *dv, double vol, R &rc, R &rv, R *icav)
 {
     rc = 0.0; rv = 0.0;
-    icav[0] = icav[1] = icav[2] = icav[3] = 0.0;
+    icav[0] = icav[1] = icav[2] = icav[3] = R(0.0);
     if (pc[4] >= P.t_crit) return;
     if (P.ncav == 2) kunz(P, pc, dv, vol, rc, rv, icav);
     else if (P.ncav == 3) singhal(P, pc, dv, vol, rc, rv, icav);

@@ -27,7 +27,11 @@ AUSMPWP = pytest.mark.skip(reason="AUSMPW+ is piecewise at every face; its face 
 
 CASES = ["lam_roe_tvd", "lam_roe_mlp", "lam_ausm_tvd_vanleer", "sst_cav_ausm_mlp", "sst_cav_roe_first",
          "lam_roe_2block", "lam_roe_bdf2", "sst_cav_bdf1", "sst_cav_roem_mlp",
-         pytest.param("lam_ausmpwp_tvd", marks=AUSMPWP), "sst_ddes", "sst_cav_3block"]
+         pytest.param("lam_ausmpwp_tvd", marks=AUSMPWP), "sst_ddes", "sst_cav_3block",
+         "sst_schnerr_sauer", "sst_lee"]
+# directions that must settle per case: Lee's rates switch on T - T_sat(p) in
+# every cell of the vapour pocket, so fewer random directions are kink-free
+NEED = {"sst_lee": 3}
 
 
 _OPEN = []
@@ -182,7 +186,7 @@ def test_vjp_matches_central_differences(pkg, name):
     rb = seeds(case, res0, rng)
     pb, _ = s.rhs_vjp(rb)
     dirs = directions(case, pvs, rng, 8)
-    worst = check_directions(lambda p: s.residual_of(p, fr["mut"], fr["qq"]), pvs, pb, rb, dirs, 5)
+    worst = check_directions(lambda p: s.residual_of(p, fr["mut"], fr["qq"]), pvs, pb, rb, dirs, NEED.get(name, 5))
     print(f"case {name}: worst relative difference {worst:.2e} over {len(dirs)} directions", flush=True)
     s.close()
 
@@ -300,3 +304,16 @@ def test_autograd_harness():
                        text=True, timeout=240)
     print(r.stdout[-2000:], r.stderr[-2000:], flush=True)
     assert r.returncode == 0
+
+
+@pytest.mark.parametrize("name", ["sst_kunz", "sst_singhal"])
+def test_vjp_rejects_unverified_models(pkg, name):
+    """Kunz and Singhal (ncav 2, 3) are refused with a message, not computed."""
+    case = small_cases(pkg)[name]()
+    s = pkg.Solver(case)
+    _OPEN.append(s)
+    for ib, st in enumerate(case.init):
+        s.put_state(ib, *st)
+    rb = [np.zeros(case.npv * (b.imax - 1) * (b.jmax - 1) * (b.kmax - 1)) for b in case.blocks]
+    with pytest.raises(pkg.Cav3dError, match="Kunz and Singhal"):
+        s.rhs_vjp(rb)
