@@ -170,6 +170,24 @@ void pc_cav_model(const int *iv, const double *dv, const double *pc, const doubl
     for (int q = 0; q < 4; q++) out[2 + q] = ic[q];
 }
 
+// rv - rc of the mass-transfer model (ncav 2-5) on the cell's own EOS, and its
+// gradient over (p, u, v, w, T, y1, y2) by the dual pass the adjoint kernel runs
+void pc_cav_adj(const int *iv, const double *dv, const double *pv, double *val, double *grad)
+{
+    Par P = mkpar(iv, dv);
+    using T = Dn<1>;
+    for (int q = 0; q < 7; q++) {
+        T pc[NPV];
+        for (int n = 0; n < NPV; n++) pc[n] = T(pv[n]);
+        pc[q] = T::seed(pv[q], 0);
+        T dd[ND], full[18], rc, rv, ic[4];
+        mixture_eos<T>(pc[0] + P.pref, pc[4], pc[5], pc[6], dd, false, full);
+        cav_model<T>(P, pc, full, 1.0, rc, rv, ic);
+        val[0] = (rv - rc).v;
+        grad[q] = (rv - rc).d[0];
+    }
+}
+
 double pc_sat_t(double p) { return sat_t(p); }
 double pc_h2o_sigma(double t) { return h2o_sigma(t); }
 

@@ -43,6 +43,7 @@ def build(outdir):
     L.pc_face32.argtypes = [_I, _D, _D, _D, _D, _D, _D]
     L.pc_face_adj.argtypes = [_I, _D, _D, _D, _D, _D, _D, _D, _D, _D]
     L.pc_eos_adj.argtypes = [C.c_int, _D, _D, _D]
+    L.pc_cav_adj.argtypes = [_I, _D, _D, _D, _D]
     L.pc_visc.argtypes = [_I, _D, _D, _D, _D, _D, _D, _D, _D, _D, C.c_double, C.c_double, _D]
     return L
 
