@@ -169,7 +169,7 @@ int cav3d_import(cav3d_ctx *ctx, const char *path, int *nps, int *nts, double *t
  *             preconditioning speed)
  * FP64 throughout, also in a CAV3D_MIXED context (it is the FP64 residual's
  * VJP).  Every option cav3d_create accepts is differentiated except the
- * Kunz and Singhal mass-transfer sources (ncav 2, 3), which return an error.
+ * Singhal mass-transfer source (ncav 3), which returns an error.
  * Deterministic (fixed-order gathers, no atomics). */
 int cav3d_rhs_vjp(cav3d_ctx *ctx, const double *res_bar, double *pv_bar, double *scal_bar);
 

@@ -306,14 +306,51 @@ def test_autograd_harness():
     assert r.returncode == 0
 
 
-@pytest.mark.parametrize("name", ["sst_kunz", "sst_singhal"])
-def test_vjp_rejects_unverified_models(pkg, name):
-    """Kunz and Singhal (ncav 2, 3) are refused with a message, not computed."""
-    case = small_cases(pkg)[name]()
+def _only(arrs, npv, keep):
+    """Copies of per-block (cell, variable) arrays with every variable outside keep zeroed."""
+    out = []
+    for r in arrs:
+        a = r.reshape(-1, npv).copy()
+        m = np.zeros(npv, bool)
+        m[list(keep)] = True
+        a[:, ~m] = 0.0
+        out.append(a.ravel())
+    return out
+
+
+@pytest.mark.parametrize("name", ["sst_kunz"])
+def test_vjp_kunz_split(pkg, name):
+    """Kunz (ncav 2).  On these states a y1 step of the flux
+    equations crosses a kink at every step size tried, and a mixed direction
+    can then pass the step-agreement test by chance.  So the check is split:
+    every variable but y1 against all equations, and y1 against its own
+    equation, which holds the mass-transfer source.  The y1 bars of the flux
+    equations are checked face by face on the host
+    (tests/test_adjoint_host.py)."""
+    print(f"case {name} adjoint vs FD (split)", flush=True)
+    case, s, pvs, fr = prepared(pkg, name)
+    npv = case.npv
+    rng = np.random.default_rng(zlib.crc32(name.encode()))
+    f = lambda p: s.residual_of(p, fr["mut"], fr["qq"])
+    rb = seeds(case, f(pvs), rng)
+    pb, _ = s.rhs_vjp(rb)
+    dirs = [_only(d, npv, [n for n in range(npv) if n != 5]) for d in directions(case, pvs, rng, 8)]
+    w1 = check_directions(f, pvs, pb, rb, dirs, 4)
+    rb5 = _only(rb, npv, [5])
+    pb5, _ = s.rhs_vjp(rb5)
+    dirs5 = [_only(d, npv, [5]) for d in directions(case, pvs, rng, 4)]
+    w2 = check_directions(f, pvs, pb5, rb5, dirs5, 3)
+    print(f"case {name}: worst relative difference {max(w1, w2):.2e}", flush=True)
+    s.close()
+
+
+def test_vjp_rejects_singhal(pkg):
+    """Singhal (ncav 3) is refused with a message, not computed."""
+    case = small_cases(pkg)["sst_singhal"]()
     s = pkg.Solver(case)
     _OPEN.append(s)
     for ib, st in enumerate(case.init):
         s.put_state(ib, *st)
     rb = [np.zeros(case.npv * (b.imax - 1) * (b.jmax - 1) * (b.kmax - 1)) for b in case.blocks]
-    with pytest.raises(pkg.Cav3dError, match="Kunz and Singhal"):
+    with pytest.raises(pkg.Cav3dError, match="Singhal"):
         s.rhs_vjp(rb)
