@@ -514,9 +514,6 @@ extern "C" {
 int cav3d_rhs_vjp(cav3d_ctx *c, const double *res_bar, double *pv_bar, double *scal_bar)
 {
     if (!c || !res_bar || !pv_bar) return fail("bad argument to cav3d_rhs_vjp", -1);
-    if (c->prm.ncav == 3)
-        return fail("cav3d_rhs_vjp: the Singhal source (ncav 3) is not differentiated "
-                    "(a product differs from central differences by 1.8e-5); ncav 0-2, 4, 5 are", -2);
     int rc;
     if ((rc = adj_alloc(c))) return rc;
     DS &S = c->S;

@@ -168,8 +168,7 @@ int cav3d_import(cav3d_ctx *ctx, const char *path, int *nps, int *nts, double *t
  *             gauge pv, [1] d(res_bar . res)/d(u_ref) (low-Mach
  *             preconditioning speed)
  * FP64 throughout, also in a CAV3D_MIXED context (it is the FP64 residual's
- * VJP).  Every option cav3d_create accepts is differentiated except the
- * Singhal mass-transfer source (ncav 3), which returns an error.
+ * VJP).  Every option cav3d_create accepts is differentiated.
  * Deterministic (fixed-order gathers, no atomics). */
 int cav3d_rhs_vjp(cav3d_ctx *ctx, const double *res_bar, double *pv_bar, double *scal_bar);
 

@@ -136,15 +136,15 @@ def phi(rb, res):
     return sum(float(np.dot(a, b)) for a, b in zip(rb, res))
 
 
-def kink_aware(diff, adj):
+def kink_aware(diff, adj, steps=(1.0, 0.25, 0.0625)):
     """Compare an adjoint directional derivative with central differences
     diff(c) at steps c = 1, 1/4, 1/16: the first pair of consecutive steps
     that agree to 1e-4 (no limiter / clamp / |.| kink inside the longer one)
     is used, the adjoint is compared with the nearer of the two, allowing
     twice their difference (the differences' rounding, which shorter steps
     magnify).  Returns the relative error, or None when no pair agrees."""
-    prev = diff(1.0)
-    for c in (0.25, 0.0625):
+    prev = diff(steps[0])
+    for c in steps[1:]:
         cur = diff(c)
         if abs(prev - cur) <= 1e-4 * max(abs(prev), abs(cur)):
             miss = min(abs(prev - adj), abs(cur - adj)) - 2.0 * abs(prev - cur)
@@ -153,7 +153,7 @@ def kink_aware(diff, adj):
     return None
 
 
-def check_directions(f, pvs, pb, rb, dirs, need):
+def check_directions(f, pvs, pb, rb, dirs, need, steps=(1.0, 0.25, 0.0625)):
     """Every direction whose differences settle (kink_aware) must match the
     adjoint to TOL, and at least `need` directions must settle."""
     worst, counted = 0.0, 0
@@ -166,7 +166,7 @@ def check_directions(f, pvs, pb, rb, dirs, need):
             rm = f([p - c * v for p, v in zip(pvs, d)])
             return 0.5 * sum(float(np.dot(w, a - b)) for w, a, b in zip(rb, rp, rm)) / c
         adj = sum(float(np.dot(a, v)) for a, v in zip(pb, d))
-        err = kink_aware(diff, adj)
+        err = kink_aware(diff, adj, steps)
         if err is None:
             print(f"  direction skipped: no two steps agree (adjoint {adj:.9e})", flush=True)
             continue
@@ -318,9 +318,15 @@ def _only(arrs, npv, keep):
     return out
 
 
-@pytest.mark.parametrize("name", ["sst_kunz"])
-def test_vjp_kunz_split(pkg, name):
-    """Kunz (ncav 2).  On these states a y1 step of the flux
+# Singhal's products on these seeds are small against the residual entries
+# they difference (one direction 50x below the others), so its differences
+# are rounding-limited at the standard steps: it starts from a 4x longer step.
+SPLIT = [("sst_kunz", (1.0, 0.25, 0.0625)), ("sst_singhal", (4.0, 1.0, 0.25))]
+
+
+@pytest.mark.parametrize("name,steps", SPLIT)
+def test_vjp_kunz_singhal_split(pkg, name, steps):
+    """Kunz and Singhal (ncav 2, 3).  On these states a y1 step of the flux
     equations crosses a kink at every step size tried, and a mixed direction
     can then pass the step-agreement test by chance.  So the check is split:
     every variable but y1 against all equations, and y1 against its own
@@ -335,22 +341,10 @@ def test_vjp_kunz_split(pkg, name):
     rb = seeds(case, f(pvs), rng)
     pb, _ = s.rhs_vjp(rb)
     dirs = [_only(d, npv, [n for n in range(npv) if n != 5]) for d in directions(case, pvs, rng, 8)]
-    w1 = check_directions(f, pvs, pb, rb, dirs, 4)
+    w1 = check_directions(f, pvs, pb, rb, dirs, 4, steps)
     rb5 = _only(rb, npv, [5])
     pb5, _ = s.rhs_vjp(rb5)
     dirs5 = [_only(d, npv, [5]) for d in directions(case, pvs, rng, 4)]
-    w2 = check_directions(f, pvs, pb5, rb5, dirs5, 3)
+    w2 = check_directions(f, pvs, pb5, rb5, dirs5, 3, steps)
     print(f"case {name}: worst relative difference {max(w1, w2):.2e}", flush=True)
     s.close()
-
-
-def test_vjp_rejects_singhal(pkg):
-    """Singhal (ncav 3) is refused with a message, not computed."""
-    case = small_cases(pkg)["sst_singhal"]()
-    s = pkg.Solver(case)
-    _OPEN.append(s)
-    for ib, st in enumerate(case.init):
-        s.put_state(ib, *st)
-    rb = [np.zeros(case.npv * (b.imax - 1) * (b.jmax - 1) * (b.kmax - 1)) for b in case.blocks]
-    with pytest.raises(pkg.Cav3dError, match="Singhal"):
-        s.rhs_vjp(rb)
