@@ -1,7 +1,8 @@
 /*
  * cav3d — MI355X (gfx950) implementation of the pseudo-time iteration of the
  * all-speed two-phase solver: RHS residual (MUSCL/MLP, IAPWS-IF97 EOS,
- * preconditioned Roe / AUSM+-up, viscous flux, k-omega SST, Merkle mass
+ * preconditioned Roe / Roe-M / AUSMPW+ / AUSM+-up, viscous flux, k-omega
+ * SST with DES variants, Merkle / Kunz / Singhal / Schnerr-Sauer / Lee mass
  * transfer) plus the implicit LU-SGS update, with every grid block resident on
  * one GPU and block-interface halos filled by device copy kernels.
  *
@@ -78,12 +79,12 @@ enum { CAV3D_CLIP_AS_WRITTEN = 0, CAV3D_CLIP_SPECIES = 1 };
  * (src/config_module.f90:7-28, 90-199), integer codes unchanged. */
 typedef struct {
     int iturb;      /* -3 Euler, -2 laminar, 0 k-omega SST */
-    int nscheme;    /* 1 preconditioned Roe, 4 AUSM+-up */
+    int nscheme;    /* 1 preconditioned Roe, 2 Roe-M, 3 AUSMPW+, 4 AUSM+-up */
     int nmuscl;     /* 0 first order, 1 MUSCL-TVD, 2 MLP */
     int nlim;       /* limiter 0..10 */
     int local;      /* -1 fixed, 0 global-min, 1 local time step */
     int prec;       /* 0 none, 1 steady, 2 unsteady preconditioning */
-    int ncav;       /* 0 none, 1 Merkle */
+    int ncav;       /* 0 none, 1 Merkle, 2 Kunz, 3 Singhal, 4 Schnerr-Sauer, 5 Lee */
     int nsteady;    /* 0 steady, 1 BDF1, 2 BDF2 dual time */
     int precision;  /* CAV3D_FP64 or CAV3D_MIXED */
     int cav_clip;   /* CAV3D_CLIP_AS_WRITTEN or CAV3D_CLIP_SPECIES */
