@@ -50,6 +50,7 @@ struct AD {
     double *ub[3];          // d(phi_f)/d(u_ref) [NF]
     double *cb;             // cell-source bars on the 7-point stencil [7][npv][NC]
     double *cdb;            // cell-source property bars [ND][NC]
+    double *cub;            // cell-source d(phi_c)/d(u_ref) [NC] (Singhal's rate constants)
     double *pb;             // pv_bar [npv][NG]
     unsigned long long mask;   // stencil points a face writes (bit ll-1)
 };
@@ -258,10 +259,13 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     double rb[NPV];
     for (int n = 0; n < NPV; n++) rb[n] = n < npv ? A.rb[(long)n * S.NC + t] : 0.0;
     const bool sst = P.iturb == 0, cav_ = P.ncav >= 1, bdf = P.nsteady > 0;
-    // z = {pc[npv], rho, h, mu, the Green-Gauss gradients of u, v, w, k, omega (15)}
-    constexpr int NZ = NPV + 3 + 15;
+    // z = {pc[npv], rho, h, mu, the Green-Gauss gradients of u, v, w, k, omega (15),
+    //      u_ref (Singhal only)}
+    constexpr int NZ = NPV + 3 + 15 + 1;
     const int od = npv, og = npv + 3;
-    const int nz = (cav_ || bdf || sst) ? (sst ? og + 15 : og) : 0;
+    const int nz0 = (cav_ || bdf || sst) ? (sst ? og + 15 : og) : 0;
+    const int ou = (nz0 && P.ncav == 3) ? nz0 : -1;
+    const int nz = ou >= 0 ? nz0 + 1 : nz0;
     double zx[NZ], zb[NZ];
     for (int q = 0; q < NZ; q++) zx[q] = zb[q] = 0.0;
     const int gvar[5] = {1, 2, 3, 7, 8};
@@ -283,6 +287,7 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         zx[od] = d[D_RHO]; zx[od + 1] = d[D_H]; zx[od + 2] = d[D_MU];
         if (sst)
             for (int v = 0; v < 5; v++) gg_grad<double>(x7, gvar[v], f6, iv, zx + og + 3 * v);
+        if (ou >= 0) zx[ou] = P.uref;
         const double ydn = sst ? S.ydn[widx(B, i, j, k)] : 1.0;
         const double mut = S.mut ? S.mut[g] : 0.0;
         double q1[NPV], q2[NPV];
@@ -292,7 +297,10 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         }
         constexpr int W = 2;
         using T = Dn<W>;
-        dn_grad<W>(nz, [&](int q) -> double { return zx[q]; }, [&](auto in, int) -> T {
+        dn_grad<W>(nz, [&](int q) -> double { return zx[q]; }, [&](auto in, int base) -> T {
+            Par Pu = P;
+            const int lu = ou - base;
+            Pu.uref_seed = (ou >= 0 && lu >= 0 && lu < W) ? lu + 1 : 0;
             T pc[NPV], dt[ND];
             for (int n = 0; n < NPV; n++) pc[n] = n < npv ? in(n) : T(0.0);
             for (int m = 0; m < ND; m++) dt[m] = T(d[m]);
@@ -304,7 +312,7 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
                     if (P.ncav >= 2) {      // on the cell's full EOS, as k_combine<true>
                         T dd[ND], full[18];
                         mixture_eos<T>(pc[0] + P.pref, pc[4], pc[5], pc[6], dd, false, full);
-                        cav_model<T>(P, pc, full, vol, rc, rv, ic);
+                        cav_model<T>(Pu, pc, full, vol, rc, rv, ic);
                     } else {
                         merkle<T>(P, pc, dt, vol, rc, rv, ic);
                     }
@@ -365,6 +373,7 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     A.cdb[(long)D_RHO * S.NC + t] = zb[od];
     A.cdb[(long)D_H * S.NC + t] = zb[od + 1];
     A.cdb[(long)D_MU * S.NC + t] = zb[od + 2];
+    A.cub[t] = ou >= 0 ? zb[ou] : 0.0;
 }
 
 // pv_bar of every ghosted cell: its bars from the faces whose stencils hold
@@ -466,8 +475,8 @@ int adj_alloc(cav3d_ctx *c)
     const int npv = c->npv;
     int rc;
     if ((rc = dalloc(c, &c->a_rb, (long)npv * S.NC)) || (rc = dalloc(c, &c->a_cb, 7L * npv * S.NC)) ||
-        (rc = dalloc(c, &c->a_cdb, (long)ND * S.NC)) || (rc = dalloc(c, &c->a_pb, (long)npv * S.NG)) ||
-        (rc = dalloc(c, &c->a_sum, 8 + 4L * SUM_WG)))
+        (rc = dalloc(c, &c->a_cdb, (long)ND * S.NC)) || (rc = dalloc(c, &c->a_cub, S.NC)) || (rc = dalloc(c, &c->a_pb, (long)npv * S.NG)) ||
+        (rc = dalloc(c, &c->a_sum, 8 + 5L * SUM_WG)))
         return rc;
     for (int d = 0; d < 3; d++)
         if ((rc = dalloc(c, &c->a_fb[d], (long)NSLOT * npv * S.NF[d])) || (rc = dalloc(c, &c->a_db[d], 2L * ND * S.NF[d])) ||
@@ -484,7 +493,7 @@ int launch_vjp(cav3d_ctx *c)
     AD A;
     A.rb = c->a_rb;
     for (int d = 0; d < 3; d++) { A.fb[d] = c->a_fb[d]; A.db[d] = c->a_db[d]; A.ub[d] = c->a_ub[d]; }
-    A.cb = c->a_cb; A.cdb = c->a_cdb; A.pb = c->a_pb;
+    A.cb = c->a_cb; A.cdb = c->a_cdb; A.cub = c->a_cub; A.pb = c->a_pb;
     A.mask = adj_mask(S.P);
     const bool sensor = S.P.nscheme == 2 || S.P.nscheme == 3;
     if (sensor) {
@@ -551,18 +560,18 @@ int cav3d_rhs_vjp(cav3d_ctx *c, const double *res_bar, double *pv_bar, double *s
     }
     if (scal_bar) {
         // stage one into a_sum[8 + SUM_WG * q], stage two into a_sum[q]
-        const double *src[4] = {c->a_pb, c->a_ub[0], c->a_ub[1], c->a_ub[2]};
-        const long len[4] = {S.NG, S.NF[0], S.NF[1], S.NF[2]};
-        for (int q = 0; q < 4; q++) {
+        const double *src[5] = {c->a_pb, c->a_ub[0], c->a_ub[1], c->a_ub[2], c->a_cub};
+        const long len[5] = {S.NG, S.NF[0], S.NF[1], S.NF[2], S.NC};
+        for (int q = 0; q < 5; q++) {
             double *part = c->a_sum + 8 + (long)SUM_WG * q;
             hipLaunchKernelGGL(k_sum_fixed, dim3(SUM_WG), dim3(256), 0, c->st, src[q], len[q], part);
             hipLaunchKernelGGL(k_sum_fixed, dim3(1), dim3(256), 0, c->st, (const double *)part, (long)SUM_WG, c->a_sum + q);
         }
-        double h[4];
+        double h[5];
         HIPCHK(hipMemcpyAsync(h, c->a_sum, sizeof(h), hipMemcpyDeviceToHost, c->st));
         HIPCHK(hipStreamSynchronize(c->st));
         scal_bar[0] = h[0];
-        scal_bar[1] = (h[1] + h[2]) + h[3];
+        scal_bar[1] = ((h[1] + h[2]) + h[3]) + h[4];
     }
     return check_launch("cav3d_rhs_vjp");
 }

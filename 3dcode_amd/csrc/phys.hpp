@@ -165,6 +165,21 @@ template <int W> HD bool signbit(const Dn<W> &a) { return ::signbit(a.v); }
 template <class R> struct IsDual { static constexpr bool v = false; };
 template <int W> struct IsDual<Dn<W>> { static constexpr bool v = true; };
 
+// min(a, b) of a mass-transfer rate a and its clip b, both proportional to a
+// mass fraction s >= 0 (y1, or 1 - y1 - y2).  The value is mn's.  At a tie
+// (typically s = 0, where both vanish) a dual takes, per lane, the slope of
+// the smaller function on the feasible side: min of the two slopes where the
+// lane moves s up, max where it moves s down.
+template <class R> HD R mn_rate(R a, R b, R) { return mn(a, b); }
+template <int W> HD Dn<W> mn_rate(const Dn<W> &a, const Dn<W> &b, const Dn<W> &s)
+{
+    if (a.v != b.v) return mn(a, b);
+    Dn<W> r = a;
+#pragma unroll
+    for (int q = 0; q < W; q++) r.d[q] = s.d[q] >= 0.0 ? ::fmin(a.d[q], b.d[q]) : ::fmax(a.d[q], b.d[q]);
+    return r;
+}
+
 // the preconditioning reference speed as R; a dual carries d/d(uref) on the
 // lane Par::uref_seed names
 template <class R> struct RefSpeed {
@@ -1438,8 +1453,9 @@ HD void merkle(const Par &P, const R *pc, const R *d, double vol, R &rc, R &rv, 
         r_c = mn(r_c, d[D_RHO] * pc[4]);
         r_v = mn(r_v, d[D_RHO] * (1.0 - pc[4] - pc[5]));
     } else {                 // on the mass fractions
-        r_c = mn(r_c, d[D_RHO] * pc[5]);
-        r_v = mn(r_v, d[D_RHO] * (1.0 - pc[5] - pc[6]));
+        const R yl = 1.0 - pc[5] - pc[6];
+        r_c = mn_rate(r_c, d[D_RHO] * pc[5], pc[5]);
+        r_v = mn_rate(r_v, d[D_RHO] * yl, yl);
     }
     rc = r_c * vol;
     rv = r_v * vol;
@@ -1468,8 +1484,9 @@ template <class R> HD void cav_clip_rates(const Par &P, const R *pc, const R *dv
         r_c = mn(r_c, dv[0] * pc[4]);
         r_v = mn(r_v, dv[0] * (1.0 - pc[4] - pc[5]));
     } else {
-        r_c = mn(r_c, dv[0] * pc[5]);
-        r_v = mn(r_v, dv[0] * (1.0 - pc[5] - pc[6]));
+        const R yl = 1.0 - pc[5] - pc[6];
+        r_c = mn_rate(r_c, dv[0] * pc[5], pc[5]);
+        r_v = mn_rate(r_v, dv[0] * yl, yl);
     }
 }
 template <class R> HD void cav_flip(R *icav)
@@ -1511,7 +1528,8 @@ HD void singhal(const Par &P, const R *pc, const R *dv, double vol, R &rc, R &rv
     R pww = sat_p(pc[4]);
     R sigma = h2o_sigma(pc[4]);
     R rho1 = 1.0 / dv[2], rho2 = 1.0 / dv[3];
-    R kc = P.c_c * P.uref / sigma * dv[2] * dv[3], kv = P.c_v * P.uref / sigma * dv[2] * dv[3];
+    const R uref = RefSpeed<R>::of(P);      // cav%uref is the config uref (src/cav_module.f90:77, 228)
+    R kc = P.c_c * uref / sigma * dv[2] * dv[3], kv = P.c_v * uref / sigma * dv[2] * dv[3];
     R pc_ = mx(pc[0] + P.pref - pww, R(0.0)), pv_ = mx(pww - pc[0] - P.pref, R(0.0));
     R r_c = kc * sqrt(2.0 / 3.0 * pc_ * rho1) * pc[5];
     R r_v = kv * sqrt(2.0 / 3.0 * pv_ * rho1) * (1.0 - pc[5] - pc[6]);
@@ -1548,6 +1566,12 @@ HD void schnerr_sauer(const Par &P, const R *pc, const R *dv, double vol, R &rc,
                  * pow(y1, 2.0 / 3.0) * pow(yl, 4.0 / 3.0) * sqrt(mx(pc[0] + P.pref - pww, R(0.0)));
     R r_v = P.c_v * 3.0 * kb * k23 * dv[0] * pow(dv[3], 1.0 / 3.0) * pow(dv[2], -5.0 / 6.0)
                  * pow(y1 + y2, 2.0 / 3.0) * pow(yl, 4.0 / 3.0) * sqrt(mx(pww - pc[0] - P.pref, R(0.0)));
+    if constexpr (IsDual<R>::v) {
+        // at y1 = 0 on the condensing side r_c grows as y1^(2/3) (infinite
+        // slope, which the dual pow reports as 0), so for y1 > 0 the species
+        // clip rho*y1 is the smaller: take its slope (same value, 0)
+        if (P.cav_clip == 1 && y1 == 0.0 && pc[0] + P.pref - pww > 0.0 && yl > 0.0) r_c = dv[0] * pc[5];
+    }
     cav_clip_rates(P, pc, dv, r_c, r_v);
     rc = r_c * vol;
     rv = r_v * vol;

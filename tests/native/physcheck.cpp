@@ -170,19 +170,24 @@ void pc_cav_model(const int *iv, const double *dv, const double *pc, const doubl
     for (int q = 0; q < 4; q++) out[2 + q] = ic[q];
 }
 
-// rv - rc of the mass-transfer model (ncav 2-5) on the cell's own EOS, and its
-// gradient over (p, u, v, w, T, y1, y2) by the dual pass the adjoint kernel runs
+// rv - rc of the mass-transfer model (ncav 1-5) on the cell's own EOS, and its
+// gradient over (p, u, v, w, T, y1, y2) and u_ref by the dual passes the
+// adjoint kernel runs
 void pc_cav_adj(const int *iv, const double *dv, const double *pv, double *val, double *grad)
 {
+    // ncav 1 is Merkle on the compact properties, 2..5 the models on dv(18)
     Par P = mkpar(iv, dv);
     using T = Dn<1>;
-    for (int q = 0; q < 7; q++) {
+    for (int q = 0; q < 8; q++) {
         T pc[NPV];
         for (int n = 0; n < NPV; n++) pc[n] = T(pv[n]);
-        pc[q] = T::seed(pv[q], 0);
+        Par Pu = P;
+        if (q < 7) pc[q] = T::seed(pv[q], 0);
+        else Pu.uref_seed = 1;
         T dd[ND], full[18], rc, rv, ic[4];
         mixture_eos<T>(pc[0] + P.pref, pc[4], pc[5], pc[6], dd, false, full);
-        cav_model<T>(P, pc, full, 1.0, rc, rv, ic);
+        if (P.ncav == 1) merkle<T>(Pu, pc, dd, 1.0, rc, rv, ic);
+        else cav_model<T>(Pu, pc, full, 1.0, rc, rv, ic);
         val[0] = (rv - rc).v;
         grad[q] = (rv - rc).d[0];
     }

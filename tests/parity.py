@@ -29,7 +29,17 @@ def small_cases(pk):
         "sst_singhal": lambda: _model(pk.cases.config2(12, 8, 6), 3),
         "sst_schnerr_sauer": lambda: _model(pk.cases.config2(12, 8, 6), 4),
         "sst_lee": lambda: _model(pk.cases.config2(12, 8, 6), 5),
+        # global-minimum (local = 0) and fixed (local = -1, dt = cfl seconds)
+        # pseudo-time steps, src/timestep_module.f90:214-316
+        "lam_roe_mintime": lambda: _local(bc(12, 8, 6, False, False, 1, 1, 1, cfl=5.0, seed=14), 0, 5.0),
+        "sst_cav_fixedtime": lambda: _local(_clip(pk.cases.config2(12, 8, 6), 1), -1, 2.0e-3),
     }
+
+
+def _local(case, local, cfl):
+    case.params["local"] = local
+    case.params["cfl"] = cfl
+    return case
 
 
 def _des(case, ides):
@@ -81,15 +91,19 @@ def interior(case, ib, a, nv):
 
 def rel_err(a, b, floor=1e-8):
     """Largest per-element relative difference of a against b per column,
-    with elements below floor * (column max) compared against that floor."""
+    with elements below floor * (column max) compared against that floor.
+    A non-finite element on either side counts as an infinite difference
+    (a cell that goes NaN on both sides is a failure, not an agreement)."""
     a = np.asarray(a, dtype=np.float64)
     b = np.asarray(b, dtype=np.float64)
     if a.ndim == 1:
         a = a[:, None]
         b = b[:, None]
-    scale = np.max(np.abs(b), axis=0)
-    den = np.maximum(np.abs(b), floor * np.maximum(scale, 1e-300))
-    e = np.abs(a - b) / den
-    e[np.isnan(a) != np.isnan(b)] = np.inf
-    e[np.isnan(a) & np.isnan(b)] = 0.0
+    bad = ~(np.isfinite(a) & np.isfinite(b))
+    bf = np.where(bad, 0.0, b)
+    scale = np.max(np.abs(bf), axis=0)
+    den = np.maximum(np.abs(bf), floor * np.maximum(scale, 1e-300))
+    with np.errstate(invalid="ignore"):
+        e = np.abs(np.where(bad, 0.0, a) - bf) / den
+    e[bad] = np.inf
     return np.max(e, axis=0)

@@ -1,0 +1,63 @@
+"""The HIP library against the oracle at the benchmark sizes: one LU-SGS
+iteration of the laminar bump channel at 64x32x32 cells (Roe, MUSCL
+minmod) and of the cavitating bump at 128x64x64 cells (SST, AUSM+-up,
+MLP, Merkle) under both clip modes, from the same seeded state.
+
+Bar (FP64 path): 1e-11 relative per cell and equation, with elements below
+1e-8 of their field's largest magnitude compared against that floor; the
+residual (a difference of fluxes) against a floor of 1e-6 of its field's
+largest magnitude; the argmax cell of |dp| bit-exact."""
+import os
+
+import numpy as np
+import pytest
+
+from parity import interior, pair, rel_err
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-11
+
+
+def _run(pkg, orc, case, check_rhs=True):
+    orc.set_threads(min(16, os.cpu_count() or 1))
+    print(f"case {case.name}: {case.ncell} cells, building both sides", flush=True)
+    s, o = pair(pkg, orc, case)
+    npv = case.npv
+    if check_rhs:
+        s.setbc()
+        o.setbc()
+        rg = s.calrhs(0)[0].reshape(-1, npv)
+        ro = o.calrhs(0)[0].reshape(-1, npv)
+        e = rel_err(rg, ro, floor=1e-6)
+        print(f"  residual rel err per equation {np.array2string(e, precision=2)}", flush=True)
+        assert np.all(e < TOL), e
+    print("  one iteration on both sides", flush=True)
+    o.setqres()
+    to = o.timeinteg(1, 1, 0.0)
+    tg = s.timeinteg(1, 1, 0.0)
+    assert abs(tg - to) <= TOL * abs(to)
+    pg, dg, tvg, _ = s.get_state(0)
+    po, do, tvo, _ = o.get_state(0)
+    for what, g, r, m in (("pv", pg, po, npv), ("dv", dg, do, 18), ("tv", tvg, tvo, case.ntv)):
+        e = rel_err(interior(case, 0, g, m), interior(case, 0, r, m))
+        print(f"  {what} rel err max {e.max():.2e}", flush=True)
+        assert np.all(e < TOL), (what, e)
+    l2g, ijkg, rmg = s.residual()
+    l2o, ijko, rmo = o.residual()
+    assert np.all(np.isfinite(l2g))
+    assert np.all(rel_err(l2g, l2o) < TOL), (l2g, l2o)
+    assert list(ijkg) == list(ijko)
+    assert abs(rmg - rmo) <= TOL * abs(rmo)
+    s.close()
+
+
+def test_config1_64x32x32_one_iteration(pkg, orc):
+    _run(pkg, orc, pkg.cases.config1())
+
+
+@pytest.mark.parametrize("clip", [1, 0])
+def test_config2_128x64x64_one_iteration(pkg, orc, clip):
+    case = pkg.cases.config2()
+    case.params["cav_clip"] = clip
+    _run(pkg, orc, case, check_rhs=(clip == 1))
