@@ -1,0 +1,667 @@
+"""Closed-form pins of the oracle's composite operators: each expected value
+comes from the model's published definition or from mathematics, evaluated
+independently in numpy, never from the oracle's own expressions.
+
+  viscous flux     a linear velocity / temperature / k / omega field on a
+                   Cartesian stencil: the face gradients are exact, so the
+                   flux is tau.n, u_f.tau.n + kappa grad T.n, and the SST
+                   k / omega diffusion at the F1 = 1 and F1 = 0 limits
+                   (Menter 1994 sigma_k = 0.85 / 1.0, sigma_w = 0.5 / 0.856)
+                   -- src/vsflux_module.f90:275-425
+  Merkle           rates at T = 300 K (IF97 table 35 p_sat) on both sides of
+                   saturation, clipped and unclipped, and the coupling icav
+                   = d(r_v - r_c)/d(p, T, y1, y2) at frozen p_sat with the
+                   reference's sign flip -- src/cav_module.f90:136-171
+  SST source       uniform shear: P = mu_t S^2, the 10 beta* rho k omega
+                   production limit, destruction beta* rho k omega,
+                   omega_cut = S, the omega source at F1 = 1 and 0, and the
+                   2x2 source Jacobian with its eigen-sign correction
+                   -- src/turbsource_module.f90:269-414
+  eddy viscosity   mu_t = a1 rho k / max(a1 omega, S F2) at F2 = 0 and 1
+                   -- src/eddy_module.f90:135-187
+  BDF1 / BDF2      the residual of a uniform state: -(Q - Q^n) V/dt and
+                   -(3Q - 4Q^n + Q^(n-1)) V/(2 dt)
+                   -- src/unsteady_module.f90:96-114
+  split Jacobian   A+ + A- = d(F.n)/dq (central differences of the
+                   physical flux through the IF97 EOS) and A+ - A- =
+                   lambda Gamma_p + nu dQ/dq, Gamma_p the preconditioned
+                   dQ/dq, lambda the Weiss-Smith spectral radius
+                   -- src/jacobian_module.f90:277-403
+  coupled getx     D' x = a r with D' = Gamma + b dQ/dq + V a (icav in the
+                   y1 equation, itt in the k-omega block) assembled from
+                   dv -- src/lhs_module.f90:346-457
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+D = C.POINTER(C.c_double)
+PP = C.POINTER(D)
+
+
+def dp(a):
+    return np.ascontiguousarray(a, dtype=np.float64).ctypes.data_as(D)
+
+
+def ptrs(arrs):
+    keep = [np.ascontiguousarray(a, dtype=np.float64) for a in arrs]
+    out = (D * len(keep))(*[a.ctypes.data_as(D) for a in keep])
+    out._keep = keep
+    return out
+
+
+@pytest.fixture(scope="module")
+def OL(orc):
+    L = orc.lib()
+    L.orc_vsflux.argtypes = [C.c_void_p, D, PP, PP, D, D, D, D, D, D, D, D]
+    L.orc_merkle.argtypes = [C.c_void_p, D, D, D, D, D, D]
+    L.orc_kwsst_source.argtypes = [PP, D, D, D, D, D, D, D]
+    L.orc_kwsst_eddy.argtypes = [PP, D, D, D, D]
+    L.orc_kwsst_eddy.restype = C.c_double
+    L.orc_caljac.argtypes = [C.c_void_p, D, D, D, D, D, C.c_int, D]
+    L.orc_getx.argtypes = [C.c_void_p, PP, D, D, D, D, C.c_double, D, D, D, D]
+    return L
+
+
+def make_cfg(orc, pkg, **over):
+    case = pkg.cases.config2(6, 5, 4)
+    case.params.update(over)
+    cfg = case.cfg_dict()
+    cfg.update({k: v for k, v in over.items() if k in cfg or k in ("npv", "ntv", "ngrd")})
+    cf = orc.Cfg()
+    for k, v in cfg.items():
+        if k == "bc_pv":
+            cf.bc_pv = (C.c_double * 9)(*v)
+        else:
+            setattr(cf, k, v)
+    return cfg, cf
+
+
+def close(a, b, rtol, scale=None):
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    s = np.max(np.abs(b)) if scale is None else scale
+    return np.all(np.abs(a - b) <= rtol * s), np.max(np.abs(a - b)) / max(s, 1e-300)
+
+
+# ------------------------------------------------------------------ geometry
+H3 = np.array([2.0e-3, 1.5e-3, 1.0e-3])          # Cartesian cell sizes dx, dy, dz
+VOL = float(np.prod(H3))
+
+
+def face_stencil(field):
+    """x(38, 9) of the i-face between cells (0,0,0) and (1,0,0): ll = 1..18
+    are (dn in {0,1} fastest, de, dt in -1..1), 19..27 (-1, de, dt),
+    28..36 (2, de, dt), 37 (-2,0,0), 38 (3,0,0)  (src/rhs_module.f90:288-312)."""
+    x = np.zeros((38, 9))
+    q = 0
+    for dt in (-1, 0, 1):
+        for de in (-1, 0, 1):
+            for dn in (0, 1):
+                x[q] = field(np.array([dn, de, dt]) * H3)
+                q += 1
+    for dn in (-1, 2):
+        for dt in (-1, 0, 1):
+            for de in (-1, 0, 1):
+                x[q] = field(np.array([dn, de, dt]) * H3)
+                q += 1
+    x[36] = field(np.array([-2, 0, 0]) * H3)
+    x[37] = field(np.array([3, 0, 0]) * H3)
+    return x
+
+
+def cell_stencil(field):
+    """x(7, 9): cell, then i-1, i+1, j-1, j+1, k-1, k+1 in the reference order
+    (i-1, cell, i+1, j-1, j+1, k-1, k+1)."""
+    offs = [(-1, 0, 0), (0, 0, 0), (1, 0, 0), (0, -1, 0), (0, 1, 0), (0, 0, -1), (0, 0, 1)]
+    return np.array([field(np.array(o) * H3) for o in offs])
+
+
+def cart_faces():
+    """the cell's six face vectors cx(i-1), cx(i), ex(j-1), ex(j), tx(k-1), tx(k)"""
+    ax = np.array([H3[1] * H3[2], 0, 0]), np.array([0, H3[0] * H3[2], 0]), np.array([0, 0, H3[0] * H3[1]])
+    return [ax[0], ax[0], ax[1], ax[1], ax[2], ax[2]]
+
+
+def linear_field(rng, base):
+    G = rng.standard_normal((9, 3)) * np.array([0, 400, 400, 400, 300, 0, 0, 20, 4e4])[:, None]
+
+    def f(x):
+        return base + G @ x
+    return f, G
+
+
+# ------------------------------------------------------------------ viscous flux
+@pytest.mark.parametrize("turb,f1", [(False, None), (True, 1.0), (True, 0.0)])
+def test_viscous_flux_of_linear_fields(orc, pkg, OL, turb, f1):
+    cfg, cf = make_cfg(orc, pkg, iturb=0 if turb else -2, npv=9 if turb else 7, ntv=3 if turb else 2)
+    rng = np.random.default_rng(31)
+    for trial in range(20):
+        base = np.array([0.0, 10.0, 0.5, -0.3, 300.0, 0.0, 0.0, 0.5, 3.0e3])
+        field, G = linear_field(rng, base)
+        x = face_stencil(field)
+        n3 = np.array([H3[1] * H3[2], 0.0, 0.0])
+        e4 = [np.array([0.0, H3[0] * H3[2], 0.0])] * 4
+        t4 = [np.array([0.0, 0.0, H3[0] * H3[1]])] * 4
+        mu, kap, mut, cp = 8.5e-4 * (1 + 0.1 * trial), 0.61, 3.0e-3 * turb, 4180.0
+        rho = 996.0
+        ydns = 1.0e-9 if f1 == 1.0 else 1.0e6
+        grd = np.array([VOL, 0, 0, 0, ydns])
+        dv = np.zeros(18)
+        dv[0], dv[11] = rho, cp
+        tv = np.array([mu, kap, mut])
+        fx = np.zeros(9)
+        OL.orc_vsflux(C.byref(cf), dp(n3), ptrs(e4), ptrs(t4), dp(grd), dp(grd), dp(x), dp(dv), dp(dv), dp(tv),
+                      dp(tv), dp(fx))
+        Gu = G[1:4]                                   # du_i/dx_j
+        gT, gk, go = G[4], G[7], G[8]
+        mue = mu + mut
+        kape = kap + mut * cp / 0.9                   # turbulent Prandtl number 0.9
+        tau = mue * (Gu + Gu.T - 2.0 / 3.0 * np.trace(Gu) * np.eye(3))
+        uf = field(np.array([0.5 * H3[0], 0, 0]))[1:4]
+        exp = np.zeros(9)
+        exp[1:4] = tau @ n3
+        exp[4] = uf @ (tau @ n3) + kape * (gT @ n3)
+        if turb:
+            sk, so = (0.85, 0.5) if f1 == 1.0 else (1.0, 0.856)
+            exp[7] = (mu + sk * mut) * (gk @ n3)
+            exp[8] = (mu + so * mut) * (go @ n3)
+        npv = cfg["npv"]
+        ok, err = close(fx[:npv], exp[:npv], 1e-11, scale=np.max(np.abs(exp[1:npv])))
+        assert ok, (trial, err, fx[:npv], exp[:npv])
+
+
+# ------------------------------------------------------------------ Merkle
+PSAT_300 = 3.53658941e3          # IAPWS-IF97 table 35, p_s(300 K)
+
+
+def _merkle(OL, cf, p_abs, y1, dvv, pref):
+    x = np.zeros((7, 9))
+    x[1, 0] = p_abs - pref
+    x[1, 4] = 300.0
+    x[1, 5] = y1
+    rc, rv, ic = C.c_double(), C.c_double(), np.zeros(4)
+    OL.orc_merkle(C.byref(cf), dp(x), dp(dvv), dp(np.array([VOL, 0, 0, 0, 0.0])), C.byref(rc), C.byref(rv), dp(ic))
+    return rc.value, rv.value, ic
+
+
+@pytest.mark.parametrize("clip", [1, 0])
+def test_merkle_rates_and_coupling(orc, pkg, OL, clip):
+    rng = np.random.default_rng(5 + clip)
+    for trial in range(40):
+        cc = 10.0 ** rng.uniform(-6, 2)               # unclipped and clipped rates
+        cvv = 10.0 ** rng.uniform(-6, 2)
+        cfg, cf = make_cfg(orc, pkg, c_c=cc, c_v=cvv, cav_clip=clip)
+        pref, dpr, tr = cfg["pref"], cfg["dp_ref"], cfg["t_ref"]
+        side = trial % 2                              # 0 condensing (p > p_sat), 1 vaporising
+        p_abs = PSAT_300 * (rng.uniform(1.5, 20.0) if side == 0 else rng.uniform(0.05, 0.9))
+        y1 = rng.uniform(1e-4, 0.5)
+        dvv = np.zeros(18)
+        dvv[0] = rng.uniform(100.0, 1000.0)
+        dvv[6:10] = rng.standard_normal(4) * np.array([1e-6, 0.3, 1e2, 1e2])
+        rho = dvv[0]
+        # Merkle et al. (1998): condensation c_c rho y_v max(p - p_v, 0), evaporation
+        # c_v rho y_l max(p_v - p, 0), both scaled by dp_ref t_ref
+        rate_c = cc * rho * y1 * max(p_abs - PSAT_300, 0.0) * dpr * tr
+        rate_v = cvv * rho * (1.0 - y1) * max(PSAT_300 - p_abs, 0.0) * dpr * tr
+        cap_c = rho * (y1 if clip else 300.0)
+        cap_v = rho * ((1.0 - y1) if clip else (1.0 - 300.0 - y1))
+        rc, rv, ic = _merkle(OL, cf, p_abs, y1, dvv, pref)
+        assert abs(rc - min(rate_c, cap_c) * VOL) <= 1e-8 * max(abs(rc), 1e-300), (trial, rc, rate_c, cap_c)
+        assert abs(rv - min(rate_v, cap_v) * VOL) <= 1e-8 * max(abs(rv), abs(cap_v) * VOL, 1e-300), (trial, rv)
+        clipped = (rate_c > cap_c) if side == 0 else (rate_v > cap_v)
+        if clipped or clip == 0:
+            # a clipped rate, and the as-written clip (a negative cap on the
+            # evaporation rate is always active), keep the unclipped rate's
+            # linearisation, which is not the gradient of the clipped rate
+            continue
+        # icav: gradient of (r_v - r_c) per unit volume over (p, T, y1, y2) with
+        # p_sat frozen and rho linear in (p, T, y1, y2) with slopes dv(7:10)
+        def net(q):
+            r = rho + dvv[6] * q[0] + dvv[7] * q[1] + dvv[8] * q[2] + dvv[9] * q[3]
+            pa, yv, yg = p_abs + q[0], y1 + q[2], q[3]
+            c = cc * r * yv * max(pa - PSAT_300, 0.0) * dpr * tr
+            v = cvv * r * (1.0 - yv - yg) * max(PSAT_300 - pa, 0.0) * dpr * tr
+            return v - c
+        g = np.zeros(4)
+        for k, h in enumerate([1e-3 * abs(p_abs - PSAT_300), 1e-3, 1e-6, 1e-6]):
+            e = np.zeros(4)
+            e[k] = h
+            g[k] = (net(e) - net(-e)) / (2 * h)
+        s = 1.0 if g[2] >= 0.0 else -1.0             # coupling flipped to a non-negative y1 entry
+        ok, err = close(ic, s * g, 1e-7, scale=np.max(np.abs(g)))
+        assert ok, (trial, side, err, ic, s * g)
+
+
+# ------------------------------------------------------------------ SST source, eddy viscosity
+def _shear_state(rng, ydns, div_free=True):
+    k, om = rng.uniform(1e-3, 5e-2), rng.uniform(50.0, 5e3)
+    G = np.zeros((9, 3))
+    G[1, 1] = rng.uniform(-2e3, 2e3)                 # du/dy: a uniform shear
+    if not div_free:
+        G[1, 0], G[2, 1], G[3, 2] = rng.standard_normal(3) * 200.0
+    base = np.array([0.0, 10.0, 0.0, 0.0, 300.0, 0.0, 0.0, k, om])
+    x7 = cell_stencil(lambda x: base + G @ x)
+    return k, om, G, x7, np.array([VOL, 0, 0, 0, ydns])
+
+
+BETA_STAR = 0.09
+
+
+@pytest.mark.parametrize("f1", [1.0, 0.0])
+def test_sst_source_uniform_shear(orc, pkg, OL, f1):
+    rng = np.random.default_rng(17 + int(f1))
+    f6 = cart_faces()
+    limited = unlimited = 0
+    for trial in range(60):
+        k, om, G, x7, grd = _shear_state(rng, 1.0e-9 if f1 == 1.0 else 1.0e6)
+        rho, mu = 996.0, 8.5e-4
+        mut = rho * k / om * (1.0 if trial % 2 == 0 else 10.0 ** rng.uniform(-1, 1.5))
+        dvv = np.zeros(18)
+        dvv[0] = rho
+        tv = np.array([mu, 0.6, mut])
+        src, itt, oc = np.zeros(2), np.zeros(4), C.c_double()
+        OL.orc_kwsst_source(ptrs(f6), dp(grd), dp(x7), dp(dvv), dp(tv), dp(src), dp(itt), C.byref(oc))
+        S = abs(G[1, 1])                              # sqrt(2 S_ij S_ij) of a simple shear
+        P = mut * S * S
+        Plim = min(P, 10.0 * BETA_STAR * rho * k * om)   # Menter (2003) production limiter
+        limited += P > Plim
+        unlimited += P <= Plim
+        dest = BETA_STAR * rho * k * om
+        gam, bet = (5.0 / 9.0, 0.075) if f1 == 1.0 else (0.44, 0.0828)
+        exp = np.array([(Plim - dest) * VOL, (gam * rho / mut * Plim - bet * rho * om * om) * VOL])
+        assert np.all(np.abs(src - exp) <= 1e-11 * np.abs(exp).max() + 1e-13 * dest * VOL), (trial, src, exp)
+        assert abs(oc.value - S) <= 1e-12 * S
+        if P <= Plim and trial % 2 == 0:
+            # with mu_t = rho k / omega (the SST relation the source Jacobian
+            # assumes): S_k = (rho k/om) S^2 - beta* rho k om and
+            # S_w = gamma rho S^2 - beta rho om^2 differentiated in (k, omega),
+            # each row multiplied by the sign of its eigenvalue (larger first)
+            J = np.array([[rho * S * S / om - BETA_STAR * rho * om, -rho * k * S * S / om ** 2 - BETA_STAR * rho * k],
+                          [0.0, -2.0 * bet * rho * om]])
+            lam = np.sort(np.linalg.eigvals(J).real)[::-1]
+            J[0] *= np.sign(lam[0])
+            J[1] *= np.sign(lam[1])
+            assert np.all(np.abs(itt - J.ravel()) <= 1e-12 * np.abs(J).max()), (trial, itt, J)
+    assert limited > 5 and unlimited > 5
+
+
+@pytest.mark.parametrize("f2", [0.0, 1.0])
+def test_sst_eddy_viscosity_limits(orc, pkg, OL, f2):
+    rng = np.random.default_rng(23 + int(f2))
+    f6 = cart_faces()
+    branch = [0, 0]
+    for trial in range(60):
+        k, om, G, x7, grd = _shear_state(rng, 1.0e6 if f2 == 0.0 else 1.0e-9, div_free=(trial % 2 == 0))
+        rho, mu = 996.0, 8.5e-4
+        dvv = np.zeros(18)
+        dvv[0] = rho
+        mt = OL.orc_kwsst_eddy(ptrs(f6), dp(grd), dp(x7), dp(dvv), dp(np.array([mu, 0.6, 0.0])))
+        Gu = G[1:4]
+        Sij = 0.5 * (Gu + Gu.T)
+        S = np.sqrt(2.0 * np.sum(Sij * Sij))
+        a1 = 0.31                                      # Menter's a1
+        exp = a1 * rho * k / max(a1 * om, S * f2)
+        branch[int(S * f2 > a1 * om)] += 1
+        assert abs(mt - exp) <= 1e-12 * exp, (trial, mt, exp)
+    if f2 == 1.0:
+        assert min(branch) > 5
+
+
+# ------------------------------------------------------------------ BDF
+@pytest.mark.parametrize("nsteady", [1, 2])
+def test_bdf_source_weights(orc, pkg, nsteady):
+    """The residual of a uniform state on a straight channel with far-field
+    sides is the dual-time source alone (equal face fluxes cancel exactly)."""
+    cs = pkg.cases
+    b = cs.bump_channel(6, 5, 4, bump=0.0)
+    I, J, K = b.imax, b.jmax, b.kmax
+    b.bc_desc = [cs.face_bc("farfield", f, I, J, K) for f in ("imin", "imax", "jmin", "jmax", "kmin", "kmax")]
+    dt_phy = 2.5e-4
+    p = cs.make_params(-2, 1, 1, 1, 2, 0, nsteady, 5.0, 101325.0, 10.0, dt_phy=dt_phy)
+    cs.finish_block(b, 4)
+    case = cs.Case("bdf", p, [b], 7, 2, 4)
+    pv = np.zeros((K + 5, J + 5, I + 5, 7))
+    pv[..., 1] = 10.0
+    pv[..., 4] = 300.0
+    dv, tv = cs.eos_host(101325.0, 300.0, 0.0, 0.0)
+    rho, h = dv[0, 0], dv[0, 1]
+    Q = np.array([rho, rho * 10.0, 0.0, 0.0, rho * (h + 50.0) - 101325.0, 0.0, 0.0])
+    nc = (I - 1) * (J - 1) * (K - 1)
+    rng = np.random.default_rng(nsteady)
+    q1 = Q * (1.0 + 1e-3 * rng.standard_normal((nc, 7)))
+    q2 = Q * (1.0 + 1e-3 * rng.standard_normal((nc, 7)))
+    qq = np.ascontiguousarray(np.stack([q1, q2], axis=1)).ravel()
+    o = orc.Oracle(case)
+    o.put_state(0, pv.ravel(), np.broadcast_to(dv[0], (K + 5, J + 5, I + 5, 18)).copy().ravel(),
+                np.broadcast_to(tv[0], (K + 5, J + 5, I + 5, 2)).copy().ravel(), qq)
+    o.setbc()
+    res = o.calrhs(0)[0].reshape(-1, 7)
+    vol = b.grd.reshape(K + 1, J + 1, I + 1, 4)[1:K, 1:J, 1:I, 0].reshape(-1)
+    if nsteady == 1:
+        exp = -(Q[None] - q1) * (vol / dt_phy)[:, None]
+    else:
+        exp = -(3.0 * Q[None] - 4.0 * q1 + q2) * (vol / (2.0 * dt_phy))[:, None]
+    scale = np.abs(exp).max(axis=0)
+    scale[scale == 0] = 1.0
+    assert np.all(np.abs(res - exp) <= 1e-9 * scale), np.max(np.abs(res - exp) / scale, axis=0)
+
+
+# ------------------------------------------------------------------ Jacobians and the diagonal inverse
+def _state(rng, npv, pref):
+    q = np.zeros(9)
+    q[0] = rng.uniform(-2e4, 2e4)
+    q[1:4] = rng.uniform(-12, 12, 3)
+    q[4] = rng.uniform(285, 330)
+    q[5] = rng.uniform(1e-4, 0.05)
+    q[6] = rng.uniform(0.0, 1e-3)
+    q[7:9] = rng.uniform(1e-3, 0.1), rng.uniform(10, 1e3)
+    if npv == 7:
+        q[7:] = 0.0
+    return q
+
+
+def _conserved(orc, q, pref, npv):
+    d = orc.deteos_simple(q[0] + pref, q[4], q[5], q[6])
+    rho, h = d[0], d[1]
+    H = h + 0.5 * q[1:4] @ q[1:4]
+    Q = np.zeros(npv)
+    Q[0] = rho
+    Q[1:4] = rho * q[1:4]
+    Q[4] = rho * H - (q[0] + pref)
+    Q[5:npv] = rho * q[5:npv]
+    return Q, d
+
+
+def _flux_n(orc, q, pref, npv, n3):
+    d = orc.deteos_simple(q[0] + pref, q[4], q[5], q[6])
+    rho, h = d[0], d[1]
+    U = q[1:4] @ n3
+    H = h + 0.5 * q[1:4] @ q[1:4]
+    F = np.zeros(npv)
+    F[0] = rho * U
+    F[1:4] = rho * q[1:4] * U + (q[0] + pref) * n3
+    F[4] = rho * H * U
+    F[5:npv] = rho * q[5:npv] * U
+    return F
+
+
+def _num_jac(fun, q, npv):
+    steps = np.array([1.0, 1e-3, 1e-3, 1e-3, 1e-4, 1e-7, 1e-7, 1e-7, 1e-4])
+    J = np.zeros((npv, npv))
+    for l in range(npv):
+        e = np.zeros(9)
+        e[l] = steps[l] * max(1.0, abs(q[l]))
+        J[:, l] = (fun(q + e) - fun(q - e)) / (2 * e[l])
+    return J
+
+
+def _gamma(dv, q, pref, npv, beta):
+    """d Q / d q of Q = (rho, rho u, rho v, rho w, rho H - p, rho y1, rho y2, rho k,
+    rho omega) in closed form from the EOS derivatives, with d rho / dp
+    replaced by beta (beta = rho_p: the plain Jacobian)."""
+    rho, h = dv[0], dv[1]
+    rp, rT, ry1, ry2, hp, hT, hy1, hy2 = dv[6:14]
+    H = h + 0.5 * q[1:4] @ q[1:4]
+    r1 = np.zeros(npv)
+    r1[[0, 4, 5, 6]] = beta, rT, ry1, ry2
+    G = np.zeros((npv, npv))
+    G[0] = r1
+    for i in range(3):
+        G[1 + i] = q[1 + i] * r1
+        G[1 + i, 1 + i] += rho
+    G[4] = H * r1
+    G[4, [0, 1, 2, 3, 4, 5, 6]] += rho * np.array([hp, q[1], q[2], q[3], hT, hy1, hy2])
+    G[4, 0] -= 1.0
+    for n in range(5, npv):
+        G[n] = q[n] * r1
+        G[n, n] += rho
+    return G
+
+
+def _sndp2(cfg, c2, u2):
+    return min(c2, max(u2, cfg["uref"] ** 2))
+
+
+def _lam(cfg, dv, q, S):
+    """1.05 (|u'| + c''): the spectral radius of the Weiss-Smith preconditioned
+    system through area vector S, with c'^2 = min(c^2, max(|u|^2, U_ref^2))."""
+    c2 = dv[5]
+    cp2 = _sndp2(cfg, c2, q[1:4] @ q[1:4])
+    U = q[1:4] @ S
+    up = 0.5 * (1.0 + cp2 / c2) * U
+    d = 0.5 * np.sqrt(U * U * (1.0 - cp2 / c2) ** 2 + 4.0 * cp2 * (S @ S))
+    return 1.05 * (abs(up) + d)
+
+
+@pytest.mark.parametrize("iturb", [-3, 0])
+def test_split_jacobian_structure(orc, pkg, OL, iturb):
+    npv = 9 if iturb == 0 else 7
+    cfg, cf = make_cfg(orc, pkg, iturb=iturb, npv=npv, ntv=3 if iturb == 0 else 0, prec=1)
+    pref = cfg["pref"]
+    rng = np.random.default_rng(41 + iturb)
+    for trial in range(15):
+        q = _state(rng, npv, pref)
+        dv = orc.deteos_simple(q[0] + pref, q[4], q[5], q[6])
+        n3 = rng.standard_normal(3) * 1e-4
+        tv = np.array([8.5e-4, 0.6, 2e-3])
+        grd = np.array([VOL, 0, 0, 0, 1e-3])
+        A = {}
+        for sg in (1, -1):
+            a = np.zeros(81)
+            OL.orc_caljac(C.byref(cf), dp(n3), dp(grd), dp(q), dp(dv), dp(tv), sg, dp(a))
+            A[sg] = a[:npv * npv].reshape(npv, npv).T
+        Jf = _num_jac(lambda z: _flux_n(orc, z, pref, npv, n3), q, npv)
+        ok, err = close(A[1] + A[-1], Jf, 1e-6, scale=None)
+        rows = np.abs(Jf).max(axis=1, keepdims=True) + 1e-300
+        assert np.all(np.abs(A[1] + A[-1] - Jf) <= 1e-6 * rows), (trial, np.max(np.abs(A[1] + A[-1] - Jf) / rows))
+        c2 = dv[5]
+        beta = 1.0 / _sndp2(cfg, c2, q[1:4] @ q[1:4]) - 1.0 / c2 + dv[6]
+        Gp = _gamma(dv, q, pref, npv, beta)          # preconditioned: the inviscid dissipation
+        G0 = _gamma(dv, q, pref, npv, dv[6])         # plain: the viscous share
+        Dm = A[1] - A[-1]
+        M = np.stack([Gp.ravel(), G0.ravel()], axis=1)
+        (lam, vis), *_ = np.linalg.lstsq(M, Dm.ravel(), rcond=None)
+        fit = lam * Gp + vis * G0
+        assert np.all(np.abs(Dm - fit) <= 1e-11 * np.abs(fit).max(axis=1, keepdims=True) + 1e-300), trial
+        assert abs(lam - _lam(cfg, dv, q, n3)) <= 1e-11 * lam, (lam, _lam(cfg, dv, q, n3))
+        if iturb == -3:
+            assert vis == pytest.approx(0.0, abs=1e-12 * lam)
+        else:
+            mu, kap, mut = tv
+            ev = max(dv[6] * (kap + dv[11] * mut / 0.9) / (dv[7] + dv[11] * dv[6] * dv[0] - dv[10] * dv[7] * dv[0]),
+                     4.0 / 3.0 * (mu + mut) / dv[0])
+            assert abs(vis - 2.0 * ev * (n3 @ n3) / VOL) <= 1e-10 * abs(vis)
+
+
+def test_coupled_diagonal_inverse(orc, pkg, OL):
+    """getx with cavitation and k-omega couplings: D' x = a r for random r,
+    D' = Gamma + b dQ/dq + V a C, C holding icav in the y1 equation over
+    (p, T, y1, y2) and itt in the k-omega block; a = 1/(V/dt + sum_d lambda_d)
+    over the cell's mean face vectors, b = (nsteady b_bdf V/dt_phy + 2 nu
+    sum_d |S_d|^2 / V) a."""
+    rng = np.random.default_rng(77)
+    for trial in range(30):
+        nsteady = trial % 3
+        cfg, cf = make_cfg(orc, pkg, iturb=0, npv=9, ntv=3, prec=1, nsteady=nsteady, dt_phy=3e-4)
+        pref, npv = cfg["pref"], 9
+        q = _state(rng, npv, pref)
+        dv = orc.deteos_simple(q[0] + pref, q[4], q[5], q[6])
+        f6 = [rng.standard_normal(3) * 1e-4 + np.array([1e-4, 0, 0]) * (1 - (m % 2) * 0.2) for m in range(6)]
+        vol = rng.uniform(1e-9, 1e-7)
+        grd = np.array([vol, 0, 0, 0, 1e-3])
+        tv = np.array([8.5e-4, 0.6, 2e-3])
+        dt = rng.uniform(1e-6, 1e-4)
+        icav = rng.standard_normal(4) * np.array([1e-3, 1e1, 1e4, 1e4])
+        icav[2] = abs(icav[2])
+        itt = rng.standard_normal(4) * 1e2
+        r = rng.standard_normal(9) * np.array([1e-3, 1e-2, 1e-2, 1e-2, 1e2, 1e-5, 1e-6, 1e-5, 1e-2])
+        x = np.zeros(9)
+        OL.orc_getx(C.byref(cf), ptrs(f6), dp(grd), dp(q), dp(dv), dp(tv), dt, dp(icav), dp(itt), dp(r), dp(x))
+        S = [0.5 * (f6[2 * d] + f6[2 * d + 1]) for d in range(3)]
+        a = 1.0 / (vol / dt + sum(_lam(cfg, dv, q, s) for s in S))
+        # viscous spectral radius of the SST path (src/lhs_module.f90:459-534,
+        # src/jacobian_module.f90:425-435)
+        mu, kap, mut = tv
+        ev = max(dv[6] * (kap + dv[11] * mut / 0.9) / (dv[7] + dv[11] * dv[6] * dv[0] - dv[10] * dv[7] * dv[0]),
+                 4.0 / 3.0 * (mu + mut) / dv[0])
+        bdf = {0: 0.0, 1: 1.0, 2: 1.5}[nsteady]
+        b = (bdf * vol / cfg["dt_phy"] + 2.0 * ev * sum(s @ s for s in S) / vol) * a
+        c2 = dv[5]
+        beta = 1.0 / _sndp2(cfg, c2, q[1:4] @ q[1:4]) - 1.0 / c2 + dv[6]
+        Dm = _gamma(dv, q, pref, npv, beta) + b * _gamma(dv, q, pref, npv, dv[6])
+        Dm[5, [0, 4, 5, 6]] += icav * vol * a
+        Dm[7:9, 7:9] += itt.reshape(2, 2) * vol * a
+        lhs = Dm @ x
+        rel = np.abs(lhs - a * r) / (np.abs(Dm) @ np.abs(x) + 1e-300)
+        assert np.all(rel < 1e-10), (trial, nsteady, rel)
+
+
+# ------------------------------------------------------------------ ghost-cell conventions and update clips
+def _bc_case(pkg):
+    """SST block with a subsonic inlet (imin), subsonic outlet (imax),
+    viscous k-omega wall (jmin), symmetry plane (jmax) and far field
+    (kmin, kmax); a random interior state around the free stream."""
+    cs = pkg.cases
+    case = cs.config2(8, 6, 5)
+    b = case.blocks[0]
+    I, J, K = b.imax, b.jmax, b.kmax
+    b.bc_desc = [cs.face_bc("inflow_subsonic", "imin", I, J, K), cs.face_bc("outflow_subsonic", "imax", I, J, K),
+                 cs.face_bc("wall", "jmin", I, J, K), cs.face_bc("symmetry", "jmax", I, J, K),
+                 cs.face_bc("farfield", "kmin", I, J, K), cs.face_bc("farfield", "kmax", I, J, K)]
+    cs.finish_block(b, 5)
+    case.params["cav_clip"] = 1
+    pv, dv, tv, qq = case.init[0]
+    P = pv.reshape(K + 5, J + 5, I + 5, 9).copy()
+    rng = np.random.default_rng(3)
+    inner = (slice(3, K + 2), slice(3, J + 2), slice(3, I + 2))
+    P[inner + (1,)] += rng.uniform(-2.0, 2.0, P[inner + (1,)].shape)
+    P[inner + (2,)] += rng.uniform(-2.0, 2.0, P[inner + (2,)].shape)
+    P[inner + (3,)] += rng.uniform(-2.0, 2.0, P[inner + (3,)].shape)
+    P[inner + (0,)] += rng.uniform(-3e3, 3e3, P[inner + (0,)].shape)
+    cell = P[inner].reshape(-1, 9)
+    d, t = cs.eos_host(cell[:, 0] + case.params["pref"], cell[:, 4], cell[:, 5], cell[:, 6])
+    Dv = dv.reshape(K + 5, J + 5, I + 5, 18).copy()
+    Dv[inner] = d.reshape(K - 1, J - 1, I - 1, 18)
+    Tv = tv.reshape(K + 5, J + 5, I + 5, 3).copy()
+    Tv[inner + (slice(0, 2),)] = t.reshape(K - 1, J - 1, I - 1, 2)
+    case.init[0] = (P.ravel(), Dv.ravel(), Tv.ravel(), qq)
+    return case
+
+
+def test_ghost_conventions(orc, pkg):
+    """After the boundary fill, the face value (mean of a ghost cell and its
+    mirror) obeys the boundary's physics: no-slip with k = 0 and Menter's
+    omega_w = 10 * 6 nu / (beta_1 d^2) at the viscous wall (beta_1 = 0.075, d
+    the wall distance of the first cell); zero normal and unchanged
+    tangential velocity on the symmetry plane; the back pressure and the
+    outgoing acoustic characteristic p + rho a u_n at the subsonic outlet;
+    both acoustic characteristics (the incoming one from the free stream)
+    and the free stream's T, tangential velocity, k and omega at the
+    subsonic inlet.  a is the preconditioned sound speed
+    sqrt(min(c^2, max(u_n^2, U_ref^2)))."""
+    case = _bc_case(pkg)
+    p = case.params
+    b = case.blocks[0]
+    I, J, K = b.imax, b.jmax, b.kmax
+    o = orc.Oracle(case)
+    o.put_state(0, *case.init[0])
+    o.setbc()
+    pv, dv, tv, _ = o.get_state(0)
+    P = pv.reshape(K + 5, J + 5, I + 5, 9)
+    Dv = dv.reshape(K + 5, J + 5, I + 5, 18)
+    Tv = tv.reshape(K + 5, J + 5, I + 5, 3)
+    grd = b.grd.reshape(K + 1, J + 1, I + 1, 5)
+    cx = b.cx.reshape(K + 1, J + 1, I, 3)
+    ex = b.ex.reshape(K + 1, J, I + 1, 3)
+    at = lambda A, i, j, k: A[k + 1, j + 1, i + 1]       # ghosted (Fortran indices)
+    inf = np.array(p["bc_pv"])
+    uref2 = p["uref"] ** 2
+    for k in range(2, K + 1):
+        for i in range(2, I + 1):
+            for depth in (1, 2, 3):
+                # viscous k-omega wall at jmin: ghost j = 2 - depth mirrors j = 1 + depth
+                g, m = at(P, i, 2 - depth, k), at(P, i, 1 + depth, k)
+                f = 0.5 * (g + m)
+                assert np.all(np.abs(f[1:4]) <= 1e-14 * np.abs(m[1:4]).max())        # no slip
+                assert f[7] == 0.0 or abs(f[7]) <= 1e-15 * abs(m[7])                  # k = 0
+                nu = at(Tv, i, 2, k)[0] / at(Dv, i, 2, k)[0]
+                d = grd[k - 1, 1, i - 1, 4]
+                assert abs(f[8] - 10.0 * 6.0 * nu / (0.075 * d * d)) <= 1e-12 * f[8]
+                assert g[0] == m[0] and g[4] == m[4]                                # zero normal gradient
+                # symmetry at jmax: ghost j = J + depth mirrors j = J + 1 - depth
+                g, m = at(P, i, J + depth, k), at(P, i, J + 1 - depth, k)
+                s = at(P, i, J, k)
+                n = ex[k - 1, J - 1, i - 1]
+                nh = n / np.linalg.norm(n)
+                v = 0.5 * (g[1:4] + m[1:4])
+                vs_t = s[1:4] - (s[1:4] @ nh) * nh
+                if depth == 1:
+                    assert abs(v @ nh) <= 1e-12 * np.linalg.norm(s[1:4])
+                    assert np.all(np.abs(v - vs_t) <= 1e-12 * np.linalg.norm(s[1:4]))
+    for k in range(2, K + 1):
+        for j in range(2, J + 1):
+            for depth in (1, 2, 3):
+                # subsonic outlet at imax: ghost i = I + depth, mirror I + 1 - depth, surface I
+                g, m, s = at(P, I + depth, j, k), at(P, I + 1 - depth, j, k), at(P, I, j, k)
+                ds = at(Dv, I, j, k)
+                n = cx[k - 1, j - 1, I - 1]
+                nh = n / np.linalg.norm(n)
+                un = s[1:4] @ nh
+                a = np.sqrt(min(ds[5], max(un * un, uref2)))
+                f = 0.5 * (g + m)
+                assert abs(f[0] - (p["bc_pressure"] - inf[0])) <= 1e-9 * p["pref"]
+                # p + rho a u_n carried out of the domain from the surface cell
+                lhs = (f[0] + inf[0]) + ds[0] * a * (f[1:4] @ nh)
+                rhs = (s[0] + inf[0]) + ds[0] * a * un
+                assert abs(lhs - rhs) <= 1e-12 * abs(rhs)
+                assert np.all(np.abs((f[1:4] - (f[1:4] @ nh) * nh) - (s[1:4] - un * nh)) <= 1e-12 * 20)
+                # subsonic inlet at imin: ghost i = 2 - depth, mirror 1 + depth, surface 2
+                g, m, s = at(P, 2 - depth, j, k), at(P, 1 + depth, j, k), at(P, 2, j, k)
+                ds = at(Dv, 2, j, k)
+                n = -cx[k - 1, j - 1, 0]                                                # outward
+                nh = n / np.linalg.norm(n)
+                un = s[1:4] @ nh
+                a = np.sqrt(min(ds[5], max(un * un, uref2)))
+                f = 0.5 * (g + m)
+                pinf = p["bc_pressure"] - inf[0]
+                # outgoing p + rho a u_n from the interior, incoming p - rho a u_n from the free stream
+                assert abs((f[0] + ds[0] * a * (f[1:4] @ nh)) - (s[0] + ds[0] * a * un)) <= 1e-9 * p["pref"]
+                assert abs((f[0] - ds[0] * a * (f[1:4] @ nh)) - (pinf - ds[0] * a * (inf[1:4] @ nh))) <= 1e-9 * p["pref"]
+                vt = f[1:4] - (f[1:4] @ nh) * nh
+                assert np.all(np.abs(vt - (inf[1:4] - (inf[1:4] @ nh) * nh)) <= 1e-12 * 20)
+                assert np.all(np.abs(f[[4, 7, 8]] - inf[[4, 7, 8]]) <= 1e-12 * np.abs(inf[[4, 7, 8]]))
+
+
+def test_update_clips(orc, pkg):
+    """After a step from states pushed to the bounds, every interior cell has
+    p >= 10 - p_ref, 0 <= y1, y2 <= 1, k >= 1e-16 and omega >= max(1e-16,
+    omega_cut), and every clip that bit is exact (the bound itself)."""
+    case = pkg.cases.config2(8, 6, 5)
+    case.params["cav_clip"] = 1
+    b = case.blocks[0]
+    I, J, K = b.imax, b.jmax, b.kmax
+    pv, dv, tv, qq = case.init[0]
+    P = pv.reshape(K + 5, J + 5, I + 5, 9).copy()
+    inner = (slice(3, K + 2), slice(3, J + 2), slice(3, I + 2))
+    rng = np.random.default_rng(9)
+    sel = rng.random(P[inner + (0,)].shape) < 0.3
+    P[inner + (5,)] = np.where(sel, 1e-14, P[inner + (5,)])       # vapour fractions the update overshoots
+    P[inner + (7,)] = np.where(sel, 1e-17, P[inner + (7,)])
+    cell = P[inner].reshape(-1, 9)
+    d, t = pkg.cases.eos_host(cell[:, 0] + case.params["pref"], cell[:, 4], cell[:, 5], cell[:, 6])
+    Dv = dv.reshape(K + 5, J + 5, I + 5, 18).copy()
+    Dv[inner] = d.reshape(K - 1, J - 1, I - 1, 18)
+    o = orc.Oracle(case)
+    o.put_state(0, P.ravel(), Dv.ravel(), tv, qq)
+    o.setbc()
+    oc = o.calrhs(0)[3]
+    o.setqres()
+    o.timeinteg(1, 1, 0.0)
+    q = o.get_state(0)[0].reshape(K + 5, J + 5, I + 5, 9)[inner].reshape(-1, 9)
+    pref = case.params["pref"]
+    assert np.all(q[:, 0] >= 10.0 - pref)
+    assert np.all((q[:, 5] >= 0.0) & (q[:, 5] <= 1.0) & (q[:, 6] >= 0.0) & (q[:, 6] <= 1.0))
+    assert np.all(q[:, 7] >= 1e-16)
+    assert np.all(q[:, 8] >= np.maximum(1e-16, oc))
+    assert np.sum(q[:, 5] == 0.0) > 0 and np.sum(q[:, 7] == 1e-16) > 0     # the clips bit
