@@ -20,7 +20,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from .solver import BC, eos_host
+from .solver import BC, SURFACE, eos_host
 
 H = 0.1  # channel height [m]
 
@@ -552,3 +552,132 @@ def split_i(case, cuts):
     out = Case(case.name + f"_split{n}", dict(case.params), blocks, case.npv, case.ntv, case.ngrd)
     out.init = init
     return out
+
+
+# ------------------------------------------------------------ NACA0015 C-grid (config 3)
+def naca0015_thickness(x, c):
+    """Half thickness of the NACA 0015 section with the closed trailing edge
+    (last coefficient -0.1036), chord c."""
+    s = np.clip(x / c, 0.0, 1.0)
+    return 0.75 * c * (0.2969 * np.sqrt(s) - 0.1260 * s - 0.3516 * s ** 2 + 0.2843 * s ** 3 - 0.1036 * s ** 4)
+
+
+def _geometric(n, first):
+    """n+1 points from 0 to 1, the first step `first`, growing by a constant ratio."""
+    lo, hi = 1.0 + 1e-12, 2.0
+    for _ in range(200):
+        r = 0.5 * (lo + hi)
+        if first * (r ** n - 1.0) / (r - 1.0) > 1.0:
+            hi = r
+        else:
+            lo = r
+    r = 0.5 * (lo + hi)
+    s = np.concatenate([[0.0], first * np.cumsum(r ** np.arange(n))])
+    return s / s[-1]
+
+
+def naca_cgrid(ni, nj, nk, chord=0.1, span=None, radius=5.0, wake=5.0, first=2.0e-6):
+    """Four blocks of ni x nj x nk cells around a NACA 0015 section extruded
+    over `span` (default one chord): block 0 the lower wake (outflow -> trailing
+    edge), 1 the lower surface (trailing -> leading edge), 2 the upper surface,
+    3 the upper wake.  i runs along the C line, j from the section / wake line
+    out to the far field at about `radius` chords, k along the span.  The j
+    lines leave the section along its outward normal where that normal leans
+    upstream, straight down (up) elsewhere, with geometric clustering from a
+    first spacing `first` [m] (y+ about 1 at Re 1.2e6 for the default).
+    Boundaries: the section is BCWall (the force-integration surface), the
+    outer boundary and both outflow planes BCFarfield, the span ends
+    BCSymmetryPlane; blocks 0-1, 1-2, 2-3 join at aligned i faces and the
+    wake cut joins block 0's jmin face to block 3's with i reversed (CGNS
+    transform (-1, -2, 3))."""
+    span = chord if span is None else span
+    R, Lw = radius * chord, wake * chord
+    # inner C line, vertices along i for each block
+    tw = _geometric(ni, 1.0 / (4.0 * ni))[::-1]              # clustered toward the trailing edge
+    x_wake = chord + Lw * tw                                 # outflow -> trailing edge
+    beta = np.linspace(0.0, math.pi, ni + 1)
+    x_foil = chord * 0.5 * (1.0 + np.cos(beta))              # trailing -> leading edge, cosine spacing
+    lines = []
+    for side in (-1.0, 1.0):
+        xf = x_foil if side < 0 else x_foil[::-1]
+        yf = side * naca0015_thickness(xf, chord)
+        xw = x_wake if side < 0 else x_wake[::-1]
+        lines.append((np.stack([xf, yf], 1), np.stack([xw, np.zeros_like(xw)], 1)))
+    (lo_foil, lo_wake), (up_foil, up_wake) = lines
+    inner = [lo_wake, lo_foil, up_foil, up_wake]
+    # j-line directions: the outward surface normal where it leans upstream,
+    # else straight out (monotone along the C line, so the lines do not cross)
+    dirs = []
+    for q, P in enumerate(inner):
+        side = -1.0 if q < 2 else 1.0
+        d = np.zeros_like(P)
+        d[:, 1] = side
+        if q in (1, 2):
+            t = np.gradient(P, axis=0)
+            n = np.stack([-t[:, 1], t[:, 0]], 1)              # outward for the C line's direction
+            n /= np.linalg.norm(n, axis=1, keepdims=True)
+            lean = n[:, 0] < 0.0                                # pointing upstream
+            d[lean] = n[lean]
+            d[np.isclose(P[:, 0], 0.0)] = np.array([-1.0, 0.0])   # the leading edge
+            d[np.isclose(P[:, 0], chord)] = np.array([0.0, side])  # the trailing edge
+        dirs.append(d)
+    sj = _geometric(nj, first / R)
+    z = np.linspace(0.0, span, nk + 1)
+    I, J, K = ni + 1, nj + 1, nk + 1
+    blocks = []
+    for q in range(4):
+        P, d = inner[q], dirs[q]
+        L = R - np.abs(P[:, 1])
+        X = np.zeros((I, J, K, 3))
+        X[..., 0] = (P[:, 0][:, None] + sj[None, :] * (L * d[:, 0])[:, None])[:, :, None]
+        X[..., 1] = (P[:, 1][:, None] + sj[None, :] * (L * d[:, 1])[:, None])[:, :, None]
+        X[..., 2] = z[None, None, :]
+        bcs = [face_bc("farfield", "jmax", I, J, K), face_bc("symmetry", "kmin", I, J, K),
+               face_bc("symmetry", "kmax", I, J, K)]
+        if q in (1, 2):
+            w = face_bc("wall", "jmin", I, J, K)
+            w[0] |= SURFACE
+            bcs.append(w)
+        if q == 0:
+            bcs.append(face_bc("farfield", "imin", I, J, K))
+        if q == 3:
+            bcs.append(face_bc("farfield", "imax", I, J, K))
+        b = Block(I, J, K, X, bcs, [])
+        if q > 0:
+            b.conn_desc.append(connection([1, 1, 1, 1, J, K], [I, 1, 1, I, J, K], [1, 2, 3], q - 1))
+        if q < 3:
+            b.conn_desc.append(connection([I, 1, 1, I, J, K], [1, 1, 1, 1, J, K], [1, 2, 3], q + 1))
+        if q == 0:
+            b.conn_desc.append(connection([1, 1, 1, I, 1, K], [I, 1, 1, 1, 1, K], [-1, -2, 3], 3))
+        if q == 3:
+            b.conn_desc.append(connection([1, 1, 1, I, 1, K], [I, 1, 1, 1, 1, K], [-1, -2, 3], 0))
+        blocks.append(b)
+    return blocks
+
+
+def config3(ni=128, nj=160, nk=96, seed=3, cfl=50.0):
+    """Cavitating NACA 0015 wing section: 4 blocks of ni x nj x nk cells on a
+    C-grid (chord 0.1 m, span one chord), angle of attack 8 deg, U = 12 m/s,
+    sigma = 1.0 (p_ref = p_sat(300 K) + sigma rho U^2 / 2), Re = 1.2e6 on the
+    chord, k-omega SST (1 % intensity, mu_t/mu = 10 at the inflow), AUSM+-up,
+    MLP, Merkle, preconditioned LU-SGS at CFL 50; uniform free stream with
+    1e-3 conserved-variable noise.  Metrics come from the vertex coordinates
+    (Solver(case, metrics="device")); the blocks carry no host metrics."""
+    uref, chord = 12.0, 0.1
+    rho_l = float(eos_host(1.0e5, 300.0, 0.0, 0.0)[0][0, 0])
+    pref = sat_pressure(300.0) + 1.0 * 0.5 * rho_l * uref ** 2
+
+    def sst_inflow(rho, mu):
+        k = 1.5 * (0.01 * uref) ** 2
+        emut = 10.0 * mu
+        return k, rho * k / emut, emut
+
+    p = make_params(0, 4, 2, 8, 1, 1, 0, cfl, pref, uref, aoa=8.0, l_chord=chord, l_character=chord,
+                    l_domain=chord, turb_ref=sst_inflow)
+    p["cav_clip"] = 1
+    p["area"] = chord * chord
+    p["aoa"] = 8.0
+    blocks = naca_cgrid(ni, nj, nk, chord=chord)
+    case = Case(f"config3_4x{ni}x{nj}x{nk}", p, blocks, 9, 3, 5)
+    init_state(case, seed)
+    return case
