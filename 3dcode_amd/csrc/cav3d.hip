@@ -1378,6 +1378,7 @@ struct cav3d_ctx {
     std::vector<Con> cons;
     std::vector<std::vector<int>> bcdesc;   // per block: nbc*7 raw
     hipStream_t st = nullptr;
+    int dev = 0;              // device current at creation (the context's device)
     double *stage = nullptr;
     size_t stage_n = 0;
     int nwg_update = 0;
@@ -1703,6 +1704,7 @@ int cav3d_create(const cav3d_params *prm, int nblocks, const cav3d_block *blocks
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail("no HIP device", -4);
     cav3d_ctx *c = new cav3d_ctx();
+    hipGetDevice(&c->dev);
     c->prm = *prm;
     c->nb = nblocks;
     c->npv = prm->iturb == 0 ? 9 : 7;

@@ -137,9 +137,11 @@ __global__ __launch_bounds__(256) void k_interior_from_aos(const double *src, in
     for (int a = 0; a < m; a++) dst[a * nstride + g] = src[(g - B.g0) * m + a];
 }
 
-std::vector<FSeg> surface_segments(const cav3d_ctx *c, long &nface, bool &ok)
+// the CAV3D_BC_SURFACE ranges of every block, in block and list order, handed
+// to push(FSeg) (empty ranges skipped); nface counts their faces
+template <class Push>
+void scan_surface(const cav3d_ctx *c, long &nface, bool &ok, Push push)
 {
-    std::vector<FSeg> out;
     nface = 0;
     ok = true;
     for (int b = 0; b < c->nb; b++) {
@@ -163,10 +165,103 @@ std::vector<FSeg> surface_segments(const cav3d_ctx *c, long &nface, bool &ok)
             for (int a = 0; a < 3; a++) s.n *= std::max(0, s.hi[a] - s.lo[a] + 1);
             s.start = nface;
             nface += s.n;
-            if (s.n > 0) out.push_back(s);
+            if (s.n > 0) push(s);
         }
     }
+}
+
+std::vector<FSeg> surface_segments(const cav3d_ctx *c, long &nface, bool &ok)
+{
+    std::vector<FSeg> out;
+    scan_surface(c, nface, ok, [&](const FSeg &s) { out.push_back(s); });
     return out;
+}
+
+// the integrals' scaling of the eight sums: vapour and gas volume, pressure
+// and viscous force coefficients, cl and cd (clcd_writing)
+void integrals_out(const double *acc, double rhoref, double uref, double area, double aoa, double *out)
+{
+    const double ref = 2.0 / rhoref / (uref * uref) / area;
+    out[0] = acc[0];
+    out[1] = acc[1];
+    for (int a = 0; a < 6; a++) out[2 + a] = acc[2 + a] * ref;
+    const double fx = out[2] + out[5], fy = out[3] + out[6];
+    out[8] = fy * cos(aoa) - fx * sin(aoa);
+    out[9] = fy * sin(aoa) + fx * cos(aoa);
+}
+
+// ---- the allocation-free integrals (cav3d_integrals_noalloc)
+// Its device memory is one file-scope array per GPU, sized once here:
+//   [0, 4096)       the surface-segment table (at most NOALLOC_MAXSEG ranges)
+//   [4096, 4352)    the per-block vertex offsets
+//   [4352, 4608)    the eight final sums
+//   [4608, end)     the per-workgroup partial sums (2 per cell workgroup,
+//                   6 per surface-face workgroup)
+// Every call on a GPU uses that GPU's array and leaves its contents behind;
+// each call writes every entry it reads before reading it, so nothing needs
+// clearing, and calls that overlap on one GPU race.
+constexpr long NOALLOC_SCRATCH_BYTES = 2L << 20;
+constexpr int NOALLOC_MAXSEG = 64;
+constexpr long NOALLOC_SEG_OFF = 0, NOALLOC_V0_OFF = 4096, NOALLOC_SUM_OFF = 4352, NOALLOC_PART_OFF = 4608;
+constexpr long NOALLOC_PART_CAP = (NOALLOC_SCRATCH_BYTES - NOALLOC_PART_OFF) / (long)sizeof(double);
+static_assert(sizeof(FSeg) * NOALLOC_MAXSEG <= NOALLOC_V0_OFF - NOALLOC_SEG_OFF, "segment table fits");
+static_assert(sizeof(long) * MAXB <= NOALLOC_SUM_OFF - NOALLOC_V0_OFF, "vertex offsets fit");
+
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+// device address of this GPU's scratch, looked up once per device with the
+// device current (the array's host-side name is not a device address)
+constexpr int NOALLOC_MAXDEV = 64;
+unsigned char *g_scratch_ptr[NOALLOC_MAXDEV] = {};
+
+int scratch_of(int dev, unsigned char **out)
+{
+    if (dev < 0 || dev >= NOALLOC_MAXDEV) return fail("cav3d_integrals_noalloc: device ordinal out of range", -1);
+    if (!g_scratch_ptr[dev]) {
+        int cur = 0;
+        HIPCHK(hipGetDevice(&cur));
+        if (cur != dev) HIPCHK(hipSetDevice(dev));
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != dev) hipSetDevice(cur);
+        if (e != hipSuccess || !p)
+            return fail(std::string("cav3d_integrals_noalloc: hipGetSymbolAddress(scratch): ") + hipGetErrorString(e), -3);
+        g_scratch_ptr[dev] = (unsigned char *)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+
+// rows of W partial sums added in row order into acc (thread 0), staged
+// through LDS in chunks of 256 rows by the whole workgroup
+template <int W>
+__device__ __forceinline__ void seq_rows(const double *p, long rows, double *acc, double *buf)
+{
+    for (long r0 = 0; r0 < rows; r0 += 256) {
+        const int nr = (int)((rows - r0) < 256 ? (rows - r0) : 256);
+        __syncthreads();
+        for (int q = threadIdx.x; q < nr * W; q += blockDim.x) buf[q] = p[r0 * W + q];
+        __syncthreads();
+        if (threadIdx.x == 0)
+            for (int r = 0; r < nr; r++)
+#pragma unroll
+                for (int a = 0; a < W; a++) acc[a] += buf[r * W + a];
+    }
+}
+
+// the eight sums of the partials in the order cav3d_integrals adds them on the
+// host (workgroup order per quantity), so both entry points agree bit for bit
+__global__ __launch_bounds__(256) void k_post_final(const double *part, long nwc, long nwf, double *sum8)
+{
+    __shared__ double buf[256 * 6];
+    double ac[2] = {0.0, 0.0}, af[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    seq_rows<2>(part, nwc, ac, buf);
+    seq_rows<6>(part + 2 * nwc, nwf, af, buf);
+    if (threadIdx.x == 0) {
+        sum8[0] = ac[0];
+        sum8[1] = ac[1];
+        for (int a = 0; a < 6; a++) sum8[2 + a] = af[a];
+    }
 }
 
 // ---- restart file records of one block (src/variable_module.f90:126-178)
@@ -224,14 +319,55 @@ int cav3d_integrals(cav3d_ctx *c, double rhoref, double uref, double area, doubl
     double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (long q = 0; q < nwc; q++) for (int a = 0; a < 2; a++) acc[a] += hp[2 * q + a];
     for (long q = 0; q < nwf; q++) for (int a = 0; a < 6; a++) acc[2 + a] += hp[2 * nwc + 6 * q + a];
-    const double ref = 2.0 / rhoref / (uref * uref) / area;
-    out[0] = acc[0];
-    out[1] = acc[1];
-    for (int a = 0; a < 6; a++) out[2 + a] = acc[2 + a] * ref;
-    const double fx = out[2] + out[5], fy = out[3] + out[6];
-    out[8] = fy * cos(aoa) - fx * sin(aoa);
-    out[9] = fy * sin(aoa) + fx * cos(aoa);
+    integrals_out(acc, rhoref, uref, area, aoa, out);
     return check_launch("cav3d_integrals");
+}
+
+int cav3d_integrals_noalloc(cav3d_ctx *c, double rhoref, double uref, double area, double aoa, double *out)
+{
+    if (!c || !out) return fail("bad argument to cav3d_integrals_noalloc", -1);
+    DS &S = c->S;
+    FSeg segs[NOALLOC_MAXSEG];
+    int nseg = 0;
+    long nface;
+    bool ok;
+    scan_surface(c, nface, ok, [&](const FSeg &s) {
+        if (nseg < NOALLOC_MAXSEG) segs[nseg] = s;
+        nseg++;
+    });
+    if (!ok) return fail("cav3d_integrals_noalloc: a surface range is not a block face inside the cells", -2);
+    if (nseg > NOALLOC_MAXSEG)
+        return fail("cav3d_integrals_noalloc: " + std::to_string(nseg) + " surface ranges, at most " +
+                    std::to_string(NOALLOC_MAXSEG), -2);
+    if (nface > 0 && !c->xv) return fail("cav3d_integrals_noalloc: surface forces need the vertex coordinates (cav3d_set_grid)", -1);
+    const long nwc = (long)nblk(S.NC, POST_WG), nwf = nface > 0 ? (long)nblk(nface, POST_WG) : 0;
+    if (2 * nwc + 6 * nwf > NOALLOC_PART_CAP)
+        return fail("cav3d_integrals_noalloc: " + std::to_string(2 * nwc + 6 * nwf) + " partial sums exceed the " +
+                    std::to_string(NOALLOC_PART_CAP) + " of its fixed device array (" +
+                    std::to_string(NOALLOC_SCRATCH_BYTES) + " bytes); use cav3d_integrals", -2);
+    unsigned char *ptr = nullptr;
+    int rc = scratch_of(c->dev, &ptr);
+    if (rc) return rc;
+    FSeg *dseg = (FSeg *)(ptr + NOALLOC_SEG_OFF);
+    long *dv0 = (long *)(ptr + NOALLOC_V0_OFF);
+    double *sum8 = (double *)(ptr + NOALLOC_SUM_OFF);
+    double *dpart = (double *)(ptr + NOALLOC_PART_OFF);
+    long v0[MAXB] = {};
+    for (int b = 0; b < c->nb && b < (int)c->v0.size(); b++) v0[b] = c->v0[b];
+    hipLaunchKernelGGL(k_post_cells, dim3(nwc), dim3(POST_WG), 0, c->st, S, (const double *)S.vol, dpart);
+    if (nface > 0) {
+        HIPCHK(hipMemcpyAsync(dseg, segs, nseg * sizeof(FSeg), hipMemcpyHostToDevice, c->st));
+        HIPCHK(hipMemcpyAsync(dv0, v0, sizeof(v0), hipMemcpyHostToDevice, c->st));
+        hipLaunchKernelGGL(k_post_faces, dim3(nwf), dim3(POST_WG), 0, c->st, S, (const FSeg *)dseg, nseg, nface,
+                           (const double *)c->xv, c->NV, (const long *)dv0, (const double *)c->xc, dpart + 2 * nwc);
+    }
+    hipLaunchKernelGGL(k_post_final, dim3(1), dim3(256), 0, c->st, (const double *)dpart, nwc, nwf, sum8);
+    double acc[8];
+    HIPCHK(hipMemcpyAsync(acc, sum8, sizeof(acc), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    if ((rc = check_launch("cav3d_integrals_noalloc"))) return rc;
+    integrals_out(acc, rhoref, uref, area, aoa, out);
+    return 0;
 }
 
 int cav3d_export(cav3d_ctx *c, const char *path, int nt_phy, int nt, double time)

@@ -30,7 +30,7 @@ SURFACE = 0x100   # CAV3D_BC_SURFACE: OR'ed into a kind, marks the force-integra
 
 EXPORTS = [
     "cav3d_create", "cav3d_set_metrics", "cav3d_set_grid", "cav3d_build_metrics", "cav3d_get_metrics",
-    "cav3d_integrals", "cav3d_export", "cav3d_import", "cav3d_rhs_vjp",
+    "cav3d_integrals", "cav3d_integrals_noalloc", "cav3d_export", "cav3d_import", "cav3d_rhs_vjp",
     "cav3d_put_state", "cav3d_get_state", "cav3d_timeinteg",
     "cav3d_setbc", "cav3d_calrhs", "cav3d_timestep", "cav3d_residual", "cav3d_unsteady_shift",
     "cav3d_eos", "cav3d_eos_host", "cav3d_profile", "cav3d_synchronize", "cav3d_last_error",
@@ -74,6 +74,7 @@ def lib(build_if_missing: bool = True):
     L.cav3d_build_metrics.argtypes = [C.c_void_p]
     L.cav3d_get_metrics.argtypes = [C.c_void_p, C.c_int, _D, _D, _D, _D]
     L.cav3d_integrals.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_double, _D]
+    L.cav3d_integrals_noalloc.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_double, _D]
     L.cav3d_export.argtypes = [C.c_void_p, C.c_char_p, C.c_int, C.c_int, C.c_double]
     L.cav3d_import.argtypes = [C.c_void_p, C.c_char_p, _I, _I, _D]
     L.cav3d_rhs_vjp.argtypes = [C.c_void_p, _D, _D, _D]
@@ -264,6 +265,12 @@ class Solver:
         """vapour volume, gas volume, pressure force (3), viscous force (3), cl, cd."""
         out = np.zeros(10)
         _check(lib().cav3d_integrals(self.h, rhoref, uref, area, aoa, _p(out)))
+        return out
+
+    def integrals_noalloc(self, rhoref, uref, area, aoa=0.0):
+        """The same integrals through cav3d_integrals_noalloc (no device allocation)."""
+        out = np.zeros(10)
+        _check(lib().cav3d_integrals_noalloc(self.h, rhoref, uref, area, aoa, _p(out)))
         return out
 
     def export(self, path, nt_phy, nt, time):

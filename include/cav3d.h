@@ -144,6 +144,17 @@ int cav3d_get_metrics(cav3d_ctx *ctx, int b, double *cx, double *ex, double *tx,
  * Surface forces need the vertex coordinates (cav3d_set_grid). */
 int cav3d_integrals(cav3d_ctx *ctx, double rhoref, double uref, double area, double aoa, double *out);
 
+/* cav3d_integrals without any allocation: no counterpart in the reference.
+ * For callers that cannot allocate device memory (code under stream capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments and
+ * results as cav3d_integrals (bit for bit); its partial sums, segment table
+ * and final sums live in one fixed 2 MiB device array per GPU instead of
+ * buffers allocated per call.  Limits: at most 64 CAV3D_BC_SURFACE ranges and
+ * 2*ceil(cells/256) + 6*ceil(surface faces/256) <= 261568 partial sums
+ * (about 33 M cells); beyond them it returns an error.  Every call on a GPU
+ * shares that GPU's array: calls that overlap on one GPU race. */
+int cav3d_integrals_noalloc(cav3d_ctx *ctx, double rhoref, double uref, double area, double aoa, double *out);
+
 /* Restart file of every block, the reference's out_<iter>.dat layout
  * (export_variable, src/variable_module.f90:108-182): per block, int32 nt_phy,
  * int32 nt, float64 time, pv(npv,-1:imax+3,..), emut(-1:imax+3,..) when
