@@ -370,9 +370,15 @@ def init_state(case, seed, pocket=None):
 
 
 # ------------------------------------------------------------------ the configs
-def bump_case(ni, nj, nk, turbulent, cavitating, nscheme, nmuscl, nlim, cfl=5.0, sigma=0.8, uref=10.0,
-              nsteady=0, dt_phy=1.0, prec=1, seed=1, name="bump"):
-    """Bump channel of height H=0.1 m with water at 300 K."""
+def bump_params(turbulent, cavitating, nscheme, nmuscl, nlim, cfl=5.0, sigma=0.8, uref=10.0, nsteady=0,
+                dt_phy=1.0, prec=1):
+    """Run parameters of the bump channel (water at 300 K) as functions of the
+    cavitation number sigma and the inflow speed U_in = uref:
+    p_ref = p_sat(300 K) + sigma rho_l U^2 / 2 (cavitating) or 101325 Pa,
+    the boundary state (p_ref, U_in, 0, 0, 300 K, 0, 0, k, omega) with
+    k = 1.5 (0.01 U)^2 and omega = rho k / (10 mu) (SST), and the
+    mass-transfer scales dp_ref = 2/(rho_ref U^2), t_ref = U/H
+    (src/cav_module.f90:81-82) at rho_ref = rho(p_ref, 300 K)."""
     iturb = 0 if turbulent else -2
     if cavitating:
         rho_l = float(eos_host(1.0e5, 300.0, 0.0, 0.0)[0][0, 0])
@@ -390,6 +396,55 @@ def bump_case(ni, nj, nk, turbulent, cavitating, nscheme, nmuscl, nlim, cfl=5.0,
 
     p = make_params(iturb, nscheme, nmuscl, nlim, prec, ncav, nsteady, cfl, pref, uref, dt_phy=dt_phy,
                     turb_ref=sst_inflow if turbulent else None)
+    p["bump"] = dict(turbulent=turbulent, cavitating=cavitating, nscheme=nscheme, nmuscl=nmuscl, nlim=nlim, cfl=cfl,
+                     sigma=sigma, uref=uref, nsteady=nsteady, dt_phy=dt_phy, prec=prec)
+    return p
+
+
+# the run parameters the residual reads, in the order of cav3d_rhs_vjp_params
+PARAM_KEYS = ["pref", "uref", "dp_ref", "t_ref"] + [f"bc_pv{q}" for q in range(9)] + ["bc_pressure"]
+
+
+def param_vector(p):
+    return np.array([p["pref"], p["uref"], p["dp_ref"], p["t_ref"], *list(p["bc_pv"]), p["bc_pressure"]])
+
+
+def param_derivatives(case, which, rel=1e-5):
+    """d(run parameters)/d(which) for which = "sigma" or "uref" (U_in) of a
+    bump-channel case, in PARAM_KEYS order: central differences of the
+    host-side parameter map bump_params (smooth closed forms and the EOS at
+    the reference state), step rel * |value|."""
+    b = dict(case.params["bump"])
+    v0 = b[which]
+    h = rel * abs(v0)
+    b[which] = v0 + h
+    hi = param_vector(bump_params(**b))
+    b[which] = v0 - h
+    lo = param_vector(bump_params(**b))
+    return (hi - lo) / (2.0 * h)
+
+
+def with_param(case, which, value):
+    """A copy of a bump-channel case (same grid and state) whose run
+    parameters are those at `which` = value."""
+    import copy
+    b = dict(case.params["bump"])
+    b[which] = value
+    p = bump_params(**b)
+    for k in ("cav_clip", "precision", "ides"):
+        if k in case.params:
+            p[k] = case.params[k]
+    p["ncav"] = case.params["ncav"]
+    c2 = copy.copy(case)
+    c2.params = p
+    return c2
+
+
+def bump_case(ni, nj, nk, turbulent, cavitating, nscheme, nmuscl, nlim, cfl=5.0, sigma=0.8, uref=10.0,
+              nsteady=0, dt_phy=1.0, prec=1, seed=1, name="bump"):
+    """Bump channel of height H=0.1 m with water at 300 K."""
+    p = bump_params(turbulent, cavitating, nscheme, nmuscl, nlim, cfl=cfl, sigma=sigma, uref=uref, nsteady=nsteady,
+                    dt_phy=dt_phy, prec=prec)
     npv = 9 if turbulent else 7
     ntv = 3 if turbulent else 2
     ngrd = 5 if turbulent else 4

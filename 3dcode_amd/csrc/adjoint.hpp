@@ -50,7 +50,7 @@ struct AD {
     double *ub[3];          // d(phi_f)/d(u_ref) [NF]
     double *cb;             // cell-source bars on the 7-point stencil [7][npv][NC]
     double *cdb;            // cell-source property bars [ND][NC]
-    double *cub;            // cell-source d(phi_c)/d(u_ref) [NC] (Singhal's rate constants)
+    double *csb;            // cell-source d(phi_c)/d(u_ref, dp_ref, t_ref) [3][NC] (the mass-transfer scales)
     double *pb;             // pv_bar [npv][NG]
     unsigned long long mask;   // stencil points a face writes (bit ll-1)
 };
@@ -260,12 +260,14 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     for (int n = 0; n < NPV; n++) rb[n] = n < npv ? A.rb[(long)n * S.NC + t] : 0.0;
     const bool sst = P.iturb == 0, cav_ = P.ncav >= 1, bdf = P.nsteady > 0;
     // z = {pc[npv], rho, h, mu, the Green-Gauss gradients of u, v, w, k, omega (15),
-    //      u_ref (Singhal only)}
-    constexpr int NZ = NPV + 3 + 15 + 1;
+    //      the scalars the mass-transfer rates read: u_ref (Singhal), dp_ref
+    //      (Merkle, Kunz's evaporation), t_ref (Merkle, Kunz)}
+    constexpr int NZ = NPV + 3 + 15 + 3;
     const int od = npv, og = npv + 3;
     const int nz0 = (cav_ || bdf || sst) ? (sst ? og + 15 : og) : 0;
-    const int ou = (nz0 && P.ncav == 3) ? nz0 : -1;
-    const int nz = ou >= 0 ? nz0 + 1 : nz0;
+    const bool sc = nz0 && (P.ncav == 1 || P.ncav == 2 || P.ncav == 3);
+    const int osc = nz0;                                   // u_ref, dp_ref, t_ref lanes
+    const int nz = sc ? nz0 + 3 : nz0;
     double zx[NZ], zb[NZ];
     for (int q = 0; q < NZ; q++) zx[q] = zb[q] = 0.0;
     const int gvar[5] = {1, 2, 3, 7, 8};
@@ -287,7 +289,7 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         zx[od] = d[D_RHO]; zx[od + 1] = d[D_H]; zx[od + 2] = d[D_MU];
         if (sst)
             for (int v = 0; v < 5; v++) gg_grad<double>(x7, gvar[v], f6, iv, zx + og + 3 * v);
-        if (ou >= 0) zx[ou] = P.uref;
+        if (sc) { zx[osc] = P.uref; zx[osc + 1] = P.dp_ref; zx[osc + 2] = P.t_ref; }
         const double ydn = sst ? S.ydn[widx(B, i, j, k)] : 1.0;
         const double mut = S.mut ? S.mut[g] : 0.0;
         double q1[NPV], q2[NPV];
@@ -299,8 +301,10 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         using T = Dn<W>;
         dn_grad<W>(nz, [&](int q) -> double { return zx[q]; }, [&](auto in, int base) -> T {
             Par Pu = P;
-            const int lu = ou - base;
-            Pu.uref_seed = (ou >= 0 && lu >= 0 && lu < W) ? lu + 1 : 0;
+            auto lane_of = [&](int q) { const int l = q - base; return (sc && l >= 0 && l < W) ? l + 1 : 0; };
+            Pu.uref_seed = lane_of(osc);
+            Pu.dpref_seed = lane_of(osc + 1);
+            Pu.tref_seed = lane_of(osc + 2);
             T pc[NPV], dt[ND];
             for (int n = 0; n < NPV; n++) pc[n] = n < npv ? in(n) : T(0.0);
             for (int m = 0; m < ND; m++) dt[m] = T(d[m]);
@@ -314,10 +318,10 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
                         mixture_eos<T>(pc[0] + P.pref, pc[4], pc[5], pc[6], dd, false, full);
                         cav_model<T>(Pu, pc, full, vol, rc, rv, ic);
                     } else {
-                        merkle<T>(P, pc, dt, vol, rc, rv, ic);
+                        merkle<T>(Pu, pc, dt, vol, rc, rv, ic);
                     }
                 } else {
-                    merkle<T>(P, pc, dt, vol, rc, rv, ic);
+                    merkle<T>(Pu, pc, dt, vol, rc, rv, ic);
                 }
                 phi = phi + rb[5] * (rv - rc);
             }
@@ -373,7 +377,7 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     A.cdb[(long)D_RHO * S.NC + t] = zb[od];
     A.cdb[(long)D_H * S.NC + t] = zb[od + 1];
     A.cdb[(long)D_MU * S.NC + t] = zb[od + 2];
-    A.cub[t] = ou >= 0 ? zb[ou] : 0.0;
+    for (int q = 0; q < 3; q++) A.csb[(long)q * S.NC + t] = sc ? zb[osc + q] : 0.0;
 }
 
 // pv_bar of every ghosted cell: its bars from the faces whose stencils hold
@@ -456,6 +460,22 @@ __global__ __launch_bounds__(256) void k_sum_fixed(const double *a, long n, doub
     if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
 
+// deterministic sum over q of w[q] * (a[q] - b[q]) * inv_h2, strided as k_sum_fixed
+__global__ __launch_bounds__(256) void k_dotdiff_fixed(const double *w, const double *a, const double *b, long n,
+                                                       double inv_h2, double *part)
+{
+    __shared__ double sh[256];
+    double s = 0.0;
+    for (long q = (long)blockIdx.x * 256 + threadIdx.x; q < n; q += 256L * gridDim.x) s += w[q] * ((a[q] - b[q]) * inv_h2);
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int v = 128; v > 0; v >>= 1) {
+        if ((int)threadIdx.x < v) sh[threadIdx.x] += sh[threadIdx.x + v];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
 // stencil points a face's bars cover under the configuration
 unsigned long long adj_mask(const Par &P)
 {
@@ -475,8 +495,8 @@ int adj_alloc(cav3d_ctx *c)
     const int npv = c->npv;
     int rc;
     if ((rc = dalloc(c, &c->a_rb, (long)npv * S.NC)) || (rc = dalloc(c, &c->a_cb, 7L * npv * S.NC)) ||
-        (rc = dalloc(c, &c->a_cdb, (long)ND * S.NC)) || (rc = dalloc(c, &c->a_cub, S.NC)) || (rc = dalloc(c, &c->a_pb, (long)npv * S.NG)) ||
-        (rc = dalloc(c, &c->a_sum, 8 + 5L * SUM_WG)))
+        (rc = dalloc(c, &c->a_cdb, (long)ND * S.NC)) || (rc = dalloc(c, &c->a_csb, 3L * S.NC)) || (rc = dalloc(c, &c->a_pb, (long)npv * S.NG)) ||
+        (rc = dalloc(c, &c->a_sum, 16 + 16L * SUM_WG)))
         return rc;
     for (int d = 0; d < 3; d++)
         if ((rc = dalloc(c, &c->a_fb[d], (long)NSLOT * npv * S.NF[d])) || (rc = dalloc(c, &c->a_db[d], 2L * ND * S.NF[d])) ||
@@ -493,7 +513,7 @@ int launch_vjp(cav3d_ctx *c)
     AD A;
     A.rb = c->a_rb;
     for (int d = 0; d < 3; d++) { A.fb[d] = c->a_fb[d]; A.db[d] = c->a_db[d]; A.ub[d] = c->a_ub[d]; }
-    A.cb = c->a_cb; A.cdb = c->a_cdb; A.cub = c->a_cub; A.pb = c->a_pb;
+    A.cb = c->a_cb; A.cdb = c->a_cdb; A.csb = c->a_csb; A.pb = c->a_pb;
     A.mask = adj_mask(S.P);
     const bool sensor = S.P.nscheme == 2 || S.P.nscheme == 3;
     if (sensor) {
@@ -516,13 +536,9 @@ int launch_vjp(cav3d_ctx *c)
     return 0;
 }
 
-}  // namespace
-
-extern "C" {
-
-int cav3d_rhs_vjp(cav3d_ctx *c, const double *res_bar, double *pv_bar, double *scal_bar)
+// every block's res_bar (the calrhs res layout, blocks concatenated) into a_rb
+int upload_res_bar(cav3d_ctx *c, const double *res_bar)
 {
-    if (!c || !res_bar || !pv_bar) return fail("bad argument to cav3d_rhs_vjp", -1);
     int rc;
     if ((rc = adj_alloc(c))) return rc;
     DS &S = c->S;
@@ -539,6 +555,29 @@ int cav3d_rhs_vjp(cav3d_ctx *c, const double *res_bar, double *pv_bar, double *s
         HIPCHK(hipStreamSynchronize(c->st));
         off += (long)npv * B.nc;
     }
+    return 0;
+}
+
+// the fixed-order sum of n values at src into a_sum[slot]
+void sum_into(cav3d_ctx *c, const double *src, long n, int slot)
+{
+    double *part = c->a_sum + 16 + (long)SUM_WG * slot;
+    hipLaunchKernelGGL(k_sum_fixed, dim3(SUM_WG), dim3(256), 0, c->st, src, n, part);
+    hipLaunchKernelGGL(k_sum_fixed, dim3(1), dim3(256), 0, c->st, (const double *)part, (long)SUM_WG, c->a_sum + slot);
+}
+
+}  // namespace
+
+extern "C" {
+
+int cav3d_rhs_vjp(cav3d_ctx *c, const double *res_bar, double *pv_bar, double *scal_bar)
+{
+    if (!c || !res_bar || !pv_bar) return fail("bad argument to cav3d_rhs_vjp", -1);
+    int rc;
+    if ((rc = upload_res_bar(c, res_bar))) return rc;
+    DS &S = c->S;
+    const int npv = c->npv;
+    long off = 0;
     if (c->profile) hipEventRecord(c->ev[10], c->st);
     launch_vjp(c);
     if (c->profile) {
@@ -559,14 +598,9 @@ int cav3d_rhs_vjp(cav3d_ctx *c, const double *res_bar, double *pv_bar, double *s
         off += (long)npv * B.ng;
     }
     if (scal_bar) {
-        // stage one into a_sum[8 + SUM_WG * q], stage two into a_sum[q]
-        const double *src[5] = {c->a_pb, c->a_ub[0], c->a_ub[1], c->a_ub[2], c->a_cub};
+        const double *src[5] = {c->a_pb, c->a_ub[0], c->a_ub[1], c->a_ub[2], c->a_csb};
         const long len[5] = {S.NG, S.NF[0], S.NF[1], S.NF[2], S.NC};
-        for (int q = 0; q < 5; q++) {
-            double *part = c->a_sum + 8 + (long)SUM_WG * q;
-            hipLaunchKernelGGL(k_sum_fixed, dim3(SUM_WG), dim3(256), 0, c->st, src[q], len[q], part);
-            hipLaunchKernelGGL(k_sum_fixed, dim3(1), dim3(256), 0, c->st, (const double *)part, (long)SUM_WG, c->a_sum + q);
-        }
+        for (int q = 0; q < 5; q++) sum_into(c, src[q], len[q], q);
         double h[5];
         HIPCHK(hipMemcpyAsync(h, c->a_sum, sizeof(h), hipMemcpyDeviceToHost, c->st));
         HIPCHK(hipStreamSynchronize(c->st));
@@ -574,6 +608,126 @@ int cav3d_rhs_vjp(cav3d_ctx *c, const double *res_bar, double *pv_bar, double *s
         scal_bar[1] = ((h[1] + h[2]) + h[3]) + h[4];
     }
     return check_launch("cav3d_rhs_vjp");
+}
+
+int cav3d_rhs_vjp_params(cav3d_ctx *c, const double *res_bar, double *par_bar)
+{
+    if (!c || !res_bar || !par_bar) return fail("bad argument to cav3d_rhs_vjp_params", -1);
+    int rc;
+    if ((rc = upload_res_bar(c, res_bar))) return rc;
+    DS &S = c->S;
+    const int npv = c->npv;
+    if (!c->a_save) {
+        if ((rc = dalloc(c, &c->a_save, (long)(2 * npv + ND + 1) * S.NG))) return rc;
+    }
+    launch_vjp(c);
+    // the partials at fixed ghost cells
+    sum_into(c, c->a_pb, S.NG, 0);
+    sum_into(c, c->a_ub[0], S.NF[0], 1);
+    sum_into(c, c->a_ub[1], S.NF[1], 2);
+    sum_into(c, c->a_ub[2], S.NF[2], 3);
+    for (int q = 0; q < 3; q++) sum_into(c, c->a_csb + (long)q * S.NC, S.NC, 4 + q);
+    // the boundary fill's share: for a fixed interior every ghost cell is an
+    // affine function of the boundary state bc_pv, bc_pressure (the mirror,
+    // surface and neighbour cells and the in/out mode do not move), so the
+    // product of the ghost-cell bars with its derivative is a central
+    // difference of the fill, exact up to rounding
+    double *save_pv = c->a_save, *save_dc = save_pv + (long)npv * S.NG, *save_mut = save_dc + (long)ND * S.NG;
+    double *plus = save_mut + S.NG;
+    HIPCHK(hipMemcpyAsync(save_pv, S.pv, (long)npv * S.NG * sizeof(double), hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(save_dc, S.dc, (long)ND * S.NG * sizeof(double), hipMemcpyDeviceToDevice, c->st));
+    if (S.mut) HIPCHK(hipMemcpyAsync(save_mut, S.mut, S.NG * sizeof(double), hipMemcpyDeviceToDevice, c->st));
+    const Par P0 = S.P;
+    double hs[10];
+    for (int m = 0; m < 10; m++) {
+        double &theta = (m < 9) ? S.P.bc_pv[m] : S.P.bc_pressure;
+        const double v0 = theta;
+        const double h = v0 != 0.0 ? 1.0e-4 * fabs(v0) : 1.0e-6;
+        hs[m] = h;
+        theta = v0 + h;
+        launch_setbc(c);
+        HIPCHK(hipMemcpyAsync(plus, S.pv, (long)npv * S.NG * sizeof(double), hipMemcpyDeviceToDevice, c->st));
+        theta = v0 - h;
+        launch_setbc(c);
+        theta = v0;
+        double *part = c->a_sum + 16 + (long)SUM_WG * (7 + (m % 8));
+        hipLaunchKernelGGL(k_dotdiff_fixed, dim3(SUM_WG), dim3(256), 0, c->st, (const double *)c->a_pb, (const double *)plus,
+                           (const double *)S.pv, (long)npv * S.NG, 1.0 / (2.0 * h), part);
+        hipLaunchKernelGGL(k_sum_fixed, dim3(1), dim3(256), 0, c->st, (const double *)part, (long)SUM_WG, c->a_sum + 7 + m % 8);
+        if (m == 7 || m == 9) {
+            // a_sum[7..14] hold up to eight results: copy them out before reuse
+            double hm[8];
+            HIPCHK(hipMemcpyAsync(hm, c->a_sum + 7, sizeof(hm), hipMemcpyDeviceToHost, c->st));
+            HIPCHK(hipStreamSynchronize(c->st));
+            const int first = m == 7 ? 0 : 8;
+            for (int q = first; q <= m; q++) par_bar[4 + q] = hm[q % 8];
+        }
+    }
+    S.P = P0;
+    HIPCHK(hipMemcpyAsync(S.pv, save_pv, (long)npv * S.NG * sizeof(double), hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(S.dc, save_dc, (long)ND * S.NG * sizeof(double), hipMemcpyDeviceToDevice, c->st));
+    if (S.mut) HIPCHK(hipMemcpyAsync(S.mut, save_mut, S.NG * sizeof(double), hipMemcpyDeviceToDevice, c->st));
+    double h[7];
+    HIPCHK(hipMemcpyAsync(h, c->a_sum, sizeof(h), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    par_bar[0] = h[0];
+    par_bar[1] = ((h[1] + h[2]) + h[3]) + h[4];
+    par_bar[2] = h[5];
+    par_bar[3] = h[6];
+    return check_launch("cav3d_rhs_vjp_params");
+}
+
+int cav3d_objective(cav3d_ctx *c, double w_vap, double w_fx, double *J, double *pv_bar, double *dJ_dpref)
+{
+    if (!c || !J) return fail("bad argument to cav3d_objective", -1);
+    DS &S = c->S;
+    long nface;
+    bool ok;
+    std::vector<FSeg> segs = surface_segments(c, nface, ok);
+    if (!ok) return fail("cav3d_objective: a surface range is not a block face inside the cells", -2);
+    if (nface > 0 && w_fx != 0.0 && !c->xv) return fail("cav3d_objective: the x-force needs the vertex coordinates (cav3d_set_grid)", -1);
+    int rc;
+    if ((rc = adj_alloc(c))) return rc;
+    if (c->a_obj_n < S.NC + nface) {
+        if (c->a_obj) { HIPCHK(hipStreamSynchronize(c->st)); HIPCHK(hipFree(c->a_obj)); c->a_obj = nullptr; }
+        HIPCHK(hipMalloc((void **)&c->a_obj, 2 * (S.NC + nface) * sizeof(double)));
+        c->a_obj_n = S.NC + nface;
+    }
+    double *val = c->a_obj, *dpr = c->a_obj + c->a_obj_n;
+    // pv_bar: zero on the ghost layers, the cells' shares, then each range's faces
+    hipLaunchKernelGGL(k_fill, dim3(nblk((long)c->npv * S.NG, 256)), dim3(256), 0, c->st, c->a_pb, 0.0, (long)c->npv * S.NG);
+    hipLaunchKernelGGL(k_obj_cells, dim3(nblk(S.NC, POST_WG)), dim3(POST_WG), 0, c->st, S, (const double *)S.vol, w_vap,
+                       c->a_pb, val, dpr);
+    long off = S.NC;
+    if (w_fx != 0.0)
+        for (const FSeg &sg : segs) {
+            hipLaunchKernelGGL(k_obj_faces, dim3(nblk(sg.n, POST_WG)), dim3(POST_WG), 0, c->st, S, sg, (const double *)c->xv,
+                               c->NV, c->v0[sg.b], (const double *)c->xc, w_fx, c->a_pb, val + off, dpr + off);
+            off += sg.n;
+        }
+    sum_into(c, val, off, 0);
+    sum_into(c, dpr, off, 1);
+    double h[2];
+    HIPCHK(hipMemcpyAsync(h, c->a_sum, sizeof(h), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    *J = h[0];
+    if (dJ_dpref) *dJ_dpref = h[1];
+    if (pv_bar) {
+        const int npv = c->npv;
+        long maxn = 0;
+        for (int b = 0; b < c->nb; b++) maxn = std::max(maxn, (long)npv * c->blk[b].ng);
+        if ((rc = ensure_stage(c, maxn))) return rc;
+        long o = 0;
+        for (int b = 0; b < c->nb; b++) {
+            const BlkD &B = c->blk[b];
+            hipLaunchKernelGGL(k_soa_to_aos, dim3(nblk(B.ng, 256)), dim3(256), 0, c->st, (const double *)c->a_pb, npv, S.NG,
+                               B.g0, c->stage, B.ng);
+            HIPCHK(hipMemcpyAsync(pv_bar + o, c->stage, (long)npv * B.ng * sizeof(double), hipMemcpyDeviceToHost, c->st));
+            HIPCHK(hipStreamSynchronize(c->st));
+            o += (long)npv * B.ng;
+        }
+    }
+    return check_launch("cav3d_objective");
 }
 
 }  // extern "C"

@@ -1401,8 +1401,9 @@ struct cav3d_ctx {
     long xbuf_n = 0;
     // adjoint work arrays (allocated by the first cav3d_rhs_vjp)
     bool adj_ready = false;
-    double *a_rb = nullptr, *a_fb[3] = {}, *a_db[3] = {}, *a_ub[3] = {}, *a_cb = nullptr, *a_cdb = nullptr, *a_cub = nullptr,
-           *a_pb = nullptr, *a_sum = nullptr;
+    double *a_rb = nullptr, *a_fb[3] = {}, *a_db[3] = {}, *a_ub[3] = {}, *a_cb = nullptr, *a_cdb = nullptr, *a_csb = nullptr,
+           *a_pb = nullptr, *a_sum = nullptr, *a_save = nullptr, *a_obj = nullptr;
+    long a_obj_n = 0;
 };
 
 namespace {
@@ -1884,6 +1885,7 @@ void cav3d_destroy(cav3d_ctx *c)
     for (void *p : c->allocs) hipFree(p);
     if (c->stage) hipFree(c->stage);
     if (c->xbuf) hipFree(c->xbuf);
+    if (c->a_obj) hipFree(c->a_obj);
     for (int q = 0; q < 14; q++) if (c->ev[q]) hipEventDestroy(c->ev[q]);
     if (c->st) hipStreamDestroy(c->st);
     delete c;

@@ -35,6 +35,7 @@ enum { D_RHO = 0, D_H, D_C2, D_RP, D_RT, D_RY1, D_RY2, D_HP, D_HT, D_HY1, D_HY2,
 struct Par {
     int npv, ntv, iturb, nscheme, nmuscl, nlim, local, prec, ncav, nsteady, cav_clip, ides;
     int uref_seed;   // dual lane + 1 that carries d/d(uref) in the adjoint's passes (0: none)
+    int dpref_seed, tref_seed;   // the same for the mass-transfer scales dp_ref and t_ref
     double cfl, pref, uref, str, dt_phy, c_v, c_c, dp_ref, t_ref, t_crit;
     double bc_pv[9], bc_pressure, emutref, bdf_b;
 };
@@ -187,6 +188,16 @@ template <class R> struct RefSpeed {
 };
 template <int W> struct RefSpeed<Dn<W>> {
     HD static Dn<W> of(const Par &P) { return Dn<W>::seed(P.uref, P.uref_seed - 1); }
+};
+// the mass-transfer scales dp_ref = 2/(rho_ref uref^2) and t_ref = uref/l
+// (src/cav_module.f90:81-82) as R, seeded like RefSpeed
+template <class R> struct CavScale {
+    HD static R dp(const Par &P) { return R(P.dp_ref); }
+    HD static R t(const Par &P) { return R(P.t_ref); }
+};
+template <int W> struct CavScale<Dn<W>> {
+    HD static Dn<W> dp(const Par &P) { return Dn<W>::seed(P.dp_ref, P.dpref_seed - 1); }
+    HD static Dn<W> t(const Par &P) { return Dn<W>::seed(P.t_ref, P.tref_seed - 1); }
 };
 
 // the face EOS skips a phase whose mass fraction is zero (its terms are exact
@@ -1447,8 +1458,9 @@ HD void merkle(const Par &P, const R *pc, const R *d, double vol, R &rc, R &rv, 
     R pww = sat_p(pc[4]);
     R irho = 1.0 / d[D_RHO];
     R pc_ = mx(pc[0] + P.pref - pww, R(0.0)), pv_ = mx(pww - pc[0] - P.pref, R(0.0));
-    R r_c = P.c_c * d[D_RHO] * pc[5] * pc_ * P.dp_ref * P.t_ref;
-    R r_v = P.c_v * d[D_RHO] * (1.0 - pc[5] - pc[6]) * pv_ * P.dp_ref * P.t_ref;
+    const R dpr = CavScale<R>::dp(P), tr = CavScale<R>::t(P);
+    R r_c = P.c_c * d[D_RHO] * pc[5] * pc_ * dpr * tr;
+    R r_v = P.c_v * d[D_RHO] * (1.0 - pc[5] - pc[6]) * pv_ * dpr * tr;
     if (P.cav_clip == 0) {   // as written in src/cav_module.f90:151-152 (pc[4] is T)
         r_c = mn(r_c, d[D_RHO] * pc[4]);
         r_v = mn(r_v, d[D_RHO] * (1.0 - pc[4] - pc[5]));
@@ -1502,9 +1514,10 @@ HD void kunz(const Par &P, const R *pc, const R *dv, double vol, R &rc, R &rv, R
     R pww = sat_p(pc[4]);
     R rho1 = 1.0 / dv[0];
     R av = dv[0] * pc[5] / dv[3], ag = dv[0] * pc[6] / dv[4], al = 1.0 - av - ag;
-    R r_c = P.c_c * dv[3] * av * ((al + ag) * (al + ag)) * P.t_ref;
+    const R dpr = CavScale<R>::dp(P), tr = CavScale<R>::t(P);
+    R r_c = P.c_c * dv[3] * av * ((al + ag) * (al + ag)) * tr;
     R pv_ = mx(pww - pc[0] - P.pref, R(0.0));
-    R r_v = P.c_v * dv[0] * (1.0 - pc[5] - pc[6]) * pv_ * P.dp_ref * P.t_ref;
+    R r_v = P.c_v * dv[0] * (1.0 - pc[5] - pc[6]) * pv_ * dpr * tr;
     cav_clip_rates(P, pc, dv, r_c, r_v);
     rc = r_c * vol;
     rv = r_v * vol;

@@ -264,6 +264,102 @@ __global__ __launch_bounds__(256) void k_post_final(const double *part, long nwc
     }
 }
 
+// ---- the config-4 objective J = w_vap (vapour volume) + w_fx (x-force on the
+// surface ranges, N) and its gradient over pv.  Each interior cell's vapour
+// volume rho y1 / rho_v vol and each surface face's x-force (the integrand of
+// k_post_faces) are differentiated through the EOS by one 4-lane dual pass
+// over (p, T, y1, y2); the velocity enters the wall shear linearly.
+__global__ __launch_bounds__(POST_WG) void k_obj_cells(DS S, const double *vol, double w_vap, double *pb, double *val,
+                                                       double *dpref)
+{
+    long t = (long)blockIdx.x * POST_WG + threadIdx.x;
+    if (t >= S.NC) return;
+    const BlkD &B = S.b[find_blk(S, t, 0)];
+    int i, j, k;
+    cell_of(B, t - B.c0, i, j, k);
+    const long g = gidx(B, i, j, k);
+    using T = Dn<4>;
+    T d[ND], full[18];
+    mixture_eos<T>(T::seed(S.pv[g] + S.P.pref, 0), T::seed(S.pv[4 * S.NG + g], 1), T::seed(S.pv[5 * S.NG + g], 2),
+                   T::seed(S.pv[6 * S.NG + g], 3), d, false, full);
+    const double w = vol[widx(B, i, j, k)];
+    const T a = full[0] * T::seed(S.pv[5 * S.NG + g], 2) / full[3] * w;
+    val[t] = w_vap * a.v;
+    dpref[t] = w_vap * a.d[0];
+    for (int n = 0; n < S.npv; n++) pb[(long)n * S.NG + g] = 0.0;
+    pb[g] = w_vap * a.d[0];
+    pb[4 * S.NG + g] = w_vap * a.d[1];
+    pb[5 * S.NG + g] = w_vap * a.d[2];
+    pb[6 * S.NG + g] = w_vap * a.d[3];
+}
+
+// the x-force of the faces of one surface range; adds its gradient to the
+// adjacent interior cell (a range holds each cell once, so no two threads of
+// a launch touch the same cell)
+__global__ __launch_bounds__(POST_WG) void k_obj_faces(DS S, FSeg s, const double *xvert, long nv, long v0b,
+                                                       const double *xc, double w_fx, double *pb, double *val,
+                                                       double *dpref)
+{
+    long r = (long)blockIdx.x * POST_WG + threadIdx.x;
+    if (r >= s.n) return;
+    const BlkD &B = S.b[s.b];
+    const int n0 = s.hi[0] - s.lo[0] + 1, n1 = s.hi[1] - s.lo[1] + 1;
+    const int p[3] = {s.lo[0] + (int)(r % n0), s.lo[1] + (int)((r / n0) % n1), s.lo[2] + (int)(r / ((long)n0 * n1))};
+    const int dim[3] = {B.I, B.J, B.K};
+    const int ax = s.ax;
+    int fi[3] = {p[0], p[1], p[2]}, ci[3] = {p[0], p[1], p[2]};
+    fi[ax] = s.side ? dim[ax] : 1;
+    ci[ax] = s.side ? dim[ax] : 2;
+    const int vplane = s.side ? dim[ax] + 1 : 2;
+    double cxv[3];
+    face_vec(S, B, ax, fi[0], fi[1], fi[2], cxv);
+    if (s.side) for (int c = 0; c < 3; c++) cxv[c] = -cxv[c];
+    const int t1 = ax == 0 ? 1 : 0, t2 = ax == 2 ? 1 : 2;
+    const XB X{xvert + v0b, nv, B.I, B.J, B.K};
+    double x[3];
+    for (int c = 0; c < 3; c++) {
+        double acc = 0.0;
+        for (int m = 0; m < 4; m++) {
+            int u[3];
+            u[ax] = vplane;
+            u[t1] = p[t1] + (m & 1);
+            u[t2] = p[t2] + (m >> 1);
+            const double a = xv(X, c, u[0], u[1], u[2]);
+            acc = m == 0 ? a : acc + a;
+        }
+        x[c] = 0.25 * acc;
+    }
+    const long g = gidx(B, ci[0], ci[1], ci[2]);
+    const long w = widx(B, ci[0], ci[1], ci[2]);
+    const double pg = S.pv[g], u = S.pv[S.NG + g], vv = S.pv[2 * S.NG + g], ww = S.pv[3 * S.NG + g];
+    const double dl = cxv[0] * cxv[0] + cxv[1] * cxv[1] + cxv[2] * cxv[2];
+    const double vel = cxv[0] * u + cxv[1] * vv + cxv[2] * ww;
+    const double gx = xc[w], gy = xc[S.NW + w], gz = xc[2 * S.NW + w];
+    const double dist = sqrt((gx - x[0]) * (gx - x[0]) + (gy - x[1]) * (gy - x[1]) + (gz - x[2]) * (gz - x[2]));
+    const double sdl = sqrt(dl);
+    const double ut = u - cxv[0] * vel / dl;
+    using T = Dn<4>;
+    T vis(0.0);
+    if (S.P.iturb >= -2) {
+        T d[ND];
+        mixture_eos<T>(T::seed(pg + S.P.pref, 0), T::seed(S.pv[4 * S.NG + g], 1), T::seed(S.pv[5 * S.NG + g], 2),
+                       T::seed(S.pv[6 * S.NG + g], 3), d, true);
+        vis = d[D_MU];
+        if (S.P.iturb == 0) vis = vis + S.mut[g];
+    }
+    const T fx = vis * ut / dist * sdl;
+    val[r] = w_fx * (-pg * cxv[0] + fx.v);
+    dpref[r] = w_fx * fx.d[0];
+    const double c = vis.v / dist * sdl;       // d(shear)/d(u, v, w) = c d(ut)/d(u, v, w)
+    pb[g] += w_fx * (-cxv[0] + fx.d[0]);
+    pb[S.NG + g] += w_fx * c * (1.0 - cxv[0] * cxv[0] / dl);
+    pb[2 * S.NG + g] += w_fx * c * (-cxv[0] * cxv[1] / dl);
+    pb[3 * S.NG + g] += w_fx * c * (-cxv[0] * cxv[2] / dl);
+    pb[4 * S.NG + g] += w_fx * fx.d[1];
+    pb[5 * S.NG + g] += w_fx * fx.d[2];
+    pb[6 * S.NG + g] += w_fx * fx.d[3];
+}
+
 // ---- restart file records of one block (src/variable_module.f90:126-178)
 struct Rec {
     long pv, emut, qq, rcav;
@@ -458,5 +554,6 @@ int cav3d_import(cav3d_ctx *c, const char *path, int *nps, int *nts, double *tim
     if (timeprev) *timeprev = tp;
     return check_launch("cav3d_import");
 }
+
 
 }  // extern "C"

@@ -31,6 +31,7 @@ SURFACE = 0x100   # CAV3D_BC_SURFACE: OR'ed into a kind, marks the force-integra
 EXPORTS = [
     "cav3d_create", "cav3d_set_metrics", "cav3d_set_grid", "cav3d_build_metrics", "cav3d_get_metrics",
     "cav3d_integrals", "cav3d_integrals_noalloc", "cav3d_export", "cav3d_import", "cav3d_rhs_vjp",
+    "cav3d_rhs_vjp_params", "cav3d_objective",
     "cav3d_put_state", "cav3d_get_state", "cav3d_timeinteg",
     "cav3d_setbc", "cav3d_calrhs", "cav3d_timestep", "cav3d_residual", "cav3d_unsteady_shift",
     "cav3d_eos", "cav3d_eos_host", "cav3d_profile", "cav3d_synchronize", "cav3d_last_error",
@@ -78,6 +79,8 @@ def lib(build_if_missing: bool = True):
     L.cav3d_export.argtypes = [C.c_void_p, C.c_char_p, C.c_int, C.c_int, C.c_double]
     L.cav3d_import.argtypes = [C.c_void_p, C.c_char_p, _I, _I, _D]
     L.cav3d_rhs_vjp.argtypes = [C.c_void_p, _D, _D, _D]
+    L.cav3d_rhs_vjp_params.argtypes = [C.c_void_p, _D, _D]
+    L.cav3d_objective.argtypes = [C.c_void_p, C.c_double, C.c_double, _D, _D, _D]
     L.cav3d_put_state.argtypes = [C.c_void_p, C.c_int, _D, _D, _D, _D]
     L.cav3d_get_state.argtypes = [C.c_void_p, C.c_int, _D, _D, _D, _D]
     L.cav3d_timeinteg.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, _D]
@@ -299,6 +302,51 @@ class Solver:
         sc = np.zeros(2)
         _check(lib().cav3d_rhs_vjp(self.h, _p(rb), _p(pb), _p(sc)))
         return np.split(pb, np.cumsum(ng)[:-1]), sc
+
+    def param_vjp(self, res_bars):
+        """d(res_bar . res)/d(run parameters) with the ghost layers refilled by
+        the boundary fill (cav3d_rhs_vjp_params), in the order of
+        cases.PARAM_KEYS: p_ref, u_ref, dp_ref, t_ref, bc_pv[0..8],
+        bc_pressure.  The current ghost layers must be the boundary fill of
+        the interior (call setbc first)."""
+        rb = np.ascontiguousarray(np.concatenate([np.asarray(r, dtype=np.float64).ravel() for r in res_bars]))
+        out = np.zeros(14)
+        _check(lib().cav3d_rhs_vjp_params(self.h, _p(rb), _p(out)))
+        return out
+
+    def objective(self, w_vap=1.0, w_fx=1.0):
+        """J = w_vap (vapour volume) + w_fx (x-force on the surface ranges, N),
+        dJ/dpv per block (pv layout, ghosted) and dJ/d(p_ref) (cav3d_objective)."""
+        ng = [self.npv * (b.imax + 5) * (b.jmax + 5) * (b.kmax + 5) for b in self.case.blocks]
+        pb = np.zeros(sum(ng))
+        J, dp = C.c_double(), C.c_double()
+        _check(lib().cav3d_objective(self.h, w_vap, w_fx, C.byref(J), _p(pb), C.byref(dp)))
+        return J.value, np.split(pb, np.cumsum(ng)[:-1]), dp.value
+
+    def put_with_eos(self, pvs, muts=None, qqs=None):
+        """Load ghosted states pvs with the cell properties from the device EOS
+        and the eddy viscosity muts / BDF history qqs as given."""
+        pref = self.case.params["pref"]
+        for ib, pv in enumerate(pvs):
+            a = np.asarray(pv, dtype=np.float64).reshape(-1, self.npv)
+            dv, t2 = self.eos(a[:, 0] + pref, a[:, 4], a[:, 5], a[:, 6])
+            tv = None
+            if self.ntv:
+                tv = np.zeros((a.shape[0], self.ntv))
+                tv[:, :2] = t2
+                if self.ntv == 3:
+                    tv[:, 2] = muts[ib]
+            qq = None if qqs is None else qqs[ib]
+            self.put_state(ib, np.ascontiguousarray(a.ravel()), np.ascontiguousarray(dv.ravel()),
+                           None if tv is None else np.ascontiguousarray(tv.ravel()), qq)
+
+    def residual_after_setbc(self, pvs, muts=None, qqs=None):
+        """Every block's residual for the states pvs with the ghost layers
+        refilled by the boundary fill (the map d/d parameter differentiates),
+        cell properties from the EOS, eddy viscosity and BDF history as given."""
+        self.residual_of(pvs, muts, qqs)
+        self.setbc()
+        return [self.calrhs(ib)[0] for ib in range(len(pvs))]
 
     def frozen(self):
         """The parts of the current state the adjoint's map holds fixed: the

@@ -184,6 +184,30 @@ int cav3d_import(cav3d_ctx *ctx, const char *path, int *nps, int *nts, double *t
  * Deterministic (fixed-order gathers, no atomics). */
 int cav3d_rhs_vjp(cav3d_ctx *ctx, const double *res_bar, double *pv_bar, double *scal_bar);
 
+/* Derivatives of res_bar . res with respect to the run parameters the
+ * residual reads, with the ghost layers refilled by the boundary fill
+ * (cav3d_setbc) from the same interior.  No counterpart in the reference.
+ *   par_bar  out, 14 values: [0] p_ref (at fixed gauge pv), [1] u_ref,
+ *            [2] dp_ref, [3] t_ref (the mass-transfer scales), [4..12]
+ *            bc_pv[0..8], [13] bc_pressure (the boundary state)
+ * The ghost layers must hold the boundary fill of the interior when it is
+ * called; they hold it again on return.  The total derivative with respect
+ * to a quantity the parameters are functions of (a cavitation number, an
+ * inflow speed) is par_bar . d(parameters)/d(quantity). */
+int cav3d_rhs_vjp_params(cav3d_ctx *ctx, const double *res_bar, double *par_bar);
+
+/* The steady-adjoint objective of the cavitating channel and its gradient:
+ *   J = w_vap * (vapour volume: sum over interior cells of rho y1/rho_v vol)
+ *     + w_fx  * (x-force on the CAV3D_BC_SURFACE ranges in N: the gauge
+ *                pressure and wall-shear integrands of clcd_writing,
+ *                src/postvariable_module.f90:507-633, unscaled)
+ *   pv_bar   out (may be NULL): dJ/dpv in the pv layout, blocks concatenated
+ *            (the eddy viscosity held fixed; ghost entries zero)
+ *   dJ_dpref out (may be NULL): dJ/d(p_ref) at fixed gauge pv
+ * No counterpart in the reference.  The force needs the vertex
+ * coordinates (cav3d_set_grid) when w_fx != 0. */
+int cav3d_objective(cav3d_ctx *ctx, double w_vap, double w_fx, double *J, double *pv_bar, double *dJ_dpref);
+
 /* pv(npv,-1:imax+3,-1:jmax+3,-1:kmax+3), dv(18,...), tv(ntv,...),
  * qq(npv,2,2:imax,2:jmax,2:kmax).  dv/tv may be NULL: then dv and tv(1:2)
  * are evaluated from pv on the device and tv(3) is set to emutref. qq may be
