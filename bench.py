@@ -7,7 +7,6 @@ update with EOS, eddy viscosity, norms) over all cells, with the state
 resident in HBM.  Prints one JSON line.
 """
 import argparse
-import subprocess
 import json
 import os
 import sys
@@ -172,6 +171,54 @@ def mixed_precision_run(pk, case, args, ms64):
     return out
 
 
+def host_cpu():
+    """CPU model, logical CPUs this process may use, and whether SMT is on."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    smt = None
+    try:
+        smt = open("/sys/devices/system/cpu/smt/active").read().strip() == "1"
+    except OSError:
+        pass
+    return {"model": model, "logical_cpus_usable": len(os.sched_getaffinity(0)), "smt_active": smt}
+
+
+def cpu_baseline(case, size):
+    """The oracle (FP64 C restatement, OpenMP over cells and hyperplanes) on
+    the same case on the host: one iteration as a warm-up, then the best of
+    several timed iterations, with one thread and with every usable core
+    (at most 16, the host share of one GPU)."""
+    from oracle import oracle as orc
+    usable = len(os.sched_getaffinity(0))
+    out = {"unit": "Mcell-iter/s", "kind": "port", "host": host_cpu(), "runs": {}}
+    for threads, reps in ((min(16, usable), 5), (1, 2)):
+        orc.set_threads(threads)
+        o = orc.Oracle(case)
+        for ib, st in enumerate(case.init):
+            o.put_state(ib, *st)
+        times = []
+        for it in range(reps + 1):
+            c0 = time.perf_counter()
+            o.timeinteg(1, it + 1, 0.0)
+            times.append(time.perf_counter() - c0)
+        o.close()
+        best = min(times[1:])
+        out["runs"][str(threads)] = {"value": round(case.ncell / best / 1e6, 5), "best_s": round(best, 4),
+                                     "timed_iterations": reps}
+    top = str(min(16, usable))
+    out["value"] = out["runs"][top]["value"]
+    out["cores"] = int(top)
+    out["sample"] = (f"the same {size} case, FP64 C restatement of the step; one warm-up iteration then the "
+                     f"best of 5 ({top} threads) and of 2 (1 thread) timed iterations")
+    return out
+
+
 def adjoint_cost(pk, s, case, reps=3):
     """Device time of one discrete-adjoint product (cav3d_rhs_vjp) against one
     evaluation of the residual it differentiates (cav3d_calrhs), on the
@@ -192,15 +239,187 @@ def adjoint_cost(pk, s, case, reps=3):
             "vjp_mcell_per_s": round(case.ncell / (v * 1e-3) / 1e6, 3), "arithmetic": "f64"}
 
 
+def alpha_field(pk, case, pv):
+    """Vapour volume fraction rho y1 / rho_v of the interior cells (the vof
+    field of src/postvariable_module.f90:282-295) from the host EOS."""
+    a = pv.reshape(-1, case.npv)
+    dv, _ = pk.eos_host(a[:, 0] + case.params["pref"], a[:, 4], a[:, 5], a[:, 6])
+    with np.errstate(invalid="ignore", divide="ignore"):
+        al = np.where(dv[:, 3] > 0, dv[:, 0] * a[:, 5] / dv[:, 3], 0.0)
+    return al
+
+
+def mixed_study(pk, args):
+    """Mixed precision against FP64 on the 128x64x64 cavitating bump over
+    `iters` iterations from the same state: residual histories, the
+    per-field difference of the final states (p, u, v, w, T, alpha, k,
+    omega) relative to each field's largest magnitude, the integrated vapour
+    volume, and the wall time each mode needs to reach the residual level
+    both reach (the higher of the two final l2 of du).  With --oracle-iters
+    M (M <= iters), the FP64 and the mixed states after M iterations are
+    also compared with the oracle's (the FP64 C restatement of the step) per
+    field, alpha and vapour volume included."""
+    if args.oracle_iters > args.iters:
+        sys.exit("--oracle-iters must not exceed --iters")
+    ni, nj, nk = (int(x) for x in args.size.split("x"))
+    case = pk.cases.config2(ni, nj, nk)
+    case.params["cav_clip"] = 1
+    p = case.params
+    b = case.blocks[0]
+    inner = (slice(3, b.kmax + 2), slice(3, b.jmax + 2), slice(3, b.imax + 2))
+    runs = {}
+    for prec, name in ((0, "fp64"), (1, "mixed")):
+        s = pk.Solver(case, precision=prec)
+        s.set_grid()
+        for ib, st in enumerate(case.init):
+            s.put_state(ib, *st)
+        hist, times, snap = [], [], None
+        s.synchronize()
+        t0 = time.perf_counter()
+        for it in range(args.iters):
+            s.timeinteg(1, it + 1, 0.0, want_time=False)
+            if (it + 1) % 10 == 0:
+                l2 = s.residual()[0]
+                times.append(time.perf_counter() - t0)
+                hist.append([it + 1, float(l2[0]), float(l2[1])])
+            if (it + 1) % 100 == 0:
+                print(f"# {name} iteration {it + 1}: l2 dp {hist[-1][1]:.3e} du {hist[-1][2]:.3e}", file=sys.stderr,
+                      flush=True)
+            if it + 1 == args.oracle_iters:
+                ts = time.perf_counter()
+                snap = s.get_state(0)[0].copy()
+                t0 += time.perf_counter() - ts          # the copy is not part of the run
+        pv = s.get_state(0)[0]
+        out = s.integrals(p["rhoref"], p["uref"], 0.01, 0.0)
+        s.close()
+        runs[name] = {"hist": hist, "times": times, "pv": pv, "vap": float(out[0]), "snap": snap}
+    P64 = runs["fp64"]["pv"].reshape(b.kmax + 5, b.jmax + 5, b.imax + 5, case.npv)[inner].reshape(-1, case.npv)
+    P32 = runs["mixed"]["pv"].reshape(b.kmax + 5, b.jmax + 5, b.imax + 5, case.npv)[inner].reshape(-1, case.npv)
+    err = {}
+    for n, f in enumerate(FIELDS):
+        if f in ("y1", "y2"):
+            continue
+        sc = float(np.max(np.abs(P64[:, n])))
+        err[f] = float(np.max(np.abs(P64[:, n] - P32[:, n])) / sc) if sc > 0 else 0.0
+    a64, a32 = alpha_field(pk, case, P64.ravel()), alpha_field(pk, case, P32.ravel())
+    err["alpha"] = float(np.max(np.abs(a64 - a32)) / max(float(np.max(np.abs(a64))), 1e-300))
+    target = max(runs["fp64"]["hist"][-1][2], runs["mixed"]["hist"][-1][2])
+    def time_to(name):
+        for (it, _, du), t in zip(runs[name]["hist"], runs[name]["times"]):
+            if du <= target:
+                return it, t
+        return None, None
+    i64, t64 = time_to("fp64")
+    i32, t32 = time_to("mixed")
+    res = {"iterations": args.iters, "field_max_rel_diff": {k: float(f"{v:.3e}") for k, v in err.items()},
+           "vapour_volume": {"fp64": runs["fp64"]["vap"], "mixed": runs["mixed"]["vap"],
+                             "rel_diff": abs(runs["mixed"]["vap"] - runs["fp64"]["vap"]) / abs(runs["fp64"]["vap"])},
+           "matched_residual": {"l2_du_target": target, "fp64": {"iterations": i64, "seconds": t64},
+                                "mixed": {"iterations": i32, "seconds": t32}},
+           "history_every_10": {"fp64": runs["fp64"]["hist"], "mixed": runs["mixed"]["hist"]}}
+    if args.oracle_iters:
+        from oracle import oracle as orc
+        orc.set_threads(min(16, len(os.sched_getaffinity(0))))
+        o = orc.Oracle(case)
+        for ib, st in enumerate(case.init):
+            o.put_state(ib, *st)
+        t0 = time.perf_counter()
+        for it in range(args.oracle_iters):
+            o.setqres()
+            o.timeinteg(1, it + 1, 0.0)
+            if (it + 1) % 50 == 0:
+                print(f"# oracle iteration {it + 1} ({time.perf_counter() - t0:.0f} s)", file=sys.stderr, flush=True)
+        po = o.get_state(0)[0]
+        vo = float(o.integrals(p["rhoref"], p["uref"], 0.01, 0.0)[0])
+        Po = po.reshape(b.kmax + 5, b.jmax + 5, b.imax + 5, case.npv)[inner].reshape(-1, case.npv)
+        ao = alpha_field(pk, case, Po.ravel())
+        res["vs_oracle_after"] = args.oracle_iters
+        for name in ("fp64", "mixed"):
+            Pg = runs[name]["snap"].reshape(b.kmax + 5, b.jmax + 5, b.imax + 5, case.npv)[inner].reshape(-1, case.npv)
+            d = {f: float(f"{float(np.max(np.abs(Pg[:, n] - Po[:, n])) / max(float(np.max(np.abs(Po[:, n]))), 1e-300)):.3e}")
+                 for n, f in enumerate(FIELDS[:case.npv]) if f != "y2"}
+            ag = alpha_field(pk, case, Pg.ravel())
+            d["alpha"] = float(f"{float(np.max(np.abs(ag - ao)) / max(float(np.max(np.abs(ao))), 1e-300)):.3e}")
+            res[f"{name}_vs_oracle_max_rel_diff"] = d
+        res["oracle_vapour_volume"] = vo
+        if args.oracle_iters == args.iters:
+            for name in ("fp64", "mixed"):
+                res["vapour_volume"][f"{name}_vs_oracle_rel_diff"] = abs(runs[name]["vap"] - vo) / abs(vo)
+    print(json.dumps(res), flush=True)
+
+
+def naca_run(pk, args):
+    """The NACA 0015 C-grid workload: 4 blocks of 128x160x96 cells (7.9 M)
+    resident on the GPU, metrics built on the device from the vertices.
+    For FP64 and for the mixed mode from the same initial state: `warmup`
+    untimed and `steps` timed iterations (Mcell-iter/s), then more iterations
+    up to `iters` in total with the residual norms every 25, the vapour
+    volume, lift and drag coefficients every 250 and at the end."""
+    ni, nj, nk = (int(x) for x in args.size.split("x"))
+    t0 = time.time()
+    case = pk.cases.config3(ni, nj, nk, cfl=args.cfl)
+    print(f"# case {case.name} ({case.ncell} cells) built in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    p = case.params
+    res = {}
+    for prec, name in ((0, "fp64"), (1, "mixed")):
+        s = pk.Solver(case, precision=prec, metrics="device")
+        for ib, st in enumerate(case.init):
+            s.put_state(ib, *st)
+        first = None
+        hist, ihist = [], []
+        n = max(args.iters, args.warmup + args.steps)
+        t1 = None
+        for it in range(n):
+            if it == args.warmup:
+                s.synchronize()
+                t1 = time.perf_counter()
+            s.timeinteg(1, it + 1, 0.0, want_time=False)
+            if it == args.warmup + args.steps - 1:
+                s.synchronize()
+                ms = (time.perf_counter() - t1) * 1e3 / args.steps
+            if it == 0 or (it + 1) % 25 == 0 or it == n - 1:
+                l2 = s.residual()[0]
+                first = l2 if first is None else first
+                hist.append([it + 1, float(l2[0]), float(l2[1])])
+                print(f"# {name} iteration {it + 1}: l2 dp {l2[0]:.3e} du {l2[1]:.3e}", file=sys.stderr, flush=True)
+                if not np.all(np.isfinite(l2)):
+                    break
+            if (it + 1) % 250 == 0:
+                q = s.integrals(p["rhoref"], p["uref"], p["area"], np.radians(p["aoa"]))
+                ihist.append([it + 1, float(q[0]), float(q[8]), float(q[9])])
+        out = s.integrals(p["rhoref"], p["uref"], p["area"], np.radians(p["aoa"]))
+        s.close()
+        last = hist[-1]
+        res[name] = {"ms_per_step": round(ms, 3), "value": round(case.ncell / (ms * 1e-3) / 1e6, 3),
+                     "iterations": last[0], "l2_drop_p": last[1] / float(first[0]), "l2_drop_u": last[2] / float(first[1]),
+                     "vapour_volume": float(out[0]), "cl": float(out[8]), "cd": float(out[9]), "history": hist,
+                     "integrals_every_250": ihist}
+    a, b = res["fp64"], res["mixed"]
+    rel = {k: abs(b[k] - a[k]) / max(abs(a[k]), 1e-300) for k in ("vapour_volume", "cl", "cd")}
+    print(json.dumps({"metric": "Mcell-iter/s (pseudo-time iteration, NACA 0015 C-grid)", "unit": "Mcell-iter/s",
+                      "config": {"workload": f"cavitating NACA 0015 C-grid, 4 blocks of {args.size} cells, k-omega "
+                                             f"SST, AUSM+-up, MLP, Merkle, AoA 8 deg, sigma 1.0, CFL {args.cfl}",
+                                 "cells": case.ncell},
+                      "fp64": a, "mixed": b, "mixed_vs_fp64_rel_diff": rel,
+                      "speedup_mixed": round(a["ms_per_step"] / b["ms_per_step"], 4)}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--size", default="128x64x64")
+    ap.add_argument("--size", default=None, help="cells per block, NIxNJxNK (default 128x64x64; 128x160x96 "
+                                                 "with --config 3)")
+    ap.add_argument("--config", type=int, default=2, choices=[2, 3],
+                    help="2: the cavitating bump channel (the default line); 3: the NACA 0015 C-grid run")
+    ap.add_argument("--cfl", type=float, default=50.0, help="CFL of the --config 3 run")
+    ap.add_argument("--iters", type=int, default=0, help="total iterations of the --config 3 run / --mixed-study")
+    ap.add_argument("--mixed-study", action="store_true",
+                    help="run only the mixed-vs-FP64 study on the config-2 workload for --iters iterations")
+    ap.add_argument("--oracle-iters", type=int, default=0, help="with --mixed-study: compare FP64 with the oracle "
+                                                                  "after this many iterations")
     ap.add_argument("--cpu-baseline", type=int, default=1, help="1: time the oracle on the host")
-    ap.add_argument("--adjoint-only", action="store_true",
-                    help="child mode: print the adjoint/primal device-time line for --size after --warmup steps")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="run only a plain device-to-device copy of MIB MiB (5 times): the calibration "
                          "kernel for the FETCH_SIZE / WRITE_SIZE counters")
@@ -210,6 +429,8 @@ def main():
                     help="write the HBM-traffic summary from four rocprofv3 --pmc output folders "
                          "(bench FETCH_SIZE, bench WRITE_SIZE, copy probe FETCH_SIZE, copy probe WRITE_SIZE)")
     args = ap.parse_args()
+    if args.size is None:
+        args.size = "128x160x96" if args.config == 3 else "128x64x64"
     if args.pmc_summary:
         pmc_summary(args, *args.pmc_summary)
         return
@@ -239,19 +460,13 @@ def main():
 
     import importlib
     pk = importlib.import_module("3dcode_amd")
-    ni, nj, nk = (int(x) for x in args.size.split("x"))
-    if args.adjoint_only:
-        case = pk.cases.config2(ni, nj, nk)
-        case.params["cav_clip"] = 1
-        s = pk.Solver(case)
-        for ib, st in enumerate(case.init):
-            s.put_state(ib, *st)
-        # the initial state with its ghost layers filled: no pseudo-time step
-        # (the parent process holds the device's cooperative-launch queue)
-        s.setbc()
-        print(json.dumps(adjoint_cost(pk, s, case)), flush=True)
-        s.close()
+    if args.config == 3:
+        naca_run(pk, args)
         return
+    if args.mixed_study:
+        mixed_study(pk, args)
+        return
+    ni, nj, nk = (int(x) for x in args.size.split("x"))
     t0 = time.time()
     case = pk.cases.config2(ni, nj, nk)
     case.params["cav_clip"] = 1
@@ -283,7 +498,6 @@ def main():
         acc += s.profile(True)[:8]
     s.profile(False)
     phase = acc / nprof
-    s.close()
 
     # dominant kernel: the forward LU-SGS sweep, one launch over all cells.  Its
     # algorithmic bytes are the forward-sweep pass of the compulsory-traffic
@@ -312,30 +526,17 @@ def main():
     grid_ms = grid_metrics_ms(pk, case, 3)
     mixed = mixed_precision_run(pk, case, args, ms)
 
-    cpu = None
-    if args.cpu_baseline:
-        from oracle import oracle as orc
-        threads = min(16, os.cpu_count() or 1)
-        orc.set_threads(threads)
-        o = orc.Oracle(case)
-        for ib, st in enumerate(case.init):
-            o.put_state(ib, *st)
-        c0 = time.perf_counter()
-        o.timeinteg(1, 1, 0.0)
-        c1 = time.perf_counter()
-        cpu = {"value": round(ncell / (c1 - c0) / 1e6, 5), "unit": "Mcell-iter/s", "cores": threads,
-               "kind": "port", "sample": f"one iteration of the same {args.size} case, FP64 C restatement, OpenMP"}
+    cpu = cpu_baseline(case, args.size) if args.cpu_baseline else None
 
-    # the discrete adjoint's cost, measured in a child process (a fresh context
-    # and a fresh HIP runtime; a failure there cannot take this run down)
+    # the discrete adjoint's cost on the stepped state the timed run left
+    # (warmup + steps + profiled iterations), its ghost layers refilled
     try:
-        r = subprocess.run([sys.executable, "-u", os.path.abspath(__file__), "--adjoint-only", "--size", args.size,
-                            "--warmup", str(args.warmup)], capture_output=True, text=True, timeout=300)
-        lines = [x for x in r.stdout.splitlines() if x.startswith("{")]
-        adjoint = json.loads(lines[-1]) if r.returncode == 0 and lines else \
-            {"error": f"rc {r.returncode}: {(r.stderr or '')[-300:]}"}
-    except Exception as e:      # noqa: BLE001 - reported in the JSON line
+        s.setbc()
+        adjoint = adjoint_cost(pk, s, case)
+        adjoint["state"] = f"after {args.warmup + args.steps + nprof} pseudo-time iterations"
+    except pk.Cav3dError as e:
         adjoint = {"error": str(e)[:300]}
+    s.close()
 
     out = {
         "metric": "Mcell-iter/s (pseudo-time iteration: residual + LU-SGS, FP64)",

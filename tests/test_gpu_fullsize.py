@@ -3,6 +3,13 @@ iteration of the laminar bump channel at 64x32x32 cells (Roe, MUSCL
 minmod) and of the cavitating bump at 128x64x64 cells (SST, AUSM+-up,
 MLP, Merkle) under both clip modes, from the same seeded state.
 
+The as-written clip (src/cav_module.f90:151-152 clips the temperature slot
+to [0, 1]) drives the temperature negative in 2679 cells within that one
+iteration, the reference's own divergence; the transport properties of a
+negative temperature are NaN.  There the NaN cells must be the same cells on
+both sides and every finite entry is compared; everywhere else a non-finite
+entry is a failure.
+
 Bar (FP64 path): 1e-11 relative per cell and equation, with elements below
 1e-8 of their field's largest magnitude compared against that floor; the
 residual (a difference of fluxes) against a floor of 1e-6 of its field's
@@ -19,7 +26,7 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-11
 
 
-def _run(pkg, orc, case, check_rhs=True):
+def _run(pkg, orc, case, check_rhs=True, nan_tv=False):
     orc.set_threads(min(16, os.cpu_count() or 1))
     print(f"case {case.name}: {case.ncell} cells, building both sides", flush=True)
     s, o = pair(pkg, orc, case)
@@ -40,7 +47,14 @@ def _run(pkg, orc, case, check_rhs=True):
     pg, dg, tvg, _ = s.get_state(0)
     po, do, tvo, _ = o.get_state(0)
     for what, g, r, m in (("pv", pg, po, npv), ("dv", dg, do, 18), ("tv", tvg, tvo, case.ntv)):
-        e = rel_err(interior(case, 0, g, m), interior(case, 0, r, m))
+        g, r = interior(case, 0, g, m), interior(case, 0, r, m)
+        if what == "tv" and nan_tv:
+            bad = ~np.isfinite(r)
+            print(f"  tv non-finite entries {bad.sum()}", flush=True)
+            assert bad.any() and np.array_equal(bad, ~np.isfinite(g))
+            keep = ~bad.any(axis=1)
+            g, r = g[keep], r[keep]
+        e = rel_err(g, r)
         print(f"  {what} rel err max {e.max():.2e}", flush=True)
         assert np.all(e < TOL), (what, e)
     l2g, ijkg, rmg = s.residual()
@@ -60,4 +74,4 @@ def test_config1_64x32x32_one_iteration(pkg, orc):
 def test_config2_128x64x64_one_iteration(pkg, orc, clip):
     case = pkg.cases.config2()
     case.params["cav_clip"] = clip
-    _run(pkg, orc, case, check_rhs=(clip == 1))
+    _run(pkg, orc, case, check_rhs=(clip == 1), nan_tv=(clip == 0))
