@@ -631,36 +631,51 @@ int cav3d_rhs_vjp_params(cav3d_ctx *c, const double *res_bar, double *par_bar)
     // affine function of the boundary state bc_pv, bc_pressure (the mirror,
     // surface and neighbour cells and the in/out mode do not move), so the
     // product of the ghost-cell bars with its derivative is a central
-    // difference of the fill, exact up to rounding
+    // difference of the fill, exact up to rounding.  The characteristic
+    // fills also read u_ref (the preconditioned speed of sound at the
+    // surface cell), smoothly where the surface cell's normal speed is not
+    // u_ref: the eleventh difference.  The fills also read the surface
+    // cells' density and speed of sound, which follow p_ref through the EOS:
+    // the twelfth difference refills from the interior properties
+    // re-evaluated at p_ref +- h.  The fills evaluate the ghost EOS at
+    // bc_pv[0] + gauge pressure, so p_ref is exact together with bc_pv[0]
+    // when the two are equal (a change of p_ref that moves the boundary
+    // state's reference pressure with it).
     double *save_pv = c->a_save, *save_dc = save_pv + (long)npv * S.NG, *save_mut = save_dc + (long)ND * S.NG;
     double *plus = save_mut + S.NG;
     HIPCHK(hipMemcpyAsync(save_pv, S.pv, (long)npv * S.NG * sizeof(double), hipMemcpyDeviceToDevice, c->st));
     HIPCHK(hipMemcpyAsync(save_dc, S.dc, (long)ND * S.NG * sizeof(double), hipMemcpyDeviceToDevice, c->st));
     if (S.mut) HIPCHK(hipMemcpyAsync(save_mut, S.mut, S.NG * sizeof(double), hipMemcpyDeviceToDevice, c->st));
     const Par P0 = S.P;
-    double hs[10];
-    for (int m = 0; m < 10; m++) {
-        double &theta = (m < 9) ? S.P.bc_pv[m] : S.P.bc_pressure;
+    double hs[12], share_u = 0.0, share_p = 0.0;
+    for (int m = 0; m < 12; m++) {
+        double &theta = (m < 9) ? S.P.bc_pv[m] : (m == 9 ? S.P.bc_pressure : (m == 10 ? S.P.uref : S.P.pref));
         const double v0 = theta;
         const double h = v0 != 0.0 ? 1.0e-4 * fabs(v0) : 1.0e-6;
         hs[m] = h;
         theta = v0 + h;
+        if (m == 11) hipLaunchKernelGGL(k_eos_cells, dim3(nblk(S.NC, 256)), dim3(256), 0, c->st, S, 1, 0);
         launch_setbc(c);
         HIPCHK(hipMemcpyAsync(plus, S.pv, (long)npv * S.NG * sizeof(double), hipMemcpyDeviceToDevice, c->st));
         theta = v0 - h;
+        if (m == 11) hipLaunchKernelGGL(k_eos_cells, dim3(nblk(S.NC, 256)), dim3(256), 0, c->st, S, 1, 0);
         launch_setbc(c);
         theta = v0;
         double *part = c->a_sum + 16 + (long)SUM_WG * (7 + (m % 8));
         hipLaunchKernelGGL(k_dotdiff_fixed, dim3(SUM_WG), dim3(256), 0, c->st, (const double *)c->a_pb, (const double *)plus,
                            (const double *)S.pv, (long)npv * S.NG, 1.0 / (2.0 * h), part);
         hipLaunchKernelGGL(k_sum_fixed, dim3(1), dim3(256), 0, c->st, (const double *)part, (long)SUM_WG, c->a_sum + 7 + m % 8);
-        if (m == 7 || m == 9) {
+        if (m == 7 || m == 11) {
             // a_sum[7..14] hold up to eight results: copy them out before reuse
             double hm[8];
             HIPCHK(hipMemcpyAsync(hm, c->a_sum + 7, sizeof(hm), hipMemcpyDeviceToHost, c->st));
             HIPCHK(hipStreamSynchronize(c->st));
             const int first = m == 7 ? 0 : 8;
-            for (int q = first; q <= m; q++) par_bar[4 + q] = hm[q % 8];
+            for (int q = first; q <= m; q++) {
+                if (q < 10) par_bar[4 + q] = hm[q % 8];
+                else if (q == 10) share_u = hm[q % 8];
+                else share_p = hm[q % 8];
+            }
         }
     }
     S.P = P0;
@@ -670,8 +685,8 @@ int cav3d_rhs_vjp_params(cav3d_ctx *c, const double *res_bar, double *par_bar)
     double h[7];
     HIPCHK(hipMemcpyAsync(h, c->a_sum, sizeof(h), hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
-    par_bar[0] = h[0];
-    par_bar[1] = ((h[1] + h[2]) + h[3]) + h[4];
+    par_bar[0] = h[0] + share_p;
+    par_bar[1] = (((h[1] + h[2]) + h[3]) + h[4]) + share_u;
     par_bar[2] = h[5];
     par_bar[3] = h[6];
     return check_launch("cav3d_rhs_vjp_params");

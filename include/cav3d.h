@@ -190,6 +190,11 @@ int cav3d_rhs_vjp(cav3d_ctx *ctx, const double *res_bar, double *pv_bar, double 
  *   par_bar  out, 14 values: [0] p_ref (at fixed gauge pv), [1] u_ref,
  *            [2] dp_ref, [3] t_ref (the mass-transfer scales), [4..12]
  *            bc_pv[0..8], [13] bc_pressure (the boundary state)
+ * The characteristic boundary fills evaluate the ghost cells' EOS at
+ * bc_pv[0] plus the gauge pressure, where the residual's cells use p_ref: a
+ * case whose bc_pv[0] equals p_ref (the reference's reference state) gets
+ * the derivative of a change that moves both as par_bar[0] + par_bar[4];
+ * either one alone is not the derivative of the filled map.
  * The ghost layers must hold the boundary fill of the interior when it is
  * called; they hold it again on return.  The total derivative with respect
  * to a quantity the parameters are functions of (a cavitation number, an

@@ -446,6 +446,81 @@ def test_sigma_and_inflow_speed_gradients(pkg, name):
     s.close()
 
 
+def _with_key(case, key, value):
+    """A copy of the case (same grid and state) with one run parameter of
+    cases.PARAM_KEYS set to value; "pref" moves bc_pv[0] with it (the
+    boundary state's reference pressure, equal to p_ref in these cases)."""
+    c2 = copy.copy(case)
+    c2.params = dict(case.params)
+    bc = list(c2.params["bc_pv"])
+    if key.startswith("bc_pv"):
+        bc[int(key[5:])] = value
+    else:
+        c2.params[key] = value
+        if key == "pref":
+            bc[0] = value
+    c2.params["bc_pv"] = bc
+    return c2
+
+
+@pytest.mark.parametrize("name", ["sst_cav_ausm_mlp", "sst_cav_roe_first"])
+def test_parameter_partials(pkg, name):
+    """Each run parameter's partial d(res_bar . res)/d theta from
+    cav3d_rhs_vjp_params (ghost layers refilled by the boundary fill)
+    against central differences over contexts built with that parameter
+    shifted, on the state of test_sigma_and_inflow_speed_gradients.  p_ref
+    is moved together with bc_pv[0] (their partials added): the fills
+    evaluate the ghost EOS at bc_pv[0], so only that combination is the
+    derivative of the filled map.  u_ref and the boundary velocity bc_pv[1..3]
+    are not checked one by one: the characteristic fill reflects the
+    interior velocity about the boundary velocity, so at the inflow faces the
+    averaged speed sits at U_in = u_ref, on the switch max(|u|, u_ref) of the
+    preconditioning, where each partial alone is one-sided; moved together,
+    as U_in moves them, both branches have the same slope
+    (test_sigma_and_inflow_speed_gradients)."""
+    print(f"case {name}: run-parameter partials", flush=True)
+    case, s, _, fr = prepared(pkg, name)
+    nb = len(case.blocks)
+    pvs = [s.get_state(ib)[0].copy() for ib in range(nb)]
+    for pv in pvs:
+        pv.reshape(-1, case.npv)[:, 1:4] *= 0.95
+    rng = np.random.default_rng(zlib.crc32(("partials" + name).encode()))
+    rb = seeds(case, s.residual_after_setbc(pvs, fr["mut"], fr["qq"]), rng)
+    par = s.param_vjp(rb)
+    theta = pkg.cases.param_vector(case.params)
+    bad, settled = [], 0
+    for q, key in enumerate(pkg.cases.PARAM_KEYS):
+        if key in ("uref", "bc_pv0", "bc_pv1", "bc_pv2", "bc_pv3", "bc_pv5", "bc_pv6") or \
+                (key == "t_ref" and case.params["ncav"] == 1):
+            continue
+        adj_q = float(par[q] + par[4]) if key == "pref" else float(par[q])
+        v0 = float(theta[q])
+        scale = abs(v0) if v0 != 0.0 else 1.0
+
+        def diff(c, key=key, v0=v0, scale=scale):
+            h = 1e-6 * scale * c
+            out = []
+            for sg in (1.0, -1.0):
+                s2 = pkg.Solver(_with_key(case, key, v0 + sg * h))
+                try:
+                    out.append(s2.residual_after_setbc(pvs, fr["mut"], fr["qq"]))
+                finally:
+                    s2.close()
+            return sum(float(np.dot(w, a - b)) for w, a, b in zip(rb, *out)) / (2.0 * h)
+        # floor: 1e-3 of the largest parameter sensitivity |par theta|, in this
+        # parameter's units (a partial far below that is rounding-limited)
+        err = kink_aware(diff, adj_q, scale=1e-3 * float(np.abs(par * np.abs(theta)).max()) / scale)
+        print(f"  d/d{key}: adjoint {adj_q:.9e} rel err {err}", flush=True)
+        if err is None:
+            continue
+        settled += 1
+        if err >= TOL:
+            bad.append((key, adj_q, err))
+    assert not bad, bad
+    assert settled >= 4, settled
+    s.close()
+
+
 def test_objective_and_its_gradient(pkg):
     """The config-4 objective, vapour volume + x-force on the bump wall:
     its value against cav3d_integrals, dJ/dpv against central differences
