@@ -52,7 +52,10 @@ struct AD {
     double *cdb;            // cell-source property bars [ND][NC]
     double *csb;            // cell-source d(phi_c)/d(u_ref, dp_ref, t_ref) [3][NC] (the mass-transfer scales)
     double *pb;             // pv_bar [npv][NG]
+    double *mtb[3];         // eddy-viscosity bars of a face's two cells [2][NF] (dmut)
     unsigned long long mask;   // stencil points a face writes (bit ll-1)
+    int dmut;               // 1: the interior cells' eddy viscosity is the SST function of
+                            // the state (differentiated); 0: held fixed
 };
 
 __device__ __forceinline__ bool interior_cell(const BlkD &B, int i, int j, int k)
@@ -147,8 +150,12 @@ __global__ __launch_bounds__(128) void k_vface_vjp(DS S, AD A)
         g[n] = n < npv ? rbar(S, A, B, i, j, k, n) - rbar(S, A, B, ip, jp, kp, n) : 0.0;
         any = any || g[n] != 0.0;
     }
-    if (!any) return;
     const bool turb = P.iturb == 0;
+    const bool dm = A.dmut && turb;
+    if (!any) {
+        if (dm) { A.mtb[D][t] = 0.0; A.mtb[D][NF + t] = 0.0; }
+        return;
+    }
     const int nv = turb ? 6 : 4;
     const int vars[6] = {1, 2, 3, 4, 7, 8};
     auto xs = [&](int ll, int n) -> double { return S.pv[(long)n * S.NG + g9 + stencil_off<D>(B, ll)]; };
@@ -193,16 +200,18 @@ __global__ __launch_bounds__(128) void k_vface_vjp(DS S, AD A)
     face_vec(S, B, T, p1[0], p1[1], p1[2], t4[3]);
     const long wl = widx(B, i, j, k), wr = widx(B, ip, jp, kp);
     const double gl[2] = {S.vol[wl], S.ydn ? S.ydn[wl] : 1.0}, gr[2] = {S.vol[wr], S.ydn ? S.ydn[wr] : 1.0};
-    // z = {g3[nv][3], cav[ncv], muL, kapL, muR, kapR, [hTL, hTR, rhoL, rhoR]}
-    constexpr int NZ = 18 + 5 + 8;
+    // z = {g3[nv][3], cav[ncv], muL, kapL, muR, kapR, [hTL, hTR, rhoL, rhoR, [mutL, mutR]]}
+    constexpr int NZ = 18 + 5 + 8 + 2;
     const int og = 3 * nv, oc = og + ncv;
-    const int nz = oc + (turb ? 8 : 4);
+    const int nz = oc + (turb ? (dm ? 10 : 8) : 4);
     double zx[NZ], zb[NZ];
     for (int q = 0; q < nv; q++) for (int c = 0; c < 3; c++) zx[3 * q + c] = g3[q][c];
     for (int q = 0; q < ncv; q++) zx[og + q] = cav[q];
     zx[oc] = dl[D_MU]; zx[oc + 1] = dl[D_KAP]; zx[oc + 2] = dr[D_MU]; zx[oc + 3] = dr[D_KAP];
     if (turb) { zx[oc + 4] = dl[D_HT]; zx[oc + 5] = dr[D_HT]; zx[oc + 6] = dl[D_RHO]; zx[oc + 7] = dr[D_RHO]; }
-    visc_adj(P, npv, nv, ncv, nz, zx, n3, e4, t4, gl, gr, dl, dr, ml, mr, g, zb);
+    if (dm) { zx[oc + 8] = ml; zx[oc + 9] = mr; }
+    visc_adj(P, npv, nv, ncv, nz, zx, n3, e4, t4, gl, gr, dl, dr, ml, mr, dm ? oc + 8 : -1, g, zb);
+    if (dm) { A.mtb[D][t] = zb[oc + 8]; A.mtb[D][NF + t] = zb[oc + 9]; }
     // the differences and averages are linear in the stencil
     double *fb = A.fb[D];
     constexpr double C1[18] = {-1, -1, 0, 0, 1, 1, -2, -2, 0, 0, 2, 2, -1, -1, 0, 0, 1, 1};
@@ -261,13 +270,15 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     const bool sst = P.iturb == 0, cav_ = P.ncav >= 1, bdf = P.nsteady > 0;
     // z = {pc[npv], rho, h, mu, the Green-Gauss gradients of u, v, w, k, omega (15),
     //      the scalars the mass-transfer rates read: u_ref (Singhal), dp_ref
-    //      (Merkle, Kunz's evaporation), t_ref (Merkle, Kunz)}
-    constexpr int NZ = NPV + 3 + 15 + 3;
+    //      (Merkle, Kunz's evaporation), t_ref (Merkle, Kunz), [the eddy viscosity]}
+    constexpr int NZ = NPV + 3 + 15 + 3 + 1;
     const int od = npv, og = npv + 3;
     const int nz0 = (cav_ || bdf || sst) ? (sst ? og + 15 : og) : 0;
     const bool sc = nz0 && (P.ncav == 1 || P.ncav == 2 || P.ncav == 3);
     const int osc = nz0;                                   // u_ref, dp_ref, t_ref lanes
-    const int nz = sc ? nz0 + 3 : nz0;
+    const bool dm = A.dmut && sst;                         // the eddy viscosity is a state function
+    const int omu = sc ? nz0 + 3 : nz0;                    // its lane
+    const int nz = omu + (dm ? 1 : 0);
     double zx[NZ], zb[NZ];
     for (int q = 0; q < NZ; q++) zx[q] = zb[q] = 0.0;
     const int gvar[5] = {1, 2, 3, 7, 8};
@@ -292,6 +303,7 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         if (sc) { zx[osc] = P.uref; zx[osc + 1] = P.dp_ref; zx[osc + 2] = P.t_ref; }
         const double ydn = sst ? S.ydn[widx(B, i, j, k)] : 1.0;
         const double mut = S.mut ? S.mut[g] : 0.0;
+        if (dm) zx[omu] = mut;
         double q1[NPV], q2[NPV];
         for (int n = 0; n < NPV; n++) {
             q1[n] = (bdf && n < npv) ? S.qq[(long)n * S.NC + t] : 0.0;
@@ -330,8 +342,8 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
                 for (int v = 0; v < 5; v++)
                     for (int c = 0; c < 3; c++) gr[v][c] = in(og + 3 * v + c);
                 T src[2], it[4], oc;
-                sst_source_g<MODELS, T>(P, gr[0], gr[1], gr[2], gr[3], gr[4], pc[7], pc[8], f6, vol, ydn, dt, T(mut),
-                                     src, it, oc);
+                sst_source_g<MODELS, T>(P, gr[0], gr[1], gr[2], gr[3], gr[4], pc[7], pc[8], f6, vol, ydn, dt,
+                                     dm ? in(omu) : T(mut), src, it, oc);
                 phi = phi + rb[7] * src[0] + rb[8] * src[1];
             }
             if (bdf) {
@@ -349,6 +361,31 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
             }
             return phi;
         }, zb);
+        if (dm) {
+            // the cell's eddy-viscosity bar: its sources' share and its six
+            // viscous faces' (the left cell of its + faces, the right cell of
+            // its - faces), back to rho, mu, k, omega and the velocity
+            // gradients through mu_t = sst_eddy_g (src/eddy_module.f90:135-187)
+            double mb = zb[omu];
+            for (int D = 0; D < 3; D++) {
+                mb += A.mtb[D][fidx(B, D, i, j, k)];
+                mb += A.mtb[D][S.NF[D] + fidx(B, D, i - (D == 0), j - (D == 1), k - (D == 2))];
+            }
+            if (mb != 0.0) {
+                double ex[13], eb[13];
+                ex[0] = d[D_RHO]; ex[1] = d[D_MU]; ex[2] = x7[1][7]; ex[3] = x7[1][8];
+                for (int q = 0; q < 9; q++) ex[4 + q] = zx[og + q];
+                dn_grad<4>(13, [&](int q) -> double { return ex[q]; }, [&](auto in, int) -> Dn<4> {
+                    const Dn<4> gu[3] = {in(4), in(5), in(6)}, gv[3] = {in(7), in(8), in(9)}, gw[3] = {in(10), in(11), in(12)};
+                    return sst_eddy_g<Dn<4>>(in(0), in(1), in(2), in(3), gu, gv, gw, ydn);
+                }, eb);
+                zb[od] += mb * eb[0];
+                zb[od + 2] += mb * eb[1];
+                zb[7] += mb * eb[2];
+                zb[8] += mb * eb[3];
+                for (int q = 0; q < 9; q++) zb[og + q] += mb * eb[4 + q];
+            }
+        }
     }
     // outputs: centre (q = 1) and neighbour bars, the cell's property bars;
     // a gradient component g[c] = iv * sum_q w_q[c] x_q with the Green-Gauss
@@ -500,19 +537,22 @@ int adj_alloc(cav3d_ctx *c)
         return rc;
     for (int d = 0; d < 3; d++)
         if ((rc = dalloc(c, &c->a_fb[d], (long)NSLOT * npv * S.NF[d])) || (rc = dalloc(c, &c->a_db[d], 2L * ND * S.NF[d])) ||
-            (rc = dalloc(c, &c->a_ub[d], S.NF[d])))
+            (rc = dalloc(c, &c->a_ub[d], S.NF[d])) || (rc = dalloc(c, &c->a_mtb[d], 2L * S.NF[d])))
             return rc;
     HIPCHK(hipStreamSynchronize(c->st));
     c->adj_ready = true;
     return 0;
 }
 
-int launch_vjp(cav3d_ctx *c)
+int launch_vjp(cav3d_ctx *c, int dmut = 0)
 {
     DS &S = c->S;
     AD A;
     A.rb = c->a_rb;
-    for (int d = 0; d < 3; d++) { A.fb[d] = c->a_fb[d]; A.db[d] = c->a_db[d]; A.ub[d] = c->a_ub[d]; }
+    A.dmut = dmut;
+    for (int d = 0; d < 3; d++) {
+        A.fb[d] = c->a_fb[d]; A.db[d] = c->a_db[d]; A.ub[d] = c->a_ub[d]; A.mtb[d] = c->a_mtb[d];
+    }
     A.cb = c->a_cb; A.cdb = c->a_cdb; A.csb = c->a_csb; A.pb = c->a_pb;
     A.mask = adj_mask(S.P);
     const bool sensor = S.P.nscheme == 2 || S.P.nscheme == 3;
@@ -570,16 +610,16 @@ void sum_into(cav3d_ctx *c, const double *src, long n, int slot)
 
 extern "C" {
 
-int cav3d_rhs_vjp(cav3d_ctx *c, const double *res_bar, double *pv_bar, double *scal_bar)
+static int rhs_vjp_run(cav3d_ctx *c, const double *res_bar, double *pv_bar, double *scal_bar, int dmut, const char *name)
 {
-    if (!c || !res_bar || !pv_bar) return fail("bad argument to cav3d_rhs_vjp", -1);
+    if (!c || !res_bar || !pv_bar) return fail(std::string("bad argument to ") + name, -1);
     int rc;
     if ((rc = upload_res_bar(c, res_bar))) return rc;
     DS &S = c->S;
     const int npv = c->npv;
     long off = 0;
     if (c->profile) hipEventRecord(c->ev[10], c->st);
-    launch_vjp(c);
+    launch_vjp(c, dmut);
     if (c->profile) {
         hipEventRecord(c->ev[11], c->st);
         hipEventSynchronize(c->ev[11]);
@@ -587,7 +627,7 @@ int cav3d_rhs_vjp(cav3d_ctx *c, const double *res_bar, double *pv_bar, double *s
         hipEventElapsedTime(&ms, c->ev[10], c->ev[11]);
         c->prof_ms[8] = ms;
     }
-    if ((rc = check_launch("cav3d_rhs_vjp"))) return rc;
+    if ((rc = check_launch(name))) return rc;
     off = 0;
     for (int b = 0; b < c->nb; b++) {
         const BlkD &B = c->blk[b];
@@ -607,7 +647,27 @@ int cav3d_rhs_vjp(cav3d_ctx *c, const double *res_bar, double *pv_bar, double *s
         scal_bar[0] = h[0];
         scal_bar[1] = ((h[1] + h[2]) + h[3]) + h[4];
     }
-    return check_launch("cav3d_rhs_vjp");
+    return check_launch(name);
+}
+
+int cav3d_rhs_vjp(cav3d_ctx *c, const double *res_bar, double *pv_bar, double *scal_bar)
+{
+    return rhs_vjp_run(c, res_bar, pv_bar, scal_bar, 0, "cav3d_rhs_vjp");
+}
+
+int cav3d_rhs_vjp_eddy(cav3d_ctx *c, const double *res_bar, double *pv_bar, double *scal_bar)
+{
+    if (c && c->prm.iturb != 0) return fail("cav3d_rhs_vjp_eddy: the eddy viscosity is a state function only under k-omega SST", -1);
+    return rhs_vjp_run(c, res_bar, pv_bar, scal_bar, 1, "cav3d_rhs_vjp_eddy");
+}
+
+int cav3d_eddy(cav3d_ctx *c)
+{
+    if (!c) return fail("bad argument to cav3d_eddy", -1);
+    if (c->prm.iturb != 0) return fail("cav3d_eddy: the eddy viscosity is computed only under k-omega SST", -1);
+    const DS &S = c->S;
+    hipLaunchKernelGGL(k_eddy, dim3(nblk(S.NC, 256)), dim3(256), 0, c->st, S);
+    return check_launch("cav3d_eddy");
 }
 
 int cav3d_rhs_vjp_params(cav3d_ctx *c, const double *res_bar, double *par_bar)

@@ -274,10 +274,13 @@ __host__ __device__ __attribute__((always_inline)) inline void face_adj(const Pa
 
 // dual passes of one face's viscous flux over its inputs zx (the layout of
 // k_vface_vjp); inlined into each direction's kernel (as a called function
-// its locals took 4.9 KB of private memory per lane, inlined 0.9 KB)
+// its locals took 4.9 KB of private memory per lane, inlined 0.9 KB).  omt:
+// the index of the two cells' eddy viscosities in zx when they are inputs,
+// or -1 (held at ml, mr)
 __host__ __device__ __attribute__((always_inline)) inline void visc_adj(const Par &P, int npv, int nv, int ncv, int nz, const double *zx, const double *n3,
                                       const double e4[4][3], const double t4[4][3], const double *gl, const double *gr,
-                                      const double *dl, const double *dr, double ml, double mr, const double *g, double *zb)
+                                      const double *dl, const double *dr, double ml, double mr, int omt, const double *g,
+                                      double *zb)
 {
     const bool turb = P.iturb == 0;
     const int og = 3 * nv, oc = og + ncv;
@@ -295,7 +298,8 @@ __host__ __device__ __attribute__((always_inline)) inline void visc_adj(const Pa
             for (int a = 0; a < 4; a++) for (int c = 0; c < 3; c++) { et[a][c] = Tn(e4[a][c]); tt[a][c] = Tn(t4[a][c]); }
             glt[0] = Tn(gl[0]); glt[1] = Tn(gl[1]); grt[0] = Tn(gr[0]); grt[1] = Tn(gr[1]);
             Tn fv[NPV];
-            visc_flux<Tn>(P, nt, et, tt, glt, grt, gt, cv, dlt, drt, Tn(ml), Tn(mr), fv);
+            const Tn mlt = omt >= 0 ? in(omt) : Tn(ml), mrt = omt >= 0 ? in(omt + 1) : Tn(mr);
+            visc_flux<Tn>(P, nt, et, tt, glt, grt, gt, cv, dlt, drt, mlt, mrt, fv);
             Tn phi(0.0);
             for (int n = 0; n < npv; n++) phi = phi + g[n] * fv[n];
             return phi;

@@ -1752,6 +1752,22 @@ HD void sst_source(const Par &P, const R x7[7][NPV], const double f6[6][3], doub
 }
 
 // SST eddy viscosity mu_t = 0.31 rho k / max(0.31 omega, S F2)
+// the SST eddy viscosity from the cell's rho, mu, k, omega and its velocity
+// gradients gu, gv, gw (src/eddy_module.f90:135-187); R = double or a dual
+// number (the adjoint differentiates it with respect to all eleven inputs)
+template <class R>
+HD R sst_eddy_g(R rho, R mu, R k, R om, const R *gu, const R *gv, const R *gw, double ydns)
+{
+    R t1 = sqrt(k) / (0.09 * om * ydns);
+    R t2 = 500.0 * mu / rho / (om * (ydns * ydns));
+    R a2 = mx(2.0 * t1, t2);
+    R bf2 = tanh(a2 * a2);
+    R sij = gu[0] * gu[0] + gv[1] * gv[1] + gw[2] * gw[2]
+          + 0.5 * ((gu[1] + gv[0]) * (gu[1] + gv[0]) + (gu[2] + gw[0]) * (gu[2] + gw[0]) + (gv[2] + gw[1]) * (gv[2] + gw[1]));
+    R sss = sqrt(2.0 * sij);
+    return 0.31 * rho * k / mx(0.31 * om, sss * bf2);
+}
+
 HD double sst_eddy(const double x7[7][NPV], const double f6[6][3], double vol, double ydns, const double *d)
 {
     double iv = 1.0 / vol;
@@ -1759,15 +1775,7 @@ HD double sst_eddy(const double x7[7][NPV], const double f6[6][3], double vol, d
     gg_grad(x7, 1, f6, iv, gu);
     gg_grad(x7, 2, f6, iv, gv);
     gg_grad(x7, 3, f6, iv, gw);
-    double k = x7[1][7], om = x7[1][8];
-    double t1 = sqrt(k) / (0.09 * om * ydns);
-    double t2 = 500.0 * d[D_MU] / d[D_RHO] / (om * (ydns * ydns));
-    double a2 = mx(2.0 * t1, t2);
-    double bf2 = tanh(a2 * a2);
-    double sij = gu[0] * gu[0] + gv[1] * gv[1] + gw[2] * gw[2]
-               + 0.5 * ((gu[1] + gv[0]) * (gu[1] + gv[0]) + (gu[2] + gw[0]) * (gu[2] + gw[0]) + (gv[2] + gw[1]) * (gv[2] + gw[1]));
-    double sss = sqrt(2.0 * sij);
-    return 0.31 * d[D_RHO] * k / mx(0.31 * om, sss * bf2);
+    return sst_eddy_g<double>(d[D_RHO], d[D_MU], x7[1][7], x7[1][8], gu, gv, gw, ydns);
 }
 
 }  // namespace cav

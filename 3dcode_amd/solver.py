@@ -31,7 +31,7 @@ SURFACE = 0x100   # CAV3D_BC_SURFACE: OR'ed into a kind, marks the force-integra
 EXPORTS = [
     "cav3d_create", "cav3d_set_metrics", "cav3d_set_grid", "cav3d_build_metrics", "cav3d_get_metrics",
     "cav3d_integrals", "cav3d_integrals_noalloc", "cav3d_export", "cav3d_import", "cav3d_rhs_vjp",
-    "cav3d_rhs_vjp_params", "cav3d_objective",
+    "cav3d_rhs_vjp_params", "cav3d_objective", "cav3d_rhs_vjp_eddy", "cav3d_eddy",
     "cav3d_put_state", "cav3d_get_state", "cav3d_timeinteg",
     "cav3d_setbc", "cav3d_calrhs", "cav3d_timestep", "cav3d_residual", "cav3d_unsteady_shift",
     "cav3d_eos", "cav3d_eos_host", "cav3d_profile", "cav3d_synchronize", "cav3d_last_error",
@@ -80,6 +80,8 @@ def lib(build_if_missing: bool = True):
     L.cav3d_import.argtypes = [C.c_void_p, C.c_char_p, _I, _I, _D]
     L.cav3d_rhs_vjp.argtypes = [C.c_void_p, _D, _D, _D]
     L.cav3d_rhs_vjp_params.argtypes = [C.c_void_p, _D, _D]
+    L.cav3d_rhs_vjp_eddy.argtypes = [C.c_void_p, _D, _D, _D]
+    L.cav3d_eddy.argtypes = [C.c_void_p]
     L.cav3d_objective.argtypes = [C.c_void_p, C.c_double, C.c_double, _D, _D, _D]
     L.cav3d_put_state.argtypes = [C.c_void_p, C.c_int, _D, _D, _D, _D]
     L.cav3d_get_state.argtypes = [C.c_void_p, C.c_int, _D, _D, _D, _D]
@@ -286,8 +288,10 @@ class Solver:
         return nps.value, nts.value, tp.value
 
     # ---- adjoint
-    def rhs_vjp(self, res_bars):
-        """(d res/d pv)^T res_bar over every block (cav3d_rhs_vjp).
+    def rhs_vjp(self, res_bars, eddy=False):
+        """(d res/d pv)^T res_bar over every block (cav3d_rhs_vjp; with
+        eddy=True cav3d_rhs_vjp_eddy, the interior eddy viscosity a function
+        of the state).
 
         res_bars: one array per block in calrhs's res layout.  Returns the
         per-block pv_bar arrays (pv layout, ghosted) and [d/d p_ref, d/d u_ref]
@@ -300,7 +304,8 @@ class Solver:
         ng = [self.npv * (b.imax + 5) * (b.jmax + 5) * (b.kmax + 5) for b in blocks]
         pb = np.zeros(sum(ng))
         sc = np.zeros(2)
-        _check(lib().cav3d_rhs_vjp(self.h, _p(rb), _p(pb), _p(sc)))
+        fn = lib().cav3d_rhs_vjp_eddy if eddy else lib().cav3d_rhs_vjp
+        _check(fn(self.h, _p(rb), _p(pb), _p(sc)))
         return np.split(pb, np.cumsum(ng)[:-1]), sc
 
     def param_vjp(self, res_bars):
@@ -339,6 +344,26 @@ class Solver:
             qq = None if qqs is None else qqs[ib]
             self.put_state(ib, np.ascontiguousarray(a.ravel()), np.ascontiguousarray(dv.ravel()),
                            None if tv is None else np.ascontiguousarray(tv.ravel()), qq)
+
+    def eddy(self):
+        """The interior cells' SST eddy viscosity from the current state (cav3d_eddy)."""
+        _check(lib().cav3d_eddy(self.h))
+
+    def residual_of_eddy(self, pvs, muts, qqs=None):
+        """The map cav3d_rhs_vjp_eddy differentiates: residual_of with the
+        interior cells' eddy viscosity recomputed from the state (the ghost
+        cells' taken from muts)."""
+        pref = self.case.params["pref"]
+        for ib, pv in enumerate(pvs):
+            a = np.asarray(pv, dtype=np.float64).reshape(-1, self.npv)
+            dv, t2 = self.eos(a[:, 0] + pref, a[:, 4], a[:, 5], a[:, 6])
+            tv = np.zeros((a.shape[0], self.ntv))
+            tv[:, :2] = t2
+            tv[:, 2] = muts[ib]
+            self.put_state(ib, np.ascontiguousarray(a.ravel()), np.ascontiguousarray(dv.ravel()),
+                           np.ascontiguousarray(tv.ravel()), None if qqs is None else qqs[ib])
+        self.eddy()
+        return [self.calrhs(ib)[0] for ib in range(len(pvs))]
 
     def residual_after_setbc(self, pvs, muts=None, qqs=None):
         """Every block's residual for the states pvs with the ghost layers

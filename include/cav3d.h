@@ -184,6 +184,19 @@ int cav3d_import(cav3d_ctx *ctx, const char *path, int *nps, int *nts, double *t
  * Deterministic (fixed-order gathers, no atomics). */
 int cav3d_rhs_vjp(cav3d_ctx *ctx, const double *res_bar, double *pv_bar, double *scal_bar);
 
+/* cav3d_rhs_vjp with the interior cells' eddy viscosity a function of the
+ * state instead of held: mu_t = the SST eddy viscosity of the cell's rho,
+ * mu, k, omega and its Green-Gauss velocity gradients over the 7-point
+ * stencil (t_eddy%calmut, src/eddy_module.f90:135-187), as cav3d_eddy
+ * computes it.  The ghost cells' eddy viscosity stays an input held fixed.
+ * k-omega SST (iturb 0) only.  No counterpart in the reference. */
+int cav3d_rhs_vjp_eddy(cav3d_ctx *ctx, const double *res_bar, double *pv_bar, double *scal_bar);
+
+/* t_eddy%calmut (src/eddy_module.f90:135-187): the interior cells' eddy
+ * viscosity from the current pv and cell properties (the step computes it
+ * after every update).  k-omega SST only; does not synchronise. */
+int cav3d_eddy(cav3d_ctx *ctx);
+
 /* Derivatives of res_bar . res with respect to the run parameters the
  * residual reads, with the ghost layers refilled by the boundary fill
  * (cav3d_setbc) from the same interior.  No counterpart in the reference.

@@ -104,6 +104,13 @@ module cav3d_iface
       import; type(c_ptr), value :: ctx; real(c_double), intent(in) :: res_bar(*)
       real(c_double), intent(out) :: pv_bar(*), scal_bar(2)
     end function
+    integer(c_int) function cav3d_rhs_vjp_eddy(ctx, res_bar, pv_bar, scal_bar) bind(c)
+      import; type(c_ptr), value :: ctx; real(c_double), intent(in) :: res_bar(*)
+      real(c_double), intent(out) :: pv_bar(*), scal_bar(2)
+    end function
+    integer(c_int) function cav3d_eddy(ctx) bind(c)
+      import; type(c_ptr), value :: ctx
+    end function
     integer(c_int) function cav3d_rhs_vjp_params(ctx, res_bar, par_bar) bind(c)
       import; type(c_ptr), value :: ctx; real(c_double), intent(in) :: res_bar(*)
       real(c_double), intent(out) :: par_bar(14)
