@@ -45,14 +45,14 @@ def _close_contexts():
         _OPEN.pop().close()
 
 
-def prepared(pk, name, steps=2, generic=True):
+def prepared(pk, name, steps=2, generic=True, make=None):
     """A case after a few pseudo-time steps, the solver holding it, and its
     frozen parts.  The ghosted state is then made generic: ghost layers that
     copy the interior and the spanwise velocity's exact zeros give limiter
     ratios of zero differences, which the 1e-30 regularisation sets and no
     finite step resolves; every ghosted cell is an input of the map, so any
     values are admissible."""
-    case = small_cases(pk)[name]()
+    case = make() if make is not None else small_cases(pk)[name]()
     s = pk.Solver(case)
     _OPEN.append(s)
     for ib, st in enumerate(case.init):
@@ -225,6 +225,62 @@ def test_vjp_sixty_four_directions_config2(pkg):
     rb = seeds(case, res0, rng)
     pb, _ = s.rhs_vjp(rb)
     check_directions(lambda p: s.residual_of(p, fr["mut"], fr["qq"]), pvs, pb, rb, directions(case, pvs, rng, 64), 48)
+    s.close()
+
+
+@pytest.mark.parametrize("name", ["sst_cav_ausm_mlp", "sst_ddes"])
+def test_vjp_eddy_matches_central_differences(pkg, name):
+    """cav3d_rhs_vjp_eddy: the map recomputes the interior cells' eddy
+    viscosity from the state (cav3d_eddy, src/eddy_module.f90:135-187) and
+    holds the ghost cells'; 8 directions over every ghosted entry.  The
+    eddy viscosity's share must be visible: the same directions' products
+    of the fixed-mu_t adjoint differ from it by more than the tolerance."""
+    print(f"case {name} eddy-viscosity adjoint vs FD", flush=True)
+    case, s, pvs, fr = prepared(pkg, name)
+    rng = np.random.default_rng(zlib.crc32(("eddy" + name).encode()))
+    f = lambda p: s.residual_of_eddy(p, fr["mut"], fr["qq"])
+    rb = seeds(case, f(pvs), rng)
+    pb, _ = s.rhs_vjp(rb, eddy=True)
+    f(pvs)
+    pb0, _ = s.rhs_vjp(rb)
+    dirs = directions(case, pvs, rng, 8)
+    worst = check_directions(f, pvs, pb, rb, dirs, 5)
+    gap = max(abs(sum(float(np.dot(a - b, v)) for a, b, v in zip(pb, pb0, d)))
+              / abs(sum(float(np.dot(a, v)) for a, v in zip(pb, d))) for d in dirs)
+    print(f"case {name}: worst relative difference {worst:.2e}; eddy share up to {gap:.2e} of a product", flush=True)
+    assert gap > 10 * TOL
+    s.close()
+
+
+def test_vjp_full_size_config2(pkg):
+    """The config-2 workload at its benchmark size (128x64x64, 524,288
+    cells, SST, Merkle, AUSM+-up, MLP) after two steps: 8 seeded directions,
+    each supported on its own random 12x12x12 box of ghosted cells (over the
+    whole grid every step of a random direction crosses limiter switches in
+    some of the half-million cells, so no three steps agree), at least 6
+    settled."""
+    print("case config2 128x64x64 adjoint vs FD", flush=True)
+
+    def make():
+        c = pkg.cases.config2()
+        c.params["cav_clip"] = 1
+        return c
+    case, s, pvs, fr = prepared(pkg, "config2_128x64x64", make=make)
+    rng = np.random.default_rng(128)
+    res0 = s.residual_of(pvs, fr["mut"], fr["qq"])
+    rb = seeds(case, res0, rng)
+    pb, _ = s.rhs_vjp(rb)
+    b = case.blocks[0]
+    shape = (b.kmax + 5, b.jmax + 5, b.imax + 5, case.npv)
+    dirs = []
+    for d in directions(case, pvs, rng, 8):
+        a = d[0].reshape(shape)
+        k0, j0, i0 = (int(rng.integers(0, n - 12)) for n in shape[:3])
+        m = np.zeros(shape[:3], dtype=bool)
+        m[k0:k0 + 12, j0:j0 + 12, i0:i0 + 12] = True
+        dirs.append([np.where(m[..., None], a, 0.0).ravel()])
+    worst = check_directions(lambda p: s.residual_of(p, fr["mut"], fr["qq"]), pvs, pb, rb, dirs, 6)
+    print(f"case config2 128x64x64: worst relative difference {worst:.2e}", flush=True)
     s.close()
 
 
