@@ -145,8 +145,10 @@ int cav3d_get_metrics(cav3d_ctx *ctx, int b, double *cx, double *ex, double *tx,
 int cav3d_integrals(cav3d_ctx *ctx, double rhoref, double uref, double area, double aoa, double *out);
 
 /* cav3d_integrals without any allocation: no counterpart in the reference.
- * For callers that cannot allocate device memory (code under stream capture,
- * tight serving loops, harnesses that forbid hipMalloc).  Same arguments and
+ * For callers that cannot allocate device memory (tight serving loops,
+ * harnesses that forbid hipMalloc).  Like cav3d_integrals it copies its
+ * results to the host and synchronises the context's stream, so it is not
+ * for code under stream capture.  Same arguments and
  * results as cav3d_integrals (bit for bit); its partial sums, segment table
  * and final sums live in one fixed 2 MiB device array per GPU instead of
  * buffers allocated per call.  Limits: at most 64 CAV3D_BC_SURFACE ranges and
