@@ -7,12 +7,27 @@
  *   preconditioned Roe            src/flux_module.f90:198-276, 643-695
  *   AUSM+-up                      src/flux_module.f90:548-641
  *   viscous flux + SST F1         src/vsflux_module.f90:63-99, 182-425
+ *   Roe-M, AUSMPW+                src/flux_module.f90:278-418, 420-546
  *   Merkle mass transfer          src/cav_module.f90:67-97, 136-171
+ *   Kunz, Singhal, Schnerr-Sauer,
+ *   Lee mass transfer             src/cav_module.f90:173-356
  *   k-omega SST source            src/turbsource_module.f90:269-414
+ *   DES/DDES/IDDES/S-IDDES length src/turbsource_module.f90:416-576
  *   SST eddy viscosity            src/eddy_module.f90:135-187
  *   split flux Jacobians          src/jacobian_module.f90:182-465
  *   diagonal inverse getx         src/lhs_module.f90:346-534
  * Stencil arrays are x[ll][n] with ll,n 0-based (= Fortran x(ll+1,n+1)).
+ *
+ * Pins (tests/test_oracle_closed_forms.py, test_oracle_invariants.py,
+ * test_phys_host.py): flux consistency and antisymmetry, exactness of every
+ * limiter on linear data, the viscous stress of linear fields, the SST
+ * source and eddy viscosity at their limits, the five mass-transfer models'
+ * rates from their publications (no non-condensable gas) and their
+ * linearisations against finite differences, the DES family's lengths and
+ * shielding functions from their publications, the Jacobian splitting, getx
+ * against an assembled inverse.  Parity unpinned: the mass-transfer rates'
+ * non-condensable-gas (y2 > 0) terms, which follow the reference's source
+ * text only.
  */
 #include "orc_int.h"
 #include <math.h>

@@ -452,6 +452,126 @@ def test_des_length_closed_forms(OL, orc, pkg):
         assert c3 == c4
 
 
+def _shielded_source(OL, orc, cfg, ides, k, om, grad, h, d, mut):
+    """k source (ides 0 and ides) of a cube cell of side h at wall distance d,
+    uniform k and omega, and the linear velocity field u = u_c + grad (x - x_c)
+    sampled at the six neighbours (the 7-point Green-Gauss gradient of a
+    linear field on a uniform cube is exact); returns the two sources, rho
+    and mu"""
+    pv = np.array([0.0, 0.4, 0.2, -0.1, 300.0, 0.0, 0.0, k, om])
+    dvv, tv2 = orc.deteos_prop(pv[0] + cfg["pref"], pv[4], pv[5], pv[6])
+    tv = np.array([tv2[0], tv2[1], mut])
+    grd = np.array([h ** 3, 0, 0, 0, d])
+    f6 = [np.eye(3)[q // 2] * h * h for q in range(6)]
+    x7 = np.tile(pv, (7, 1))                  # rows: i-1, cell, i+1, j-1, j+1, k-1, k+1
+    for row, axis, sgn in [(0, 0, -1), (2, 0, 1), (3, 1, -1), (4, 1, 1), (5, 2, -1), (6, 2, 1)]:
+        x7[row, 1:4] = pv[1:4] + sgn * h * grad[:, axis]
+    out = []
+    for i in (0, ides):
+        src, itt, oc = np.zeros(2), np.zeros(4), C.c_double()
+        OL.orc_kwsst_source_des(i, ptrs(f6), dp(grd), dp(x7), dp(dvv), dp(tv), dp(src), dp(itt), C.byref(oc))
+        out.append(src[0])
+    return out, dvv[0], tv2[0]
+
+
+def _menter_f1(rho, mu, k, om, d):
+    """Menter's blending F1 = tanh(arg1^4) with uniform k and omega (CD_kw at
+    its floor 1e-10): arg1 = min(max(sqrt(k)/(beta* omega d), 500 nu/(d^2
+    omega)), 4 rho sigma_w2 k/(CD_kw d^2)), beta* = 0.09, sigma_w2 = 0.856"""
+    arg1 = min(max(math.sqrt(k) / (0.09 * om * d), 500.0 * mu / rho / (d * d * om)),
+               4.0 * rho * 0.856 * k / (1e-10 * d * d))
+    return math.tanh(arg1 ** 4)
+
+
+def _check_length(OL, orc, cfg, ides, k, om, grad, h, d, mut, l_want):
+    (s0, s1), rho, _ = _shielded_source(OL, orc, cfg, ides, k, om, grad, h, d, mut)
+    l_rans = math.sqrt(k) / (0.09 * om)
+    vol = h ** 3
+    want = s0 + rho * k ** 1.5 * (1.0 / l_rans - 1.0 / l_want) * vol
+    assert s1 == pytest.approx(want, rel=1e-9, abs=1e-11 * rho * k ** 1.5 / min(l_rans, l_want) * vol), (ides, d, h)
+
+
+def _grad_with_norm(rng, norm):
+    g = rng.standard_normal((3, 3))
+    return g * (norm / np.linalg.norm(g))
+
+
+def test_ddes_shielding_closed_form(OL, orc, pkg):
+    """DDES (Spalart et al. 2006) with the SST blend of C_DES (0.78 F1 +
+    0.61 (1 - F1)) away from its limits: l = l_RANS - f_d max(0, l_RANS -
+    C_DES h_max), f_d = 1 - tanh((20 r_d)^3), r_d = (mu + mu_t) / (kappa^2
+    d^2 |grad u|) with kappa = 0.41 and |grad u| the Frobenius norm of the
+    velocity gradient (sqrt(U_ij U_ij)).  The reference's r_d takes the
+    dynamic viscosities mu + mu_t where the paper has nu + nu_t; the pin
+    follows the reference.  r_d is drawn so f_d runs over (0, 1)."""
+    cfg = pkg.cases.config2(8, 6, 6).cfg_dict()
+    rng = np.random.default_rng(41)
+    k, om = 1e-2, 100.0
+    l_rans = math.sqrt(k) / (0.09 * om)
+    seen = []
+    for trial in range(24):
+        d = l_rans * math.exp(rng.uniform(math.log(0.2), math.log(3.0)))
+        h = l_rans * rng.uniform(0.2, 1.4)
+        _, rho, mu = _shielded_source(OL, orc, cfg, 0, k, om, np.eye(3), h, d, 0.0)
+        mut = mu * math.exp(rng.uniform(math.log(0.5), math.log(50.0)))
+        r_d = rng.uniform(0.01, 0.08)
+        grad = _grad_with_norm(rng, (mu + mut) / (0.41 ** 2 * d * d * r_d))
+        f1 = _menter_f1(rho, mu, k, om, d)
+        fd = 1.0 - math.tanh((20.0 * r_d) ** 3)
+        l_les = (0.78 * f1 + 0.61 * (1.0 - f1)) * h
+        _check_length(OL, orc, cfg, 2, k, om, grad, h, d, mut, l_rans - fd * max(0.0, l_rans - l_les))
+        seen.append((fd, f1))
+    fds, f1s = np.array(seen).T
+    assert fds.min() < 0.05 and fds.max() > 0.95 and np.any((fds > 0.2) & (fds < 0.8))
+    assert f1s.min() < 0.2 and f1s.max() > 0.99
+
+
+def test_iddes_shielding_closed_form(OL, orc, pkg):
+    """SST-IDDES (Gritskevich et al. 2012; Shur et al. 2008), ides 3, and its
+    form without the elevating function f_e, ides 4:
+      l = f~_d (1 + f_e) l_RANS + (1 - f~_d) C_DES Delta,
+      Delta = min(C_w max(d, h_max), h_max), C_w = 0.15,
+      alpha = 0.25 - d / h_max, f_B = min(2 exp(-9 alpha^2), 1),
+      f~_d = max(1 - f_dt, f_B), f_dt = 1 - tanh((20 r_dt)^3),
+      f_e = f_e2 max(f_e1 - 1, 0), f_e1 = 2 exp(-11.09 alpha^2) (alpha >= 0)
+      or 2 exp(-9 alpha^2), f_e2 = 1 - max(f_t, f_l),
+      f_t = tanh((c_t^2 r_dt)^3), f_l = tanh((c_l^2 r_dl)^10), c_t = 1.87, c_l = 5,
+      r_dt = mu_t / (kappa^2 d^2 |grad u|), r_dl = mu / (kappa^2 d^2 |grad u|)
+    (dynamic viscosities, as in the reference's DDES).  The draws cover both
+    signs of alpha, both branches of f~_d and f_l, f_t away from 0 and 1."""
+    cfg = pkg.cases.config2(8, 6, 6).cfg_dict()
+    rng = np.random.default_rng(42)
+    k, om = 1e-2, 100.0
+    l_rans = math.sqrt(k) / (0.09 * om)
+    seen = []
+    for trial in range(32):
+        d = l_rans * math.exp(rng.uniform(math.log(0.2), math.log(3.0)))
+        h = d / rng.uniform(0.05, 1.2)
+        _, rho, mu = _shielded_source(OL, orc, cfg, 0, k, om, np.eye(3), h, d, 0.0)
+        r_dt = math.exp(rng.uniform(math.log(0.01), math.log(0.6)))
+        r_dl = rng.uniform(0.015, 0.06)
+        mut = mu * r_dt / r_dl
+        grad = _grad_with_norm(rng, mut / (0.41 ** 2 * d * d * r_dt))
+        f1 = _menter_f1(rho, mu, k, om, d)
+        cdes = 0.78 * f1 + 0.61 * (1.0 - f1)
+        delta = min(0.15 * max(d, h), h)
+        alpha = 0.25 - d / h
+        fb = min(2.0 * math.exp(-9.0 * alpha * alpha), 1.0)
+        fdt = 1.0 - math.tanh((20.0 * r_dt) ** 3)
+        fdd = max(1.0 - fdt, fb)
+        fe1 = 2.0 * math.exp((-11.09 if alpha >= 0.0 else -9.0) * alpha * alpha)
+        ft = math.tanh((1.87 ** 2 * r_dt) ** 3)
+        fl = math.tanh((5.0 ** 2 * r_dl) ** 10)
+        fe = (1.0 - max(ft, fl)) * max(fe1 - 1.0, 0.0)
+        _check_length(OL, orc, cfg, 3, k, om, grad, h, d, mut, fdd * (1.0 + fe) * l_rans + (1.0 - fdd) * cdes * delta)
+        _check_length(OL, orc, cfg, 4, k, om, grad, h, d, mut, fdd * l_rans + (1.0 - fdd) * cdes * delta)
+        seen.append((alpha, fb > 1.0 - fdt, fl, ft, fe))
+    a, fbwin, fl, ft, fe = np.array(seen).T
+    assert a.min() < 0.0 < a.max()
+    assert 0 < fbwin.sum() < len(fbwin)
+    assert fl.min() < 0.1 and fl.max() > 0.9 and ft.max() > 0.3 and fe.max() > 0.05
+
+
 @pytest.mark.parametrize("nscheme,nmuscl,nlim", [(1, 1, 1), (1, 0, 0), (4, 2, 8), (2, 2, 8)])
 def test_fp32_face_within_single_precision(L, OL, orc, pkg, nscheme, nmuscl, nlim):
     """The mixed-precision face (reconstruction, face EOS, flux in FP32 on
