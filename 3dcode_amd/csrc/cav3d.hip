@@ -901,18 +901,34 @@ struct WinRow {
     const double *p;
     __device__ double operator[](int e) const { return p[e * WIN]; }
 };
-constexpr int SWEEP_THREADS = 256 + 64 + 64;   // 16 pencil groups of 16 lanes, a store and a poll wavefront
-constexpr int NEDGE = 2 * 4 * NPV;   // cross-tile dq values a tile reads per step: two edges of four pencils
+constexpr int SWEEP_THREADS = 256 + 64 + 64 + 64;   // 16 pencil groups of 16 lanes, a store, a poll and a loader wavefront
+constexpr int NEDGE = 2 * 4 * NPV;
+// publication order of a tile's pencils g = gj + TJ gk in the forward sweep,
+// four bits each: the downstream edges gk = 3 and gj = 3 first (the backward
+// sweep takes 15 - g)
+constexpr unsigned long long PUB_ORDER = 0xA986542107B3FEDCull;
+// Stage loads, by the loader wavefront, as LDS-DMA (global memory to LDS,
+// 16 bytes a lane, no registers).  A pencil at sweep distance d reads window
+// d >> 2 (its cell) and window (d + 1) >> 2 (the lookahead); while d & 3 is
+// 0, 1 or 2 both are window W = d >> 2 and the other buffer, window W + 1, is
+// free.  The loader issues quarters 0 and 1 of window W + 1 at d & 3 = 0 and
+// quarters 2 and 3 at d & 3 = 1, and waits for each step's transfers before
+// the barrier that ends the next step, so every transfer lands while its
+// buffer is free and window W + 1 is complete when it first becomes the
+// lookahead (d & 3 = 3).  A window's four cells lie in ascending i in LDS in
+// both sweeps, as the transfers copy them.   // cross-tile dq values a tile reads per step: two edges of four pencils
 static_assert(TJ == 4 && TK == 4, "the poll wavefront maps an edge of four pencils");
 
 // Division of labour inside a tile's workgroup:
 //  - four pencil wavefronts (16 groups of 16 lanes, lane = row) run the
-//    hyperplane steps; they read only LDS and the stage prefetch (plain
-//    loads), so their waits cover nothing slower;
+//    hyperplane steps; they read only LDS and issue no global memory
+//    access, so nothing they wait for is slower than LDS;
 //  - a poll wavefront reads, with device-coherent loads, the dq of the
 //    upstream cells in other tiles one step ahead of use and hands it over
 //    through LDS (it is the only wavefront that waits on other tiles);
-//  - a store wavefront writes each hyperplane's dq (and dcv) one step later.
+//  - a store wavefront writes each hyperplane's dq (and dcv) as soon as the
+//    pencil wavefronts signal that it is in LDS;
+//  - a loader wavefront fills the stage windows ahead of the pencils.
 // A step's Jacobian coefficients are formed during the step before from its
 // stage without branching on the row, so the dependent chain of a step is
 // three row products, the exchange of y and D^-1.
@@ -934,6 +950,7 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_sweep_tile(DS S, const int *t
     __shared__ double win[2][TJ * TK][ST_N][WIN];   // stage windows (see the pencil wavefronts)
     __shared__ const double *src[TJ * TK][ST_N];
     __shared__ int gave_up;
+    __shared__ int ready;       // pencil wavefronts done with the current hyperplane (counts up over the tile)
     const int npv = S.npv;
     const int sg = FWD ? 1 : -1;
     const double sgn = FWD ? 1.0 : -1.0;
@@ -957,12 +974,16 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_sweep_tile(DS S, const int *t
             return jj <= jend && kk <= kend && ii - jj - kk >= 2 && ii - jj - kk <= B.I;
         };
         if (wave == 4) {
-            // store wavefront: hyperplane s is published during step s+1,
-            // while the pencil wavefronts compute
+            // store wavefront: hyperplane s is published as soon as the four
+            // pencil wavefronts have written it to LDS, the seven pencils on
+            // the tile's downstream edges (the ones other tiles wait for) in
+            // the first 63 lanes
             auto publish = [&](int s) {
                 const int pb = s & 1;
                 for (int e = threadIdx.x - 256; e < TJ * TK * NPV; e += 64) {
-                    const int g = e / NPV, n = e - g * NPV;
+                    const int r = e / NPV, n = e - r * NPV;
+                    const int gf = (int)((PUB_ORDER >> (4 * r)) & 15);
+                    const int g = FWD ? gf : TJ * TK - 1 - gf;
                     if (n < npv && active(g, s)) {
                         const int jj = j0 + g % TJ, kk = k0 + g / TJ;
                         const long w = n * NW + widx(B, s - jj - kk, jj, kk);
@@ -971,14 +992,66 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_sweep_tile(DS S, const int *t
                     }
                 }
             };
+            if (threadIdx.x == 256) ready = 0;
             __syncthreads();
+            int want = 0;
             for (int s = sa;; s += sg) {
-                if (s != sa) publish(s - sg);
+                want += 4;
+                for (long spins = 0; __hip_atomic_load(&ready, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) < want;) {
+                    if (++spins > POLL_LIMIT) {      // cannot happen: the pencil wavefronts wait on nothing before signalling
+                        gave_up = 1;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+                asm volatile("" ::: "memory");
+                publish(s);
                 lds_barrier();
                 if (s == sb) break;
             }
-            publish(sb);
             if (gave_up && threadIdx.x == 256) atomicOr(err, 1);
+            __syncthreads();
+            continue;
+        }
+        if (wave == 6) {
+            // loader wavefront: in a step, the eight pencils with d & 3 = 0
+            // or 1 each get two quarters (32 items x 4 cells, 1 KB each) of
+            // their next window, one 16-byte transfer a lane per quarter:
+            // lane L moves item L / 2, cells 2 (L & 1) and 2 (L & 1) + 1.
+            // The pencils' prologue fills window 0 and the source table is
+            // theirs, so the loader starts after the first barrier.
+            const int l = threadIdx.x - 384, li = l >> 1, lc = 2 * (l & 1);
+            const int BI = B.I;
+            __syncthreads();
+            int d0 = FWD ? sa - j0 - k0 - 2 : BI - (sa - j0 - k0);      // distance of pencil 0 (it grows by one a step)
+            for (int s = sa;; s += sg) {
+                if (s != sb) {
+#pragma unroll
+                    for (int g = 0; g < TJ * TK; g++) {
+                        const int t = (g & 3) + (g >> 2), d = FWD ? d0 - t : d0 + t;
+                        if ((d & 3) > 1) continue;
+                        const int w1 = (d >> 2) + 1;
+                        // first cell of the lane's pair; a pair that lies
+                        // beyond the pencil is clamped to cells 1..I+1 (its
+                        // slots hold no cell of the pencil)
+                        const int c0 = min(max((FWD ? 2 + 4 * w1 : BI - 4 * w1 - 3) + lc, 1), BI);
+#pragma unroll
+                        for (int h = 0; h < 2; h++) {
+                            const int q = 2 * (d & 3) + h;
+                            const double *from = src[g][32 * q + li] + c0;
+                            __builtin_amdgcn_global_load_lds((const GLOBAL void *)from,
+                                                             (__attribute__((address_space(3))) void *)&win[w1 & 1][g][32 * q][0],
+                                                             16, 0, 0);
+                        }
+                    }
+                }
+                d0++;
+                // the transfers of the previous step have landed (16 issued since)
+                asm volatile("s_waitcnt vmcnt(16)" ::: "memory");
+                lds_barrier();
+                if (s == sb) break;
+            }
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
             __syncthreads();
             continue;
         }
@@ -1066,15 +1139,12 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_sweep_tile(DS S, const int *t
         // Stage windows: window m holds cells at sweep distance 4m..4m+3 of
         // the pencil (distance d = i - 2 forward, I - i backward), item-major
         // with the four cells adjacent, in LDS buffer m & 1.  Four adjacent
-        // lanes load one item's four cells (one cache line between them);
-        // quarter q of a window (items 32q..32q+31) is loaded in one step and
-        // written to LDS at the top of the next.  During the step at distance
-        // d the quarter (d+1) & 3 of window ((d+1) >> 2) + 1 is loaded, so a
-        // window is complete one step before its first cell is the step's
-        // lookahead, and it reuses the buffer of the window two back only
-        // after that window's last read.
+        // lanes load one item's four cells (one cache line between them).
+        // The pencils fill window 0 here; the loader wavefront fills the
+        // rest during the steps.
         const int l4 = lane >> 2, c4 = lane & 3;
-        auto wcell = [&](int m, int c) { return FWD ? 2 + 4 * m + c : BI - 4 * m - c; };
+        auto wcell = [&](int m, int c) { return FWD ? 2 + 4 * m + c : BI - 4 * m - 3 + c; };
+        auto slot = [](int dd) { return FWD ? (dd & 3) : 3 - (dd & 3); };     // the cell at distance dd in its window
         auto fetchq = [&](int m, int qq, double *v) {
             const int ic = min(max(wcell(m, c4), 2), BI);
 #pragma unroll
@@ -1085,7 +1155,7 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_sweep_tile(DS S, const int *t
             for (int r = 0; r < 8; r++) win[m & 1][grp][32 * qq + 4 * r + l4][c4] = v[r];
         };
         // prologue: window 0 complete, window -1's buffer zeroed (steps before
-        // the pencil's first cell read it), the first pending quarter in flight
+        // the pencil's first cell read it)
         for (int qq = 0; qq < 4; qq++) {
             double v[8];
             fetchq(0, qq, v);
@@ -1093,15 +1163,12 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_sweep_tile(DS S, const int *t
         }
         for (int e = lane; e < ST_N * WIN; e += 16) (&win[1][grp][0][0])[e] = 0.0;
         const int d0 = FWD ? sa - j - k - 2 : BI - (sa - j - k);
-        int pm = (d0 >> 2) + 1, pq = d0 & 3;                          // the quarter in flight
-        double nx[8];
-        fetchq(pm, pq, nx);
         if (lane < NPV) lds((sa - sg) & 1, grp)[lane] = 0.0;       // the first step's upstream reads zeros
         double ci[8], cj[8], ck[8];                                  // coefficients of the coming step
 #pragma unroll
         for (int q = 0; q < 8; q++) ci[q] = 0.0;
         {
-            const WinRow s0{&win[(d0 >> 2) & 1][grp][0][d0 & 3]};
+            const WinRow s0{&win[(d0 >> 2) & 1][grp][0][slot(d0)]};
             jac_row_coef(npv, stage_jnb(s0, ST_J, lane, npv), sg, lane, sgn, cj);
             jac_row_coef(npv, stage_jnb(s0, ST_K, lane, npv), sg, lane, sgn, ck);
         }
@@ -1110,14 +1177,9 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_sweep_tile(DS S, const int *t
             const int i = s - j - k;
             const int d = FWD ? i - 2 : BI - i;
             const bool act = live && i >= 2 && i <= BI;
-            // the quarter loaded during the previous step, then the next one
-            putq(pm, pq, nx);
-            pm = ((d + 1) >> 2) + 1;
-            pq = (d + 1) & 3;
-            fetchq(pm, pq, nx);
             // dq of the three upstream neighbours: in the tile from the
             // previous hyperplane, from another tile through the poll wavefront
-            const WinRow st{&win[(d >> 2) & 1][grp][0][d & 3]};
+            const WinRow st{&win[(d >> 2) & 1][grp][0][slot(d)]};
             const int pb = (s - sg) & 1;
             // (indices into the hand-over object, not pointers chosen among
             // several, so the compiler sees these reads cannot touch the stage)
@@ -1144,12 +1206,15 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_sweep_tile(DS S, const int *t
                 lds(s & 1, grp)[lane] = act ? xn : 0.0;
                 if (FWD) ylds[s & 1][grp][lane] = y;
             }
+            // this wavefront's four pencils are in LDS: tell the store wavefront
+            asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+            if ((threadIdx.x & 63) == 0) __hip_atomic_fetch_add(&ready, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             __builtin_amdgcn_sched_barrier(0);
             // coefficients of the next step (independent of dq): this cell's
             // row through its face to the next cell, and the j / k neighbours
             // of the next cell
             jac_row_coef(npv, stage_jnb(st, ST_PV, lane, npv), sg, lane, sgn, ci);
-            const WinRow s1{&win[((d + 1) >> 2) & 1][grp][0][(d + 1) & 3]};
+            const WinRow s1{&win[((d + 1) >> 2) & 1][grp][0][slot(d + 1)]};
             jac_row_coef(npv, stage_jnb(s1, ST_J, lane, npv), sg, lane, sgn, cj);
             jac_row_coef(npv, stage_jnb(s1, ST_K, lane, npv), sg, lane, sgn, ck);
             lds_barrier();
@@ -1409,13 +1474,19 @@ struct cav3d_ctx {
 
 namespace {
 
+// Every device array carries DPAD zeroed bytes before and after it: the
+// sweeps' stage transfers move cells in pairs and may read one element past
+// either end of an array.
+constexpr size_t DPAD = 256;
 template <class T> int dalloc(cav3d_ctx *c, T **p, size_t n)
 {
     *p = nullptr;
     if (n == 0) n = 1;
-    HIPCHK(hipMalloc((void **)p, n * sizeof(T)));
-    HIPCHK(hipMemsetAsync(*p, 0, n * sizeof(T), c->st));
-    c->allocs.push_back(*p);
+    char *raw = nullptr;
+    HIPCHK(hipMalloc((void **)&raw, n * sizeof(T) + 2 * DPAD));
+    HIPCHK(hipMemsetAsync(raw, 0, n * sizeof(T) + 2 * DPAD, c->st));
+    c->allocs.push_back(raw);
+    *p = (T *)(raw + DPAD);
     return 0;
 }
 
