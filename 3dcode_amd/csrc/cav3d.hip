@@ -1217,6 +1217,10 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_sweep_tile(DS S, const int *t
             const WinRow s1{&win[((d + 1) >> 2) & 1][grp][0][slot(d + 1)]};
             jac_row_coef(npv, stage_jnb(s1, ST_J, lane, npv), sg, lane, sgn, cj);
             jac_row_coef(npv, stage_jnb(s1, ST_K, lane, npv), sg, lane, sgn, ck);
+            // formed before the barrier (while the poll wavefront may still be
+            // waiting), not after it on the next step's path
+#pragma unroll
+            for (int q = 0; q < 8; q++) asm volatile("" : "+v"(ci[q]), "+v"(cj[q]), "+v"(ck[q]));
             lds_barrier();
             if (s == sb) break;
         }
