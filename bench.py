@@ -78,8 +78,8 @@ def pmc_summary(args, dfetch, dwrite, dcfetch, dcwrite):
     lines = [
         "# HBM traffic per dispatch, FP64 config-2 workload (cavitating bump 128x64x64, SST, AUSM+-up, MLP, Merkle)",
         "# from four rocprofv3 counter runs on one MI355X (run from /tmp with TMPDIR=/tmp, R = repository root):",
-        "#   " + cmd.format(c="FETCH_SIZE", a="--steps 2 --warmup 1 --cpu-baseline 0"),
-        "#   " + cmd.format(c="WRITE_SIZE", a="--steps 2 --warmup 1 --cpu-baseline 0"),
+        "#   " + cmd.format(c="FETCH_SIZE", a="--steps 2 --warmup 1 --cpu-baseline 0 --legs 0"),
+        "#   " + cmd.format(c="WRITE_SIZE", a="--steps 2 --warmup 1 --cpu-baseline 0 --legs 0"),
         "#   " + cmd.format(c="FETCH_SIZE", a="--copy-probe 256"),
         "#   " + cmd.format(c="WRITE_SIZE", a="--copy-probe 256"),
         "# summarised by: python3 bench.py --pmc-summary <fetch dir> <write dir> <copy fetch dir> <copy write dir>",
@@ -420,6 +420,9 @@ def main():
     ap.add_argument("--oracle-iters", type=int, default=0, help="with --mixed-study: compare FP64 with the oracle "
                                                                   "after this many iterations")
     ap.add_argument("--cpu-baseline", type=int, default=1, help="1: time the oracle on the host")
+    ap.add_argument("--legs", type=int, default=1,
+                    help="0: only the timed iterations and their phase profile (no grid-metrics, mixed-precision "
+                         "or adjoint leg), for counter runs whose dispatch counts must be the iterations' alone")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="run only a plain device-to-device copy of MIB MiB (5 times): the calibration "
                          "kernel for the FETCH_SIZE / WRITE_SIZE counters")
@@ -523,19 +526,21 @@ def main():
                 "phase_ms": {k: round(v, 4) for k, v in zip(
                     ["timestep", "boundary", "rhs", "lusgs", "update_eddy", "total", "sweep_fwd", "sweep_bwd"], phase)}}
 
-    grid_ms = grid_metrics_ms(pk, case, 3)
-    mixed = mixed_precision_run(pk, case, args, ms)
+    grid_ms = grid_metrics_ms(pk, case, 3) if args.legs else None
+    mixed = mixed_precision_run(pk, case, args, ms) if args.legs else None
 
     cpu = cpu_baseline(case, args.size) if args.cpu_baseline else None
 
     # the discrete adjoint's cost on the stepped state the timed run left
     # (warmup + steps + profiled iterations), its ghost layers refilled
-    try:
-        s.setbc()
-        adjoint = adjoint_cost(pk, s, case)
-        adjoint["state"] = f"after {args.warmup + args.steps + nprof} pseudo-time iterations"
-    except pk.Cav3dError as e:
-        adjoint = {"error": str(e)[:300]}
+    adjoint = None
+    if args.legs:
+        try:
+            s.setbc()
+            adjoint = adjoint_cost(pk, s, case)
+            adjoint["state"] = f"after {args.warmup + args.steps + nprof} pseudo-time iterations"
+        except pk.Cav3dError as e:
+            adjoint = {"error": str(e)[:300]}
     s.close()
 
     out = {
