@@ -39,15 +39,18 @@ def measured_traffic(size, kernel=None):
     path = os.path.join(ROOT, TRAFFIC_FILE)
     if size != TRAFFIC_SIZE or not os.path.exists(path):
         return None
+    import re
     total = 0.0
     for line in open(path):
-        f = line.split()
-        if len(f) >= 8 and f[0] == "kernel":
-            b = float(f[5]) + float(f[7])
-            if kernel is not None and f[1] == kernel:
+        # kernel names may hold spaces ("k_face<1, false, double>")
+        m = re.match(r"kernel (.+) dispatches (\d+) fetch_bytes (\S+) write_bytes (\S+)\s*$", line)
+        if m:
+            name, n = m.group(1), int(m.group(2))
+            b = float(m.group(3)) + float(m.group(4))
+            if kernel is not None and name == kernel:
                 return int(b)
-            if f[1] not in SETUP_KERNELS:
-                total += b * int(f[3]) / TRAFFIC_ITERS
+            if name not in SETUP_KERNELS:
+                total += b * n / TRAFFIC_ITERS
     return None if kernel is not None else int(total)
 
 
