@@ -903,10 +903,6 @@ struct WinRow {
 };
 constexpr int SWEEP_THREADS = 256 + 64 + 64 + 64;   // 16 pencil groups of 16 lanes, a store, a poll and a loader wavefront
 constexpr int NEDGE = 2 * 4 * NPV;
-// publication order of a tile's pencils g = gj + TJ gk in the forward sweep,
-// four bits each: the downstream edges gk = 3 and gj = 3 first (the backward
-// sweep takes 15 - g)
-constexpr unsigned long long PUB_ORDER = 0xA986542107B3FEDCull;
 // Stage loads, by the loader wavefront, as LDS-DMA (global memory to LDS,
 // 16 bytes a lane, no registers).  A pencil at sweep distance d reads window
 // d >> 2 (its cell) and window (d + 1) >> 2 (the lookahead); while d & 3 is
@@ -926,8 +922,9 @@ static_assert(TJ == 4 && TK == 4, "the poll wavefront maps an edge of four penci
 //  - a poll wavefront reads, with device-coherent loads, the dq of the
 //    upstream cells in other tiles one step ahead of use and hands it over
 //    through LDS (it is the only wavefront that waits on other tiles);
-//  - a store wavefront writes each hyperplane's dq (and dcv) as soon as the
-//    pencil wavefronts signal that it is in LDS;
+//  - a store wavefront writes each hyperplane's dq (and dcv) one step later,
+//    except the dq of the pencils on the tile's downstream edges, which the
+//    pencil wavefronts store as soon as they form it;
 //  - a loader wavefront fills the stage windows ahead of the pencils.
 // A step's Jacobian coefficients are formed during the step before from its
 // stage without branching on the row, so the dependent chain of a step is
@@ -950,7 +947,6 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_sweep_tile(DS S, const int *t
     __shared__ double win[2][TJ * TK][ST_N][WIN];   // stage windows (see the pencil wavefronts)
     __shared__ const double *src[TJ * TK][ST_N];
     __shared__ int gave_up;
-    __shared__ int ready;       // pencil wavefronts done with the current hyperplane (counts up over the tile)
     const int npv = S.npv;
     const int sg = FWD ? 1 : -1;
     const double sgn = FWD ? 1.0 : -1.0;
@@ -969,46 +965,38 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_sweep_tile(DS S, const int *t
         // the tile's hyperplane range in sweep order
         const int sa = FWD ? 2 + j0 + k0 : B.I + jend + kend;
         const int sb = FWD ? B.I + jend + kend : 2 + j0 + k0;
+        // the pencils whose dq the downstream tiles read (j + 1 and k + 1 edges forward)
+        auto down_edge = [](int g) {
+            return FWD ? (g % TJ == TJ - 1 || g / TJ == TK - 1) : (g % TJ == 0 || g / TJ == 0);
+        };
         auto active = [&](int g, int ii) {
             const int jj = j0 + g % TJ, kk = k0 + g / TJ;
             return jj <= jend && kk <= kend && ii - jj - kk >= 2 && ii - jj - kk <= B.I;
         };
         if (wave == 4) {
-            // store wavefront: hyperplane s is published as soon as the four
-            // pencil wavefronts have written it to LDS, the seven pencils on
-            // the tile's downstream edges (the ones other tiles wait for) in
-            // the first 63 lanes
+            // store wavefront: hyperplane s is published during step s+1,
+            // while the pencil wavefronts compute; the dq of the pencils on
+            // the tile's downstream edges (the values other tiles wait for)
+            // the pencil wavefronts store themselves, as soon as it is formed
             auto publish = [&](int s) {
                 const int pb = s & 1;
                 for (int e = threadIdx.x - 256; e < TJ * TK * NPV; e += 64) {
-                    const int r = e / NPV, n = e - r * NPV;
-                    const int gf = (int)((PUB_ORDER >> (4 * r)) & 15);
-                    const int g = FWD ? gf : TJ * TK - 1 - gf;
+                    const int g = e / NPV, n = e - g * NPV;
                     if (n < npv && active(g, s)) {
                         const int jj = j0 + g % TJ, kk = k0 + g / TJ;
                         const long w = n * NW + widx(B, s - jj - kk, jj, kk);
                         if (FWD) S.dcv[w] = ylds[pb][g][n];
-                        st_dev(dq + w, lds(pb, g)[n]);
+                        if (!down_edge(g)) st_dev(dq + w, lds(pb, g)[n]);
                     }
                 }
             };
-            if (threadIdx.x == 256) ready = 0;
             __syncthreads();
-            int want = 0;
             for (int s = sa;; s += sg) {
-                want += 4;
-                for (long spins = 0; __hip_atomic_load(&ready, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) < want;) {
-                    if (++spins > POLL_LIMIT) {      // cannot happen: the pencil wavefronts wait on nothing before signalling
-                        gave_up = 1;
-                        break;
-                    }
-                    __builtin_amdgcn_s_sleep(1);
-                }
-                asm volatile("" ::: "memory");
-                publish(s);
+                if (s != sa) publish(s - sg);
                 lds_barrier();
                 if (s == sb) break;
             }
+            publish(sb);
             if (gave_up && threadIdx.x == 256) atomicOr(err, 1);
             __syncthreads();
             continue;
@@ -1125,6 +1113,7 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_sweep_tile(DS S, const int *t
         const int ln = lane < NPV ? lane : 0;
         const int j = j0 + gj, k = k0 + gk;
         const bool live = (j <= jend && k <= kend);
+        const bool edge = down_edge(grp);
         const int BI = B.I;
         const int jm = j - sg, km = k - sg;
         const bool lj = jm >= j0 && jm <= jend, lk = km >= k0 && km <= kend;
@@ -1205,10 +1194,10 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_sweep_tile(DS S, const int *t
             if (lane < NPV) {
                 lds(s & 1, grp)[lane] = act ? xn : 0.0;
                 if (FWD) ylds[s & 1][grp][lane] = y;
+                // the pencil wavefronts have no loads in flight, so this store
+                // costs only its issue
+                if (edge && act && mine) st_dev(dq + lane * NW + widx(B, i, j, k), xn);
             }
-            // this wavefront's four pencils are in LDS: tell the store wavefront
-            asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-            if ((threadIdx.x & 63) == 0) __hip_atomic_fetch_add(&ready, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             __builtin_amdgcn_sched_barrier(0);
             // coefficients of the next step (independent of dq): this cell's
             // row through its face to the next cell, and the j / k neighbours
