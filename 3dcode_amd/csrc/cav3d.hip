@@ -920,8 +920,8 @@ static_assert(TJ == 4 && TK == 4, "the poll wavefront maps an edge of four penci
 //    hyperplane steps; they read only LDS and issue no global memory
 //    access, so nothing they wait for is slower than LDS;
 //  - a poll wavefront reads, with device-coherent loads, the dq of the
-//    upstream cells in other tiles one step ahead of use and hands it over
-//    through LDS (it is the only wavefront that waits on other tiles);
+//    upstream cells in other tiles during the step before its use and hands
+//    it over through LDS (it is the only wavefront that waits on other tiles);
 //  - a store wavefront writes each hyperplane's dq (and dcv) one step later,
 //    except the dq of the pencils on the tile's downstream edges, which the
 //    pencil wavefronts store as soon as they form it;
@@ -1094,12 +1094,13 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_sweep_tile(DS S, const int *t
             };
             issue(sa);
             deliver(sa);
-            issue(sa + sg);
             __syncthreads();
             for (int s = sa;; s += sg) {
+                // the next step's values are read at the start of this step,
+                // while the pencils compute, as late as they can be
                 if (s != sb) {
+                    issue(s + sg);
                     deliver(s + sg);
-                    issue(s + 2 * sg);
                 }
                 lds_barrier();
                 if (s == sb) break;
