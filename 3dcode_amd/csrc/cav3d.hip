@@ -947,6 +947,7 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_sweep_tile(DS S, const int *t
     __shared__ double win[2][TJ * TK][ST_N][WIN];   // stage windows (see the pencil wavefronts)
     __shared__ const double *src[TJ * TK][ST_N];
     __shared__ int gave_up;
+    __shared__ int polled;      // steps whose cross-tile reads the poll wavefront has issued (counts up over the tile)
     const int npv = S.npv;
     const int sg = FWD ? 1 : -1;
     const double sgn = FWD ? 1.0 : -1.0;
@@ -990,6 +991,7 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_sweep_tile(DS S, const int *t
                     }
                 }
             };
+            if (threadIdx.x == 256) polled = 0;
             __syncthreads();
             for (int s = sa;; s += sg) {
                 if (s != sa) publish(s - sg);
@@ -1012,7 +1014,18 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_sweep_tile(DS S, const int *t
             const int BI = B.I;
             __syncthreads();
             int d0 = FWD ? sa - j0 - k0 - 2 : BI - (sa - j0 - k0);      // distance of pencil 0 (it grows by one a step)
+            int nstep = 0;
             for (int s = sa;; s += sg) {
+                // the poll wavefront's reads of this step enter the memory
+                // pipeline first, then the transfers
+                nstep++;
+                for (long spins = 0; __hip_atomic_load(&polled, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) < nstep;) {
+                    if (++spins > POLL_LIMIT) {      // cannot happen: the poll wavefront counts every step
+                        gave_up = 1;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
                 if (s != sb) {
 #pragma unroll
                     for (int g = 0; g < TJ * TK; g++) {
@@ -1098,10 +1111,9 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_sweep_tile(DS S, const int *t
             for (int s = sa;; s += sg) {
                 // the next step's values are read at the start of this step,
                 // while the pencils compute, as late as they can be
-                if (s != sb) {
-                    issue(s + sg);
-                    deliver(s + sg);
-                }
+                if (s != sb) issue(s + sg);
+                if (threadIdx.x == 320) __hip_atomic_fetch_add(&polled, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                if (s != sb) deliver(s + sg);
                 lds_barrier();
                 if (s == sb) break;
             }
