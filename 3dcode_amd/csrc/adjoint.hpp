@@ -52,6 +52,7 @@ struct AD {
     double *cdb;            // cell-source property bars [ND][NC]
     double *csb;            // cell-source d(phi_c)/d(u_ref, dp_ref, t_ref) [3][NC] (the mass-transfer scales)
     double *pb;             // pv_bar [npv][NG]
+    double *gdb;            // a ghosted cell's gathered property bars [ND][NG], for the EOS pass
     double *mtb[3];         // eddy-viscosity bars of a face's two cells [2][NF] (dmut)
     unsigned long long mask;   // stencil points a face writes (bit ll-1)
     int dmut;               // 1: the interior cells' eddy viscosity is the SST function of
@@ -418,9 +419,10 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 }
 
 // pv_bar of every ghosted cell: its bars from the faces whose stencils hold
-// it and from the sources of its neighbours, in a fixed order, then its
-// property bars through the EOS.
-__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_adj_gather(DS S, AD A)
+// it and from the sources of its neighbours, in a fixed order, and its
+// gathered property bars (k_adj_eos takes those through the EOS: the dual
+// pass needs the registers this memory-bound gather does not).
+__global__ __launch_bounds__(256) void k_adj_gather(DS S, AD A)
 {
     long gl = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (gl >= S.NG) return;
@@ -468,14 +470,30 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         const long ct = cidx(B, i, j, k);
         for (int m = 0; m < ND; m++) dcb[m] += A.cdb[(long)m * S.NC + ct];
     }
-    bool any = false;
-    for (int m = 0; m < ND; m++) any = any || dcb[m] != 0.0;
-    if (any) {
-        double x4[4] = {S.pv[gl] + P.pref, S.pv[4 * S.NG + gl], S.pv[5 * S.NG + gl], S.pv[6 * S.NG + gl]}, gx[4];
-        eos_adj(P.iturb != -3, x4, dcb, gx);
-        acc[0] += gx[0]; acc[4] += gx[1]; acc[5] += gx[2]; acc[6] += gx[3];
-    }
     for (int n = 0; n < npv; n++) A.pb[(long)n * S.NG + gl] = acc[n];
+    for (int m = 0; m < ND; m++) A.gdb[(long)m * S.NG + gl] = dcb[m];
+}
+
+// the gathered property bars of a ghosted cell back to (p, T, y1, y2),
+// added to the pv_bar k_adj_gather left
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_adj_eos(DS S, AD A)
+{
+    long gl = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gl >= S.NG) return;
+    const Par &P = S.P;
+    double dcb[ND];
+    bool any = false;
+    for (int m = 0; m < ND; m++) {
+        dcb[m] = A.gdb[(long)m * S.NG + gl];
+        any = any || dcb[m] != 0.0;
+    }
+    if (!any) return;
+    double x4[4] = {S.pv[gl] + P.pref, S.pv[4 * S.NG + gl], S.pv[5 * S.NG + gl], S.pv[6 * S.NG + gl]}, gx[4];
+    eos_adj(P.iturb != -3, x4, dcb, gx);
+    A.pb[gl] += gx[0];
+    A.pb[4 * S.NG + gl] += gx[1];
+    A.pb[5 * S.NG + gl] += gx[2];
+    A.pb[6 * S.NG + gl] += gx[3];
 }
 
 // deterministic sum of n doubles in two stages: workgroup b sums its strided
@@ -533,7 +551,7 @@ int adj_alloc(cav3d_ctx *c)
     int rc;
     if ((rc = dalloc(c, &c->a_rb, (long)npv * S.NC)) || (rc = dalloc(c, &c->a_cb, 7L * npv * S.NC)) ||
         (rc = dalloc(c, &c->a_cdb, (long)ND * S.NC)) || (rc = dalloc(c, &c->a_csb, 3L * S.NC)) || (rc = dalloc(c, &c->a_pb, (long)npv * S.NG)) ||
-        (rc = dalloc(c, &c->a_sum, 16 + 16L * SUM_WG)))
+        (rc = dalloc(c, &c->a_gdb, (long)ND * S.NG)) || (rc = dalloc(c, &c->a_sum, 16 + 16L * SUM_WG)))
         return rc;
     for (int d = 0; d < 3; d++)
         if ((rc = dalloc(c, &c->a_fb[d], (long)NSLOT * npv * S.NF[d])) || (rc = dalloc(c, &c->a_db[d], 2L * ND * S.NF[d])) ||
@@ -553,7 +571,7 @@ int launch_vjp(cav3d_ctx *c, int dmut = 0)
     for (int d = 0; d < 3; d++) {
         A.fb[d] = c->a_fb[d]; A.db[d] = c->a_db[d]; A.ub[d] = c->a_ub[d]; A.mtb[d] = c->a_mtb[d];
     }
-    A.cb = c->a_cb; A.cdb = c->a_cdb; A.csb = c->a_csb; A.pb = c->a_pb;
+    A.cb = c->a_cb; A.cdb = c->a_cdb; A.csb = c->a_csb; A.pb = c->a_pb; A.gdb = c->a_gdb;
     A.mask = adj_mask(S.P);
     const bool sensor = S.P.nscheme == 2 || S.P.nscheme == 3;
     if (sensor) {
@@ -572,7 +590,8 @@ int launch_vjp(cav3d_ctx *c, int dmut = 0)
     }
     if (S.P.ides > 0 || S.P.ncav >= 2) hipLaunchKernelGGL(k_cell_vjp<true>, dim3(nblk(S.NC, 128)), dim3(128), 0, c->st, S, A);
     else hipLaunchKernelGGL(k_cell_vjp<false>, dim3(nblk(S.NC, 128)), dim3(128), 0, c->st, S, A);
-    hipLaunchKernelGGL(k_adj_gather, dim3(nblk(S.NG, 128)), dim3(128), 0, c->st, S, A);
+    hipLaunchKernelGGL(k_adj_gather, dim3(nblk(S.NG, 256)), dim3(256), 0, c->st, S, A);
+    hipLaunchKernelGGL(k_adj_eos, dim3(nblk(S.NG, 128)), dim3(128), 0, c->st, S, A);
     return 0;
 }
 
