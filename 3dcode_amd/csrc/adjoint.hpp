@@ -46,6 +46,8 @@ namespace {
 struct AD {
     const double *rb;       // res_bar [npv][NC]
     double *fb[3];          // bars of a face's stencil points [NSLOT][npv][NF]
+    double *rsb[3];         // a face's reconstruction seeds [4][npv][NF]: left, right, left cell, right cell
+    double *snb[3];         // a face's sensor-pressure bars [18][NF] (Roe-M, AUSMPW+)
     double *db[3];          // cell-property bars of a face's two cells [2][ND][NF]
     double *ub[3];          // d(phi_f)/d(u_ref) [NF]
     double *cb;             // cell-source bars on the 7-point stencil [7][npv][NC]
@@ -87,6 +89,9 @@ __device__ __forceinline__ bool face_exists(const BlkD &B, int D, int i, int j, 
     return i >= lo[0] && i <= B.I && j >= lo[1] && j <= B.J && k >= lo[2] && k <= B.K;
 }
 
+// Inviscid part of a face up to its reconstruction: the seeds of the
+// reconstruction (k_face_rvjp takes them to the stencil points), the two
+// cells' property bars and the u_ref derivative.
 template <int D, bool SENSOR>
 __global__ __launch_bounds__(128) void k_face_vjp(DS S, AD A)
 {
@@ -107,11 +112,10 @@ __global__ __launch_bounds__(128) void k_face_vjp(DS S, AD A)
         g[n] = n < npv ? rbar(S, A, B, ip, jp, kp, n) - rbar(S, A, B, i, j, k, n) : 0.0;
         any = any || g[n] != 0.0;
     }
-    double *fb = A.fb[D], *db = A.db[D];
+    double *rsb = A.rsb[D], *db = A.db[D];
     if (!any) {
-        for (int ll = 1; ll <= NSLOT; ll++)
-            if ((A.mask >> (ll - 1)) & 1ull)
-                for (int n = 0; n < npv; n++) fb[((long)(ll - 1) * npv + n) * NF + t] = 0.0;
+        for (int m = 0; m < 4 * npv; m++) rsb[m * NF + t] = 0.0;
+        if (SENSOR) for (int q = 0; q < 18; q++) A.snb[D][q * NF + t] = 0.0;
         for (int m = 0; m < 2 * ND; m++) db[m * NF + t] = 0.0;
         A.ub[D][t] = 0.0;
         return;
@@ -123,9 +127,42 @@ __global__ __launch_bounds__(128) void k_face_vjp(DS S, AD A)
     face_vec(S, B, D, i, j, k, n3);
     const double cellp[6] = {S.dc[D_RHO * S.NG + g9], S.dc[D_H * S.NG + g9], S.dc[D_C2 * S.NG + g9],
                              S.dc[D_RHO * S.NG + gq], S.dc[D_H * S.NG + gq], S.dc[D_C2 * S.NG + gq]};
-    double ub;
-    face_adj<SENSOR>(P, npv, f0, S.NG, off, n3, cellp, g, A.mask, fb + t, (long)npv * NF, NF, db + t, NF, ub);
+    double ub, xlb[NPV], xrb[NPV], c9b[NPV], c10b[NPV], senb[18];
+    face_adj_core<SENSOR>(P, npv, f0, S.NG, off, n3, cellp, g, xlb, xrb, c9b, c10b, senb, db + t, NF, ub);
+    for (int n = 0; n < npv; n++) {
+        rsb[(0 * npv + n) * NF + t] = xlb[n];
+        rsb[(1 * npv + n) * NF + t] = xrb[n];
+        rsb[(2 * npv + n) * NF + t] = c9b[n];
+        rsb[(3 * npv + n) * NF + t] = c10b[n];
+    }
+    if (SENSOR) for (int q = 0; q < 18; q++) A.snb[D][q * NF + t] = senb[q];
     A.ub[D][t] = ub;
+}
+
+// The reconstruction part of a face's inviscid adjoint, one thread per face
+// and variable: the seeds k_face_vjp left, back to the face's stencil points.
+template <int D, bool SENSOR>
+__global__ __launch_bounds__(256) void k_face_rvjp(DS S, AD A)
+{
+    const long NF = S.NF[D];
+    long u = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= (long)S.npv * NF) return;
+    const int n = (int)(u / NF);
+    const long t = u - (long)n * NF;
+    int i, j, k;
+    const BlkD &B = face_of<D>(S, t, i, j, k);
+    const int npv = S.npv;
+    const double *rsb = A.rsb[D];
+    const double *f0 = S.pv + (long)n * S.NG + gidx(B, i, j, k);
+    const double *senb = nullptr;
+    double sn18[18];
+    if (SENSOR && n == 0) {
+        for (int q = 0; q < 18; q++) sn18[q] = A.snb[D][q * NF + t];
+        senb = sn18;
+    }
+    face_adj_recon(S.P, n, [&](int ll) -> double { return f0[stencil_off<D>(B, ll)]; }, rsb[(0 * npv + n) * NF + t],
+                   rsb[(1 * npv + n) * NF + t], rsb[(2 * npv + n) * NF + t], rsb[(3 * npv + n) * NF + t], senb, A.mask,
+                   A.fb[D] + (long)n * NF + t, (long)npv * NF);
 }
 
 // Viscous part of a face, added to the inviscid bars (stencil points 1..18,
@@ -555,6 +592,8 @@ int adj_alloc(cav3d_ctx *c)
         return rc;
     for (int d = 0; d < 3; d++)
         if ((rc = dalloc(c, &c->a_fb[d], (long)NSLOT * npv * S.NF[d])) || (rc = dalloc(c, &c->a_db[d], 2L * ND * S.NF[d])) ||
+            (rc = dalloc(c, &c->a_rsb[d], 4L * npv * S.NF[d])) ||
+            (rc = dalloc(c, &c->a_snb[d], (S.P.nscheme == 2 || S.P.nscheme == 3) ? 18L * S.NF[d] : 1L)) ||
             (rc = dalloc(c, &c->a_ub[d], S.NF[d])) || (rc = dalloc(c, &c->a_mtb[d], 2L * S.NF[d])))
             return rc;
     HIPCHK(hipStreamSynchronize(c->st));
@@ -570,18 +609,26 @@ int launch_vjp(cav3d_ctx *c, int dmut = 0)
     A.dmut = dmut;
     for (int d = 0; d < 3; d++) {
         A.fb[d] = c->a_fb[d]; A.db[d] = c->a_db[d]; A.ub[d] = c->a_ub[d]; A.mtb[d] = c->a_mtb[d];
+        A.rsb[d] = c->a_rsb[d]; A.snb[d] = c->a_snb[d];
     }
     A.cb = c->a_cb; A.cdb = c->a_cdb; A.csb = c->a_csb; A.pb = c->a_pb; A.gdb = c->a_gdb;
     A.mask = adj_mask(S.P);
     const bool sensor = S.P.nscheme == 2 || S.P.nscheme == 3;
+    const int npv = S.npv;
     if (sensor) {
         hipLaunchKernelGGL((k_face_vjp<0, true>), dim3(nblk(S.NF[0], 128)), dim3(128), 0, c->st, S, A);
         hipLaunchKernelGGL((k_face_vjp<1, true>), dim3(nblk(S.NF[1], 128)), dim3(128), 0, c->st, S, A);
         hipLaunchKernelGGL((k_face_vjp<2, true>), dim3(nblk(S.NF[2], 128)), dim3(128), 0, c->st, S, A);
+        hipLaunchKernelGGL((k_face_rvjp<0, true>), dim3(nblk(npv * S.NF[0], 256)), dim3(256), 0, c->st, S, A);
+        hipLaunchKernelGGL((k_face_rvjp<1, true>), dim3(nblk(npv * S.NF[1], 256)), dim3(256), 0, c->st, S, A);
+        hipLaunchKernelGGL((k_face_rvjp<2, true>), dim3(nblk(npv * S.NF[2], 256)), dim3(256), 0, c->st, S, A);
     } else {
         hipLaunchKernelGGL((k_face_vjp<0, false>), dim3(nblk(S.NF[0], 128)), dim3(128), 0, c->st, S, A);
         hipLaunchKernelGGL((k_face_vjp<1, false>), dim3(nblk(S.NF[1], 128)), dim3(128), 0, c->st, S, A);
         hipLaunchKernelGGL((k_face_vjp<2, false>), dim3(nblk(S.NF[2], 128)), dim3(128), 0, c->st, S, A);
+        hipLaunchKernelGGL((k_face_rvjp<0, false>), dim3(nblk(npv * S.NF[0], 256)), dim3(256), 0, c->st, S, A);
+        hipLaunchKernelGGL((k_face_rvjp<1, false>), dim3(nblk(npv * S.NF[1], 256)), dim3(256), 0, c->st, S, A);
+        hipLaunchKernelGGL((k_face_rvjp<2, false>), dim3(nblk(npv * S.NF[2], 256)), dim3(256), 0, c->st, S, A);
     }
     if (S.P.iturb >= -2) {
         hipLaunchKernelGGL((k_vface_vjp<0>), dim3(nblk(S.NF[0], 128)), dim3(128), 0, c->st, S, A);

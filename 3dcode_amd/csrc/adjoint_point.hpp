@@ -58,18 +58,19 @@ template <class R, class Z> HD R c2_average(const Par &P, Z z)
     return face_c2<R>(R(0.5) * (z(0) + z(1)) + R(P.pref), R(0.5) * (z(2) + z(3)), ya1, ya2);
 }
 
-// The inviscid adjoint of one face, independent of its direction: f0 points
-// at the left cell's pressure, off[ll-1] is stencil point ll's offset and ng
-// the component stride; cellp = {rho, h, c^2} of the left and right cells
-// (read by first-order faces); g the seed.  Out: sbo[(ll-1)*ss + n*sn] the bar
-// of stencil point ll of variable n for every ll in mask (bit ll-1),
-// dcbo[(side*ND + m)*dstr] the two cells' property bars, ub the u_ref
-// derivative.  Inlined into its kernel: as a called function its locals
+// The inviscid adjoint of one face up to its reconstruction, independent of
+// its direction: f0 points at the left cell's pressure, off[ll-1] is stencil
+// point ll's offset and ng the component stride; cellp = {rho, h, c^2} of the
+// left and right cells (read by first-order faces); g the seed.  Out: the
+// seeds of the reconstruction (xlb, xrb: the reconstructed values; c9b, c10b:
+// the cells' own values where a clamp chose them), senb the sensor
+// pressures' bars (sensor schemes), dcbo[(side*ND + m)*dstr] the two cells'
+// property bars, ub the u_ref derivative.  Inlined into its kernel: as a called function its locals
 // lived in 5.3 KB of private memory per lane, inlined in 1.4 KB.
 template <bool SENSOR>
-__host__ __device__ __attribute__((always_inline)) inline void face_adj(const Par &P, int npv, const double *f0, long ng, const long *off, const double *n3,
-                                      const double *cellp, const double *g, unsigned long long mask, double *sbo,
-                                      long ss, long sn, double *dcbo, long dstr, double &ub)
+__host__ __device__ __attribute__((always_inline)) inline void face_adj_core(const Par &P, int npv, const double *f0, long ng, const long *off,
+                                      const double *n3, const double *cellp, const double *g, double *xlb, double *xrb,
+                                      double *c9b, double *c10b, double *senb, double *dcbo, long dstr, double &ub)
 {
     auto xs = [&](int ll, int n) -> double { return f0[(long)n * ng + off[ll - 1]]; };
     // forward recompute: reconstruction, clamps, face EOS, averaged c^2
@@ -196,81 +197,106 @@ __host__ __device__ __attribute__((always_inline)) inline void face_adj(const Pa
         if (SENSOR) { dcb[D_C2] = c2Lb; dcb[ND + D_C2] = c2Rb; }
     }
     // through the clamps to the raw reconstruction or the cells
-    double xlb[NPV], xrb[NPV], c9b[NPV], c10b[NPV];
     for (int n = 0; n < NPV; n++) {
         xlb[n] = ol[n] == 0 ? qlb[n] : 0.0;
         xrb[n] = orr[n] == 0 ? qrb[n] : 0.0;
         c9b[n] = ol[n] == 1 ? qlb[n] : 0.0;
         c10b[n] = orr[n] == 1 ? qrb[n] : 0.0;
     }
-    // reconstruction, one variable at a time, and the stencil bars
-    constexpr int TVD6[6] = {9, 10, 23, 32, 37, 38};
-    for (int n = 0; n < npv; n++) {
-        double sb[NSLOT];
-        for (int q = 0; q < NSLOT; q++) sb[q] = 0.0;
-        if (P.nmuscl == 0) {
-            sb[8] += xlb[n];
-            sb[9] += xrb[n];
-        } else if (xlb[n] != 0.0 || xrb[n] != 0.0) {
-            const double wl = xlb[n], wr = xrb[n];
-            if (P.nmuscl == 2) {
-                // the points the MLP value can depend on: the fourteen it reads
-                // directly and the extremum each min / max group selects (the
-                // first one, as mn / mx keep it); the others enter as constants
-                int sup[NSLOT], ns = 0;
-                unsigned long long seen = 0;
-                auto add = [&](int ll) { if (!((seen >> ll) & 1ull)) { seen |= 1ull << ll; sup[ns++] = ll; } };
-                constexpr int DIRECT[14] = {9, 10, 23, 32, 37, 38, 3, 4, 7, 8, 11, 12, 15, 16};
-                for (int q = 0; q < 14; q++) add(DIRECT[q]);
-                auto ext = [&](int lo, int hi, int step) {
-                    int amin = lo, amax = lo;
-                    double vmin = xs(lo, n), vmax = vmin;
-                    for (int q = lo + step; q <= hi; q += step) {
-                        const double v = xs(q, n);
-                        if (v < vmin) { vmin = v; amin = q; }
-                        if (v > vmax) { vmax = v; amax = q; }
-                    }
-                    add(amin);
-                    add(amax);
-                };
-                ext(1, 18, 1); ext(19, 27, 1); ext(28, 36, 1); ext(1, 17, 2); ext(2, 18, 2);
-                // four support points per pass, each read compared against them
-                for (int q = 0; q < ns; q += 4) {
-                    int sl4[4];
-                    for (int l = 0; l < 4; l++) sl4[l] = q + l < ns ? sup[q + l] : -1;
-                    Dn<4> a, b;
-                    muscl_var<Dn<4>>(P.nlim, true, [&](int ll) -> Dn<4> {
-                        Dn<4> r(xs(ll, n));
-                        for (int l = 0; l < 4; l++)
-                            if (ll == sl4[l]) r.d[l] = 1.0;
-                        return r;
-                    }, a, b);
-                    const Dn<4> phi = wl * a + wr * b;
-                    for (int l = 0; l < 4; l++)
-                        if (sl4[l] > 0) sb[sl4[l] - 1] += phi.d[l];
-                }
-            } else {
-                double gx[6];
-                dn_grad<6>(6, [&](int q) -> double { return xs(TVD6[q], n); }, [&](auto in, int) -> Dn<6> {
-                    Dn<6> a, b;
-                    muscl_var<Dn<6>>(P.nlim, false, [&](int ll) -> Dn<6> {
-                        return in(ll == 9 ? 0 : ll == 10 ? 1 : ll == 23 ? 2 : ll == 32 ? 3 : ll == 37 ? 4 : 5);
-                    }, a, b);
-                    return wl * a + wr * b;
-                }, gx);
-                for (int q = 0; q < 6; q++) sb[TVD6[q] - 1] += gx[q];
-            }
-        }
-        sb[8] += c9b[n];
-        sb[9] += c10b[n];
-        if (SENSOR && n == 0)
-            for (int q = 0; q < 18; q++) sb[q] += zb[o + 7 + q];
-        for (int ll = 1; ll <= NSLOT; ll++)
-            if ((mask >> (ll - 1)) & 1ull) sbo[(long)(ll - 1) * ss + (long)n * sn] = sb[ll - 1];
-    }
+    if (SENSOR) for (int q = 0; q < 18; q++) senb[q] = zb[o + 7 + q];
     for (int m = 0; m < 2 * ND; m++) dcbo[(long)m * dstr] = dcb[m];
 }
 
+// The reconstruction part of one face's inviscid adjoint, for variable n
+// (xs(ll): its value at stencil point ll): the seeds wl, wr of its left and
+// right reconstructed values and c9bn, c10bn of the two cells' own values
+// (where a clamp chose them) go back to the stencil points; senb (variable
+// 0 of the sensor schemes, else null) adds the sensor pressures' bars.
+// Out: sbo[(ll-1)*ss] for every ll in mask.
+template <class XS>
+__host__ __device__ __attribute__((always_inline)) inline void face_adj_recon(const Par &P, int n, XS xs1, double wl, double wr,
+                                      double c9bn, double c10bn, const double *senb, unsigned long long mask,
+                                      double *sbo, long ss)
+{
+    constexpr int TVD6[6] = {9, 10, 23, 32, 37, 38};
+    auto xs = [&](int ll, int) -> double { return xs1(ll); };
+    double sb[NSLOT];
+    for (int q = 0; q < NSLOT; q++) sb[q] = 0.0;
+    if (P.nmuscl == 0) {
+        sb[8] += wl;
+        sb[9] += wr;
+    } else if (wl != 0.0 || wr != 0.0) {
+        if (P.nmuscl == 2) {
+            // the points the MLP value can depend on: the fourteen it reads
+            // directly and the extremum each min / max group selects (the
+            // first one, as mn / mx keep it); the others enter as constants
+            int sup[NSLOT], ns = 0;
+            unsigned long long seen = 0;
+            auto add = [&](int ll) { if (!((seen >> ll) & 1ull)) { seen |= 1ull << ll; sup[ns++] = ll; } };
+            constexpr int DIRECT[14] = {9, 10, 23, 32, 37, 38, 3, 4, 7, 8, 11, 12, 15, 16};
+            for (int q = 0; q < 14; q++) add(DIRECT[q]);
+            auto ext = [&](int lo, int hi, int step) {
+                int amin = lo, amax = lo;
+                double vmin = xs(lo, n), vmax = vmin;
+                for (int q = lo + step; q <= hi; q += step) {
+                    const double v = xs(q, n);
+                    if (v < vmin) { vmin = v; amin = q; }
+                    if (v > vmax) { vmax = v; amax = q; }
+                }
+                add(amin);
+                add(amax);
+            };
+            ext(1, 18, 1); ext(19, 27, 1); ext(28, 36, 1); ext(1, 17, 2); ext(2, 18, 2);
+            // four support points per pass, each read compared against them
+            for (int q = 0; q < ns; q += 4) {
+                int sl4[4];
+                for (int l = 0; l < 4; l++) sl4[l] = q + l < ns ? sup[q + l] : -1;
+                Dn<4> a, b;
+                muscl_var<Dn<4>>(P.nlim, true, [&](int ll) -> Dn<4> {
+                    Dn<4> r(xs(ll, n));
+                    for (int l = 0; l < 4; l++)
+                        if (ll == sl4[l]) r.d[l] = 1.0;
+                    return r;
+                }, a, b);
+                const Dn<4> phi = wl * a + wr * b;
+                for (int l = 0; l < 4; l++)
+                    if (sl4[l] > 0) sb[sl4[l] - 1] += phi.d[l];
+            }
+        } else {
+            double gx[6];
+            dn_grad<6>(6, [&](int q) -> double { return xs(TVD6[q], n); }, [&](auto in, int) -> Dn<6> {
+                Dn<6> a, b;
+                muscl_var<Dn<6>>(P.nlim, false, [&](int ll) -> Dn<6> {
+                    return in(ll == 9 ? 0 : ll == 10 ? 1 : ll == 23 ? 2 : ll == 32 ? 3 : ll == 37 ? 4 : 5);
+                }, a, b);
+                return wl * a + wr * b;
+            }, gx);
+            for (int q = 0; q < 6; q++) sb[TVD6[q] - 1] += gx[q];
+        }
+    }
+    sb[8] += c9bn;
+    sb[9] += c10bn;
+    if (senb)
+        for (int q = 0; q < 18; q++) sb[q] += senb[q];
+    for (int ll = 1; ll <= NSLOT; ll++)
+        if ((mask >> (ll - 1)) & 1ull) sbo[(long)(ll - 1) * ss] = sb[ll - 1];
+}
+
+// The inviscid adjoint of one face: face_adj_core, then face_adj_recon for
+// every variable (the kernels run the two parts as separate passes).  Out
+// as face_adj_core, and sbo[(ll-1)*ss + n*sn] the bar of stencil point ll of
+// variable n for every ll in mask (bit ll-1).
+template <bool SENSOR>
+__host__ __device__ __attribute__((always_inline)) inline void face_adj(const Par &P, int npv, const double *f0, long ng, const long *off, const double *n3,
+                                      const double *cellp, const double *g, unsigned long long mask, double *sbo,
+                                      long ss, long sn, double *dcbo, long dstr, double &ub)
+{
+    double xlb[NPV], xrb[NPV], c9b[NPV], c10b[NPV], senb[18];
+    face_adj_core<SENSOR>(P, npv, f0, ng, off, n3, cellp, g, xlb, xrb, c9b, c10b, senb, dcbo, dstr, ub);
+    for (int n = 0; n < npv; n++)
+        face_adj_recon(P, n, [&](int ll) -> double { return f0[(long)n * ng + off[ll - 1]]; }, xlb[n], xrb[n], c9b[n],
+                       c10b[n], SENSOR && n == 0 ? senb : nullptr, mask, sbo + (long)n * sn, ss);
+}
 
 // dual passes of one face's viscous flux over its inputs zx (the layout of
 // k_vface_vjp); inlined into each direction's kernel (as a called function

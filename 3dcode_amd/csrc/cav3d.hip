@@ -1472,7 +1472,7 @@ struct cav3d_ctx {
     long xbuf_n = 0;
     // adjoint work arrays (allocated by the first cav3d_rhs_vjp)
     bool adj_ready = false;
-    double *a_gdb = nullptr;
+    double *a_gdb = nullptr, *a_rsb[3] = {}, *a_snb[3] = {};
     double *a_rb = nullptr, *a_fb[3] = {}, *a_db[3] = {}, *a_ub[3] = {}, *a_mtb[3] = {}, *a_cb = nullptr, *a_cdb = nullptr,
            *a_csb = nullptr,
            *a_pb = nullptr, *a_sum = nullptr, *a_save = nullptr, *a_obj = nullptr;
