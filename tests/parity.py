@@ -33,6 +33,10 @@ def small_cases(pk):
         # pseudo-time steps, src/timestep_module.f90:214-316
         "lam_roe_mintime": lambda: _local(bc(12, 8, 6, False, False, 1, 1, 1, cfl=5.0, seed=14), 0, 5.0),
         "sst_cav_fixedtime": lambda: _local(_clip(pk.cases.config2(12, 8, 6), 1), -1, 2.0e-3),
+        # blocks three to five cells thick in i (the fewest a block may have): the sweeps' stage transfers
+        # move cells in pairs, clamped at both ends of such short pencils
+        "lam_roe_thin_blocks": lambda: pk.cases.split_i(pk.cases.config1(12, 8, 6), [4, 8]),
+        "sst_cav_thin_blocks": lambda: _clip(pk.cases.split_i(pk.cases.config2(12, 8, 6), [4, 8]), 1),
     }
 
 

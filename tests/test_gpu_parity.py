@@ -15,7 +15,7 @@ NAMES = ["lam_roe_tvd", "lam_roe_mlp", "lam_ausm_tvd_vanleer", "sst_cav_ausm_mlp
          "sst_cav_as_written", "lam_roe_2block", "sst_cav_3block", "lam_roe_bdf2", "sst_cav_bdf1",
          "sst_kunz", "sst_singhal", "sst_schnerr_sauer", "sst_lee", "sst_cav_roem_mlp", "lam_ausmpwp_tvd",
          "sst_cav_ausmpwp_first", "sst_des", "sst_ddes", "sst_iddes", "sst_siddes", "lam_roe_mintime",
-         "sst_cav_fixedtime"]
+         "sst_cav_fixedtime", "lam_roe_thin_blocks", "sst_cav_thin_blocks"]
 
 
 def _case(pkg, name):
