@@ -789,8 +789,8 @@ __device__ __forceinline__ void st_dev(double *p, double v)
 // through the device-coherent dq array, waited for with a bounded poll.
 // Everything a cell reads that does not depend on the sweep (its residual,
 // state, properties, diagonal-inverse terms, and the Jacobian operands of its
-// j and k neighbours) is fetched one hyperplane ahead into an LDS stage, so
-// a step's critical path is the dq hand-over and the arithmetic.
+// j and k neighbours) is moved ahead of use into an LDS stage, so a step's
+// critical path is the dq hand-over and the arithmetic.
 // Tiles wait only on tiles upstream in j and k, every tile is resident
 // (cooperative launch) and a workgroup takes its tiles in topological order,
 // so the chain always advances.
@@ -902,7 +902,7 @@ struct WinRow {
     __device__ double operator[](int e) const { return p[e * WIN]; }
 };
 constexpr int SWEEP_THREADS = 256 + 64 + 64 + 64;   // 16 pencil groups of 16 lanes, a store, a poll and a loader wavefront
-constexpr int NEDGE = 2 * 4 * NPV;
+constexpr int NEDGE = 2 * 4 * NPV;   // cross-tile dq values a tile reads per step: two edges of four pencils
 // Stage loads, by the loader wavefront, as LDS-DMA (global memory to LDS,
 // 16 bytes a lane, no registers).  A pencil at sweep distance d reads window
 // d >> 2 (its cell) and window (d + 1) >> 2 (the lookahead); while d & 3 is
@@ -912,13 +912,14 @@ constexpr int NEDGE = 2 * 4 * NPV;
 // the barrier that ends the next step, so every transfer lands while its
 // buffer is free and window W + 1 is complete when it first becomes the
 // lookahead (d & 3 = 3).  A window's four cells lie in ascending i in LDS in
-// both sweeps, as the transfers copy them.   // cross-tile dq values a tile reads per step: two edges of four pencils
+// both sweeps, as the transfers copy them.
 static_assert(TJ == 4 && TK == 4, "the poll wavefront maps an edge of four pencils");
 
 // Division of labour inside a tile's workgroup:
 //  - four pencil wavefronts (16 groups of 16 lanes, lane = row) run the
-//    hyperplane steps; they read only LDS and issue no global memory
-//    access, so nothing they wait for is slower than LDS;
+//    hyperplane steps; they read only LDS (their one global access is the
+//    store of a downstream-edge pencil's dq), so nothing they wait for is
+//    slower than LDS;
 //  - a poll wavefront reads, with device-coherent loads, the dq of the
 //    upstream cells in other tiles during the step before its use and hands
 //    it over through LDS (it is the only wavefront that waits on other tiles);
