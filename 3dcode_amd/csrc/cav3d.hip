@@ -923,9 +923,9 @@ static_assert(TJ == 4 && TK == 4, "the poll wavefront maps an edge of four penci
 //  - a poll wavefront reads, with device-coherent loads, the dq of the
 //    upstream cells in other tiles during the step before its use and hands
 //    it over through LDS (it is the only wavefront that waits on other tiles);
-//  - a store wavefront writes each hyperplane's dq (and dcv) one step later,
-//    except the dq of the pencils on the tile's downstream edges, which the
-//    pencil wavefronts store as soon as they form it;
+//  - a store wavefront writes each hyperplane's dcv (forward) or dq
+//    (backward) one step later; the dq of the pencils on the tile's
+//    downstream edges the pencil wavefronts store as soon as they form it;
 //  - a loader wavefront fills the stage windows ahead of the pencils.
 // A step's Jacobian coefficients are formed during the step before from its
 // stage without branching on the row, so the dependent chain of a step is
@@ -944,7 +944,10 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_sweep_tile(DS S, const int *t
     auto lds = [&](int par, int g) { return hand + lds_at(par, g); };
     auto xch = [&](int par, int a, int g) { return hand + xch_at(par, a, g); };
     double *const zrow = hand + HZ;
-    __shared__ double ylds[2][TJ * TK][NPV];     // dcv of the hyperplane (forward sweep)
+    // dcv of the last YR hyperplanes (forward sweep), by s mod YR: the store
+    // wavefront writes a pencil's dcv four cells at a time
+    constexpr int YR = 5;
+    __shared__ double ylds[FWD ? YR : 1][TJ * TK][NPV];
     __shared__ double win[2][TJ * TK][ST_N][WIN];   // stage windows (see the pencil wavefronts)
     __shared__ const double *src[TJ * TK][ST_N];
     __shared__ int gave_up;
@@ -976,19 +979,31 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_sweep_tile(DS S, const int *t
             return jj <= jend && kk <= kend && ii - jj - kk >= 2 && ii - jj - kk <= B.I;
         };
         if (wave == 4) {
-            // store wavefront: hyperplane s is published during step s+1,
-            // while the pencil wavefronts compute; the dq of the pencils on
+            // store wavefront: hyperplane s is written out during step s+1,
+            // while the pencil wavefronts compute.  The dq of the pencils on
             // the tile's downstream edges (the values other tiles wait for)
-            // the pencil wavefronts store themselves, as soon as it is formed
+            // the pencil wavefronts store themselves, as soon as it is
+            // formed; no other tile reads the rest.  Forward, that rest is
+            // read by nothing (the backward sweep starts from dcv), so only
+            // dcv is written, a pencil's four cells of a stage window (or
+            // its last few) together and non-temporal (the backward sweep
+            // reads them after this launch); backward, the rest of dq is the
+            // update's input.
             auto publish = [&](int s) {
                 const int pb = s & 1;
                 for (int e = threadIdx.x - 256; e < TJ * TK * NPV; e += 64) {
                     const int g = e / NPV, n = e - g * NPV;
                     if (n < npv && active(g, s)) {
-                        const int jj = j0 + g % TJ, kk = k0 + g / TJ;
-                        const long w = n * NW + widx(B, s - jj - kk, jj, kk);
-                        if (FWD) S.dcv[w] = ylds[pb][g][n];
-                        if (!down_edge(g)) st_dev(dq + w, lds(pb, g)[n]);
+                        const int jj = j0 + g % TJ, kk = k0 + g / TJ, i = s - jj - kk;
+                        if constexpr (FWD) {
+                            const int m = (i - 2) & 3;          // cells i - m .. i of the window
+                            if (m == 3 || i == B.I) {
+                                double *o = S.dcv + n * NW + widx(B, i - m, jj, kk);
+                                for (int c = 0; c <= m; c++) __builtin_nontemporal_store(ylds[(s - m + c) % YR][g][n], o + c);
+                            }
+                        } else if (!down_edge(g)) {
+                            dq[n * NW + widx(B, i, jj, kk)] = lds(pb, g)[n];
+                        }
                     }
                 }
             };
@@ -1207,7 +1222,7 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_sweep_tile(DS S, const int *t
                                              st[ST_DC + D_RT], yv, lane, y, st[ST_PV + ln]);
             if (lane < NPV) {
                 lds(s & 1, grp)[lane] = act ? xn : 0.0;
-                if (FWD) ylds[s & 1][grp][lane] = y;
+                if constexpr (FWD) ylds[s % YR][grp][lane] = y;
                 // the pencil wavefronts have no loads in flight, so this store
                 // costs only its issue
                 if (edge && act && mine) st_dev(dq + lane * NW + widx(B, i, j, k), xn);
@@ -1238,7 +1253,10 @@ __global__ __launch_bounds__(256) void k_fill_bits(double *p, long n, unsigned l
 }
 
 // ------------------------------------------------------------ update + norms
-__global__ __launch_bounds__(256) void k_update(DS S)
+// Two wavefronts per SIMD: left to itself the compiler gives the cell's EOS
+// 268 registers, one wavefront per SIMD, and the loads go unhidden (159 us
+// against 99 us with a small spill, 524288 cells).
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_update(DS S)
 {
     __shared__ double sred[4][NPV + 1];
     __shared__ long sidx[4];
