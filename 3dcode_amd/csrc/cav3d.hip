@@ -68,7 +68,9 @@ struct DS {           // device state, passed by value
     long NG, NC, NW, NF[3], NCX, NEX, NTX, NBC[3], NHALO;
     double *pv, *dc, *mut, *vol, *ydn, *cx, *ex, *tx;
     double *res, *dt, *icav, *itt, *ocut, *rcav, *qq;
-    double *dqf, *dqb, *dcv;
+    double *dcv;
+    double *xqf, *xqb; // dq of the sweeps [NW][NPV], a cell's values adjacent: forward
+                       // the cross-tile values only, backward every cell's
     double *gx;       // diagonal-inverse terms [NGX][NC]
     double *zero;     // zeros, one more than the longest pencil: the sweeps' source for absent items
     double *jc;       // Jacobian cell terms [2][NC]: beta, h
@@ -786,7 +788,9 @@ __device__ __forceinline__ void st_dev(double *p, double v)
 // walks its hyperplanes s = i + j + k in sweep order with one barrier per
 // hyperplane.  A pencil's neighbour inside the tile hands over its dq through
 // LDS (double-buffered by the parity of s); a neighbour in another tile
-// through the device-coherent dq array, waited for with a bounded poll.
+// through a device-coherent exchange array that holds a cell's values side by
+// side (one write of 72 bytes, not nine 8-byte writes to nine lines), waited
+// for with a bounded poll.
 // Everything a cell reads that does not depend on the sweep (its residual,
 // state, properties, diagonal-inverse terms, and the Jacobian operands of its
 // j and k neighbours) is moved ahead of use into an LDS stage, so a step's
@@ -955,7 +959,7 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_sweep_tile(DS S, const int *t
     const int npv = S.npv;
     const int sg = FWD ? 1 : -1;
     const double sgn = FWD ? 1.0 : -1.0;
-    double *dq = FWD ? S.dqf : S.dqb;
+    double *const xq = FWD ? S.xqf : S.xqb;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 15, grp = (threadIdx.x >> 4) & 15;
     constexpr int PER = ST_N / 16;                                  // stage items per lane
@@ -980,15 +984,15 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_sweep_tile(DS S, const int *t
         };
         if (wave == 4) {
             // store wavefront: hyperplane s is written out during step s+1,
-            // while the pencil wavefronts compute.  The dq of the pencils on
-            // the tile's downstream edges (the values other tiles wait for)
-            // the pencil wavefronts store themselves, as soon as it is
-            // formed; no other tile reads the rest.  Forward, that rest is
-            // read by nothing (the backward sweep starts from dcv), so only
-            // dcv is written, a pencil's four cells of a stage window (or
-            // its last few) together and non-temporal (the backward sweep
-            // reads them after this launch); backward, the rest of dq is the
-            // update's input.
+            // while the pencil wavefronts compute.  The dq that other tiles
+            // wait for (the pencils on the tile's downstream edges) the
+            // pencil wavefronts store themselves into the exchange array, as
+            // soon as it is formed.  Forward, dq is read by nothing else (the
+            // backward sweep starts from dcv), so only dcv is written, a
+            // pencil's four cells of a stage window (or its last few)
+            // together and non-temporal (the backward sweep reads them after
+            // this launch); backward, the exchange array also takes the other
+            // pencils' dq, the update's input.
             auto publish = [&](int s) {
                 const int pb = s & 1;
                 for (int e = threadIdx.x - 256; e < TJ * TK * NPV; e += 64) {
@@ -1002,7 +1006,7 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_sweep_tile(DS S, const int *t
                                 for (int c = 0; c <= m; c++) __builtin_nontemporal_store(ylds[(s - m + c) % YR][g][n], o + c);
                             }
                         } else if (!down_edge(g)) {
-                            dq[n * NW + widx(B, i, jj, kk)] = lds(pb, g)[n];
+                            xq[widx(B, i, jj, kk) * NPV + n] = lds(pb, g)[n];
                         }
                     }
                 }
@@ -1077,14 +1081,14 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_sweep_tile(DS S, const int *t
             // (edge a = q / 36: the j (a = 0) or k (a = 1) neighbours; pencil
             // e = q / 9 % 4 along the edge; row n = q % 9)
             const int l64 = threadIdx.x - 320;
-            const double *dummy = dq + widx(B, min(max(sa - j0 - k0, 2), B.I), j0, k0);
+            const double *dummy = xq + widx(B, min(max(sa - j0 - k0, 2), B.I), j0, k0) * NPV;
             auto item = [&](int q, int s, bool &need) -> const double * {
                 const int a = q / (4 * NPV), e = (q / NPV) % 4, n = q % NPV;
                 const int g = a == 0 ? e * TJ + (FWD ? 0 : TJ - 1) : (FWD ? 0 : TK - 1) * TJ + e;
                 const int jj = j0 + g % TJ, kk = k0 + g / TJ;
                 const int jn = a == 0 ? jj - sg : jj, kn = a == 0 ? kk : kk - sg;
                 need = q < NEDGE && n < npv && active(g, s) && jn >= 2 && jn <= B.J && kn >= 2 && kn <= B.K;
-                return need ? dq + n * NW + widx(B, s - jj - kk, jn, kn) : dummy;
+                return need ? xq + widx(B, s - jj - kk, jn, kn) * NPV + n : dummy;
             };
             bool n0, n1;
             const double *p0, *p1;
@@ -1225,7 +1229,7 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_sweep_tile(DS S, const int *t
                 if constexpr (FWD) ylds[s % YR][grp][lane] = y;
                 // the pencil wavefronts have no loads in flight, so this store
                 // costs only its issue
-                if (edge && act && mine) st_dev(dq + lane * NW + widx(B, i, j, k), xn);
+                if (edge && act && mine) st_dev(xq + widx(B, i, j, k) * NPV + lane, xn);
             }
             __builtin_amdgcn_sched_barrier(0);
             // coefficients of the next step (independent of dq): this cell's
@@ -1274,7 +1278,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         cell_of(B, t - B.c0, i, j, k);
         long g = gidx(B, i, j, k), w = widx(B, i, j, k);
         double p[NPV], old[NPV];
-        for (int n = 0; n < NPV; n++) if (n < npv) { old[n] = S.pv[n * S.NG + g]; p[n] = old[n] + S.dqb[n * S.NW + w]; }
+        for (int n = 0; n < NPV; n++) if (n < npv) { old[n] = S.pv[n * S.NG + g]; p[n] = old[n] + S.xqb[w * NPV + n]; }
         p[0] = mx(-P.pref + 1.0e1, p[0]);
         p[5] = mn(1.0, mx(0.0, p[5]));
         p[6] = mn(1.0, mx(0.0, p[6]));
@@ -1732,9 +1736,9 @@ int launch_sweeps(cav3d_ctx *c)
 {
     DS &S = c->S;
     hipLaunchKernelGGL(k_lhs_prep, dim3(nblk(S.NC, 128)), dim3(128), 0, c->st, S);
-    const long nq = (long)S.npv * S.NW;
-    hipLaunchKernelGGL(k_fill_bits, dim3(nblk(nq, 256)), dim3(256), 0, c->st, S.dqf, nq, DQ_UNSET);
-    hipLaunchKernelGGL(k_fill_bits, dim3(nblk(nq, 256)), dim3(256), 0, c->st, S.dqb, nq, DQ_UNSET);
+    const long nq = (long)NPV * S.NW;
+    hipLaunchKernelGGL(k_fill_bits, dim3(nblk(nq, 256)), dim3(256), 0, c->st, S.xqf, nq, DQ_UNSET);
+    hipLaunchKernelGGL(k_fill_bits, dim3(nblk(nq, 256)), dim3(256), 0, c->st, S.xqb, nq, DQ_UNSET);
     const int *tiles = c->tiles;
     int ntiles = c->ntiles;
     int *err = c->sweep_err;
@@ -1906,7 +1910,8 @@ int cav3d_create(const cav3d_params *prm, int nblocks, const cav3d_block *blocks
     if ((rc = dalloc(c, &S.pv, npv * S.NG)) || (rc = dalloc(c, &S.dc, (long)ND * S.NG)) ||
         (rc = dalloc(c, &S.vol, S.NW)) || (rc = dalloc(c, &c->xc, 3 * S.NW)) || (rc = dalloc(c, &S.cx, 3 * S.NCX)) || (rc = dalloc(c, &S.ex, 3 * S.NEX)) ||
         (rc = dalloc(c, &S.tx, 3 * S.NTX)) || (rc = dalloc(c, &S.res, npv * S.NC)) || (rc = dalloc(c, &S.dt, S.NC)) ||
-        (rc = dalloc(c, &S.dqf, npv * S.NW)) || (rc = dalloc(c, &S.dqb, npv * S.NW)) || (rc = dalloc(c, &S.dcv, npv * S.NW)) ||
+        (rc = dalloc(c, &S.dcv, npv * S.NW)) ||
+        (rc = dalloc(c, &S.xqf, (long)NPV * S.NW)) || (rc = dalloc(c, &S.xqb, (long)NPV * S.NW)) ||
         (rc = dalloc(c, &S.scal, 32))) { cav3d_destroy(c); return rc; }
     for (int d = 0; d < 3; d++)
         if ((rc = dalloc(c, &S.ff[d], 2 * npv * S.NF[d]))) { cav3d_destroy(c); return rc; }
