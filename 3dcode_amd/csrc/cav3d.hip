@@ -995,17 +995,37 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_sweep_tile(DS S, const int *t
             // pencils' dq, the update's input.
             auto publish = [&](int s) {
                 const int pb = s & 1;
-                for (int e = threadIdx.x - 256; e < TJ * TK * NPV; e += 64) {
-                    const int g = e / NPV, n = e - g * NPV;
-                    if (n < npv && active(g, s)) {
-                        const int jj = j0 + g % TJ, kk = k0 + g / TJ, i = s - jj - kk;
-                        if constexpr (FWD) {
-                            const int m = (i - 2) & 3;          // cells i - m .. i of the window
-                            if (m == 3 || i == B.I) {
+                if constexpr (FWD) {
+                    // the four pencils whose window closes at hyperplane s
+                    // (j + k = s - 5 mod 4) write its four cells, formed at
+                    // hyperplanes s - 3 .. s; lane -> (pencil, row, cell), so
+                    // that four adjacent lanes write one row's 32 bytes
+                    const int r0 = (s - j0 - k0 - 5) & 3;
+                    for (int v = threadIdx.x - 256; v < 4 * NPV * 4; v += 64) {
+                        const int p = v / (4 * NPV), n = (v >> 2) % NPV, c = v & 3;
+                        // the class's pencil with gj = p: gj + gk = r0, or r0 + 4 past the tile's anti-diagonal
+                        const int gj = p, gk = r0 + 4 * (p > r0) - p;
+                        const int jj = j0 + gj, kk = k0 + gk, i = s - jj - kk;
+                        if (n < npv && jj <= jend && kk <= kend && i >= 5 && i <= B.I)
+                            __builtin_nontemporal_store(ylds[(s - 3 + c) % YR][gj + TJ * gk][n], S.dcv + n * NW + widx(B, i - 3 + c, jj, kk));
+                    }
+                    // a pencil that ends at hyperplane s in the middle of a window
+                    if (s >= B.I + j0 + k0) {
+                        for (int e = threadIdx.x - 256; e < TJ * TK * NPV; e += 64) {
+                            const int g = e / NPV, n = e - g * NPV;
+                            const int jj = j0 + g % TJ, kk = k0 + g / TJ, i = s - jj - kk;
+                            const int m = (i - 2) & 3;
+                            if (n < npv && i == B.I && m != 3 && jj <= jend && kk <= kend) {
                                 double *o = S.dcv + n * NW + widx(B, i - m, jj, kk);
                                 for (int c = 0; c <= m; c++) __builtin_nontemporal_store(ylds[(s - m + c) % YR][g][n], o + c);
                             }
-                        } else if (!down_edge(g)) {
+                        }
+                    }
+                } else {
+                    for (int e = threadIdx.x - 256; e < TJ * TK * NPV; e += 64) {
+                        const int g = e / NPV, n = e - g * NPV;
+                        if (n < npv && active(g, s) && !down_edge(g)) {
+                            const int jj = j0 + g % TJ, kk = k0 + g / TJ, i = s - jj - kk;
                             xq[widx(B, i, jj, kk) * NPV + n] = lds(pb, g)[n];
                         }
                     }
