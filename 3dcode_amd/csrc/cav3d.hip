@@ -1067,22 +1067,41 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_sweep_tile(DS S, const int *t
                     __builtin_amdgcn_s_sleep(1);
                 }
                 if (s != sb) {
+                    // the eight pencils with d & 3 = e in {0, 1}: four per
+                    // class, pencil p of a class at gj = p on the class's
+                    // anti-diagonals t = c, c + 4 (no branch, so every
+                    // pointer read is issued before the first is waited for)
+                    const double *pp[2][4][2];
+                    int gg[2][4], ww[2][4];
 #pragma unroll
-                    for (int g = 0; g < TJ * TK; g++) {
-                        const int t = (g & 3) + (g >> 2), d = FWD ? d0 - t : d0 + t;
-                        if ((d & 3) > 1) continue;
-                        const int w1 = (d >> 2) + 1;
-                        // first cell of the lane's pair; a pair that lies
-                        // beyond the pencil is clamped to cells 1..I+1 (its
-                        // slots hold no cell of the pencil)
-                        const int c0 = min(max((FWD ? 2 + 4 * w1 : BI - 4 * w1 - 3) + lc, 1), BI);
+                    for (int e = 0; e < 2; e++) {
+                        const int c = (FWD ? d0 - e : e - d0) & 3;
 #pragma unroll
-                        for (int h = 0; h < 2; h++) {
-                            const int q = 2 * (d & 3) + h;
-                            const double *from = src[g][32 * q + li] + c0;
-                            __builtin_amdgcn_global_load_lds((const GLOBAL void *)from,
-                                                             (__attribute__((address_space(3))) void *)&win[w1 & 1][g][32 * q][0],
-                                                             16, 0, 0);
+                        for (int p = 0; p < 4; p++) {
+                            const int t = c + 4 * (p > c), g = p + TJ * (t - p);
+                            const int d = FWD ? d0 - t : d0 + t;
+                            gg[e][p] = g;
+                            ww[e][p] = (d >> 2) + 1;
+#pragma unroll
+                            for (int h = 0; h < 2; h++) pp[e][p][h] = src[g][32 * (2 * e + h) + li];
+                        }
+                    }
+#pragma unroll
+                    for (int e = 0; e < 2; e++) {
+#pragma unroll
+                        for (int p = 0; p < 4; p++) {
+                            const int g = gg[e][p], w1 = ww[e][p];
+                            // first cell of the lane's pair; a pair that lies
+                            // beyond the pencil is clamped to cells 1..I+1 (its
+                            // slots hold no cell of the pencil)
+                            const int c0 = min(max((FWD ? 2 + 4 * w1 : BI - 4 * w1 - 3) + lc, 1), BI);
+#pragma unroll
+                            for (int h = 0; h < 2; h++) {
+                                const int q = 2 * e + h;
+                                __builtin_amdgcn_global_load_lds((const GLOBAL void *)(pp[e][p][h] + c0),
+                                                                 (__attribute__((address_space(3))) void *)&win[w1 & 1][g][32 * q][0],
+                                                                 16, 0, 0);
+                            }
                         }
                     }
                 }
