@@ -520,7 +520,7 @@ __device__ __forceinline__ long stencil_off(const BlkD &B, int ll)
 // pressure keeps the low-Mach pressure differences; the metastable-vapour
 // switch compares against the FP64 saturation pressure).
 template <int D, bool SENSOR, class R>
-__global__ __launch_bounds__(128) void k_face(DS S)
+__global__ __launch_bounds__(256) void k_face(DS S)
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= S.NF[D]) return;
@@ -562,7 +562,7 @@ __global__ __launch_bounds__(128) void k_face(DS S)
 // Viscous flux of every face of direction D (src/vsflux_module.f90:182-398),
 // with the metric averaging of src/rhs_module.f90:340-357.  R as k_face.
 template <int D, class R>
-__global__ __launch_bounds__(128) void k_vface(DS S)
+__global__ __launch_bounds__(256) void k_vface(DS S)
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= S.NF[D]) return;
@@ -1736,9 +1736,9 @@ int launch_rhs(cav3d_ctx *c)
     const bool mixed = c->prm.precision == CAV3D_MIXED && S.P.nscheme != 3;
 #define CAV3D_FACES(SEN, R)                                                                             \
     do {                                                                                                \
-        hipLaunchKernelGGL((k_face<0, SEN, R>), dim3(nblk(S.NF[0], 128)), dim3(128), 0, c->st, S);      \
-        hipLaunchKernelGGL((k_face<1, SEN, R>), dim3(nblk(S.NF[1], 128)), dim3(128), 0, c->st, S);      \
-        hipLaunchKernelGGL((k_face<2, SEN, R>), dim3(nblk(S.NF[2], 128)), dim3(128), 0, c->st, S);      \
+        hipLaunchKernelGGL((k_face<0, SEN, R>), dim3(nblk(S.NF[0], 256)), dim3(256), 0, c->st, S);      \
+        hipLaunchKernelGGL((k_face<1, SEN, R>), dim3(nblk(S.NF[1], 256)), dim3(256), 0, c->st, S);      \
+        hipLaunchKernelGGL((k_face<2, SEN, R>), dim3(nblk(S.NF[2], 256)), dim3(256), 0, c->st, S);      \
     } while (0)
     if (sensor && mixed) CAV3D_FACES(true, float);
     else if (sensor) CAV3D_FACES(true, double);
@@ -1747,17 +1747,17 @@ int launch_rhs(cav3d_ctx *c)
 #undef CAV3D_FACES
     if (c->prm.iturb >= -2) {
         if (c->prm.precision == CAV3D_MIXED) {
-            hipLaunchKernelGGL((k_vface<0, float>), dim3(nblk(S.NF[0], 128)), dim3(128), 0, c->st, S);
-            hipLaunchKernelGGL((k_vface<1, float>), dim3(nblk(S.NF[1], 128)), dim3(128), 0, c->st, S);
-            hipLaunchKernelGGL((k_vface<2, float>), dim3(nblk(S.NF[2], 128)), dim3(128), 0, c->st, S);
+            hipLaunchKernelGGL((k_vface<0, float>), dim3(nblk(S.NF[0], 256)), dim3(256), 0, c->st, S);
+            hipLaunchKernelGGL((k_vface<1, float>), dim3(nblk(S.NF[1], 256)), dim3(256), 0, c->st, S);
+            hipLaunchKernelGGL((k_vface<2, float>), dim3(nblk(S.NF[2], 256)), dim3(256), 0, c->st, S);
         } else {
-            hipLaunchKernelGGL((k_vface<0, double>), dim3(nblk(S.NF[0], 128)), dim3(128), 0, c->st, S);
-            hipLaunchKernelGGL((k_vface<1, double>), dim3(nblk(S.NF[1], 128)), dim3(128), 0, c->st, S);
-            hipLaunchKernelGGL((k_vface<2, double>), dim3(nblk(S.NF[2], 128)), dim3(128), 0, c->st, S);
+            hipLaunchKernelGGL((k_vface<0, double>), dim3(nblk(S.NF[0], 256)), dim3(256), 0, c->st, S);
+            hipLaunchKernelGGL((k_vface<1, double>), dim3(nblk(S.NF[1], 256)), dim3(256), 0, c->st, S);
+            hipLaunchKernelGGL((k_vface<2, double>), dim3(nblk(S.NF[2], 256)), dim3(256), 0, c->st, S);
         }
     }
-    if (S.P.ncav >= 2 || S.P.ides > 0) hipLaunchKernelGGL(k_combine<true>, dim3(nblk(S.NC, 128)), dim3(128), 0, c->st, S);
-    else hipLaunchKernelGGL(k_combine<false>, dim3(nblk(S.NC, 128)), dim3(128), 0, c->st, S);
+    if (S.P.ncav >= 2 || S.P.ides > 0) hipLaunchKernelGGL(k_combine<true>, dim3(nblk(S.NC, 256)), dim3(256), 0, c->st, S);
+    else hipLaunchKernelGGL(k_combine<false>, dim3(nblk(S.NC, 256)), dim3(256), 0, c->st, S);
     return 0;
 }
 
