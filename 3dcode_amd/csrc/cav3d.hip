@@ -1069,8 +1069,9 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_sweep_tile(DS S, const int *t
                 if (s != sb) {
                     // the eight pencils with d & 3 = e in {0, 1}: four per
                     // class, pencil p of a class at gj = p on the class's
-                    // anti-diagonals t = c, c + 4 (no branch, so every
-                    // pointer read is issued before the first is waited for)
+                    // anti-diagonals t = c, c + 4 (no branch between them, so
+                    // the pointer reads can be issued ahead of the transfers
+                    // that use them, not each waited for in turn)
                     const double *pp[2][4][2];
                     int gg[2][4], ww[2][4];
 #pragma unroll
