@@ -1512,6 +1512,10 @@ struct cav3d_ctx {
     std::vector<Con> cons;
     std::vector<std::vector<int>> bcdesc;   // per block: nbc*7 raw
     hipStream_t st = nullptr;
+    // the residual's j and k directions run on two more streams, forked from
+    // and joined back to st by events (default priority, like st)
+    hipStream_t side[2] = {};
+    hipEvent_t fork = nullptr, join[2] = {};
     int dev = 0;              // device current at creation (the context's device)
     double *stage = nullptr;
     size_t stage_n = 0;
@@ -1735,11 +1739,18 @@ int launch_rhs(cav3d_ctx *c)
     // AUSMPW+ stays FP64 in the mixed mode: its pressure weight
     // (p + p_ref + rho c^2)/(rho c^2) - 1 is ~1e-5 in water and cancels in FP32
     const bool mixed = c->prm.precision == CAV3D_MIXED && S.P.nscheme != 3;
+    // the three directions are independent (each writes its own face
+    // fluxes): i on st, j and k on the two side streams, joined before the
+    // combine, so that one direction's tail overlaps the others
+    hipStream_t sd[3] = {c->st, c->side[0], c->side[1]};
+    hipEventRecord(c->fork, c->st);
+    hipStreamWaitEvent(c->side[0], c->fork, 0);
+    hipStreamWaitEvent(c->side[1], c->fork, 0);
 #define CAV3D_FACES(SEN, R)                                                                             \
     do {                                                                                                \
-        hipLaunchKernelGGL((k_face<0, SEN, R>), dim3(nblk(S.NF[0], 256)), dim3(256), 0, c->st, S);      \
-        hipLaunchKernelGGL((k_face<1, SEN, R>), dim3(nblk(S.NF[1], 256)), dim3(256), 0, c->st, S);      \
-        hipLaunchKernelGGL((k_face<2, SEN, R>), dim3(nblk(S.NF[2], 256)), dim3(256), 0, c->st, S);      \
+        hipLaunchKernelGGL((k_face<0, SEN, R>), dim3(nblk(S.NF[0], 256)), dim3(256), 0, sd[0], S);      \
+        hipLaunchKernelGGL((k_face<1, SEN, R>), dim3(nblk(S.NF[1], 256)), dim3(256), 0, sd[1], S);      \
+        hipLaunchKernelGGL((k_face<2, SEN, R>), dim3(nblk(S.NF[2], 256)), dim3(256), 0, sd[2], S);      \
     } while (0)
     if (sensor && mixed) CAV3D_FACES(true, float);
     else if (sensor) CAV3D_FACES(true, double);
@@ -1748,27 +1759,41 @@ int launch_rhs(cav3d_ctx *c)
 #undef CAV3D_FACES
     if (c->prm.iturb >= -2) {
         if (c->prm.precision == CAV3D_MIXED) {
-            hipLaunchKernelGGL((k_vface<0, float>), dim3(nblk(S.NF[0], 256)), dim3(256), 0, c->st, S);
-            hipLaunchKernelGGL((k_vface<1, float>), dim3(nblk(S.NF[1], 256)), dim3(256), 0, c->st, S);
-            hipLaunchKernelGGL((k_vface<2, float>), dim3(nblk(S.NF[2], 256)), dim3(256), 0, c->st, S);
+            hipLaunchKernelGGL((k_vface<0, float>), dim3(nblk(S.NF[0], 256)), dim3(256), 0, sd[0], S);
+            hipLaunchKernelGGL((k_vface<1, float>), dim3(nblk(S.NF[1], 256)), dim3(256), 0, sd[1], S);
+            hipLaunchKernelGGL((k_vface<2, float>), dim3(nblk(S.NF[2], 256)), dim3(256), 0, sd[2], S);
         } else {
-            hipLaunchKernelGGL((k_vface<0, double>), dim3(nblk(S.NF[0], 256)), dim3(256), 0, c->st, S);
-            hipLaunchKernelGGL((k_vface<1, double>), dim3(nblk(S.NF[1], 256)), dim3(256), 0, c->st, S);
-            hipLaunchKernelGGL((k_vface<2, double>), dim3(nblk(S.NF[2], 256)), dim3(256), 0, c->st, S);
+            hipLaunchKernelGGL((k_vface<0, double>), dim3(nblk(S.NF[0], 256)), dim3(256), 0, sd[0], S);
+            hipLaunchKernelGGL((k_vface<1, double>), dim3(nblk(S.NF[1], 256)), dim3(256), 0, sd[1], S);
+            hipLaunchKernelGGL((k_vface<2, double>), dim3(nblk(S.NF[2], 256)), dim3(256), 0, sd[2], S);
         }
     }
+    hipEventRecord(c->join[0], c->side[0]);
+    hipEventRecord(c->join[1], c->side[1]);
+    hipStreamWaitEvent(c->st, c->join[0], 0);
+    hipStreamWaitEvent(c->st, c->join[1], 0);
     if (S.P.ncav >= 2 || S.P.ides > 0) hipLaunchKernelGGL(k_combine<true>, dim3(nblk(S.NC, 256)), dim3(256), 0, c->st, S);
     else hipLaunchKernelGGL(k_combine<false>, dim3(nblk(S.NC, 256)), dim3(256), 0, c->st, S);
     return 0;
 }
 
-int launch_timestep(cav3d_ctx *c)
+int launch_timestep(cav3d_ctx *c, hipStream_t st)
 {
     DS &S = c->S;
-    hipLaunchKernelGGL(k_reset_scal, dim3(1), dim3(64), 0, c->st, S);
-    hipLaunchKernelGGL(k_timestep, dim3(nblk(S.NC, 256)), dim3(256), 0, c->st, S);
-    if (c->prm.local == 0) hipLaunchKernelGGL(k_dt_fill, dim3(nblk(S.NC, 256)), dim3(256), 0, c->st, S);
-    hipLaunchKernelGGL(k_time_acc, dim3(1), dim3(64), 0, c->st, S);
+    hipLaunchKernelGGL(k_reset_scal, dim3(1), dim3(64), 0, st, S);
+    hipLaunchKernelGGL(k_timestep, dim3(nblk(S.NC, 256)), dim3(256), 0, st, S);
+    if (c->prm.local == 0) hipLaunchKernelGGL(k_dt_fill, dim3(nblk(S.NC, 256)), dim3(256), 0, st, S);
+    hipLaunchKernelGGL(k_time_acc, dim3(1), dim3(64), 0, st, S);
+    return 0;
+}
+
+// the sweeps' exchange arrays back to "not yet written"
+int launch_sweep_fills(cav3d_ctx *c, hipStream_t st)
+{
+    DS &S = c->S;
+    const long nq = (long)NPV * S.NW;
+    hipLaunchKernelGGL(k_fill_bits, dim3(nblk(nq, 256)), dim3(256), 0, st, S.xqf, nq, DQ_UNSET);
+    hipLaunchKernelGGL(k_fill_bits, dim3(nblk(nq, 256)), dim3(256), 0, st, S.xqb, nq, DQ_UNSET);
     return 0;
 }
 
@@ -1776,9 +1801,6 @@ int launch_sweeps(cav3d_ctx *c)
 {
     DS &S = c->S;
     hipLaunchKernelGGL(k_lhs_prep, dim3(nblk(S.NC, 128)), dim3(128), 0, c->st, S);
-    const long nq = (long)NPV * S.NW;
-    hipLaunchKernelGGL(k_fill_bits, dim3(nblk(nq, 256)), dim3(256), 0, c->st, S.xqf, nq, DQ_UNSET);
-    hipLaunchKernelGGL(k_fill_bits, dim3(nblk(nq, 256)), dim3(256), 0, c->st, S.xqb, nq, DQ_UNSET);
     const int *tiles = c->tiles;
     int ntiles = c->ntiles;
     int *err = c->sweep_err;
@@ -1866,6 +1888,13 @@ int cav3d_create(const cav3d_params *prm, int nblocks, const cav3d_block *blocks
     P.emutref = prm->emutref;
     P.bdf_b = prm->nsteady == 1 ? 1.0 : (prm->nsteady == 2 ? 1.5 * 0.5 : 0.0);
     if (hipStreamCreate(&c->st) != hipSuccess) { delete c; return fail("hipStreamCreate failed", -3); }
+    for (int q = 0; q < 2; q++)
+        if (hipStreamCreateWithFlags(&c->side[q], hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&c->join[q], hipEventDisableTiming) != hipSuccess) {
+            cav3d_destroy(c);
+            return fail("stream set-up failed", -3);
+        }
+    if (hipEventCreateWithFlags(&c->fork, hipEventDisableTiming) != hipSuccess) { cav3d_destroy(c); return fail("stream set-up failed", -3); }
     DS &S = c->S;
     memset(&S, 0, sizeof(S));
     S.nb = nblocks;
@@ -2030,6 +2059,11 @@ void cav3d_destroy(cav3d_ctx *c)
     if (c->xbuf) hipFree(c->xbuf);
     if (c->a_obj) hipFree(c->a_obj);
     for (int q = 0; q < 14; q++) if (c->ev[q]) hipEventDestroy(c->ev[q]);
+    for (int q = 0; q < 2; q++) {
+        if (c->side[q]) { hipStreamSynchronize(c->side[q]); hipStreamDestroy(c->side[q]); }
+        if (c->join[q]) hipEventDestroy(c->join[q]);
+    }
+    if (c->fork) hipEventDestroy(c->fork);
     if (c->st) hipStreamDestroy(c->st);
     delete c;
 }
@@ -2151,7 +2185,7 @@ int cav3d_setbc(cav3d_ctx *c)
 int cav3d_timestep(cav3d_ctx *c, int b, double *dt)
 {
     if (!c || b < 0 || b >= c->nb) return fail("bad argument to cav3d_timestep", -1);
-    launch_timestep(c);
+    launch_timestep(c, c->st);
     const BlkD &B = c->blk[b];
     if (dt) HIPCHK(hipMemcpyAsync(dt, c->S.dt + B.c0, B.nc * sizeof(double), hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
@@ -2192,7 +2226,15 @@ int cav3d_timeinteg(cav3d_ctx *c, int nt_phy, int nt, double timeprev, double *t
     (void)nt;
     if (!c) return fail("null context", -1);
     if (c->profile) hipEventRecord(c->ev[0], c->st);
-    launch_timestep(c);
+    // the time step and the sweeps' exchange-array fills read nothing the
+    // boundary fill writes (interior cells and metrics only): they run on the
+    // side streams meanwhile, joined again before the combine (launch_rhs),
+    // which precedes every reader of dt and of the exchange arrays
+    hipEventRecord(c->fork, c->st);
+    hipStreamWaitEvent(c->side[0], c->fork, 0);
+    hipStreamWaitEvent(c->side[1], c->fork, 0);
+    launch_sweep_fills(c, c->side[0]);
+    launch_timestep(c, c->side[1]);
     if (c->profile) hipEventRecord(c->ev[1], c->st);
     launch_setbc(c);
     if (c->profile) hipEventRecord(c->ev[2], c->st);
