@@ -615,26 +615,32 @@ int launch_vjp(cav3d_ctx *c, int dmut = 0)
     A.mask = adj_mask(S.P);
     const bool sensor = S.P.nscheme == 2 || S.P.nscheme == 3;
     const int npv = S.npv;
-    if (sensor) {
-        hipLaunchKernelGGL((k_face_vjp<0, true>), dim3(nblk(S.NF[0], 128)), dim3(128), 0, c->st, S, A);
-        hipLaunchKernelGGL((k_face_vjp<1, true>), dim3(nblk(S.NF[1], 128)), dim3(128), 0, c->st, S, A);
-        hipLaunchKernelGGL((k_face_vjp<2, true>), dim3(nblk(S.NF[2], 128)), dim3(128), 0, c->st, S, A);
-        hipLaunchKernelGGL((k_face_rvjp<0, true>), dim3(nblk(npv * S.NF[0], 256)), dim3(256), 0, c->st, S, A);
-        hipLaunchKernelGGL((k_face_rvjp<1, true>), dim3(nblk(npv * S.NF[1], 256)), dim3(256), 0, c->st, S, A);
-        hipLaunchKernelGGL((k_face_rvjp<2, true>), dim3(nblk(npv * S.NF[2], 256)), dim3(256), 0, c->st, S, A);
-    } else {
-        hipLaunchKernelGGL((k_face_vjp<0, false>), dim3(nblk(S.NF[0], 128)), dim3(128), 0, c->st, S, A);
-        hipLaunchKernelGGL((k_face_vjp<1, false>), dim3(nblk(S.NF[1], 128)), dim3(128), 0, c->st, S, A);
-        hipLaunchKernelGGL((k_face_vjp<2, false>), dim3(nblk(S.NF[2], 128)), dim3(128), 0, c->st, S, A);
-        hipLaunchKernelGGL((k_face_rvjp<0, false>), dim3(nblk(npv * S.NF[0], 256)), dim3(256), 0, c->st, S, A);
-        hipLaunchKernelGGL((k_face_rvjp<1, false>), dim3(nblk(npv * S.NF[1], 256)), dim3(256), 0, c->st, S, A);
-        hipLaunchKernelGGL((k_face_rvjp<2, false>), dim3(nblk(npv * S.NF[2], 256)), dim3(256), 0, c->st, S, A);
-    }
-    if (S.P.iturb >= -2) {
-        hipLaunchKernelGGL((k_vface_vjp<0>), dim3(nblk(S.NF[0], 128)), dim3(128), 0, c->st, S, A);
-        hipLaunchKernelGGL((k_vface_vjp<1>), dim3(nblk(S.NF[1], 128)), dim3(128), 0, c->st, S, A);
-        hipLaunchKernelGGL((k_vface_vjp<2>), dim3(nblk(S.NF[2], 128)), dim3(128), 0, c->st, S, A);
-    }
+    // the three directions write their own face bars: i on st, j and k on
+    // the side streams, joined before the cell sources and the gather
+    hipStream_t sd[3] = {c->st, c->side[0], c->side[1]};
+    hipEventRecord(c->fork, c->st);
+    hipStreamWaitEvent(c->side[0], c->fork, 0);
+    hipStreamWaitEvent(c->side[1], c->fork, 0);
+#define CAV3D_VJP_DIR(D)                                                                                             \
+    do {                                                                                                             \
+        if (sensor) {                                                                                                \
+            hipLaunchKernelGGL((k_face_vjp<D, true>), dim3(nblk(S.NF[D], 128)), dim3(128), 0, sd[D], S, A);          \
+            hipLaunchKernelGGL((k_face_rvjp<D, true>), dim3(nblk(npv * S.NF[D], 256)), dim3(256), 0, sd[D], S, A);   \
+        } else {                                                                                                     \
+            hipLaunchKernelGGL((k_face_vjp<D, false>), dim3(nblk(S.NF[D], 128)), dim3(128), 0, sd[D], S, A);         \
+            hipLaunchKernelGGL((k_face_rvjp<D, false>), dim3(nblk(npv * S.NF[D], 256)), dim3(256), 0, sd[D], S, A);  \
+        }                                                                                                            \
+        if (S.P.iturb >= -2) hipLaunchKernelGGL((k_vface_vjp<D>), dim3(nblk(S.NF[D], 128)), dim3(128), 0, sd[D], S, A); \
+    } while (0)
+    CAV3D_VJP_DIR(1);
+    CAV3D_VJP_DIR(2);
+    CAV3D_VJP_DIR(0);
+#undef CAV3D_VJP_DIR
+    hipEventRecord(c->join[0], c->side[0]);
+    hipEventRecord(c->join[1], c->side[1]);
+    hipStreamWaitEvent(c->st, c->join[0], 0);
+    hipStreamWaitEvent(c->st, c->join[1], 0);
+    // the cell sources read every direction's eddy-viscosity bars
     if (S.P.ides > 0 || S.P.ncav >= 2) hipLaunchKernelGGL(k_cell_vjp<true>, dim3(nblk(S.NC, 128)), dim3(128), 0, c->st, S, A);
     else hipLaunchKernelGGL(k_cell_vjp<false>, dim3(nblk(S.NC, 128)), dim3(128), 0, c->st, S, A);
     hipLaunchKernelGGL(k_adj_gather, dim3(nblk(S.NG, 256)), dim3(256), 0, c->st, S, A);
