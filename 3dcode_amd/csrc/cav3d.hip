@@ -1820,7 +1820,13 @@ int launch_update(cav3d_ctx *c)
 {
     DS &S = c->S;
     hipLaunchKernelGGL(k_update, dim3(c->nwg_update), dim3(256), 0, c->st, S);
-    hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(256), 0, c->st, S, c->nwg_update);
+    // the final norm reduction reads only the update's partials: it runs on
+    // a side stream beside the eddy viscosity, and reaches st again at the
+    // next residual's join (before the next update rewrites the partials);
+    // cav3d_residual waits for it
+    hipEventRecord(c->fork, c->st);
+    hipStreamWaitEvent(c->side[0], c->fork, 0);
+    hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(256), 0, c->side[0], S, c->nwg_update);
     if (c->prm.iturb == 0) hipLaunchKernelGGL(k_eddy, dim3(nblk(S.NC, 128)), dim3(128), 0, c->st, S);
     return 0;
 }
@@ -2272,6 +2278,7 @@ int cav3d_residual(cav3d_ctx *c, double *l2, int *ijk, double *resmax)
 {
     if (!c) return fail("null context", -1);
     double h[32];
+    HIPCHK(hipStreamSynchronize(c->side[0]));     // the norms' final reduction
     HIPCHK(hipMemcpyAsync(h, c->S.scal, sizeof(h), hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
     int rc = sweep_status(c);
