@@ -955,7 +955,6 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_sweep_tile(DS S, const int *t
     __shared__ double win[2][TJ * TK][ST_N][WIN];   // stage windows (see the pencil wavefronts)
     __shared__ const double *src[TJ * TK][ST_N];
     __shared__ int gave_up;
-    __shared__ int polled;      // steps whose cross-tile reads the poll wavefront has issued (counts up over the tile)
     const int npv = S.npv;
     const int sg = FWD ? 1 : -1;
     const double sgn = FWD ? 1.0 : -1.0;
@@ -1031,7 +1030,6 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_sweep_tile(DS S, const int *t
                     }
                 }
             };
-            if (threadIdx.x == 256) polled = 0;
             __syncthreads();
             for (int s = sa;; s += sg) {
                 if (s != sa) publish(s - sg);
@@ -1054,18 +1052,10 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_sweep_tile(DS S, const int *t
             const int BI = B.I;
             __syncthreads();
             int d0 = FWD ? sa - j0 - k0 - 2 : BI - (sa - j0 - k0);      // distance of pencil 0 (it grows by one a step)
-            int nstep = 0;
             for (int s = sa;; s += sg) {
-                // the poll wavefront's reads of this step enter the memory
-                // pipeline first, then the transfers
-                nstep++;
-                for (long spins = 0; __hip_atomic_load(&polled, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) < nstep;) {
-                    if (++spins > POLL_LIMIT) {      // cannot happen: the poll wavefront counts every step
-                        gave_up = 1;
-                        break;
-                    }
-                    __builtin_amdgcn_s_sleep(1);
-                }
+                // the transfers are issued at once (issuing sixteen of them
+                // takes longer than the poll wavefront's two reads; holding
+                // them back behind those reads cost 10 us a sweep)
                 if (s != sb) {
                     // the eight pencils with d & 3 = e in {0, 1}: four per
                     // class, pencil p of a class at gj = p on the class's
@@ -1172,7 +1162,6 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_sweep_tile(DS S, const int *t
                 // the next step's values are read at the start of this step,
                 // while the pencils compute, as late as they can be
                 if (s != sb) issue(s + sg);
-                if (threadIdx.x == 320) __hip_atomic_fetch_add(&polled, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                 if (s != sb) deliver(s + sg);
                 lds_barrier();
                 if (s == sb) break;
